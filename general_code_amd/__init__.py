@@ -1,0 +1,9 @@
+"""general_code_amd -- MI355X-native Maximal Abelian gauge fixing.
+
+The compute path is libmag_gpu.so (hand-written gfx950 HIP kernels behind the
+C ABI of include/mag_gpu.h); this package is its Python front end.
+"""
+from .mag import (  # noqa: F401
+    F64, MIXED, SU2, SU3, MAG_OK, MagError, MagJob, MagParams, NotConverged,
+    device_count, lib, plaquette, su2_functional, su3_functional,
+)

@@ -1,0 +1,628 @@
+// mag_capi.hip -- C ABI of the MAG gauge-fixing path (include/mag_gpu.h).
+//
+// Host-side job management: device buffers, the per-sweep launch sequence
+// (red half-sweep, black half-sweep, one-block reduction), convergence by
+// device-resident flags (no host synchronisation per sweep), and the
+// mixed-precision schedule (FP32 sweeps with an accumulated site gauge,
+// FP64 rebuild from the FP64 master every reunit_every sweeps, FP64 rebuild
+// and FP64 polish at the end).
+#include "mag_gpu.h"
+#include "mag_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using mag::Geom;
+using mag::JobCtl;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                 \
+  do {                                                                            \
+    hipError_t e_ = (x);                                                          \
+    if (e_ != hipSuccess)                                                         \
+      return fail(e_ == hipErrorOutOfMemory ? MAG_ERR_NOMEM : MAG_ERR_HIP,        \
+                  "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__,   \
+                  __LINE__);                                                      \
+  } while (0)
+
+#define CHK(x)              \
+  do {                      \
+    int r_ = (x);           \
+    if (r_ != MAG_OK) return r_; \
+  } while (0)
+
+constexpr int kChunk = 20;  // sweeps enqueued between convergence polls
+
+int check_dims(const int* d, int64_t* V) {
+  if (!d) return fail(MAG_ERR_ARG, "dims is NULL");
+  int64_t v = 1;
+  for (int i = 0; i < 4; i++) {
+    if (d[i] < 2 || (d[i] & 1))
+      return fail(MAG_ERR_DIMS, "extent %d of dimension %d must be even and >= 2", d[i], i);
+    v *= d[i];
+  }
+  // 8 * Vh * 12 reals must index in size_t and Vh in uint32
+  if (v / 2 > (int64_t)0x7fffffff / 8) return fail(MAG_ERR_DIMS, "lattice too large");
+  *V = v;
+  return MAG_OK;
+}
+
+int check_params(const mag_params* p) {
+  if (!p) return fail(MAG_ERR_ARG, "params is NULL");
+  if (p->group != MAG_SU2 && p->group != MAG_SU3)
+    return fail(MAG_ERR_ARG, "group must be 2 or 3, got %d", p->group);
+  if (p->precision != MAG_PREC_F64 && p->precision != MAG_PREC_MIXED)
+    return fail(MAG_ERR_ARG, "precision must be MAG_PREC_F64 or MAG_PREC_MIXED");
+  if (!(p->omega > 0.0 && p->omega <= 2.0))
+    return fail(MAG_ERR_ARG, "omega must be in (0, 2], got %g", p->omega);
+  if (!(p->tol >= 0.0)) return fail(MAG_ERR_ARG, "tol must be >= 0");
+  if (p->max_sweeps < 1) return fail(MAG_ERR_ARG, "max_sweeps must be >= 1");
+  if (p->precision == MAG_PREC_MIXED) {
+    if (p->reunit_every < 1) return fail(MAG_ERR_ARG, "reunit_every must be >= 1");
+    if (!(p->tol_mixed >= 0.0)) return fail(MAG_ERR_ARG, "tol_mixed must be >= 0");
+  }
+  int64_t V;
+  return check_dims(p->dims, &V);
+}
+
+bool all_finite(const double* a, size_t n) {
+  for (size_t i = 0; i < n; i++)
+    if (!std::isfinite(a[i])) return false;
+  return true;
+}
+
+}  // namespace
+
+struct mag_job {
+  mag_params prm;
+  Geom g;
+  int64_t V = 0;
+  int nr = 0;      // reals per link in the SoA layout (4 or 12)
+  int nr_in = 0;   // reals per link in the reference layout (4 or 18)
+  size_t nreal = 0;  // reals in one link array (8 * Vh * nr)
+  int nb = 0;        // workgroups per half-sweep
+  hipStream_t st = nullptr;
+  double* U64 = nullptr;  // FP64 links (plain) / FP64 master then output (mixed)
+  float* W32 = nullptr;   // mixed: FP32 working links
+  float* G32 = nullptr;   // mixed: accumulated site gauge
+  double* part = nullptr;
+  JobCtl* ctl = nullptr;
+  double* hF = nullptr;
+  double* hR = nullptr;
+  double* hA = nullptr;
+  double* out3 = nullptr;
+  JobCtl* ctl_h = nullptr;   // pinned host mirrors
+  double* out3_h = nullptr;
+  bool loaded = false;
+  int phase = 0;             // mixed: 0 = FP32 sweeps, 1 = FP64 after rebuild
+  int fp64_start = -1;
+  int since_rebuild = 0;
+  double invF = 0, invV = 0;
+};
+
+namespace {
+
+bool mixed32(const mag_job* j) { return j->prm.precision == MAG_PREC_MIXED && j->phase == 0; }
+
+int set_device(const mag_job* j) {
+  HIPCHK(hipSetDevice(j->prm.device));
+  return MAG_OK;
+}
+
+void launch_half(mag_job* j, int parity) {
+  const int* done = &j->ctl->done;
+  if (mixed32(j)) {
+    if (j->prm.group == MAG_SU2)
+      mag::launch_su2_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, j->part, done, j->st);
+    else
+      mag::launch_su3_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, j->part, done, j->st);
+  } else {
+    if (j->prm.group == MAG_SU2)
+      mag::launch_su2_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, j->part, done, j->st);
+    else
+      mag::launch_su3_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, j->part, done, j->st);
+  }
+}
+
+void launch_rebuild(mag_job* j, bool final64) {
+  if (j->prm.group == MAG_SU2)
+    mag::launch_su2_rebuild(j->U64, j->G32, final64 ? nullptr : j->W32, final64 ? j->U64 : nullptr,
+                            j->g, j->st);
+  else
+    mag::launch_su3_rebuild(j->U64, j->G32, final64 ? nullptr : j->W32, final64 ? j->U64 : nullptr,
+                            j->g, j->st);
+  if (!final64) mag::launch_gauge_normalize(j->G32, j->g, j->prm.group, j->st);
+}
+
+// sweep tail shared by the plain and timed paths
+void enqueue_after_sweep(mag_job* j) {
+  mag::launch_reduce(j->part, 2 * j->nb, mag::RED_SWEEP, j->invF, j->invV, j->ctl, j->hF, j->hR,
+                     j->hA, nullptr, j->st);
+  if (mixed32(j) && ++j->since_rebuild >= j->prm.reunit_every) {
+    launch_rebuild(j, false);
+    j->since_rebuild = 0;
+  }
+}
+
+void enqueue_sweep(mag_job* j) {
+  launch_half(j, 0);
+  launch_half(j, 1);
+  enqueue_after_sweep(j);
+}
+
+// functional partials of the current working links
+void launch_functional(mag_job* j) {
+  if (mixed32(j)) {
+    if (j->prm.group == MAG_SU2) mag::launch_su2_functional<float>(j->W32, j->g, j->part, j->st);
+    else mag::launch_su3_functional<float>(j->W32, j->g, j->part, j->st);
+  } else {
+    if (j->prm.group == MAG_SU2) mag::launch_su2_functional<double>(j->U64, j->g, j->part, j->st);
+    else mag::launch_su3_functional<double>(j->U64, j->g, j->part, j->st);
+  }
+}
+
+int read_ctl(mag_job* j) {
+  HIPCHK(hipMemcpyAsync(j->ctl_h, j->ctl, sizeof(JobCtl), hipMemcpyDeviceToHost, j->st));
+  HIPCHK(hipStreamSynchronize(j->st));
+  return MAG_OK;
+}
+
+int write_ctl(mag_job* j) {
+  HIPCHK(hipMemcpyAsync(j->ctl, j->ctl_h, sizeof(JobCtl), hipMemcpyHostToDevice, j->st));
+  HIPCHK(hipStreamSynchronize(j->st));
+  return MAG_OK;
+}
+
+// after the configuration is in U64 (SoA FP64): reset the job state
+int reset_after_upload(mag_job* j) {
+  if (j->prm.precision == MAG_PREC_MIXED) {
+    mag::launch_f64_to_f32(j->U64, j->W32, j->nreal, j->st);
+    mag::launch_gauge_identity(j->G32, j->g, j->prm.group, j->st);
+  }
+  j->phase = 0;
+  j->fp64_start = (j->prm.precision == MAG_PREC_MIXED) ? -1 : 0;
+  j->since_rebuild = 0;
+  std::memset(j->ctl_h, 0, sizeof(JobCtl));
+  j->ctl_h->max_sweeps = j->prm.max_sweeps;
+  j->ctl_h->tol = (j->prm.precision == MAG_PREC_MIXED) ? j->prm.tol_mixed : j->prm.tol;
+  HIPCHK(hipMemcpyAsync(j->ctl, j->ctl_h, sizeof(JobCtl), hipMemcpyHostToDevice, j->st));
+  // initial functional from the FP64 links
+  if (j->prm.group == MAG_SU2) mag::launch_su2_functional<double>(j->U64, j->g, j->part, j->st);
+  else mag::launch_su3_functional<double>(j->U64, j->g, j->part, j->st);
+  mag::launch_reduce(j->part, 2 * j->nb, mag::RED_INIT, j->invF, j->invV, j->ctl, j->hF, j->hR,
+                     j->hA, nullptr, j->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(j->st));
+  j->loaded = true;
+  return MAG_OK;
+}
+
+int upload_device_aos(mag_job* j, const double* dev) {
+  if (j->prm.group == MAG_SU2) mag::launch_su2_aos_to_soa<double>(dev, j->U64, j->g, j->st);
+  else mag::launch_su3_aos_to_soa<double>(dev, j->U64, j->g, j->st);
+  return reset_after_upload(j);
+}
+
+void free_job(mag_job* j) {
+  if (!j) return;
+  hipFree(j->U64);
+  hipFree(j->W32);
+  hipFree(j->G32);
+  hipFree(j->part);
+  hipFree(j->ctl);
+  hipFree(j->hF);
+  hipFree(j->hR);
+  hipFree(j->hA);
+  hipFree(j->out3);
+  if (j->ctl_h) hipHostFree(j->ctl_h);
+  if (j->out3_h) hipHostFree(j->out3_h);
+  if (j->st) hipStreamDestroy(j->st);
+  delete j;
+}
+
+// finish the FP32 phase: links <- G U0 G^+ in FP64, F_prev <- F(links)
+int finalize_mixed(mag_job* j) {
+  if (!mixed32(j)) return MAG_OK;
+  CHK(read_ctl(j));
+  launch_rebuild(j, true);
+  j->phase = 1;
+  j->fp64_start = j->ctl_h->sweep;
+  if (j->prm.group == MAG_SU2) mag::launch_su2_functional<double>(j->U64, j->g, j->part, j->st);
+  else mag::launch_su3_functional<double>(j->U64, j->g, j->part, j->st);
+  mag::launch_reduce(j->part, 2 * j->nb, mag::RED_RESET, j->invF, j->invV, j->ctl, j->hF, j->hR,
+                     j->hA, nullptr, j->st);
+  HIPCHK(hipGetLastError());
+  CHK(read_ctl(j));
+  j->ctl_h->tol = j->prm.tol;
+  j->ctl_h->converged = 0;
+  j->ctl_h->done = (j->ctl_h->sweep >= j->ctl_h->max_sweeps) ? 1 : 0;
+  return write_ctl(j);
+}
+
+int run_until_done(mag_job* j) {
+  for (;;) {
+    CHK(read_ctl(j));
+    if (j->ctl_h->done) return MAG_OK;
+    for (int i = 0; i < kChunk; i++) enqueue_sweep(j);
+    HIPCHK(hipGetLastError());
+  }
+}
+
+}  // namespace
+
+// ====================================================================== API
+extern "C" {
+
+void mag_params_default(mag_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->group = MAG_SU2;
+  p->dims[0] = p->dims[1] = p->dims[2] = p->dims[3] = 8;
+  p->precision = MAG_PREC_F64;
+  p->device = 0;
+  p->omega = 1.7;
+  p->tol = 1e-12;
+  p->max_sweeps = 1000;
+  p->reunit_every = 10;
+  p->tol_mixed = 1e-9;
+}
+
+const char* mag_last_error(void) { return g_err.c_str(); }
+
+int mag_device_count(int* n) {
+  if (!n) return fail(MAG_ERR_ARG, "n is NULL");
+  HIPCHK(hipGetDeviceCount(n));
+  return MAG_OK;
+}
+
+int mag_job_footprint(const mag_params* p, uint64_t* bytes) {
+  CHK(check_params(p));
+  if (!bytes) return fail(MAG_ERR_ARG, "bytes is NULL");
+  int64_t V = 1;
+  for (int i = 0; i < 4; i++) V *= p->dims[i];
+  const uint64_t nr = p->group == MAG_SU2 ? 4 : 12;
+  uint64_t b = (uint64_t)V * 4 * nr * 8;  // FP64 links / master
+  if (p->precision == MAG_PREC_MIXED) b += (uint64_t)V * 4 * nr * 4 + (uint64_t)V * nr * 4;
+  const uint64_t nb = (uint64_t)((V / 2 + mag::kBlock - 1) / mag::kBlock);
+  b += nb * 2 * 3 * 8 + (uint64_t)p->max_sweeps * 3 * 8 + sizeof(JobCtl) + 3 * 8;
+  *bytes = b;
+  return MAG_OK;
+}
+
+int mag_job_create(const mag_params* p, mag_job** out) {
+  if (!out) return fail(MAG_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  CHK(check_params(p));
+  mag_job* j = new (std::nothrow) mag_job();
+  if (!j) return fail(MAG_ERR_NOMEM, "host allocation failed");
+  j->prm = *p;
+  int r = set_device(j);
+  if (r != MAG_OK) {
+    delete j;
+    return r;
+  }
+  check_dims(p->dims, &j->V);
+  j->g = mag::make_geom(p->dims);
+  j->nr = p->group == MAG_SU2 ? 4 : 12;
+  j->nr_in = p->group == MAG_SU2 ? 4 : 18;
+  j->nreal = (size_t)8 * j->g.Vh * j->nr;
+  j->nb = mag::nblocks(j->g);
+  j->invF = 1.0 / (double)(j->V * (p->group == MAG_SU2 ? 4 : 12));
+  j->invV = 1.0 / (double)j->V;
+#define ALLOC(ptr, bytes)                                                          \
+  do {                                                                             \
+    hipError_t e_ = hipMalloc((void**)&(ptr), (bytes));                            \
+    if (e_ != hipSuccess) {                                                        \
+      free_job(j);                                                                 \
+      return fail(MAG_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", (size_t)(bytes), \
+                  hipGetErrorString(e_));                                          \
+    }                                                                              \
+  } while (0)
+  hipError_t e = hipStreamCreateWithFlags(&j->st, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    free_job(j);
+    return fail(MAG_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+  }
+  ALLOC(j->U64, j->nreal * sizeof(double));
+  if (p->precision == MAG_PREC_MIXED) {
+    ALLOC(j->W32, j->nreal * sizeof(float));
+    ALLOC(j->G32, (size_t)2 * j->g.Vh * j->nr * sizeof(float));
+  }
+  ALLOC(j->part, (size_t)2 * j->nb * 3 * sizeof(double));
+  ALLOC(j->ctl, sizeof(JobCtl));
+  ALLOC(j->hF, (size_t)p->max_sweeps * sizeof(double));
+  ALLOC(j->hR, (size_t)p->max_sweeps * sizeof(double));
+  ALLOC(j->hA, (size_t)p->max_sweeps * sizeof(double));
+  ALLOC(j->out3, 3 * sizeof(double));
+#undef ALLOC
+  if (hipHostMalloc((void**)&j->ctl_h, sizeof(JobCtl)) != hipSuccess ||
+      hipHostMalloc((void**)&j->out3_h, 3 * sizeof(double)) != hipSuccess) {
+    free_job(j);
+    return fail(MAG_ERR_NOMEM, "pinned host allocation failed");
+  }
+  std::memset(j->ctl_h, 0, sizeof(JobCtl));
+  j->ctl_h->done = 1;  // nothing to sweep before an upload
+  e = hipMemcpy(j->ctl, j->ctl_h, sizeof(JobCtl), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    free_job(j);
+    return fail(MAG_ERR_HIP, "hipMemcpy failed: %s", hipGetErrorString(e));
+  }
+  *out = j;
+  return MAG_OK;
+}
+
+int mag_job_destroy(mag_job* job) {
+  if (!job) return MAG_OK;
+  hipSetDevice(job->prm.device);
+  hipStreamSynchronize(job->st);
+  free_job(job);
+  return MAG_OK;
+}
+
+int mag_job_stream(mag_job* job, void** stream) {
+  if (!job || !stream) return fail(MAG_ERR_ARG, "NULL argument");
+  *stream = (void*)job->st;
+  return MAG_OK;
+}
+
+int mag_job_upload_links(mag_job* j, const double* links) {
+  if (!j || !links) return fail(MAG_ERR_ARG, "NULL argument");
+  CHK(set_device(j));
+  const size_t n = (size_t)j->V * 4 * j->nr_in;
+  if (!all_finite(links, n)) return fail(MAG_ERR_NONFINITE, "links hold NaN or Inf");
+  double* stage = nullptr;
+  HIPCHK(hipMalloc((void**)&stage, n * sizeof(double)));
+  hipError_t e = hipMemcpyAsync(stage, links, n * sizeof(double), hipMemcpyHostToDevice, j->st);
+  int r = (e == hipSuccess) ? upload_device_aos(j, stage)
+                            : fail(MAG_ERR_HIP, "hipMemcpy H2D failed: %s", hipGetErrorString(e));
+  hipStreamSynchronize(j->st);
+  hipFree(stage);
+  return r;
+}
+
+int mag_job_upload_links_device(mag_job* j, const double* dev_links) {
+  if (!j || !dev_links) return fail(MAG_ERR_ARG, "NULL argument");
+  CHK(set_device(j));
+  return upload_device_aos(j, dev_links);
+}
+
+int mag_job_generate(mag_job* j, double eps, uint64_t seed) {
+  if (!j) return fail(MAG_ERR_ARG, "NULL job");
+  if (!std::isfinite(eps)) return fail(MAG_ERR_ARG, "eps must be finite");
+  CHK(set_device(j));
+  const size_t n = (size_t)j->V * 4 * j->nr_in;
+  double* stage = nullptr;
+  double* gs = nullptr;
+  HIPCHK(hipMalloc((void**)&stage, n * sizeof(double)));
+  if (j->prm.group == MAG_SU3) {
+    hipError_t e = hipMalloc((void**)&gs, (size_t)j->V * 18 * sizeof(double));
+    if (e != hipSuccess) {
+      hipFree(stage);
+      return fail(MAG_ERR_NOMEM, "hipMalloc failed: %s", hipGetErrorString(e));
+    }
+    mag::launch_gen_su3(stage, gs, j->g, eps, seed, j->st);
+  } else {
+    mag::launch_gen_su2(stage, j->g, eps, seed, j->st);
+  }
+  int r = upload_device_aos(j, stage);
+  hipStreamSynchronize(j->st);
+  hipFree(stage);
+  hipFree(gs);
+  return r;
+}
+
+int mag_job_sweeps(mag_job* j, int n) {
+  if (!j || n < 0) return fail(MAG_ERR_ARG, "bad argument");
+  if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  CHK(set_device(j));
+  for (int i = 0; i < n; i++) enqueue_sweep(j);
+  HIPCHK(hipGetLastError());
+  return MAG_OK;
+}
+
+int mag_job_finalize(mag_job* j) {
+  if (!j) return fail(MAG_ERR_ARG, "NULL job");
+  if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  CHK(set_device(j));
+  return finalize_mixed(j);
+}
+
+int mag_job_run(mag_job* j) {
+  if (!j) return fail(MAG_ERR_ARG, "NULL job");
+  if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  CHK(set_device(j));
+  CHK(run_until_done(j));
+  if (mixed32(j)) {
+    CHK(finalize_mixed(j));
+    CHK(run_until_done(j));
+  }
+  CHK(read_ctl(j));
+  if (!j->ctl_h->converged)
+    return fail(MAG_ERR_NOT_CONVERGED, "sweep cap %d reached before tol", j->prm.max_sweeps);
+  return MAG_OK;
+}
+
+int mag_job_status(mag_job* j, int* sweeps, int* converged, double* F) {
+  if (!j) return fail(MAG_ERR_ARG, "NULL job");
+  CHK(set_device(j));
+  CHK(read_ctl(j));
+  if (sweeps) *sweeps = j->ctl_h->sweep;
+  if (converged) *converged = j->ctl_h->converged;
+  if (F) *F = j->ctl_h->F_prev;
+  return MAG_OK;
+}
+
+int mag_job_initial_functional(mag_job* j, double* F0) {
+  if (!j || !F0) return fail(MAG_ERR_ARG, "NULL argument");
+  CHK(set_device(j));
+  CHK(read_ctl(j));
+  *F0 = j->ctl_h->F0;
+  return MAG_OK;
+}
+
+int mag_job_history(mag_job* j, double* F, double* rmax, double* ravg, int cap, int* n) {
+  if (!j || cap < 0) return fail(MAG_ERR_ARG, "bad argument");
+  CHK(set_device(j));
+  CHK(read_ctl(j));
+  int m = j->ctl_h->sweep < cap ? j->ctl_h->sweep : cap;
+  if (m > 0) {
+    if (F) HIPCHK(hipMemcpy(F, j->hF, m * sizeof(double), hipMemcpyDeviceToHost));
+    if (rmax) HIPCHK(hipMemcpy(rmax, j->hR, m * sizeof(double), hipMemcpyDeviceToHost));
+    if (ravg) HIPCHK(hipMemcpy(ravg, j->hA, m * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  if (n) *n = m;
+  return MAG_OK;
+}
+
+int mag_job_fp64_start(mag_job* j, int* sweep) {
+  if (!j || !sweep) return fail(MAG_ERR_ARG, "NULL argument");
+  *sweep = j->fp64_start;
+  return MAG_OK;
+}
+
+int mag_job_download_links(mag_job* j, double* links) {
+  if (!j || !links) return fail(MAG_ERR_ARG, "NULL argument");
+  if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  CHK(set_device(j));
+  const size_t n = (size_t)j->V * 4 * j->nr_in;
+  double* stage = nullptr;
+  HIPCHK(hipMalloc((void**)&stage, n * sizeof(double)));
+  if (mixed32(j)) {
+    if (j->prm.group == MAG_SU2) mag::launch_su2_soa_to_aos<float>(j->W32, stage, j->g, j->st);
+    else mag::launch_su3_soa_to_aos<float>(j->W32, stage, j->g, j->st);
+  } else {
+    if (j->prm.group == MAG_SU2) mag::launch_su2_soa_to_aos<double>(j->U64, stage, j->g, j->st);
+    else mag::launch_su3_soa_to_aos<double>(j->U64, stage, j->g, j->st);
+  }
+  hipError_t e = hipMemcpyAsync(links, stage, n * sizeof(double), hipMemcpyDeviceToHost, j->st);
+  hipError_t e2 = hipStreamSynchronize(j->st);
+  hipFree(stage);
+  if (e != hipSuccess || e2 != hipSuccess)
+    return fail(MAG_ERR_HIP, "download failed: %s",
+                hipGetErrorString(e != hipSuccess ? e : e2));
+  return MAG_OK;
+}
+
+int mag_job_functional(mag_job* j, double* F) {
+  if (!j || !F) return fail(MAG_ERR_ARG, "NULL argument");
+  if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  CHK(set_device(j));
+  launch_functional(j);
+  mag::launch_reduce(j->part, 2 * j->nb, mag::RED_PLAIN, j->invF, j->invV, j->ctl, j->hF, j->hR,
+                     j->hA, j->out3, j->st);
+  HIPCHK(hipMemcpyAsync(j->out3_h, j->out3, 3 * sizeof(double), hipMemcpyDeviceToHost, j->st));
+  HIPCHK(hipStreamSynchronize(j->st));
+  *F = j->out3_h[0];
+  return MAG_OK;
+}
+
+int mag_job_plaquette(mag_job* j, double* P) {
+  if (!j || !P) return fail(MAG_ERR_ARG, "NULL argument");
+  if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  CHK(set_device(j));
+  // U64 is either the current links or (mixed, FP32 phase) the master,
+  // which is gauge-equivalent: the plaquette is the same.
+  if (j->prm.group == MAG_SU2) mag::launch_su2_plaquette(j->U64, j->g, j->part, j->st);
+  else mag::launch_su3_plaquette(j->U64, j->g, j->part, j->st);
+  mag::launch_reduce(j->part, 2 * j->nb, mag::RED_PLAIN, j->invV, j->invV, j->ctl, j->hF, j->hR,
+                     j->hA, j->out3, j->st);
+  HIPCHK(hipMemcpyAsync(j->out3_h, j->out3, 3 * sizeof(double), hipMemcpyDeviceToHost, j->st));
+  HIPCHK(hipStreamSynchronize(j->st));
+  *P = j->out3_h[0];
+  return MAG_OK;
+}
+
+int mag_job_sweeps_timed(mag_job* j, int n, double* total_ms, double* kern_ms) {
+  if (!j || n < 1 || !total_ms) return fail(MAG_ERR_ARG, "bad argument");
+  if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  CHK(set_device(j));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  std::vector<hipEvent_t> ka, kb;
+  if (kern_ms) {
+    ka.resize(2 * (size_t)n);
+    kb.resize(2 * (size_t)n);
+    for (size_t i = 0; i < ka.size(); i++) {
+      HIPCHK(hipEventCreate(&ka[i]));
+      HIPCHK(hipEventCreate(&kb[i]));
+    }
+  }
+  HIPCHK(hipEventRecord(e0, j->st));
+  for (int i = 0; i < n; i++) {
+    for (int p = 0; p < 2; p++) {
+      if (kern_ms) HIPCHK(hipEventRecord(ka[2 * i + p], j->st));
+      launch_half(j, p);
+      if (kern_ms) HIPCHK(hipEventRecord(kb[2 * i + p], j->st));
+    }
+    enqueue_after_sweep(j);
+  }
+  HIPCHK(hipEventRecord(e1, j->st));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *total_ms = ms;
+  if (kern_ms) {
+    double acc = 0.0;
+    for (size_t i = 0; i < ka.size(); i++) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, ka[i], kb[i]));
+      acc += t;
+      hipEventDestroy(ka[i]);
+      hipEventDestroy(kb[i]);
+    }
+    *kern_ms = acc / (double)ka.size();
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return MAG_OK;
+}
+
+// ------------------------------------------------------------ one-shot
+static int oneshot(int group, const int* dims, const double* links, int what, double* out) {
+  if (!links || !out) return fail(MAG_ERR_ARG, "NULL argument");
+  mag_params p;
+  mag_params_default(&p);
+  p.group = group;
+  for (int i = 0; i < 4; i++) p.dims[i] = dims ? dims[i] : 0;
+  p.max_sweeps = 1;
+  CHK(check_params(&p));
+  mag_job* j = nullptr;
+  CHK(mag_job_create(&p, &j));
+  int r = mag_job_upload_links(j, links);
+  if (r == MAG_OK) r = what == 0 ? mag_job_functional(j, out) : mag_job_plaquette(j, out);
+  mag_job_destroy(j);
+  return r;
+}
+
+int mag_su2_functional(const int* dims, const double* links, double* F) {
+  return oneshot(MAG_SU2, dims, links, 0, F);
+}
+int mag_su3_functional(const int* dims, const double* links18, double* F) {
+  return oneshot(MAG_SU3, dims, links18, 0, F);
+}
+int mag_plaquette(int group, const int* dims, const double* links, double* P) {
+  if (group != MAG_SU2 && group != MAG_SU3) return fail(MAG_ERR_ARG, "group must be 2 or 3");
+  return oneshot(group, dims, links, 1, P);
+}
+
+}  // extern "C"

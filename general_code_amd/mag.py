@@ -1,0 +1,305 @@
+"""Python front end of libmag_gpu.so (ctypes over the C ABI in include/mag_gpu.h).
+
+The HIP library is the only implementation: importing this module on a
+machine without the built library, or calling into it without a GPU, raises.
+
+Arrays follow the reference layout (ilyakudrov/general_code
+lib/cpu/include/indexing.h:71-126): SU(2) links float64 [4V, 4]
+(a0 + i a.sigma, matrix.h:8-13), SU(3) links float64 [4V, 18] (row-major
+re/im, matrix.h:236-254), link index 4*site + mu with
+site = x + Lx*(y + Ly*(z + Lz*t)).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libmag_gpu.so")
+
+MAG_OK = 0
+MAG_ERR_ARG = -1
+MAG_ERR_DIMS = -2
+MAG_ERR_NOMEM = -3
+MAG_ERR_NONFINITE = -4
+MAG_ERR_NOT_CONVERGED = -5
+MAG_ERR_HIP = -6
+MAG_ERR_STATE = -7
+
+SU2, SU3 = 2, 3
+F64, MIXED = 0, 1
+
+
+class MagError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class NotConverged(MagError):
+    pass
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [
+        ("group", ctypes.c_int),
+        ("dims", ctypes.c_int * 4),
+        ("precision", ctypes.c_int),
+        ("device", ctypes.c_int),
+        ("omega", ctypes.c_double),
+        ("tol", ctypes.c_double),
+        ("max_sweeps", ctypes.c_int),
+        ("reunit_every", ctypes.c_int),
+        ("tol_mixed", ctypes.c_double),
+    ]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); every entry point of include/mag_gpu.h
+SIGNATURES = {
+    "mag_params_default": (None, [ctypes.POINTER(_Params)]),
+    "mag_last_error": (ctypes.c_char_p, []),
+    "mag_device_count": (ctypes.c_int, [_ip]),
+    "mag_job_create": (ctypes.c_int, [ctypes.POINTER(_Params), ctypes.POINTER(_vp)]),
+    "mag_job_destroy": (ctypes.c_int, [_vp]),
+    "mag_job_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "mag_job_footprint": (ctypes.c_int, [ctypes.POINTER(_Params), ctypes.POINTER(ctypes.c_uint64)]),
+    "mag_job_upload_links": (ctypes.c_int, [_vp, _dp]),
+    "mag_job_upload_links_device": (ctypes.c_int, [_vp, _vp]),
+    "mag_job_generate": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_uint64]),
+    "mag_job_sweeps": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mag_job_run": (ctypes.c_int, [_vp]),
+    "mag_job_finalize": (ctypes.c_int, [_vp]),
+    "mag_job_status": (ctypes.c_int, [_vp, _ip, _ip, _dp]),
+    "mag_job_initial_functional": (ctypes.c_int, [_vp, _dp]),
+    "mag_job_history": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, _ip]),
+    "mag_job_fp64_start": (ctypes.c_int, [_vp, _ip]),
+    "mag_job_download_links": (ctypes.c_int, [_vp, _dp]),
+    "mag_job_functional": (ctypes.c_int, [_vp, _dp]),
+    "mag_job_plaquette": (ctypes.c_int, [_vp, _dp]),
+    "mag_job_sweeps_timed": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+    "mag_su2_functional": (ctypes.c_int, [_ip, _dp, _dp]),
+    "mag_su3_functional": (ctypes.c_int, [_ip, _dp, _dp]),
+    "mag_plaquette": (ctypes.c_int, [ctypes.c_int, _ip, _dp, _dp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libmag_gpu.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m general_code_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != MAG_OK:
+        msg = lib().mag_last_error().decode(errors="replace")
+        if rc == MAG_ERR_NOT_CONVERGED:
+            raise NotConverged(rc, msg)
+        raise MagError(rc, msg)
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _dims(d):
+    return (ctypes.c_int * 4)(*[int(x) for x in d])
+
+
+def device_count() -> int:
+    n = ctypes.c_int()
+    _check(lib().mag_device_count(ctypes.byref(n)))
+    return n.value
+
+
+@dataclass
+class MagParams:
+    """Job parameters (struct mag_params of include/mag_gpu.h)."""
+    group: int = SU2
+    dims: tuple = (8, 8, 8, 8)
+    precision: int = F64
+    device: int = 0
+    omega: float = 1.7
+    tol: float = 1e-12
+    max_sweeps: int = 1000
+    reunit_every: int = 10
+    tol_mixed: float = 1e-9
+
+    def to_c(self) -> _Params:
+        p = _Params()
+        lib().mag_params_default(ctypes.byref(p))
+        p.group = int(self.group)
+        for i in range(4):
+            p.dims[i] = int(self.dims[i])
+        p.precision = int(self.precision)
+        p.device = int(self.device)
+        p.omega = float(self.omega)
+        p.tol = float(self.tol)
+        p.max_sweeps = int(self.max_sweeps)
+        p.reunit_every = int(self.reunit_every)
+        p.tol_mixed = float(self.tol_mixed)
+        return p
+
+    @property
+    def volume(self) -> int:
+        return int(np.prod([int(d) for d in self.dims]))
+
+    @property
+    def reals_in(self) -> int:
+        return 4 if self.group == SU2 else 18
+
+    def footprint(self) -> int:
+        b = ctypes.c_uint64()
+        p = self.to_c()
+        _check(lib().mag_job_footprint(ctypes.byref(p), ctypes.byref(b)))
+        return b.value
+
+
+class MagJob:
+    """One configuration resident in HBM, gauge-fixed by checkerboard
+    omega-overrelaxation on the GPU."""
+
+    def __init__(self, params: MagParams | None = None, **kw):
+        self.params = params if params is not None else MagParams(**kw)
+        self._p = self.params.to_c()
+        h = _vp()
+        _check(lib().mag_job_create(ctypes.byref(self._p), ctypes.byref(h)))
+        self._h = h
+
+    # lifecycle ------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mag_job_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # data -----------------------------------------------------------------
+    def upload(self, links: np.ndarray):
+        a = np.ascontiguousarray(links, dtype=np.float64)
+        V = self.params.volume
+        if a.size != 4 * V * self.params.reals_in:
+            raise ValueError(f"links must hold {4 * V * self.params.reals_in} reals, got {a.size}")
+        _check(lib().mag_job_upload_links(self._h, _dptr(a)))
+
+    def generate(self, eps: float, seed: int):
+        _check(lib().mag_job_generate(self._h, float(eps), int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    def download(self) -> np.ndarray:
+        V = self.params.volume
+        out = np.empty((4 * V, self.params.reals_in), dtype=np.float64)
+        _check(lib().mag_job_download_links(self._h, _dptr(out)))
+        return out
+
+    # work -----------------------------------------------------------------
+    def sweeps(self, n: int):
+        _check(lib().mag_job_sweeps(self._h, int(n)))
+
+    def run(self, raise_on_cap: bool = False) -> bool:
+        rc = lib().mag_job_run(self._h)
+        if rc == MAG_ERR_NOT_CONVERGED and not raise_on_cap:
+            return False
+        _check(rc)
+        return True
+
+    def finalize(self):
+        _check(lib().mag_job_finalize(self._h))
+
+    def sweeps_timed(self, n: int, kernel: bool = True):
+        tot = ctypes.c_double()
+        k = ctypes.c_double()
+        _check(lib().mag_job_sweeps_timed(self._h, int(n), ctypes.byref(tot),
+                                          ctypes.byref(k) if kernel else None))
+        return tot.value, (k.value if kernel else None)
+
+    # results --------------------------------------------------------------
+    def status(self):
+        s = ctypes.c_int()
+        c = ctypes.c_int()
+        F = ctypes.c_double()
+        _check(lib().mag_job_status(self._h, ctypes.byref(s), ctypes.byref(c), ctypes.byref(F)))
+        return s.value, bool(c.value), F.value
+
+    def initial_functional(self) -> float:
+        F = ctypes.c_double()
+        _check(lib().mag_job_initial_functional(self._h, ctypes.byref(F)))
+        return F.value
+
+    def history(self):
+        n_max = self.params.max_sweeps
+        F = np.zeros(n_max)
+        R = np.zeros(n_max)
+        A = np.zeros(n_max)
+        n = ctypes.c_int()
+        _check(lib().mag_job_history(self._h, _dptr(F), _dptr(R), _dptr(A), n_max,
+                                     ctypes.byref(n)))
+        m = n.value
+        return F[:m].copy(), R[:m].copy(), A[:m].copy()
+
+    def fp64_start(self) -> int:
+        s = ctypes.c_int()
+        _check(lib().mag_job_fp64_start(self._h, ctypes.byref(s)))
+        return s.value
+
+    def functional(self) -> float:
+        F = ctypes.c_double()
+        _check(lib().mag_job_functional(self._h, ctypes.byref(F)))
+        return F.value
+
+    def plaquette(self) -> float:
+        P = ctypes.c_double()
+        _check(lib().mag_job_plaquette(self._h, ctypes.byref(P)))
+        return P.value
+
+
+# one-shot helpers ---------------------------------------------------------
+def su2_functional(dims, links) -> float:
+    """MAG_functional_su2 (lib/cpu/src/mag.cpp:993-1001) on the GPU."""
+    a = np.ascontiguousarray(links, dtype=np.float64)
+    F = ctypes.c_double()
+    _check(lib().mag_su2_functional(_dims(dims), _dptr(a), ctypes.byref(F)))
+    return F.value
+
+
+def su3_functional(dims, links) -> float:
+    """mag_functional_su3 (lib/cpu/src/mag.cpp:1837-1862) on the GPU."""
+    a = np.ascontiguousarray(links, dtype=np.float64)
+    F = ctypes.c_double()
+    _check(lib().mag_su3_functional(_dims(dims), _dptr(a), ctypes.byref(F)))
+    return F.value
+
+
+def plaquette(group, dims, links) -> float:
+    """plaket (lib/cpu/include/plaket.h:84-101) on the GPU."""
+    a = np.ascontiguousarray(links, dtype=np.float64)
+    P = ctypes.c_double()
+    _check(lib().mag_plaquette(int(group), _dims(dims), _dptr(a), ctypes.byref(P)))
+    return P.value

@@ -1,0 +1,145 @@
+/*
+ * mag_gpu.h -- C ABI of the MI355X Maximal Abelian gauge (MAG) fixing path.
+ *
+ * Plain pointers and sizes only; no exceptions cross the boundary; every
+ * function returns an int status (MAG_OK = 0, negative on error, message in
+ * mag_last_error()).  Host buffers are copied; device memory is owned by the
+ * job.  Jobs are independent: calls on different jobs may come from
+ * different host threads.
+ *
+ * Link arrays at this boundary use the reference's layout
+ * (ilyakudrov/general_code lib/cpu/include/indexing.h:71-126, data.h:78-237):
+ *   link index = 4*site + mu, site = x + Lx*(y + Ly*(z + Lz*t))
+ *   SU(2): 4 doubles per link {a0,a1,a2,a3} = a0 + i a.sigma (matrix.h:8-13)
+ *   SU(3): 18 doubles per link, row-major (re,im) (matrix.h:236-254)
+ *   spins: 3 doubles per site (matrix.h:773-787)
+ * dims = {Lx, Ly, Lz, Lt}; every extent must be even (checkerboard).
+ *
+ * Reference interfaces replaced (lib/cpu/include/mag.h unless noted):
+ *   MAG_functional_su2 ............. mag.h:69-70      -> mag_su2_functional
+ *   MAG_functional_su2_spin ........ mag.h:58-60      -> mag_su2_functional_spin
+ *   mag_functional_su3 ............. mag.h:177        -> mag_su3_functional
+ *   gauge_tranformation_spins ...... mag.h:80-82      -> mag_su2_gauge_transform_spins
+ *   make_maximization_final ........ mag.h:166-169    -> mag_su2_maximize_final
+ *   make_maximization_approximate .. mag.h:151-153    -> mag_su2_maximize_approximate
+ *   make_simulated_annealing ....... mag.h:119-122    -> mag_su2_simulated_annealing
+ *   generate_spins_uniform ......... mag.h:179-199    -> mag_su2_generate_spins_uniform
+ *   plaket ......................... plaket.h:84-101  -> mag_plaquette
+ *   (none: the reference has no link-form checkerboard MAG, no SU(3) MAG
+ *    fixing, no mixed precision, batching or serving)  -> mag_job_* / mag_batch_*
+ */
+#ifndef MAG_GPU_H
+#define MAG_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define MAG_OK 0
+#define MAG_ERR_ARG (-1)           /* bad argument (null pointer, range)       */
+#define MAG_ERR_DIMS (-2)          /* odd / non-positive / too large extents   */
+#define MAG_ERR_NOMEM (-3)         /* device allocation failed                 */
+#define MAG_ERR_NONFINITE (-4)     /* input holds NaN or Inf                   */
+#define MAG_ERR_NOT_CONVERGED (-5) /* sweep cap reached before the tolerance   */
+#define MAG_ERR_HIP (-6)           /* HIP runtime error                        */
+#define MAG_ERR_STATE (-7)         /* call not valid in the job's state        */
+
+/* ------------------------------------------------------------ parameters */
+#define MAG_SU2 2
+#define MAG_SU3 3
+#define MAG_PREC_F64 0   /* FP64 links, FP64 sweeps                            */
+#define MAG_PREC_MIXED 1 /* FP32 sweeps, FP64 functional, FP64 master links,  */
+                         /* FP64 reunitarisation every reunit_every sweeps     */
+
+typedef struct mag_params {
+  int group;          /* MAG_SU2 or MAG_SU3                                    */
+  int dims[4];        /* Lx, Ly, Lz, Lt                                        */
+  int precision;      /* MAG_PREC_F64 or MAG_PREC_MIXED                        */
+  int device;         /* HIP device ordinal                                    */
+  double omega;       /* overrelaxation parameter, 0 < omega <= 2              */
+  double tol;         /* stop when |F_k - F_{k-1}| < tol |F_k|                 */
+  int max_sweeps;     /* sweep cap (all phases together)                       */
+  int reunit_every;   /* mixed: rebuild FP32 links from the FP64 master      */
+                      /*        every reunit_every sweeps                      */
+  double tol_mixed;   /* mixed: FP32 phase ends at this relative change;     */
+                      /*        FP64 sweeps then run down to tol              */
+} mag_params;
+
+/* Fill defaults: SU(2), omega 1.7, tol 1e-12, 1000 sweeps, FP64,
+ * reunit_every 10, tol_mixed 1e-9, device 0. */
+void mag_params_default(mag_params *p);
+
+const char *mag_last_error(void);
+int mag_device_count(int *n);
+
+/* ------------------------------------------------------------ jobs
+ * A job owns one configuration in HBM (parity-split SoA) and runs the
+ * checkerboard omega-overrelaxation (SU(3): Cabibbo-Marinari over the
+ * subgroups (0,1),(1,2),(0,2)) until the functional's relative change drops
+ * below tol.  All calls are asynchronous on the job's stream unless they
+ * return data to the host. */
+typedef struct mag_job mag_job;
+
+int mag_job_create(const mag_params *p, mag_job **out);
+int mag_job_destroy(mag_job *job);
+/* hipStream_t of the job (created by the job) */
+int mag_job_stream(mag_job *job, void **stream);
+/* bytes of HBM the job holds */
+int mag_job_footprint(const mag_params *p, uint64_t *bytes);
+
+/* Upload links in the reference layout (host memory) and reset the job. */
+int mag_job_upload_links(mag_job *job, const double *links);
+/* Same from a device buffer in the reference layout (no host copy). */
+int mag_job_upload_links_device(mag_job *job, const double *dev_links);
+/* Fill the job with a seeded synthetic configuration generated on the GPU:
+ * SU(2) U = exp(i eps a.sigma), a ~ N(0,1)^3; SU(3) U = exp(i eps
+ * sum_a c_a lambda_a / 2), c ~ N(0,1)^8; then a Haar random gauge
+ * transformation, all from a Philox4x32-10 stream keyed by seed. */
+int mag_job_generate(mag_job *job, double eps, uint64_t seed);
+
+/* Enqueue exactly n sweeps (fewer take effect once the job is done). */
+int mag_job_sweeps(mag_job *job, int n);
+/* Run to convergence or the sweep cap (blocking).  Mixed precision: FP32
+ * phase, final FP64 rebuild, FP64 polish.  Returns MAG_OK or
+ * MAG_ERR_NOT_CONVERGED. */
+int mag_job_run(mag_job *job);
+/* Finish a mixed-precision job's FP32 phase now (FP64 rebuild of the links
+ * from the master and the accumulated gauge); no-op for FP64 jobs. */
+int mag_job_finalize(mag_job *job);
+
+/* sweeps done, converged flag, current functional (blocking) */
+int mag_job_status(mag_job *job, int *sweeps, int *converged, double *F);
+/* initial functional F0 (blocking) */
+int mag_job_initial_functional(mag_job *job, double *F0);
+/* per-sweep history: functional, max and mean of 1 - cos(theta) (the
+ * relaxation residual of mag.cpp:1477-1512); n = entries written. */
+int mag_job_history(mag_job *job, double *F, double *rmax, double *ravg,
+                    int cap, int *n);
+/* sweep index at which a mixed job switched to FP64 (-1: not yet) */
+int mag_job_fp64_start(mag_job *job, int *sweep);
+/* Download the current links (reference layout, FP64).  A mixed job that is
+ * still in its FP32 phase returns its FP32 working links. */
+int mag_job_download_links(mag_job *job, double *links);
+/* MAG functional / plaquette of the device-resident links (blocking) */
+int mag_job_functional(mag_job *job, double *F);
+int mag_job_plaquette(mag_job *job, double *P);
+
+/* Timed sweeps for measurement: n sweeps bracketed by HIP events on the
+ * job's stream; also the mean duration of the half-sweep kernel (events
+ * around each launch) when kern_ms is not NULL.  Blocking. */
+int mag_job_sweeps_timed(mag_job *job, int n, double *total_ms,
+                         double *kern_ms);
+
+/* ------------------------------------------------------------ one-shot
+ * functionals and checks on host arrays (reference layout) */
+int mag_su2_functional(const int *dims, const double *links, double *F);
+int mag_su3_functional(const int *dims, const double *links18, double *F);
+int mag_plaquette(int group, const int *dims, const double *links, double *P);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAG_GPU_H */

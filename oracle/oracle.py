@@ -1,0 +1,351 @@
+"""ctypes front end of the CPU oracle (libmag_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation.  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module; the product (general_code_amd) never does.
+
+Arrays follow the reference layout (lib/cpu/include/indexing.h:71-126):
+SU(2) links are float64 [V*4, 4], SU(3) links float64 [V*4, 18] (row-major
+re/im, lib/cpu/include/matrix.h:236-254), spins float64 [V, 3].
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libmag_oracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc + OpenMP)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "all"],
+                       check=True, stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        sig = {
+            "or_su2_mul": (None, [_dp, _dp, _dp]),
+            "or_su2_mul_dag": (None, [_dp, _dp, _dp]),
+            "or_su2_dag_mul": (None, [_dp, _dp, _dp]),
+            "or_su2_link_functional": (ctypes.c_double, [_dp]),
+            "or_spin_contribution1": (None, [_dp, _dp, _dp]),
+            "or_spin_contribution1_conj": (None, [_dp, _dp, _dp]),
+            "or_spin_reflect_fast": (None, [_dp, _dp]),
+            "or_spin_parallel": (ctypes.c_double, [_dp, _dp]),
+            "or_spin_gauge_matrix": (None, [_dp, _dp]),
+            "or_su2_contribution_site": (None, [_dp, _dp, _ip, ctypes.c_int64, _dp]),
+            "or_su2_functional": (ctypes.c_double, [_dp, _ip]),
+            "or_su2_functional_spin": (ctypes.c_double, [_dp, _dp, _ip]),
+            "or_su2_gauge_transform_spins": (None, [_dp, _dp, _ip]),
+            "or_su2_or_sweep_lexi": (None, [_dp, _dp, _ip]),
+            "or_su2_relax_sweep_lexi": (None, [_dp, _dp, _ip, _dp, _dp]),
+            "or_su2_maximize_final": (ctypes.c_int, [_dp, _dp, _ip, ctypes.c_int,
+                                                     ctypes.c_double, ctypes.c_double,
+                                                     ctypes.c_int]),
+            "or_su2_maximize_approximate": (ctypes.c_int, [_dp, _dp, _ip, ctypes.c_int,
+                                                           ctypes.c_int, ctypes.c_int]),
+            "or_su2_spin_halfsweep": (None, [_dp, _dp, _ip, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_double, _dp, _dp]),
+            "or_su2_heatbath_halfsweep": (None, [_dp, _dp, _ip, ctypes.c_int,
+                                                 ctypes.c_double, ctypes.c_uint64,
+                                                 ctypes.c_uint32]),
+            "or_su2_link_sweep": (None, [_dp, _ip, ctypes.c_double, _dp, _dp, _dp]),
+            "or_su3_link_sweep": (None, [_dp, _ip, ctypes.c_double, _dp, _dp, _dp]),
+            "or_omega_gauge": (None, [_dp, ctypes.c_double, _dp]),
+            "or_block_field_fwd": (None, [_dp, _dp]),
+            "or_block_field_bwd": (None, [_dp, _dp]),
+            "or_su3_functional": (ctypes.c_double, [_dp, _ip]),
+            "or_su3_reunitarize": (None, [_dp]),
+            "or_su3_proj": (None, [_dp]),
+            "or_su3_mul": (None, [_dp, _dp, _dp]),
+            "or_su3_mul_dag": (None, [_dp, _dp, _dp]),
+            "or_su2_plaquette": (ctypes.c_double, [_dp, _ip]),
+            "or_su3_plaquette": (ctypes.c_double, [_dp, _ip]),
+            "or_su2_wilson_loop": (ctypes.c_double, [_dp, _ip, ctypes.c_int, ctypes.c_int]),
+            "or_su3_wilson_loop": (ctypes.c_double, [_dp, _ip, ctypes.c_int, ctypes.c_int]),
+            "or_philox4x32_10": (None, [_u32p, _u32p, _u32p]),
+            "or_gen_su2_config": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64]),
+            "or_gen_su3_config": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64]),
+            "or_gen_su3_exp": (None, [_dp, ctypes.c_double, _dp]),
+            "or_gen_spins_uniform": (None, [_dp, _ip, ctypes.c_uint64]),
+            "or_set_threads": (None, [ctypes.c_int]),
+            "or_get_threads": (ctypes.c_int, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _dims(dims):
+    arr = (ctypes.c_int * 4)(*[int(d) for d in dims])
+    return arr
+
+
+def volume(dims) -> int:
+    return int(np.prod([int(d) for d in dims]))
+
+
+# ---------------------------------------------------------------- algebra
+def su2_mul(a, b):
+    out = np.zeros(4)
+    lib().or_su2_mul(_d(np.ascontiguousarray(a, np.float64)),
+                     _d(np.ascontiguousarray(b, np.float64)), _d(out))
+    return out
+
+
+def su2_mul_dag(a, b):
+    out = np.zeros(4)
+    lib().or_su2_mul_dag(_d(np.ascontiguousarray(a, np.float64)),
+                         _d(np.ascontiguousarray(b, np.float64)), _d(out))
+    return out
+
+
+def su2_dag_mul(a, b):
+    out = np.zeros(4)
+    lib().or_su2_dag_mul(_d(np.ascontiguousarray(a, np.float64)),
+                         _d(np.ascontiguousarray(b, np.float64)), _d(out))
+    return out
+
+
+def su2_link_functional(a):
+    return lib().or_su2_link_functional(_d(np.ascontiguousarray(a, np.float64)))
+
+
+def contribution1(h, A, b):
+    h = np.array(h, np.float64)
+    lib().or_spin_contribution1(_d(h), _d(np.ascontiguousarray(A, np.float64)),
+                                _d(np.ascontiguousarray(b, np.float64)))
+    return h
+
+
+def contribution1_conj(h, A, b):
+    h = np.array(h, np.float64)
+    lib().or_spin_contribution1_conj(_d(h), _d(np.ascontiguousarray(A, np.float64)),
+                                     _d(np.ascontiguousarray(b, np.float64)))
+    return h
+
+
+def reflect_fast(s, V):
+    s = np.array(s, np.float64)
+    lib().or_spin_reflect_fast(_d(s), _d(np.ascontiguousarray(V, np.float64)))
+    return s
+
+
+def parallel(s, V):
+    s = np.array(s, np.float64)
+    d = lib().or_spin_parallel(_d(s), _d(np.ascontiguousarray(V, np.float64)))
+    return s, d
+
+
+def gauge_matrix(s):
+    g = np.zeros(4)
+    lib().or_spin_gauge_matrix(_d(np.ascontiguousarray(s, np.float64)), _d(g))
+    return g
+
+
+def omega_gauge(h, omega):
+    g = np.zeros(4)
+    lib().or_omega_gauge(_d(np.ascontiguousarray(h, np.float64)), float(omega), _d(g))
+    return g
+
+
+def block_field_fwd(B):
+    h = np.zeros(3)
+    lib().or_block_field_fwd(_d(np.ascontiguousarray(B, np.float64)), _d(h))
+    return h
+
+
+def block_field_bwd(B):
+    h = np.zeros(3)
+    lib().or_block_field_bwd(_d(np.ascontiguousarray(B, np.float64)), _d(h))
+    return h
+
+
+# ---------------------------------------------------------------- SU(2) MAG
+def contribution_site(links, spins, dims, site):
+    h = np.zeros(3)
+    lib().or_su2_contribution_site(_d(links), _d(spins), _dims(dims), int(site), _d(h))
+    return h
+
+
+def su2_functional(links, dims):
+    return lib().or_su2_functional(_d(links), _dims(dims))
+
+
+def su2_functional_spin(links, spins, dims):
+    return lib().or_su2_functional_spin(_d(links), _d(spins), _dims(dims))
+
+
+def su2_gauge_transform_spins(links, spins, dims):
+    out = links.copy()
+    lib().or_su2_gauge_transform_spins(_d(out), _d(spins), _dims(dims))
+    return out
+
+
+def su2_or_sweep_lexi(links, spins, dims):
+    lib().or_su2_or_sweep_lexi(_d(links), _d(spins), _dims(dims))
+
+
+def su2_relax_sweep_lexi(links, spins, dims):
+    mx = ctypes.c_double()
+    av = ctypes.c_double()
+    lib().or_su2_relax_sweep_lexi(_d(links), _d(spins), _dims(dims),
+                                  ctypes.byref(mx), ctypes.byref(av))
+    return mx.value, av.value
+
+
+def su2_maximize_final(links, spins, dims, or_steps, tol_max, tol_avg, max_iter=100000):
+    return lib().or_su2_maximize_final(_d(links), _d(spins), _dims(dims), int(or_steps),
+                                       float(tol_max), float(tol_avg), int(max_iter))
+
+
+def su2_maximize_approximate(links, spins, dims, or_steps, tol_digits, max_iter=100000):
+    return lib().or_su2_maximize_approximate(_d(links), _d(spins), _dims(dims),
+                                             int(or_steps), int(tol_digits), int(max_iter))
+
+
+def su2_spin_halfsweep(links, spins, dims, parity, mode, omega=1.0):
+    mx = ctypes.c_double()
+    sm = ctypes.c_double()
+    lib().or_su2_spin_halfsweep(_d(links), _d(spins), _dims(dims), int(parity), int(mode),
+                                float(omega), ctypes.byref(mx), ctypes.byref(sm))
+    return mx.value, sm.value
+
+
+def su2_heatbath_halfsweep(links, spins, dims, parity, temperature, seed, sweep):
+    lib().or_su2_heatbath_halfsweep(_d(links), _d(spins), _dims(dims), int(parity),
+                                    float(temperature), int(seed), int(sweep))
+
+
+def su2_link_sweep(links, dims, omega):
+    F = ctypes.c_double()
+    mx = ctypes.c_double()
+    av = ctypes.c_double()
+    lib().or_su2_link_sweep(_d(links), _dims(dims), float(omega), ctypes.byref(F),
+                            ctypes.byref(mx), ctypes.byref(av))
+    return F.value, mx.value, av.value
+
+
+def su3_link_sweep(links, dims, omega):
+    F = ctypes.c_double()
+    mx = ctypes.c_double()
+    av = ctypes.c_double()
+    lib().or_su3_link_sweep(_d(links), _dims(dims), float(omega), ctypes.byref(F),
+                            ctypes.byref(mx), ctypes.byref(av))
+    return F.value, mx.value, av.value
+
+
+# ---------------------------------------------------------------- SU(3)
+def su3_functional(links, dims):
+    return lib().or_su3_functional(_d(links), _dims(dims))
+
+
+def su3_reunitarize(m):
+    m = np.array(m, np.float64)
+    lib().or_su3_reunitarize(_d(m))
+    return m
+
+
+def su3_proj(m):
+    m = np.array(m, np.float64)
+    lib().or_su3_proj(_d(m))
+    return m
+
+
+def su3_mul(a, b):
+    out = np.zeros(18)
+    lib().or_su3_mul(_d(np.ascontiguousarray(a, np.float64)),
+                     _d(np.ascontiguousarray(b, np.float64)), _d(out))
+    return out
+
+
+def su3_mul_dag(a, b):
+    out = np.zeros(18)
+    lib().or_su3_mul_dag(_d(np.ascontiguousarray(a, np.float64)),
+                         _d(np.ascontiguousarray(b, np.float64)), _d(out))
+    return out
+
+
+# ---------------------------------------------------------------- checks
+def su2_plaquette(links, dims):
+    return lib().or_su2_plaquette(_d(links), _dims(dims))
+
+
+def su3_plaquette(links, dims):
+    return lib().or_su3_plaquette(_d(links), _dims(dims))
+
+
+def su2_wilson_loop(links, dims, R, T):
+    return lib().or_su2_wilson_loop(_d(links), _dims(dims), int(R), int(T))
+
+
+def su3_wilson_loop(links, dims, R, T):
+    return lib().or_su3_wilson_loop(_d(links), _dims(dims), int(R), int(T))
+
+
+# ---------------------------------------------------------------- generators
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().or_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def gen_su2_config(dims, eps, seed):
+    V = volume(dims)
+    links = np.zeros((4 * V, 4))
+    lib().or_gen_su2_config(_d(links), _dims(dims), float(eps), int(seed))
+    return links
+
+
+def gen_su3_config(dims, eps, seed):
+    V = volume(dims)
+    links = np.zeros((4 * V, 18))
+    lib().or_gen_su3_config(_d(links), _dims(dims), float(eps), int(seed))
+    return links
+
+
+def gen_su3_exp(c8, eps):
+    out = np.zeros(18)
+    lib().or_gen_su3_exp(_d(np.ascontiguousarray(c8, np.float64)), float(eps), _d(out))
+    return out
+
+
+def gen_spins_uniform(dims, seed):
+    V = volume(dims)
+    spins = np.zeros((V, 3))
+    lib().or_gen_spins_uniform(_d(spins), _dims(dims), int(seed))
+    return spins
+
+
+def set_threads(n: int):
+    lib().or_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return lib().or_get_threads()

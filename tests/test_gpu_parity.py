@@ -1,0 +1,193 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Tolerances: FP64 path -- per-sweep functional 1e-12 relative, links 1e-12
+absolute (components are O(1)), gauge invariants 1e-12.  The GPU and the
+oracle run the same checkerboard sweep order, so the only differences are
+rounding (device FMA contraction, ocml vs glibc transcendentals).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SU2_DIMS = (8, 8, 8, 8)
+SU3_DIMS = (4, 4, 4, 8)
+
+
+def _rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
+def test_su2_generator_matches_oracle(mag, oracle):
+    dims = (4, 6, 4, 8)
+    ref = oracle.gen_su2_config(dims, 0.3, 20240601)
+    with mag.MagJob(group=mag.SU2, dims=dims) as job:
+        job.generate(0.3, 20240601)
+        got = job.download()
+    assert np.max(np.abs(got - ref)) < 1e-13
+
+
+def test_su3_generator_matches_oracle(mag, oracle):
+    dims = (4, 4, 4, 6)
+    ref = oracle.gen_su3_config(dims, 0.4, 77)
+    with mag.MagJob(group=mag.SU3, dims=dims) as job:
+        job.generate(0.4, 77)
+        got = job.download()
+    assert np.max(np.abs(got - ref)) < 1e-13
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.7, 2.0])
+def test_su2_fp64_sweeps_match_oracle(mag, oracle, omega):
+    dims = SU2_DIMS
+    links = oracle.gen_su2_config(dims, 0.3, 1234)
+    n = 40
+    ref = links.copy()
+    F_ref, R_ref, A_ref = [], [], []
+    for _ in range(n):
+        F, mx, av = oracle.su2_link_sweep(ref, dims, omega)
+        F_ref.append(F)
+        R_ref.append(mx)
+        A_ref.append(av)
+    with mag.MagJob(group=mag.SU2, dims=dims, omega=omega, tol=0.0, max_sweeps=n) as job:
+        job.upload(links)
+        assert abs(job.initial_functional() - oracle.su2_functional(links, dims)) < 1e-14
+        job.sweeps(n)
+        F, R, A = job.history()
+        got = job.download()
+        P = job.plaquette()
+    assert len(F) == n
+    assert _rel(F, F_ref) < 1e-12
+    assert np.max(np.abs(np.array(A) - np.array(A_ref))) < 1e-12
+    assert np.max(np.abs(got - ref)) < 1e-12
+    assert abs(P - oracle.su2_plaquette(links, dims)) < 1e-12
+
+
+def test_su2_fp64_converges_like_oracle(mag, oracle):
+    """8^4 SU(2), eps 0.3, omega 1.7, stop at |dF/F| < 1e-12 or 1000 sweeps."""
+    dims = SU2_DIMS
+    links = oracle.gen_su2_config(dims, 0.3, 42)
+    ref = links.copy()
+    F_prev = oracle.su2_functional(ref, dims)
+    F_ref = []
+    for _ in range(1000):
+        F, _, _ = oracle.su2_link_sweep(ref, dims, 1.7)
+        F_ref.append(F)
+        if abs(F - F_prev) < 1e-12 * abs(F):
+            break
+        F_prev = F
+    with mag.MagJob(group=mag.SU2, dims=dims, omega=1.7, tol=1e-12, max_sweeps=1000) as job:
+        job.upload(links)
+        assert job.run()
+        sweeps, conv, Ffin = job.status()
+        F, _, _ = job.history()
+        got = job.download()
+    assert conv
+    m = min(len(F), len(F_ref))
+    assert abs(len(F) - len(F_ref)) <= 1
+    assert _rel(F[:m], F_ref[:m]) < 1e-12
+    if len(F) == len(F_ref):
+        assert np.max(np.abs(got - ref)) < 1e-12
+    for R, T in [(1, 1), (2, 2)]:
+        w0 = oracle.su2_wilson_loop(links, dims, R, T)
+        w1 = oracle.su2_wilson_loop(got, dims, R, T)
+        assert abs(w1 - w0) < 1e-12
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.6])
+def test_su3_fp64_sweeps_match_oracle(mag, oracle, omega):
+    dims = SU3_DIMS
+    links = oracle.gen_su3_config(dims, 0.4, 4321)
+    n = 25
+    ref = links.copy()
+    F_ref, A_ref = [], []
+    for _ in range(n):
+        F, mx, av = oracle.su3_link_sweep(ref, dims, omega)
+        F_ref.append(F)
+        A_ref.append(av)
+    with mag.MagJob(group=mag.SU3, dims=dims, omega=omega, tol=0.0, max_sweeps=n) as job:
+        job.upload(links)
+        assert abs(job.initial_functional() - oracle.su3_functional(links, dims)) < 1e-14
+        job.sweeps(n)
+        F, R, A = job.history()
+        got = job.download()
+        P = job.plaquette()
+    assert _rel(F, F_ref) < 1e-12
+    assert np.max(np.abs(np.array(A) - np.array(A_ref))) < 1e-11
+    assert np.max(np.abs(got - ref)) < 1e-12
+    assert abs(P - oracle.su3_plaquette(links, dims)) < 1e-12
+
+
+def test_su2_mixed_functional_close_to_fp64(mag, oracle):
+    """Mixed: FP32 sweeps + FP64 master rebuild + FP64 polish; the final
+    functional within 1e-10 relative of the FP64 oracle, gauge invariants
+    to 1e-12 (links are rebuilt from the FP64 master)."""
+    dims = SU2_DIMS
+    links = oracle.gen_su2_config(dims, 0.3, 42)
+    ref = links.copy()
+    F_prev = oracle.su2_functional(ref, dims)
+    for _ in range(1000):
+        F, _, _ = oracle.su2_link_sweep(ref, dims, 1.7)
+        if abs(F - F_prev) < 1e-12 * abs(F):
+            break
+        F_prev = F
+    with mag.MagJob(group=mag.SU2, dims=dims, precision=mag.MIXED, omega=1.7, tol=1e-12,
+                    max_sweeps=1000, reunit_every=10, tol_mixed=1e-8) as job:
+        job.upload(links)
+        assert job.run()
+        _, conv, Fm = job.status()
+        got = job.download()
+        start = job.fp64_start()
+    assert conv and start > 0
+    assert abs(Fm - F) / abs(F) < 1e-10
+    assert abs(oracle.su2_functional(got, dims) - F) / abs(F) < 1e-10
+    for R, T in [(1, 1), (2, 2)]:
+        assert abs(oracle.su2_wilson_loop(got, dims, R, T) -
+                   oracle.su2_wilson_loop(links, dims, R, T)) < 1e-12
+
+
+def test_su3_mixed_functional_close_to_fp64(mag, oracle):
+    dims = SU3_DIMS
+    links = oracle.gen_su3_config(dims, 0.4, 99)
+    ref = links.copy()
+    F_prev = oracle.su3_functional(ref, dims)
+    for _ in range(2000):
+        F, _, _ = oracle.su3_link_sweep(ref, dims, 1.6)
+        if abs(F - F_prev) < 1e-11 * abs(F):
+            break
+        F_prev = F
+    with mag.MagJob(group=mag.SU3, dims=dims, precision=mag.MIXED, omega=1.6, tol=1e-11,
+                    max_sweeps=2000, reunit_every=10, tol_mixed=1e-8) as job:
+        job.upload(links)
+        assert job.run()
+        _, conv, Fm = job.status()
+        got = job.download()
+    assert conv
+    assert abs(Fm - F) / abs(F) < 1e-10
+    assert abs(oracle.su3_plaquette(got, dims) - oracle.su3_plaquette(links, dims)) < 1e-12
+
+
+def test_one_shot_functionals(mag, oracle):
+    dims = (4, 4, 4, 4)
+    l2 = oracle.gen_su2_config(dims, 0.5, 5)
+    l3 = oracle.gen_su3_config(dims, 0.4, 6)
+    assert abs(mag.su2_functional(dims, l2) - oracle.su2_functional(l2, dims)) < 1e-14
+    assert abs(mag.su3_functional(dims, l3) - oracle.su3_functional(l3, dims)) < 1e-14
+    assert abs(mag.plaquette(2, dims, l2) - oracle.su2_plaquette(l2, dims)) < 1e-13
+    assert abs(mag.plaquette(3, dims, l3) - oracle.su3_plaquette(l3, dims)) < 1e-13
+
+
+def test_errors_are_reported(mag):
+    with pytest.raises(mag.MagError) as e:
+        mag.MagJob(group=mag.SU2, dims=(7, 8, 8, 8))
+    assert e.value.code == -2
+    with mag.MagJob(group=mag.SU2, dims=(4, 4, 4, 4)) as job:
+        bad = np.zeros((256, 4))
+        bad[3, 1] = np.nan
+        with pytest.raises(mag.MagError) as e:
+            job.upload(bad)
+        assert e.value.code == -4
+        with pytest.raises(mag.MagError) as e:
+            job.sweeps(1)
+        assert e.value.code == -7
