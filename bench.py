@@ -1,0 +1,189 @@
+"""MAG gauge-fixing throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
+
+A step is one checkerboard omega-overrelaxation sweep (V site updates) of a
+synthetic configuration resident in HBM.  The headline workload is SU(2)
+32^4 (eps 0.5, omega 1.7) in plain FP64; the same lattice in mixed
+precision (FP32 sweeps, FP64 functional, FP64 master rebuild every 10
+sweeps) and SU(3) 32^4 / 24^3x48 are reported beside it.  Rank 0 prints one
+JSON line.
+
+Roofline: the dominant kernel is the half-sweep kernel.  Its algorithmic
+bytes per launch are (V/2) site updates x B_site, with
+B_site = 8 links x reals/link x bytes x 2 (read + write), plus, in mixed
+precision, the site gauge read + write (2 x reals x 4 bytes).  Its mean
+launch duration is measured with HIP events around every launch on the
+job's stream (mag_job_sweeps_timed).  Peak HBM = 8.0 TB/s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def b_site(group: int, mixed: bool) -> int:
+    nr = 4 if group == 2 else 12
+    if mixed:
+        return 8 * nr * 4 * 2 + 2 * nr * 4
+    return 8 * nr * 8 * 2
+
+
+def measure(G, group, dims, precision, eps, omega, steps, warmup, seed=2024):
+    V = 1
+    for d in dims:
+        V *= d
+    job = G.MagJob(group=group, dims=dims, precision=precision, omega=omega, tol=0.0,
+                   max_sweeps=2 * steps + warmup + 4, reunit_every=10, tol_mixed=0.0)
+    try:
+        job.generate(eps, seed)
+        if warmup:
+            job.sweeps_timed(warmup, kernel=False)
+        # whole-step time without per-launch events, then the kernel's mean
+        # launch duration from a second pass with events around each launch
+        total_ms, _ = job.sweeps_timed(steps, kernel=False)
+        _, kern_ms = job.sweeps_timed(max(steps // 2, 1), kernel=True)
+        sweeps, _, F = job.status()
+    finally:
+        job.close()
+    mixed = precision == G.MIXED
+    bytes_launch = (V // 2) * b_site(group, mixed)
+    achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
+    return {
+        "value": V * steps / (total_ms * 1e-3),
+        "ms_per_step": total_ms / steps,
+        "kernel_ms": kern_ms,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": None,
+            "bytes_per_launch": bytes_launch,
+            "b_site": b_site(group, mixed),
+        },
+        "F_after": F,
+        "sweeps": sweeps,
+    }
+
+
+def cpu_baseline(dims, eps, omega, seconds):
+    """The oracle's checkerboard link sweep (FP64, OpenMP over the host's
+    cores) on the same lattice, for a bounded number of sweeps."""
+    from oracle import oracle as O
+    O.build()
+    threads = O.get_threads()
+    links = O.gen_su2_config(dims, eps, 2024)
+    V = 1
+    for d in dims:
+        V *= d
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        O.su2_link_sweep(links, dims, omega)
+        n += 1
+        el = time.perf_counter() - t0
+        if (el >= seconds and n >= 3) or n >= 2000:
+            break
+    return {
+        "value": V * n / el,
+        "unit": "site-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"SU(2) {'x'.join(map(str, dims))} FP64 checkerboard omega={omega} link "
+                  f"sweeps of the oracle (C, gcc -O3, OpenMP {threads} threads): {n} sweeps "
+                  f"in {el:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(independent MAG jobs shard one per GPU with no communication)", file=sys.stderr)
+        sys.exit(2)
+
+    import general_code_amd as G
+
+    dims = (32, 32, 32, 32)
+    eps, omega = 0.5, 1.7
+    copy_gbs = G.hbm_copy_bandwidth(1 << 30, 20)
+    print(f"[bench] copy kernel {copy_gbs:.0f} GB/s; SU(2) {dims} FP64 steps={args.steps} "
+          f"warmup={args.warmup}", file=sys.stderr, flush=True)
+    head = measure(G, 2, dims, G.F64, eps, omega, args.steps, args.warmup)
+
+    extra = {}
+    if not args.no_extra:
+        for name, group, d, prec, e, om in [
+            ("su2_32^4_mixed", 2, dims, G.MIXED, eps, omega),
+            ("su3_32^4_mixed", 3, dims, G.MIXED, 0.4, 1.6),
+            ("su3_32^4_fp64", 3, dims, G.F64, 0.4, 1.6),
+            ("su3_24^3x48_mixed", 3, (24, 24, 24, 48), G.MIXED, 0.4, 1.6),
+        ]:
+            print(f"[bench] {name}", file=sys.stderr, flush=True)
+            r = measure(G, group, d, prec, e, om, args.steps, args.warmup)
+            extra[name] = {
+                "site_updates_per_s": r["value"],
+                "ms_per_step": r["ms_per_step"],
+                "kernel_ms": r["kernel_ms"],
+                "roofline_frac": r["roofline"]["frac"],
+                "achieved_GBs": r["roofline"]["achieved"],
+                "b_site": r["roofline"]["b_site"],
+            }
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(dims, eps, omega, args.cpu_seconds)
+
+    line = {
+        "metric": "MAG site-updates/s",
+        "value": head["value"],
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Philox exp-map links + Haar gauge transform, generated on GPU)",
+        "config": {
+            "workload": "SU(2) MAG 32^4, eps 0.5, checkerboard omega-OR omega=1.7, plain FP64; "
+                        "one step = one sweep",
+            "lattice": list(dims),
+            "group": "SU(2)",
+            "omega": omega,
+        },
+        "roofline": {k: head["roofline"][k] for k in
+                     ("bound", "achieved", "peak", "unit", "frac", "traffic")},
+        "kernel": "k_su2_half<double> (one checkerboard half-sweep)",
+        "kernel_ms": head["kernel_ms"],
+        "b_site": head["roofline"]["b_site"],
+        "copy_kernel_GBs": round(copy_gbs, 1),
+        "frac_of_copy": round(head["roofline"]["achieved"] / copy_gbs, 4),
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -5,5 +5,5 @@ C ABI of include/mag_gpu.h); this package is its Python front end.
 """
 from .mag import (  # noqa: F401
     F64, MIXED, SU2, SU3, MAG_OK, MagError, MagJob, MagParams, NotConverged,
-    device_count, lib, plaquette, su2_functional, su3_functional,
+    device_count, hbm_copy_bandwidth, lib, plaquette, su2_functional, su3_functional,
 )

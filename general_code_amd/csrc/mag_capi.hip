@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -116,6 +117,7 @@ struct mag_job {
   int fp64_start = -1;
   int since_rebuild = 0;
   double invF = 0, invV = 0;
+  bool fused = true;  // reduction fused into the black half-sweep
 };
 
 namespace {
@@ -127,18 +129,31 @@ int set_device(const mag_job* j) {
   return MAG_OK;
 }
 
+mag::SweepRed sweep_red(mag_job* j) {
+  mag::SweepRed r;
+  r.ctl = j->ctl;
+  r.part = j->part;
+  r.hF = j->hF;
+  r.hR = j->hR;
+  r.hA = j->hA;
+  r.invF = j->invF;
+  r.invV = j->invV;
+  r.nparts = 2 * j->nb;
+  return r;
+}
+
 void launch_half(mag_job* j, int parity) {
-  const int* done = &j->ctl->done;
+  const mag::SweepRed r = sweep_red(j);
   if (mixed32(j)) {
     if (j->prm.group == MAG_SU2)
-      mag::launch_su2_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, j->part, done, j->st);
+      mag::launch_su2_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, r, j->fused, j->st);
     else
-      mag::launch_su3_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, j->part, done, j->st);
+      mag::launch_su3_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, r, j->fused, j->st);
   } else {
     if (j->prm.group == MAG_SU2)
-      mag::launch_su2_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, j->part, done, j->st);
+      mag::launch_su2_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, r, j->fused, j->st);
     else
-      mag::launch_su3_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, j->part, done, j->st);
+      mag::launch_su3_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, r, j->fused, j->st);
   }
 }
 
@@ -154,8 +169,9 @@ void launch_rebuild(mag_job* j, bool final64) {
 
 // sweep tail shared by the plain and timed paths
 void enqueue_after_sweep(mag_job* j) {
-  mag::launch_reduce(j->part, 2 * j->nb, mag::RED_SWEEP, j->invF, j->invV, j->ctl, j->hF, j->hR,
-                     j->hA, nullptr, j->st);
+  if (!j->fused)
+    mag::launch_reduce(j->part, 2 * j->nb, mag::RED_SWEEP, j->invF, j->invV, j->ctl, j->hF,
+                       j->hR, j->hA, nullptr, j->st);
   if (mixed32(j) && ++j->since_rebuild >= j->prm.reunit_every) {
     launch_rebuild(j, false);
     j->since_rebuild = 0;
@@ -314,6 +330,7 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   mag_job* j = new (std::nothrow) mag_job();
   if (!j) return fail(MAG_ERR_NOMEM, "host allocation failed");
   j->prm = *p;
+  if (const char* e = getenv("MAG_FUSED_REDUCE")) j->fused = atoi(e) != 0;
   int r = set_device(j);
   if (r != MAG_OK) {
     delete j;
@@ -626,3 +643,34 @@ int mag_plaquette(int group, const int* dims, const double* links, double* P) {
 }
 
 }  // extern "C"
+
+extern "C" int mag_hbm_copy_bandwidth(int device, uint64_t bytes, int iters, double* gbs) {
+  if (!gbs || iters < 1 || bytes < 16) return fail(MAG_ERR_ARG, "bad argument");
+  bytes &= ~(uint64_t)15;
+  HIPCHK(hipSetDevice(device));
+  void *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, bytes));
+  hipError_t e = hipMalloc(&b, bytes);
+  if (e != hipSuccess) {
+    hipFree(a);
+    return fail(MAG_ERR_NOMEM, "hipMalloc failed: %s", hipGetErrorString(e));
+  }
+  hipMemset(a, 0, bytes);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  mag::launch_copy(a, b, bytes, nullptr);  // warm-up
+  hipEventRecord(e0, nullptr);
+  for (int i = 0; i < iters; i++) mag::launch_copy(a, b, bytes, nullptr);
+  hipEventRecord(e1, nullptr);
+  e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  if (e != hipSuccess) return fail(MAG_ERR_HIP, "copy failed: %s", hipGetErrorString(e));
+  *gbs = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+  return MAG_OK;
+}

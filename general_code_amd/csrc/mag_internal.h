@@ -43,7 +43,21 @@ struct JobCtl {
   double tol;      // relative functional change for convergence
   double F_prev;   // functional after the last sweep (initial value first)
   double F0;       // functional of the uploaded configuration
-  double pad;
+  unsigned int arrived;  // black half-sweep workgroups done (fused reduction)
+  int pad;
+};
+
+// Everything the end-of-sweep reduction needs: partial triples of the red
+// half at part[0 .. 3*nb), of the black half at part[3*nb .. 6*nb).
+struct SweepRed {
+  JobCtl* ctl;
+  double* part;
+  double* hF;
+  double* hR;
+  double* hA;
+  double invF;  // 1/(4V) (SU(2)) or 1/(12V) (SU(3))
+  double invV;
+  int nparts;   // 2*nb
 };
 
 // reduction kernel modes
@@ -56,12 +70,15 @@ enum { RED_INIT = 0, RED_SWEEP = 1, RED_PLAIN = 2, RED_RESET = 3 };
 // residual max).  A sweep uses 2*nblocks(g) partial triples (red, black).
 inline int nblocks(const Geom& g) { return (int)((g.Vh + kBlock - 1) / kBlock); }
 
+// fused = true: the last black workgroup to finish reduces the sweep's
+// partials itself (device-scope fence + arrival counter) and no separate
+// reduction kernel is launched for the sweep.
 template <typename T>
 void launch_su2_half(T* U, T* G, const Geom& g, int parity, double omega,
-                     double* part, const int* done, hipStream_t st);
+                     const SweepRed& r, bool fused, hipStream_t st);
 template <typename T>
 void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega,
-                     double* part, const int* done, hipStream_t st);
+                     const SweepRed& r, bool fused, hipStream_t st);
 
 // per-sweep (or initial) reduction of the partials into the job history
 void launch_reduce(const double* part, int nparts, int mode, double invF,
@@ -112,4 +129,9 @@ void launch_gen_su3(double* aos, double* gauge_scratch, const Geom& g, double ep
 void launch_su2_plaquette(const double* U, const Geom& g, double* part, hipStream_t st);
 void launch_su3_plaquette(const double* U, const Geom& g, double* part, hipStream_t st);
 
+}  // namespace mag
+
+namespace mag {
+// 16-byte streaming copy of bytes (a multiple of 16)
+void launch_copy(const void* a, void* b, size_t bytes, hipStream_t st);
 }  // namespace mag

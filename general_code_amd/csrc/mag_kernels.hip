@@ -150,17 +150,79 @@ __device__ __forceinline__ void write_part(double* part, double a, double b, dou
   }
 }
 
+// ================================================================== reduction
+// Reduce nparts partial triples (F sum, residual sum, residual max) and
+// apply the result to the job's control block; called by every thread of
+// one workgroup.  Thread t combines partials t, t + kBlock, ... in order,
+// then the fixed-order workgroup tree: the result depends only on the job.
+__device__ void reduce_partials(const double* part, int nparts, int mode, double invF,
+                                double invV, JobCtl* ctl, double* hF, double* hR, double* hA,
+                                double* out3) {
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kBlock) {
+    a += part[3 * i];
+    b += part[3 * i + 1];
+    c = fmax(c, part[3 * i + 2]);
+  }
+  block_reduce3<kWaves>(a, b, c);
+  if (threadIdx.x == 0) {
+    const double F = a * invF;
+    if (mode == RED_INIT) {
+      ctl->F_prev = F;
+      ctl->F0 = F;
+    } else if (mode == RED_RESET) {
+      ctl->F_prev = F;
+    } else if (mode == RED_SWEEP) {
+      const int sw = ctl->sweep;
+      hF[sw] = F;
+      hR[sw] = c;
+      hA[sw] = b * invV;
+      const double rel = fabs(F - ctl->F_prev) / fabs(F);
+      ctl->F_prev = F;
+      ctl->sweep = sw + 1;
+      if (rel < ctl->tol) {
+        ctl->converged = 1;
+        ctl->done = 1;
+      }
+      if (sw + 1 >= ctl->max_sweeps) ctl->done = 1;
+    }
+    if (out3) {
+      out3[0] = F;
+      out3[1] = b * invV;
+      out3[2] = c;
+    }
+  }
+}
+
+// End of a black half-sweep with the fused reduction: every workgroup
+// publishes its partial (device-scope release) and bumps the arrival
+// counter; the last one to arrive (acquire) reduces the sweep.
+__device__ __forceinline__ void fused_sweep_epilogue(const SweepRed& r) {
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned prev = atomicAdd(&r.ctl->arrived, 1u);
+    is_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (is_last) {
+    __threadfence();
+    reduce_partials(r.part, r.nparts, RED_SWEEP, r.invF, r.invV, r.ctl, r.hF, r.hR, r.hA,
+                    nullptr);
+    if (threadIdx.x == 0) r.ctl->arrived = 0u;
+  }
+}
+
 // ================================================================== SU(2)
 // One checkerboard half-sweep over the sites of parity p.  The black
 // (p = 1) half also sums the MAG functional of the 8 links each site has
 // just written: every link is written exactly once in that half, so the
 // sum is the functional after the sweep with no extra pass.
-template <typename T, bool GA>
+template <typename T, bool GA, bool FUSED>
 __global__ __launch_bounds__(kBlock) void k_su2_half(T* __restrict__ U, T* __restrict__ G,
-                                                     Geom g, int p, T omega,
-                                                     double* __restrict__ part,
-                                                     const int* __restrict__ done) {
-  if (*done) return;
+                                                     Geom g, int p, T omega, SweepRed r) {
+  if (r.ctl->done) return;
+  double* part = r.part + (size_t)3 * gridDim.x * p;
   const uint32_t h = blockIdx.x * kBlock + threadIdx.x;
   double fs = 0.0, rs = 0.0, rm = 0.0;
   if (h < g.Vh) {
@@ -201,24 +263,27 @@ __global__ __launch_bounds__(kBlock) void k_su2_half(T* __restrict__ U, T* __res
   }
   block_reduce3<kWaves>(fs, rs, rm);
   write_part(part, fs, rs, rm);
+  if (FUSED && p == 1) fused_sweep_epilogue(r);
 }
 
 template <typename T>
-void launch_su2_half(T* U, T* G, const Geom& g, int parity, double omega, double* part,
-                     const int* done, hipStream_t st) {
+void launch_su2_half(T* U, T* G, const Geom& g, int parity, double omega, const SweepRed& r,
+                     bool fused, hipStream_t st) {
   const int nb = nblocks(g);
-  double* pp = part + (size_t)3 * nb * parity;
-  if (G)
-    hipLaunchKernelGGL((k_su2_half<T, true>), dim3(nb), dim3(kBlock), 0, st, U, G, g, parity,
-                       (T)omega, pp, done);
-  else
-    hipLaunchKernelGGL((k_su2_half<T, false>), dim3(nb), dim3(kBlock), 0, st, U, G, g, parity,
-                       (T)omega, pp, done);
+#define L_(GA, FU)                                                                        \
+  hipLaunchKernelGGL((k_su2_half<T, GA, FU>), dim3(nb), dim3(kBlock), 0, st, U, G, g, parity, \
+                     (T)omega, r)
+  if (G) {
+    if (fused) L_(true, true); else L_(true, false);
+  } else {
+    if (fused) L_(false, true); else L_(false, false);
+  }
+#undef L_
 }
-template void launch_su2_half<float>(float*, float*, const Geom&, int, double, double*,
-                                     const int*, hipStream_t);
-template void launch_su2_half<double>(double*, double*, const Geom&, int, double, double*,
-                                      const int*, hipStream_t);
+template void launch_su2_half<float>(float*, float*, const Geom&, int, double, const SweepRed&,
+                                     bool, hipStream_t);
+template void launch_su2_half<double>(double*, double*, const Geom&, int, double,
+                                      const SweepRed&, bool, hipStream_t);
 
 // ================================================================== SU(3)
 // Cabibbo-Marinari subgroup (I,J) update of the 8 links of one site, links
@@ -286,12 +351,11 @@ __device__ __forceinline__ double su3_link_F(const M2<T>& m) {
   return cnorm2(a) + cnorm2(b) + cnorm2(u22);
 }
 
-template <typename T, bool GA>
+template <typename T, bool GA, bool FUSED>
 __global__ __launch_bounds__(kBlock) void k_su3_half(T* __restrict__ U, T* __restrict__ G,
-                                                     Geom g, int p, T omega,
-                                                     double* __restrict__ part,
-                                                     const int* __restrict__ done) {
-  if (*done) return;
+                                                     Geom g, int p, T omega, SweepRed r) {
+  if (r.ctl->done) return;
+  double* part = r.part + (size_t)3 * gridDim.x * p;
   const uint32_t h = blockIdx.x * kBlock + threadIdx.x;
   double fs = 0.0, rs = 0.0, rm = 0.0;
   if (h < g.Vh) {
@@ -324,65 +388,35 @@ __global__ __launch_bounds__(kBlock) void k_su3_half(T* __restrict__ U, T* __res
   }
   block_reduce3<kWaves>(fs, rs, rm);
   write_part(part, fs, rs, rm);
+  if (FUSED && p == 1) fused_sweep_epilogue(r);
 }
 
 template <typename T>
-void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega, double* part,
-                     const int* done, hipStream_t st) {
+void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega, const SweepRed& r,
+                     bool fused, hipStream_t st) {
   const int nb = nblocks(g);
-  double* pp = part + (size_t)3 * nb * parity;
-  if (G)
-    hipLaunchKernelGGL((k_su3_half<T, true>), dim3(nb), dim3(kBlock), 0, st, U, G, g, parity,
-                       (T)omega, pp, done);
-  else
-    hipLaunchKernelGGL((k_su3_half<T, false>), dim3(nb), dim3(kBlock), 0, st, U, G, g, parity,
-                       (T)omega, pp, done);
+#define L_(GA, FU)                                                                        \
+  hipLaunchKernelGGL((k_su3_half<T, GA, FU>), dim3(nb), dim3(kBlock), 0, st, U, G, g, parity, \
+                     (T)omega, r)
+  if (G) {
+    if (fused) L_(true, true); else L_(true, false);
+  } else {
+    if (fused) L_(false, true); else L_(false, false);
+  }
+#undef L_
 }
-template void launch_su3_half<float>(float*, float*, const Geom&, int, double, double*,
-                                     const int*, hipStream_t);
-template void launch_su3_half<double>(double*, double*, const Geom&, int, double, double*,
-                                      const int*, hipStream_t);
+template void launch_su3_half<float>(float*, float*, const Geom&, int, double, const SweepRed&,
+                                     bool, hipStream_t);
+template void launch_su3_half<double>(double*, double*, const Geom&, int, double,
+                                      const SweepRed&, bool, hipStream_t);
 
-// ================================================================== reduction
+// one-workgroup reduction kernel (initial functional, unfused sweeps, checks)
 __global__ __launch_bounds__(kBlock) void k_reduce(const double* __restrict__ part, int nparts,
                                                    int mode, double invF, double invV,
                                                    JobCtl* ctl, double* hF, double* hR,
                                                    double* hA, double* out3) {
   if (mode == RED_SWEEP && ctl->done) return;
-  double a = 0.0, b = 0.0, c = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += kBlock) {
-    a += part[3 * i];
-    b += part[3 * i + 1];
-    c = fmax(c, part[3 * i + 2]);
-  }
-  block_reduce3<kWaves>(a, b, c);
-  if (threadIdx.x == 0) {
-    const double F = a * invF;
-    if (mode == RED_INIT) {
-      ctl->F_prev = F;
-      ctl->F0 = F;
-    } else if (mode == RED_RESET) {
-      ctl->F_prev = F;
-    } else if (mode == RED_SWEEP) {
-      const int sw = ctl->sweep;
-      hF[sw] = F;
-      hR[sw] = c;
-      hA[sw] = b * invV;
-      const double rel = fabs(F - ctl->F_prev) / fabs(F);
-      ctl->F_prev = F;
-      ctl->sweep = sw + 1;
-      if (rel < ctl->tol) {
-        ctl->converged = 1;
-        ctl->done = 1;
-      }
-      if (sw + 1 >= ctl->max_sweeps) ctl->done = 1;
-    }
-    if (out3) {
-      out3[0] = F;
-      out3[1] = b * invV;
-      out3[2] = c;
-    }
-  }
+  reduce_partials(part, nparts, mode, invF, invV, ctl, hF, hR, hA, out3);
 }
 
 void launch_reduce(const double* part, int nparts, int mode, double invF, double invV,
@@ -845,4 +879,21 @@ void launch_su3_plaquette(const double* U, const Geom& g, double* part, hipStrea
   hipLaunchKernelGGL(k_su3_plaquette, dim3(2 * nblocks(g)), dim3(kBlock), 0, st, U, g, part);
 }
 
+}  // namespace mag
+
+namespace mag {
+// streaming copy (16-byte vectors, grid-stride): the bandwidth reference
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ a, float4* __restrict__ b,
+                                              size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) b[i] = a[i];
+}
+void launch_copy(const void* a, void* b, size_t bytes, hipStream_t st) {
+  const size_t n = bytes / 16;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 1u << 20) blocks = 1u << 20;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, st, (const float4*)a,
+                     (float4*)b, n);
+}
 }  // namespace mag

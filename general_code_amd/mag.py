@@ -87,6 +87,7 @@ SIGNATURES = {
     "mag_su2_functional": (ctypes.c_int, [_ip, _dp, _dp]),
     "mag_su3_functional": (ctypes.c_int, [_ip, _dp, _dp]),
     "mag_plaquette": (ctypes.c_int, [ctypes.c_int, _ip, _dp, _dp]),
+    "mag_hbm_copy_bandwidth": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, _dp]),
 }
 
 _lib = None
@@ -122,6 +123,13 @@ def _dptr(a: np.ndarray):
 
 def _dims(d):
     return (ctypes.c_int * 4)(*[int(x) for x in d])
+
+
+def hbm_copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20, device: int = 0) -> float:
+    """GB/s (read + write) of a plain 16-byte streaming copy kernel."""
+    g = ctypes.c_double()
+    _check(lib().mag_hbm_copy_bandwidth(int(device), int(nbytes), int(iters), ctypes.byref(g)))
+    return g.value
 
 
 def device_count() -> int:

@@ -133,6 +133,12 @@ int mag_job_plaquette(mag_job *job, double *P);
 int mag_job_sweeps_timed(mag_job *job, int n, double *total_ms,
                          double *kern_ms);
 
+/* Measured HBM bandwidth of a plain streaming copy kernel (read bytes +
+ * write bytes per second / 1e9) over buffers of the given size: the
+ * practical ceiling the sweep kernels are compared with.  No counterpart in
+ * the reference (measurement aid). */
+int mag_hbm_copy_bandwidth(int device, uint64_t bytes, int iters, double *gbs);
+
 /* ------------------------------------------------------------ one-shot
  * functionals and checks on host arrays (reference layout) */
 int mag_su2_functional(const int *dims, const double *links, double *F);

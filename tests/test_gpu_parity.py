@@ -38,11 +38,13 @@ def test_su3_generator_matches_oracle(mag, oracle):
     assert np.max(np.abs(got - ref)) < 1e-13
 
 
-@pytest.mark.parametrize("omega", [1.0, 1.7, 2.0])
-def test_su2_fp64_sweeps_match_oracle(mag, oracle, omega):
+# omega = 2 (pure reflection) conserves F and is not contracting: rounding
+# differences grow about 3x per sweep, so the trajectory is compared over a
+# few sweeps only (F conservation is checked over many below).
+@pytest.mark.parametrize("omega,n", [(1.0, 40), (1.7, 40), (2.0, 6)])
+def test_su2_fp64_sweeps_match_oracle(mag, oracle, omega, n):
     dims = SU2_DIMS
     links = oracle.gen_su2_config(dims, 0.3, 1234)
-    n = 40
     ref = links.copy()
     F_ref, R_ref, A_ref = [], [], []
     for _ in range(n):
@@ -62,6 +64,17 @@ def test_su2_fp64_sweeps_match_oracle(mag, oracle, omega):
     assert np.max(np.abs(np.array(A) - np.array(A_ref))) < 1e-12
     assert np.max(np.abs(got - ref)) < 1e-12
     assert abs(P - oracle.su2_plaquette(links, dims)) < 1e-12
+
+
+def test_su2_reflection_conserves_functional(mag, oracle):
+    dims = SU2_DIMS
+    links = oracle.gen_su2_config(dims, 0.3, 5)
+    F0 = oracle.su2_functional(links, dims)
+    with mag.MagJob(group=mag.SU2, dims=dims, omega=2.0, tol=0.0, max_sweeps=50) as job:
+        job.upload(links)
+        job.sweeps(50)
+        F, _, _ = job.history()
+    assert np.max(np.abs(np.array(F) - F0)) < 1e-12
 
 
 def test_su2_fp64_converges_like_oracle(mag, oracle):

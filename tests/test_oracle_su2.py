@@ -1,0 +1,278 @@
+"""Pin the CPU oracle's SU(2) part against closed forms, invariants and an
+independent brute-force formulation (the reference ships no golden vectors
+for this path: ilyakudrov/general_code tests print numbers without
+assertions, tests/cpu/code/test_mag.cpp:159-222).
+
+Each check is chosen so that a plausible restatement mistake (a wrong sign
+in a product, a transposed adjoint matrix, a dropped neighbour term, the
+wrong parity order) fails it.
+"""
+import numpy as np
+import pytest
+
+S1 = np.array([[0, 1], [1, 0]], complex)
+S2 = np.array([[0, -1j], [1j, 0]])
+S3 = np.array([[1, 0], [0, -1]], complex)
+SIG = [S1, S2, S3]
+
+
+def mat(q):
+    """a0 + i a.sigma (lib/cpu/include/matrix.h:8-13) as a 2x2 matrix."""
+    return q[0] * np.eye(2) + 1j * (q[1] * S1 + q[2] * S2 + q[3] * S3)
+
+
+def quat(M):
+    a0 = 0.5 * np.trace(M).real
+    a = [0.5 * np.trace(-1j * s @ M).real for s in SIG]
+    return np.array([a0, *a])
+
+
+def rand_su2(rng, n=None):
+    q = rng.normal(size=(n, 4) if n else 4)
+    return q / np.linalg.norm(q, axis=-1, keepdims=True)
+
+
+def adjoint(q):
+    A = mat(q)
+    return np.array([[0.5 * np.trace(SIG[i] @ A @ SIG[j] @ A.conj().T).real
+                      for j in range(3)] for i in range(3)])
+
+
+def test_su2_products_match_2x2_matrices(oracle):
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        a, b = rand_su2(rng), rand_su2(rng)
+        A, B = mat(a), mat(b)
+        assert np.allclose(mat(oracle.su2_mul(a, b)), A @ B, atol=1e-15)
+        assert np.allclose(mat(oracle.su2_mul_dag(a, b)), A @ B.conj().T, atol=1e-15)
+        assert np.allclose(mat(oracle.su2_dag_mul(a, b)), A.conj().T @ B, atol=1e-15)
+        assert abs(np.linalg.norm(oracle.su2_mul(a, b)) - 1) < 1e-15
+
+
+def test_link_functional_closed_form(oracle):
+    rng = np.random.default_rng(2)
+    for q in rand_su2(rng, 100):
+        F = oracle.su2_link_functional(q)
+        M = mat(q)
+        assert abs(F - (1 - 2 * (q[1] ** 2 + q[2] ** 2))) < 1e-15
+        assert abs(F - (abs(M[0, 0]) ** 2 - abs(M[0, 1]) ** 2)) < 1e-15
+    assert oracle.su2_link_functional(np.array([1.0, 0, 0, 0])) == 1.0
+
+
+def test_contribution_is_adjoint_matrix(oracle):
+    rng = np.random.default_rng(3)
+    for q in rand_su2(rng, 20):
+        R = adjoint(q)
+        b = rng.normal(size=3)
+        h0 = rng.normal(size=3)
+        assert np.allclose(oracle.contribution1(h0, q, b), h0 + R @ b, atol=1e-14)
+        assert np.allclose(oracle.contribution1_conj(h0, q, b), h0 + R.T @ b, atol=1e-14)
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-14)
+        assert abs(np.linalg.det(R) - 1) < 1e-14
+
+
+def test_reflect_and_parallel(oracle):
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        s = rng.normal(size=3)
+        s /= np.linalg.norm(s)
+        V = rng.normal(size=3) * 3
+        v = V / np.linalg.norm(V)
+        r = oracle.reflect_fast(s, V)
+        assert np.allclose(r, 2 * np.dot(s, v) * v - s, atol=1e-14)
+        assert np.allclose(oracle.reflect_fast(r, V), s, atol=1e-14)
+        assert abs(np.dot(r, V) - np.dot(s, V)) < 1e-13
+        p, d = oracle.parallel(s, V)
+        assert np.allclose(p, v, atol=1e-15)
+        assert abs(d - (1 - np.dot(s, v))) < 1e-15
+
+
+def test_gauge_matrix_spin_roundtrip(oracle):
+    """row3 of the rotation of G = GetGaugeMatrix(s) is s (matrix.h:948-963),
+    and G^+ s3 G = s.sigma for the reference's convention."""
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        s = rng.normal(size=3)
+        s /= np.linalg.norm(s)
+        g = oracle.gauge_matrix(s)
+        assert abs(np.linalg.norm(g) - 1) < 1e-14
+        a0, a1, a2, a3 = g
+        row3 = np.array([2 * (a1 * a3 + a0 * a2), 2 * (a2 * a3 - a0 * a1),
+                         a0 ** 2 + a3 ** 2 - a1 ** 2 - a2 ** 2])
+        assert np.allclose(row3, s, atol=1e-13)
+        G = mat(g)
+        X = G.conj().T @ S3 @ G
+        assert np.allclose(X, s[0] * S1 + s[1] * S2 + s[2] * S3, atol=1e-13)
+    assert np.isnan(oracle.gauge_matrix(np.array([0.0, 0.0, 1.0]))).any()
+
+
+def _lattice(oracle, dims, eps, seed):
+    return oracle.gen_su2_config(dims, eps, seed)
+
+
+def _contribution_site_bruteforce(links, spins, dims, site):
+    """h(x) = sum_mu M(U_x,mu) s(x+mu) + M(U_x-mu,mu)^T s(x-mu) from 2x2
+    matrices (independent of the oracle's closed-form expressions)."""
+    L = dims
+    x = np.unravel_index(site, L[::-1])[::-1]  # (x, y, z, t)
+    h = np.zeros(3)
+    for mu in range(4):
+        xf = list(x)
+        xf[mu] = (xf[mu] + 1) % L[mu]
+        xb = list(x)
+        xb[mu] = (xb[mu] - 1) % L[mu]
+        f = np.ravel_multi_index(xf[::-1], L[::-1])
+        b = np.ravel_multi_index(xb[::-1], L[::-1])
+        h += adjoint(links[4 * site + mu]) @ spins[f]
+        h += adjoint(links[4 * b + mu]).T @ spins[b]
+    return h
+
+
+def test_contribution_site_bruteforce(oracle):
+    dims = (2, 4, 2, 4)
+    links = _lattice(oracle, dims, 0.7, 11)
+    spins = oracle.gen_spins_uniform(dims, 12)
+    for site in range(int(np.prod(dims))):
+        h = oracle.contribution_site(links, spins, dims, site)
+        assert np.allclose(h, _contribution_site_bruteforce(links, spins, dims, site), atol=1e-13)
+
+
+def test_functional_spin_equals_link_functional_after_transform(oracle):
+    """The reference author's own check (tests/cpu/code/test_mag.cpp:206-222):
+    F_spin(U, s) = F(gauge_tranformation_spins(U, s))."""
+    dims = (4, 4, 4, 4)
+    links = _lattice(oracle, dims, 0.5, 13)
+    spins = oracle.gen_spins_uniform(dims, 14)
+    Fs = oracle.su2_functional_spin(links, spins, dims)
+    Ut = oracle.su2_gauge_transform_spins(links, spins, dims)
+    assert abs(Fs - oracle.su2_functional(Ut, dims)) < 1e-13
+    assert abs(oracle.su2_plaquette(Ut, dims) - oracle.su2_plaquette(links, dims)) < 1e-13
+    # identity configuration and aligned spins: F = 1
+    ident = np.tile([1.0, 0, 0, 0], (4 * 256, 1))
+    assert oracle.su2_functional(ident, dims) == 1.0
+
+
+def test_lexicographic_reflection_sweep_conserves_functional(oracle):
+    dims = (4, 4, 4, 4)
+    links = _lattice(oracle, dims, 0.5, 15)
+    spins = oracle.gen_spins_uniform(dims, 16)
+    F0 = oracle.su2_functional_spin(links, spins, dims)
+    for _ in range(5):
+        oracle.su2_or_sweep_lexi(links, spins, dims)
+        assert abs(oracle.su2_functional_spin(links, spins, dims) - F0) < 1e-14
+
+
+def test_relaxation_is_monotone_and_maximisation_converges(oracle):
+    dims = (4, 4, 4, 4)
+    links = _lattice(oracle, dims, 0.3, 17)
+    spins = oracle.gen_spins_uniform(dims, 18)
+    F = oracle.su2_functional_spin(links, spins, dims)
+    for _ in range(10):
+        oracle.su2_relax_sweep_lexi(links, spins, dims)
+        F1 = oracle.su2_functional_spin(links, spins, dims)
+        assert F1 >= F - 1e-15
+        F = F1
+    it = oracle.su2_maximize_final(links, spins, dims, 4, 1e-10, 1e-14, 10000)
+    assert it > 0
+    dmax, davg = oracle.su2_relax_sweep_lexi(links, spins, dims)
+    assert dmax < 1e-9
+    Fa = oracle.su2_functional_spin(links, spins, dims)
+    assert Fa >= F - 1e-15
+    # approximate criterion terminates too
+    spins2 = oracle.gen_spins_uniform(dims, 19)
+    assert oracle.su2_maximize_approximate(links, spins2, dims, 4, 7, 10000) > 0
+
+
+def test_omega_gauge_rotates_by_omega_theta(oracle):
+    rng = np.random.default_rng(6)
+    for _ in range(30):
+        h = rng.normal(size=3)
+        hh = h / np.linalg.norm(h)
+        theta = np.arccos(hh[2])
+        for w in (0.5, 1.0, 1.7, 2.0):
+            g = oracle.omega_gauge(h, w)
+            G = mat(g)
+            X = G.conj().T @ S3 @ G
+            n = np.array([0.5 * np.trace(s @ X).real for s in SIG])
+            assert abs(g[3]) == 0.0
+            assert abs(np.arccos(np.clip(n[2], -1, 1)) - min(w * theta, 2 * np.pi - w * theta)) < 1e-12
+            # n stays in the plane spanned by e3 and h
+            assert abs(np.dot(n, np.cross([0, 0, 1.0], hh))) < 1e-12
+        g1 = oracle.omega_gauge(h, 1.0)
+        G1 = mat(g1)
+        X = G1.conj().T @ S3 @ G1
+        n = np.array([0.5 * np.trace(s @ X).real for s in SIG])
+        assert np.allclose(n, hh, atol=1e-13)
+
+
+def test_block_fields_match_trace_formula(oracle):
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        Bc = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+        B = np.array([Bc[0, 0].real, Bc[0, 0].imag, Bc[0, 1].real, Bc[0, 1].imag,
+                      Bc[1, 0].real, Bc[1, 0].imag, Bc[1, 1].real, Bc[1, 1].imag])
+        xf = [0.5 * np.trace(s @ Bc @ S3 @ Bc.conj().T).real for s in SIG]
+        xb = [0.5 * np.trace(s @ Bc.conj().T @ S3 @ Bc).real for s in SIG]
+        assert np.allclose(oracle.block_field_fwd(B), xf, atol=1e-13)
+        assert np.allclose(oracle.block_field_bwd(B), xb, atol=1e-13)
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.3, 1.7, 2.0])
+def test_link_sweep_equals_spin_sweep(oracle, omega):
+    """Two independent formulations of the same checkerboard sweep: spins
+    with read-only links (reference primitives, geodesic rule) and the
+    link form the GPU runs.  F per sweep must agree."""
+    dims = (4, 4, 4, 4)
+    links = _lattice(oracle, dims, 0.4, 21)
+    spins = np.tile([0.0, 0.0, 1.0], (256, 1))
+    W = links.copy()
+    n = 4 if omega == 2.0 else 12
+    for _ in range(n):
+        oracle.su2_spin_halfsweep(links, spins, dims, 0, 2, omega)
+        oracle.su2_spin_halfsweep(links, spins, dims, 1, 2, omega)
+        F, _, _ = oracle.su2_link_sweep(W, dims, omega)
+        assert abs(oracle.su2_functional_spin(links, spins, dims) - F) < 1e-13
+
+
+def test_spin_modes_reduce_to_reference_updates(oracle):
+    """omega=1 of the geodesic rule is spin::parallel, omega=2 is
+    spin::reflect_fast (to rounding: the rule goes through atan2/sincos)."""
+    dims = (4, 4, 4, 4)
+    links = _lattice(oracle, dims, 0.5, 23)
+    s0 = oracle.gen_spins_uniform(dims, 24)
+    for omega, mode in [(1.0, 1), (2.0, 0)]:
+        a, b = s0.copy(), s0.copy()
+        oracle.su2_spin_halfsweep(links, a, dims, 0, 2, omega)
+        oracle.su2_spin_halfsweep(links, b, dims, 0, mode)
+        assert np.max(np.abs(a - b)) < 1e-13
+
+
+def test_link_sweep_monotone_and_gauge_invariant(oracle):
+    dims = (4, 4, 4, 8)
+    links = _lattice(oracle, dims, 0.5, 25)
+    W = links.copy()
+    F = oracle.su2_functional(W, dims)
+    for _ in range(30):
+        F1, rmax, ravg = oracle.su2_link_sweep(W, dims, 1.7)
+        assert F1 >= F - 1e-14
+        assert 0 <= ravg <= rmax <= 2
+        F = F1
+    assert np.max(np.abs(np.linalg.norm(W, axis=1) - 1)) < 1e-13
+    assert abs(oracle.su2_plaquette(W, dims) - oracle.su2_plaquette(links, dims)) < 1e-13
+    for R, T in [(1, 1), (2, 3), (4, 4)]:
+        assert abs(oracle.su2_wilson_loop(W, dims, R, T) -
+                   oracle.su2_wilson_loop(links, dims, R, T)) < 1e-13
+
+
+def test_link_sweep_thread_independent(oracle):
+    dims = (4, 4, 8, 8)
+    links = _lattice(oracle, dims, 0.5, 27)
+    a, b = links.copy(), links.copy()
+    n0 = oracle.get_threads()
+    oracle.set_threads(1)
+    ra = [oracle.su2_link_sweep(a, dims, 1.7) for _ in range(3)]
+    oracle.set_threads(max(n0, 2))
+    rb = [oracle.su2_link_sweep(b, dims, 1.7) for _ in range(3)]
+    oracle.set_threads(n0)
+    assert ra == rb
+    assert np.array_equal(a, b)
