@@ -46,11 +46,11 @@ def measure(G, group, dims, precision, eps, omega, steps, warmup, seed=2024):
     try:
         job.generate(eps, seed)
         if warmup:
-            job.sweeps_timed(warmup, kernel=False)
-        # whole-step time without per-launch events, then the kernel's mean
-        # launch duration from a second pass with events around each launch
-        total_ms, _ = job.sweeps_timed(steps, kernel=False)
-        _, kern_ms = job.sweeps_timed(max(steps // 2, 1), kernel=True)
+            job.sweeps_timed(warmup)
+        # whole steps (sweep = 2 half-sweeps + reduction), then the dominant
+        # kernel's mean duration over back-to-back half-sweep launches
+        total_ms = job.sweeps_timed(steps)
+        kern_ms = job.halfsweep_timed(2 * steps)
         sweeps, _, F = job.status()
     finally:
         job.close()

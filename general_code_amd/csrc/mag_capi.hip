@@ -7,6 +7,7 @@
 // FP64 rebuild from the FP64 master every reunit_every sweeps, FP64 rebuild
 // and FP64 polish at the end).
 #include "mag_gpu.h"
+#include "mag_capi_util.h"
 #include "mag_internal.h"
 
 #include <cmath>
@@ -22,10 +23,10 @@ using mag::Geom;
 using mag::JobCtl;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const char* fmt, ...) {
+int mag::fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -35,20 +36,8 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHK(x)                                                                 \
-  do {                                                                            \
-    hipError_t e_ = (x);                                                          \
-    if (e_ != hipSuccess)                                                         \
-      return fail(e_ == hipErrorOutOfMemory ? MAG_ERR_NOMEM : MAG_ERR_HIP,        \
-                  "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__,   \
-                  __LINE__);                                                      \
-  } while (0)
-
-#define CHK(x)              \
-  do {                      \
-    int r_ = (x);           \
-    if (r_ != MAG_OK) return r_; \
-  } while (0)
+namespace {
+using mag::fail;
 
 constexpr int kChunk = 20;  // sweeps enqueued between convergence polls
 
@@ -99,7 +88,8 @@ struct mag_job {
   int nr = 0;      // reals per link in the SoA layout (4 or 12)
   int nr_in = 0;   // reals per link in the reference layout (4 or 18)
   size_t nreal = 0;  // reals in one link array (8 * Vh * nr)
-  int nb = 0;        // workgroups per half-sweep
+  int nb = 0;        // workgroups per half-sweep (two lanes per site)
+  int nf = 0;        // workgroups per parity of the functional / plaquette kernels
   hipStream_t st = nullptr;
   double* U64 = nullptr;  // FP64 links (plain) / FP64 master then output (mixed)
   float* W32 = nullptr;   // mixed: FP32 working links
@@ -117,7 +107,6 @@ struct mag_job {
   int fp64_start = -1;
   int since_rebuild = 0;
   double invF = 0, invV = 0;
-  bool fused = true;  // reduction fused into the black half-sweep
 };
 
 namespace {
@@ -129,31 +118,17 @@ int set_device(const mag_job* j) {
   return MAG_OK;
 }
 
-mag::SweepRed sweep_red(mag_job* j) {
-  mag::SweepRed r;
-  r.ctl = j->ctl;
-  r.part = j->part;
-  r.hF = j->hF;
-  r.hR = j->hR;
-  r.hA = j->hA;
-  r.invF = j->invF;
-  r.invV = j->invV;
-  r.nparts = 2 * j->nb;
-  return r;
-}
-
 void launch_half(mag_job* j, int parity) {
-  const mag::SweepRed r = sweep_red(j);
   if (mixed32(j)) {
     if (j->prm.group == MAG_SU2)
-      mag::launch_su2_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, r, j->fused, j->st);
+      mag::launch_su2_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, j->part, j->ctl, j->st);
     else
-      mag::launch_su3_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, r, j->fused, j->st);
+      mag::launch_su3_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, j->part, j->ctl, j->st);
   } else {
     if (j->prm.group == MAG_SU2)
-      mag::launch_su2_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, r, j->fused, j->st);
+      mag::launch_su2_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, j->part, j->ctl, j->st);
     else
-      mag::launch_su3_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, r, j->fused, j->st);
+      mag::launch_su3_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, j->part, j->ctl, j->st);
   }
 }
 
@@ -169,9 +144,8 @@ void launch_rebuild(mag_job* j, bool final64) {
 
 // sweep tail shared by the plain and timed paths
 void enqueue_after_sweep(mag_job* j) {
-  if (!j->fused)
-    mag::launch_reduce(j->part, 2 * j->nb, mag::RED_SWEEP, j->invF, j->invV, j->ctl, j->hF,
-                       j->hR, j->hA, nullptr, j->st);
+  mag::launch_reduce(j->part, 2 * j->nb, mag::RED_SWEEP, j->invF, j->invV, j->ctl, j->hF,
+                     j->hR, j->hA, nullptr, j->st);
   if (mixed32(j) && ++j->since_rebuild >= j->prm.reunit_every) {
     launch_rebuild(j, false);
     j->since_rebuild = 0;
@@ -223,7 +197,7 @@ int reset_after_upload(mag_job* j) {
   // initial functional from the FP64 links
   if (j->prm.group == MAG_SU2) mag::launch_su2_functional<double>(j->U64, j->g, j->part, j->st);
   else mag::launch_su3_functional<double>(j->U64, j->g, j->part, j->st);
-  mag::launch_reduce(j->part, 2 * j->nb, mag::RED_INIT, j->invF, j->invV, j->ctl, j->hF, j->hR,
+  mag::launch_reduce(j->part, 2 * j->nf, mag::RED_INIT, j->invF, j->invV, j->ctl, j->hF, j->hR,
                      j->hA, nullptr, j->st);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(j->st));
@@ -263,7 +237,7 @@ int finalize_mixed(mag_job* j) {
   j->fp64_start = j->ctl_h->sweep;
   if (j->prm.group == MAG_SU2) mag::launch_su2_functional<double>(j->U64, j->g, j->part, j->st);
   else mag::launch_su3_functional<double>(j->U64, j->g, j->part, j->st);
-  mag::launch_reduce(j->part, 2 * j->nb, mag::RED_RESET, j->invF, j->invV, j->ctl, j->hF, j->hR,
+  mag::launch_reduce(j->part, 2 * j->nf, mag::RED_RESET, j->invF, j->invV, j->ctl, j->hF, j->hR,
                      j->hA, nullptr, j->st);
   HIPCHK(hipGetLastError());
   CHK(read_ctl(j));
@@ -317,7 +291,7 @@ int mag_job_footprint(const mag_params* p, uint64_t* bytes) {
   const uint64_t nr = p->group == MAG_SU2 ? 4 : 12;
   uint64_t b = (uint64_t)V * 4 * nr * 8;  // FP64 links / master
   if (p->precision == MAG_PREC_MIXED) b += (uint64_t)V * 4 * nr * 4 + (uint64_t)V * nr * 4;
-  const uint64_t nb = (uint64_t)((V / 2 + mag::kBlock - 1) / mag::kBlock);
+  const uint64_t nb = (uint64_t)((V + mag::kBlock - 1) / mag::kBlock);  // sweep workgroups
   b += nb * 2 * 3 * 8 + (uint64_t)p->max_sweeps * 3 * 8 + sizeof(JobCtl) + 3 * 8;
   *bytes = b;
   return MAG_OK;
@@ -330,7 +304,6 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   mag_job* j = new (std::nothrow) mag_job();
   if (!j) return fail(MAG_ERR_NOMEM, "host allocation failed");
   j->prm = *p;
-  if (const char* e = getenv("MAG_FUSED_REDUCE")) j->fused = atoi(e) != 0;
   int r = set_device(j);
   if (r != MAG_OK) {
     delete j;
@@ -341,7 +314,8 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   j->nr = p->group == MAG_SU2 ? 4 : 12;
   j->nr_in = p->group == MAG_SU2 ? 4 : 18;
   j->nreal = (size_t)8 * j->g.Vh * j->nr;
-  j->nb = mag::nblocks(j->g);
+  j->nb = mag::sweep_blocks(j->g, p->group);
+  j->nf = mag::nblocks(j->g);
   j->invF = 1.0 / (double)(j->V * (p->group == MAG_SU2 ? 4 : 12));
   j->invV = 1.0 / (double)j->V;
 #define ALLOC(ptr, bytes)                                                          \
@@ -543,7 +517,7 @@ int mag_job_functional(mag_job* j, double* F) {
   if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
   CHK(set_device(j));
   launch_functional(j);
-  mag::launch_reduce(j->part, 2 * j->nb, mag::RED_PLAIN, j->invF, j->invV, j->ctl, j->hF, j->hR,
+  mag::launch_reduce(j->part, 2 * j->nf, mag::RED_PLAIN, j->invF, j->invV, j->ctl, j->hF, j->hR,
                      j->hA, j->out3, j->st);
   HIPCHK(hipMemcpyAsync(j->out3_h, j->out3, 3 * sizeof(double), hipMemcpyDeviceToHost, j->st));
   HIPCHK(hipStreamSynchronize(j->st));
@@ -559,7 +533,7 @@ int mag_job_plaquette(mag_job* j, double* P) {
   // which is gauge-equivalent: the plaquette is the same.
   if (j->prm.group == MAG_SU2) mag::launch_su2_plaquette(j->U64, j->g, j->part, j->st);
   else mag::launch_su3_plaquette(j->U64, j->g, j->part, j->st);
-  mag::launch_reduce(j->part, 2 * j->nb, mag::RED_PLAIN, j->invV, j->invV, j->ctl, j->hF, j->hR,
+  mag::launch_reduce(j->part, 2 * j->nf, mag::RED_PLAIN, j->invV, j->invV, j->ctl, j->hF, j->hR,
                      j->hA, j->out3, j->st);
   HIPCHK(hipMemcpyAsync(j->out3_h, j->out3, 3 * sizeof(double), hipMemcpyDeviceToHost, j->st));
   HIPCHK(hipStreamSynchronize(j->st));
@@ -567,48 +541,43 @@ int mag_job_plaquette(mag_job* j, double* P) {
   return MAG_OK;
 }
 
-int mag_job_sweeps_timed(mag_job* j, int n, double* total_ms, double* kern_ms) {
+int mag_job_sweeps_timed(mag_job* j, int n, double* total_ms) {
   if (!j || n < 1 || !total_ms) return fail(MAG_ERR_ARG, "bad argument");
   if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
   CHK(set_device(j));
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  std::vector<hipEvent_t> ka, kb;
-  if (kern_ms) {
-    ka.resize(2 * (size_t)n);
-    kb.resize(2 * (size_t)n);
-    for (size_t i = 0; i < ka.size(); i++) {
-      HIPCHK(hipEventCreate(&ka[i]));
-      HIPCHK(hipEventCreate(&kb[i]));
-    }
-  }
   HIPCHK(hipEventRecord(e0, j->st));
-  for (int i = 0; i < n; i++) {
-    for (int p = 0; p < 2; p++) {
-      if (kern_ms) HIPCHK(hipEventRecord(ka[2 * i + p], j->st));
-      launch_half(j, p);
-      if (kern_ms) HIPCHK(hipEventRecord(kb[2 * i + p], j->st));
-    }
-    enqueue_after_sweep(j);
-  }
+  for (int i = 0; i < n; i++) enqueue_sweep(j);
   HIPCHK(hipEventRecord(e1, j->st));
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventSynchronize(e1));
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   *total_ms = ms;
-  if (kern_ms) {
-    double acc = 0.0;
-    for (size_t i = 0; i < ka.size(); i++) {
-      float t = 0.f;
-      HIPCHK(hipEventElapsedTime(&t, ka[i], kb[i]));
-      acc += t;
-      hipEventDestroy(ka[i]);
-      hipEventDestroy(kb[i]);
-    }
-    *kern_ms = acc / (double)ka.size();
-  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return MAG_OK;
+}
+
+int mag_job_halfsweep_timed(mag_job* j, int n, double* avg_ms) {
+  if (!j || n < 1 || !avg_ms) return fail(MAG_ERR_ARG, "bad argument");
+  if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  CHK(set_device(j));
+  CHK(read_ctl(j));
+  if (j->ctl_h->done) return fail(MAG_ERR_STATE, "job is done: its kernels would return at once");
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, j->st));
+  for (int i = 0; i < n; i++) launch_half(j, i & 1);
+  HIPCHK(hipEventRecord(e1, j->st));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *avg_ms = ms / n;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   return MAG_OK;

@@ -43,21 +43,7 @@ struct JobCtl {
   double tol;      // relative functional change for convergence
   double F_prev;   // functional after the last sweep (initial value first)
   double F0;       // functional of the uploaded configuration
-  unsigned int arrived;  // black half-sweep workgroups done (fused reduction)
-  int pad;
-};
-
-// Everything the end-of-sweep reduction needs: partial triples of the red
-// half at part[0 .. 3*nb), of the black half at part[3*nb .. 6*nb).
-struct SweepRed {
-  JobCtl* ctl;
-  double* part;
-  double* hF;
-  double* hR;
-  double* hA;
-  double invF;  // 1/(4V) (SU(2)) or 1/(12V) (SU(3))
-  double invV;
-  int nparts;   // 2*nb
+  double pad;
 };
 
 // reduction kernel modes
@@ -69,16 +55,22 @@ enum { RED_INIT = 0, RED_SWEEP = 1, RED_PLAIN = 2, RED_RESET = 3 };
 // Partial buffers hold 3 doubles per workgroup: (F sum, residual sum,
 // residual max).  A sweep uses 2*nblocks(g) partial triples (red, black).
 inline int nblocks(const Geom& g) { return (int)((g.Vh + kBlock - 1) / kBlock); }
+// workgroups of a half-sweep: SU(2) one lane per site, SU(3) two
+inline int sweep_blocks(const Geom& g, int group) {
+  const size_t lanes = (size_t)g.Vh * (group == 2 ? 1 : 2);
+  return (int)((lanes + kBlock - 1) / kBlock);
+}
 
-// fused = true: the last black workgroup to finish reduces the sweep's
-// partials itself (device-scope fence + arrival counter) and no separate
-// reduction kernel is launched for the sweep.
+// One half-sweep of parity `parity`; partial triples go to
+// part[3*nb*parity ...], nb = sweep_blocks(g); G (mixed precision) is the
+// accumulated site gauge or nullptr.  The kernels return at once when
+// ctl->done is set.
 template <typename T>
 void launch_su2_half(T* U, T* G, const Geom& g, int parity, double omega,
-                     const SweepRed& r, bool fused, hipStream_t st);
+                     double* part, const JobCtl* ctl, hipStream_t st);
 template <typename T>
 void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega,
-                     const SweepRed& r, bool fused, hipStream_t st);
+                     double* part, const JobCtl* ctl, hipStream_t st);
 
 // per-sweep (or initial) reduction of the partials into the job history
 void launch_reduce(const double* part, int nparts, int mode, double invF,
@@ -134,4 +126,45 @@ void launch_su3_plaquette(const double* U, const Geom& g, double* part, hipStrea
 namespace mag {
 // 16-byte streaming copy of bytes (a multiple of 16)
 void launch_copy(const void* a, void* b, size_t bytes, hipStream_t st);
+}  // namespace mag
+
+namespace mag {
+// ---------------------------------------------------------------- spin form
+// Driver state of the spin-form maximisation, device resident.  The relax
+// half-sweeps return at once when stop_next or done is set; the reduction
+// that follows a relax sweep turns stop_next into done, so the reflection
+// sweeps of the iteration that met the criterion still run, as in
+// make_maximization_final (mag.cpp:1800-1816).
+struct SpinCtl {
+  int done;
+  int stop_next;
+  int iter;
+  int max_iter;
+  int converged;
+  int digits;      // make_maximization_approximate tolerance digits
+  double tol_max;  // make_maximization_final criteria
+  double tol_avg;
+  double dmax;     // last relaxation residuals
+  double davg;
+  double F_prev;   // last spin functional (approximate criterion)
+};
+
+enum { SPIN_OR = 0, SPIN_RELAX = 1, SPIN_OMEGA = 2, SPIN_HEATBATH = 3 };
+enum { SRED_RELAX_FINAL = 0, SRED_RELAX_PLAIN = 1, SRED_FUNC_INIT = 2, SRED_FUNC_APPROX = 3,
+       SRED_PLAIN = 4 };
+
+// checkerboard spin half-sweep; partial triples at part[3*nblocks*parity]
+void launch_spin_half(int mode, const double* U, double* S, const Geom& g, int parity,
+                      double param, uint64_t seed, uint32_t sweep, double* part,
+                      const SpinCtl* ctl, hipStream_t st);
+// spin functional partials, 2*nblocks triples (ctl may be nullptr)
+void launch_spin_functional(const double* U, const double* S, const Geom& g, double* part,
+                            const SpinCtl* ctl, hipStream_t st);
+void launch_spin_reduce(const double* part, int nparts, int mode, double invV, double invF,
+                        SpinCtl* ctl, double* out3, hipStream_t st);
+void launch_spin_gauge_transform(double* U, const double* S, const Geom& g, hipStream_t st);
+void launch_spins_uniform(double* aos, const Geom& g, uint64_t seed, hipStream_t st);
+// to_soa = 1: AoS -> SoA, 0: SoA -> AoS
+void launch_spins_layout(const double* aos, double* S, const Geom& g, int to_soa,
+                         hipStream_t st);
 }  // namespace mag

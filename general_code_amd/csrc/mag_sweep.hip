@@ -1,0 +1,311 @@
+// mag_sweep.hip -- checkerboard omega-overrelaxation half-sweep kernels of
+// the MAG path (gfx950) and the per-sweep reduction.
+//
+// The site update (build-defined; the reference has only the spin-form
+// omega=1 and omega=2 updates, lib/cpu/src/mag.cpp:1375-1512):
+//   h      = sum_mu [ M(U_x,mu) e3 + M(U_x-mu,mu)^T e3 ]     (mag.cpp:569-626
+//            with every spin e3: the links carry the gauge transformation)
+//   g      = geodesic omega step of e3 towards h_hat          (mag_math.h)
+//   U_x,mu <- g U_x,mu ;  U_x-mu,mu <- U_x-mu,mu g^+
+// SU(3): the same update for the Cabibbo-Marinari subgroups (0,1), (1,2),
+// (0,2) in turn, with the block field terms of mag_math.h.
+//
+// Work split: SU(2) runs one site per lane.  SU(3) runs a site on a pair of
+// adjacent lanes: lane q of the pair holds the 4 links of directions
+// mu = 2q, 2q+1 (forward and backward), sums their field terms, and the
+// pair exchanges the two partial fields with one DPP quad_perm; both lanes
+// then derive the same g and update their own links.  Control flow is
+// identical in both lanes and the register footprint per lane is halved
+// (4 links instead of 8): FP64 SU(3) goes from 1 to 3 waves per SIMD.
+//
+// Field summation order (matched by the oracle):
+//   h = [f(0) + b(0) + f(1) + b(1)] + [f(2) + b(2) + f(3) + b(3)]
+#include "mag_internal.h"
+#include "mag_layout.h"
+#include "mag_math.h"
+#include "mag_reduce.h"
+
+namespace mag {
+
+// ================================================================== reduction
+constexpr int kRedBlock = 1024;
+
+// Reduce nparts partial triples (F sum, residual sum, residual max) and
+// apply the result to the job's control block.  Thread t combines partials
+// t, t + kRedBlock, ... in order, then the fixed-order workgroup tree, so
+// the result depends only on the job's partials.
+__global__ __launch_bounds__(kRedBlock) void k_reduce(const double* __restrict__ part,
+                                                      int nparts, int mode, double invF,
+                                                      double invV, JobCtl* ctl, double* hF,
+                                                      double* hR, double* hA, double* out3) {
+  if (mode == RED_SWEEP && ctl->done) return;
+  double a = 0.0, b = 0.0, c = 0.0;
+  int i = threadIdx.x;
+  // four triples in flight per thread, combined in index order
+  for (; i + 3 * kRedBlock < nparts; i += 4 * kRedBlock) {
+    double x[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) x[k][j] = part[3 * (i + k * kRedBlock) + j];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      a += x[k][0];
+      b += x[k][1];
+      c = fmax(c, x[k][2]);
+    }
+  }
+  for (; i < nparts; i += kRedBlock) {
+    a += part[3 * i];
+    b += part[3 * i + 1];
+    c = fmax(c, part[3 * i + 2]);
+  }
+  block_reduce3<kRedBlock / 64>(a, b, c);
+  if (threadIdx.x == 0) {
+    const double F = a * invF;
+    if (mode == RED_INIT) {
+      ctl->F_prev = F;
+      ctl->F0 = F;
+    } else if (mode == RED_RESET) {
+      ctl->F_prev = F;
+    } else if (mode == RED_SWEEP) {
+      const int sw = ctl->sweep;
+      hF[sw] = F;
+      hR[sw] = c;
+      hA[sw] = b * invV;
+      const double rel = fabs(F - ctl->F_prev) / fabs(F);
+      ctl->F_prev = F;
+      ctl->sweep = sw + 1;
+      if (rel < ctl->tol) {
+        ctl->converged = 1;
+        ctl->done = 1;
+      }
+      if (sw + 1 >= ctl->max_sweeps) ctl->done = 1;
+    }
+    if (out3) {
+      out3[0] = F;
+      out3[1] = b * invV;
+      out3[2] = c;
+    }
+  }
+}
+
+void launch_reduce(const double* part, int nparts, int mode, double invF, double invV,
+                   JobCtl* ctl, double* hF, double* hR, double* hA, double* out3,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedBlock), 0, st, part, nparts, mode, invF, invV,
+                     ctl, hF, hR, hA, out3);
+}
+
+// ================================================================== helpers
+// value of the partner lane of the pair (lane ^ 1)
+__device__ __forceinline__ double pair_xchg(double v) { return dpp_f64<DPP_QUAD_XOR1>(v); }
+__device__ __forceinline__ float pair_xchg(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), DPP_QUAD_XOR1, 0xF, 0xF, true));
+}
+
+// half indices of x - mu for all mu
+__device__ __forceinline__ void back_all(const Geom& g, uint32_t h, const SiteIdx& s,
+                                         uint32_t* hb) {
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) hb[mu] = back_h(g, h, s, mu);
+}
+
+// ================================================================== SU(2)
+// One site per lane: the 8 links of an su2 site fit comfortably in
+// registers (6 waves/SIMD in FP64), and measured faster than the lane pair.
+template <typename T, bool GA>
+__global__ __launch_bounds__(kBlock) void k_su2_half(T* __restrict__ U, T* __restrict__ G,
+                                                     Geom g, int p, T omega,
+                                                     double* __restrict__ part,
+                                                     const JobCtl* __restrict__ ctl) {
+  if (ctl->done) return;
+  const uint32_t h = blockIdx.x * kBlock + threadIdx.x;
+  double fs = 0.0, rs = 0.0, rm = 0.0;
+  if (h < g.Vh) {
+    const SiteIdx s = decode(g, h, p);
+    uint32_t hb[4];
+    back_all(g, h, s, hb);
+    Q<T> F[4], B[4];
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) {
+      F[mu] = ldq(link_ptr<4>(U, g, p, mu, h));
+      B[mu] = ldq(link_ptr<4>(U, g, 1 - p, mu, hb[mu]));
+    }
+    T a0 = T(0), a1 = T(0), a2 = T(0), b0 = T(0), b1 = T(0), b2 = T(0);
+#pragma unroll
+    for (int mu = 0; mu < 2; mu++) {
+      field_fwd(F[mu], a0, a1, a2);
+      field_bwd(B[mu], a0, a1, a2);
+      field_fwd(F[mu + 2], b0, b1, b2);
+      field_bwd(B[mu + 2], b0, b1, b2);
+    }
+    T res;
+    const Q<T> gq = omega_gauge(a0 + b0, a1 + b1, a2 + b2, omega, res);
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) {
+      F[mu] = gmul(gq, F[mu]);
+      B[mu] = mul_gdag(B[mu], gq);
+      stq(link_ptr<4>(U, g, p, mu, h), F[mu]);
+      stq(link_ptr<4>(U, g, 1 - p, mu, hb[mu]), B[mu]);
+    }
+    if (GA) {
+      T* gp = site_ptr<4>(G, g, p, h);
+      stq(gp, gmul(gq, ldq(gp)));
+    }
+    if (p == 1) {
+#pragma unroll
+      for (int mu = 0; mu < 4; mu++) fs += link_F(F[mu]) + link_F(B[mu]);
+    }
+    rs = (double)res;
+    rm = (double)res;
+  }
+  block_reduce3<kWaves>(fs, rs, rm);
+  write_part(part, fs, rs, rm);
+}
+
+template <typename T>
+void launch_su2_half(T* U, T* G, const Geom& g, int parity, double omega, double* part,
+                     const JobCtl* ctl, hipStream_t st) {
+  const int nb = sweep_blocks(g, 2);
+  double* pp = part + (size_t)3 * nb * parity;
+  if (G)
+    hipLaunchKernelGGL((k_su2_half<T, true>), dim3(nb), dim3(kBlock), 0, st, U, G, g, parity,
+                       (T)omega, pp, ctl);
+  else
+    hipLaunchKernelGGL((k_su2_half<T, false>), dim3(nb), dim3(kBlock), 0, st, U, G, g, parity,
+                       (T)omega, pp, ctl);
+}
+template void launch_su2_half<float>(float*, float*, const Geom&, int, double, double*,
+                                     const JobCtl*, hipStream_t);
+template void launch_su2_half<double>(double*, double*, const Geom&, int, double, double*,
+                                      const JobCtl*, hipStream_t);
+
+// ================================================================== SU(3)
+// Cabibbo-Marinari subgroup (I,J) update of the 4 links a lane holds
+// (forward F[k], backward B[k], mu = 2q + k), rows 0,1 in registers, row 2
+// rebuilt where the subgroup touches it.  Returns 1 - cos(theta).
+template <typename T, int I, int J, bool GA>
+__device__ __forceinline__ T cm_subgroup(M2<T>* F, M2<T>* B, M2<T>& Gx, T omega) {
+  T h0 = T(0), h1 = T(0), h2 = T(0);
+  C<T> Fr2[2][3], Br2[2][3];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    if (J == 2) {
+      row2_of(F[k], Fr2[k]);
+      row2_of(B[k], Br2[k]);
+    }
+    auto fe = [&](int a, int b) -> C<T> { return a < 2 ? F[k].r[a][b] : Fr2[k][b]; };
+    auto be = [&](int a, int b) -> C<T> { return a < 2 ? B[k].r[a][b] : Br2[k][b]; };
+    blk_fwd(fe(I, I), fe(I, J), fe(J, I), fe(J, J), h0, h1, h2);
+    blk_bwd(be(I, I), be(I, J), be(J, I), be(J, J), h0, h1, h2);
+  }
+  h0 += pair_xchg(h0);
+  h1 += pair_xchg(h1);
+  h2 += pair_xchg(h2);
+  T res;
+  const Q<T> gq = omega_gauge(h0, h1, h2, omega, res);
+  // r = [[g0 + i g3, g2 + i g1], [-g2 + i g1, g0 - i g3]], g3 = 0
+  const C<T> r00{gq.a0, T(0)}, r01{gq.a2, gq.a1}, r10{-gq.a2, gq.a1}, r11{gq.a0, T(0)};
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    // forward: rows I,J <- r (rows I,J); only stored rows are written
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const C<T> a = F[k].r[I][c];
+      const C<T> b = (J < 2) ? F[k].r[J][c] : Fr2[k][c];
+      F[k].r[I][c] = cadd(cmul(r00, a), cmul(r01, b));
+      if (J < 2) F[k].r[J][c] = cadd(cmul(r10, a), cmul(r11, b));
+    }
+    // backward: cols I,J of rows 0,1 <- (cols I,J) r^+
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const C<T> a = B[k].r[c][I], b = B[k].r[c][J];
+      B[k].r[c][I] = cadd(cmulc(a, r00), cmulc(b, r01));
+      B[k].r[c][J] = cadd(cmulc(a, r10), cmulc(b, r11));
+    }
+  }
+  if (GA) {
+    C<T> g2[3];
+    if (J == 2) row2_of(Gx, g2);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const C<T> a = Gx.r[I][c];
+      const C<T> b = (J < 2) ? Gx.r[J][c] : g2[c];
+      Gx.r[I][c] = cadd(cmul(r00, a), cmul(r01, b));
+      if (J < 2) Gx.r[J][c] = cadd(cmul(r10, a), cmul(r11, b));
+    }
+  }
+  return res;
+}
+
+template <typename T, bool GA>
+__global__ __launch_bounds__(kBlock) void k_su3_half(T* __restrict__ U, T* __restrict__ G,
+                                                     Geom g, int p, T omega,
+                                                     double* __restrict__ part,
+                                                     const JobCtl* __restrict__ ctl) {
+  if (ctl->done) return;
+  const uint32_t tid = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t h = tid >> 1;
+  const int q = (int)(tid & 1u);
+  double fs = 0.0, rs = 0.0, rm = 0.0;
+  if (h < g.Vh) {
+    const SiteIdx s = decode(g, h, p);
+    uint32_t hb[4];
+    back_all(g, h, s, hb);
+    T* pf[2];
+    T* pb[2];
+    M2<T> F[2], B[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const int mu = 2 * q + k;
+      const uint32_t hbm = q ? hb[2 + k] : hb[k];
+      pf[k] = link_ptr<12>(U, g, p, mu, h);
+      pb[k] = link_ptr<12>(U, g, 1 - p, mu, hbm);
+      F[k] = ldm<T>(pf[k]);
+      B[k] = ldm<T>(pb[k]);
+    }
+    M2<T> Gx;
+    T* gp = GA ? site_ptr<12>(G, g, p, h) : nullptr;
+    if (GA) Gx = ldm<T>(gp);
+    T res = cm_subgroup<T, 0, 1, GA>(F, B, Gx, omega);
+    res += cm_subgroup<T, 1, 2, GA>(F, B, Gx, omega);
+    res += cm_subgroup<T, 0, 2, GA>(F, B, Gx, omega);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      stm(pf[k], F[k]);
+      stm(pb[k], B[k]);
+    }
+    if (GA && q == 0) stm(gp, Gx);
+    if (p == 1) {
+#pragma unroll
+      for (int k = 0; k < 2; k++) fs += su3_link_F(F[k]) + su3_link_F(B[k]);
+    }
+    if (q == 0) {
+      rs = (double)res;
+      rm = (double)res;
+    }
+  }
+  block_reduce3<kWaves>(fs, rs, rm);
+  write_part(part, fs, rs, rm);
+}
+
+template <typename T>
+void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega, double* part,
+                     const JobCtl* ctl, hipStream_t st) {
+  const int nb = sweep_blocks(g, 3);
+  double* pp = part + (size_t)3 * nb * parity;
+  if (G)
+    hipLaunchKernelGGL((k_su3_half<T, true>), dim3(nb), dim3(kBlock), 0, st, U, G, g, parity,
+                       (T)omega, pp, ctl);
+  else
+    hipLaunchKernelGGL((k_su3_half<T, false>), dim3(nb), dim3(kBlock), 0, st, U, G, g, parity,
+                       (T)omega, pp, ctl);
+}
+template void launch_su3_half<float>(float*, float*, const Geom&, int, double, double*,
+                                     const JobCtl*, hipStream_t);
+template void launch_su3_half<double>(double*, double*, const Geom&, int, double, double*,
+                                      const JobCtl*, hipStream_t);
+
+}  // namespace mag

@@ -83,12 +83,45 @@ SIGNATURES = {
     "mag_job_download_links": (ctypes.c_int, [_vp, _dp]),
     "mag_job_functional": (ctypes.c_int, [_vp, _dp]),
     "mag_job_plaquette": (ctypes.c_int, [_vp, _dp]),
-    "mag_job_sweeps_timed": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+    "mag_job_sweeps_timed": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "mag_job_halfsweep_timed": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "mag_su2_functional": (ctypes.c_int, [_ip, _dp, _dp]),
     "mag_su3_functional": (ctypes.c_int, [_ip, _dp, _dp]),
     "mag_plaquette": (ctypes.c_int, [ctypes.c_int, _ip, _dp, _dp]),
     "mag_hbm_copy_bandwidth": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, _dp]),
+    # spin form (reference mag.h surface)
+    "mag_spin_session_create": (ctypes.c_int, [_ip, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "mag_spin_session_destroy": (ctypes.c_int, [_vp]),
+    "mag_spin_session_upload_links": (ctypes.c_int, [_vp, _dp]),
+    "mag_spin_session_download_links": (ctypes.c_int, [_vp, _dp]),
+    "mag_spin_session_set_spins": (ctypes.c_int, [_vp, _dp]),
+    "mag_spin_session_get_spins": (ctypes.c_int, [_vp, _dp]),
+    "mag_spin_session_generate_spins": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "mag_spin_session_functional_spin": (ctypes.c_int, [_vp, _dp]),
+    "mag_spin_session_functional": (ctypes.c_int, [_vp, _dp]),
+    "mag_spin_session_sweep": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double,
+                                              ctypes.c_uint64, _dp, _dp]),
+    "mag_spin_session_simulated_annealing": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
+                                                            ctypes.c_double, ctypes.c_int,
+                                                            ctypes.c_int, ctypes.c_uint64]),
+    "mag_spin_session_maximize_final": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double,
+                                                       ctypes.c_double, ctypes.c_int, _ip]),
+    "mag_spin_session_maximize_approximate": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
+                                                             ctypes.c_int, _ip]),
+    "mag_spin_session_gauge_transform": (ctypes.c_int, [_vp]),
+    "mag_su2_generate_spins_uniform": (ctypes.c_int, [_ip, ctypes.c_uint64, _dp]),
+    "mag_su2_functional_spin": (ctypes.c_int, [_ip, _dp, _dp, _dp]),
+    "mag_su2_gauge_transform_spins": (ctypes.c_int, [_ip, _dp, _dp]),
+    "mag_su2_maximize_final": (ctypes.c_int, [_ip, _dp, _dp, ctypes.c_int, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_int, _ip]),
+    "mag_su2_maximize_approximate": (ctypes.c_int, [_ip, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_int, _ip]),
+    "mag_su2_simulated_annealing": (ctypes.c_int, [_ip, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.c_uint64]),
 }
+
+SPIN_OR, SPIN_RELAX, SPIN_OMEGA, SPIN_HEATBATH = 0, 1, 2, 3
 
 _lib = None
 
@@ -241,12 +274,17 @@ class MagJob:
     def finalize(self):
         _check(lib().mag_job_finalize(self._h))
 
-    def sweeps_timed(self, n: int, kernel: bool = True):
+    def sweeps_timed(self, n: int) -> float:
+        """ms for n sweeps (HIP events on the job's stream)."""
         tot = ctypes.c_double()
-        k = ctypes.c_double()
-        _check(lib().mag_job_sweeps_timed(self._h, int(n), ctypes.byref(tot),
-                                          ctypes.byref(k) if kernel else None))
-        return tot.value, (k.value if kernel else None)
+        _check(lib().mag_job_sweeps_timed(self._h, int(n), ctypes.byref(tot)))
+        return tot.value
+
+    def halfsweep_timed(self, n: int) -> float:
+        """mean ms of one half-sweep kernel over n back-to-back launches."""
+        avg = ctypes.c_double()
+        _check(lib().mag_job_halfsweep_timed(self._h, int(n), ctypes.byref(avg)))
+        return avg.value
 
     # results --------------------------------------------------------------
     def status(self):
@@ -311,3 +349,124 @@ def plaquette(group, dims, links) -> float:
     P = ctypes.c_double()
     _check(lib().mag_plaquette(int(group), _dims(dims), _dptr(a), ctypes.byref(P)))
     return P.value
+
+
+# spin form -----------------------------------------------------------------
+class SpinSession:
+    """SU(2) links and spins resident on the GPU: the spin-form MAG of the
+    reference's lib/cpu/include/mag.h (checkerboard sweep order)."""
+
+    def __init__(self, dims, device: int = 0):
+        self.dims = tuple(int(d) for d in dims)
+        self.volume = int(np.prod(self.dims))
+        h = _vp()
+        _check(lib().mag_spin_session_create(_dims(self.dims), int(device), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mag_spin_session_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload_links(self, links):
+        a = np.ascontiguousarray(links, dtype=np.float64)
+        if a.size != 16 * self.volume:
+            raise ValueError("links must hold 16*V reals")
+        _check(lib().mag_spin_session_upload_links(self._h, _dptr(a)))
+
+    def download_links(self):
+        out = np.empty((4 * self.volume, 4))
+        _check(lib().mag_spin_session_download_links(self._h, _dptr(out)))
+        return out
+
+    def set_spins(self, spins):
+        a = np.ascontiguousarray(spins, dtype=np.float64)
+        if a.size != 3 * self.volume:
+            raise ValueError("spins must hold 3*V reals")
+        _check(lib().mag_spin_session_set_spins(self._h, _dptr(a)))
+
+    def get_spins(self):
+        out = np.empty((self.volume, 3))
+        _check(lib().mag_spin_session_get_spins(self._h, _dptr(out)))
+        return out
+
+    def generate_spins(self, seed: int):
+        _check(lib().mag_spin_session_generate_spins(self._h, int(seed)))
+
+    def functional_spin(self) -> float:
+        F = ctypes.c_double()
+        _check(lib().mag_spin_session_functional_spin(self._h, ctypes.byref(F)))
+        return F.value
+
+    def functional(self) -> float:
+        F = ctypes.c_double()
+        _check(lib().mag_spin_session_functional(self._h, ctypes.byref(F)))
+        return F.value
+
+    def sweep(self, mode: int, param: float = 0.0, seed: int = 0):
+        mx = ctypes.c_double()
+        av = ctypes.c_double()
+        _check(lib().mag_spin_session_sweep(self._h, int(mode), float(param), int(seed),
+                                            ctypes.byref(mx), ctypes.byref(av)))
+        return mx.value, av.value
+
+    def simulated_annealing(self, T_init, T_final, T_step, OR_steps, thermalization_steps, seed):
+        _check(lib().mag_spin_session_simulated_annealing(
+            self._h, float(T_init), float(T_final), float(T_step), int(OR_steps),
+            int(thermalization_steps), int(seed)))
+
+    def maximize_final(self, OR_steps, tol_max, tol_avg, max_iter=100000) -> int:
+        it = ctypes.c_int()
+        _check(lib().mag_spin_session_maximize_final(self._h, int(OR_steps), float(tol_max),
+                                                     float(tol_avg), int(max_iter),
+                                                     ctypes.byref(it)))
+        return it.value
+
+    def maximize_approximate(self, OR_steps, tolerance_digits, max_iter=100000) -> int:
+        it = ctypes.c_int()
+        _check(lib().mag_spin_session_maximize_approximate(self._h, int(OR_steps),
+                                                           int(tolerance_digits), int(max_iter),
+                                                           ctypes.byref(it)))
+        return it.value
+
+    def gauge_transform(self):
+        _check(lib().mag_spin_session_gauge_transform(self._h))
+
+
+def generate_spins_uniform(dims, seed: int) -> np.ndarray:
+    """generate_spins_uniform (lib/cpu/include/mag.h:179-199) on the GPU."""
+    V = int(np.prod(dims))
+    out = np.empty((V, 3))
+    _check(lib().mag_su2_generate_spins_uniform(_dims(dims), int(seed), _dptr(out)))
+    return out
+
+
+def functional_spin(dims, links, spins) -> float:
+    """MAG_functional_su2_spin (lib/cpu/src/mag.cpp:925-953) on the GPU."""
+    F = ctypes.c_double()
+    _check(lib().mag_su2_functional_spin(_dims(dims),
+                                         _dptr(np.ascontiguousarray(links, np.float64)),
+                                         _dptr(np.ascontiguousarray(spins, np.float64)),
+                                         ctypes.byref(F)))
+    return F.value
+
+
+def gauge_transform_spins(dims, links, spins) -> np.ndarray:
+    """gauge_tranformation_spins (lib/cpu/src/mag.cpp:1045-1068); returns the
+    transformed links."""
+    out = np.array(links, dtype=np.float64, copy=True)
+    _check(lib().mag_su2_gauge_transform_spins(_dims(dims), _dptr(out),
+                                               _dptr(np.ascontiguousarray(spins, np.float64))))
+    return out

@@ -127,11 +127,68 @@ int mag_job_download_links(mag_job *job, double *links);
 int mag_job_functional(mag_job *job, double *F);
 int mag_job_plaquette(mag_job *job, double *P);
 
-/* Timed sweeps for measurement: n sweeps bracketed by HIP events on the
- * job's stream; also the mean duration of the half-sweep kernel (events
- * around each launch) when kern_ms is not NULL.  Blocking. */
-int mag_job_sweeps_timed(mag_job *job, int n, double *total_ms,
-                         double *kern_ms);
+/* Measurement: n sweeps bracketed by HIP events on the job's stream
+ * (blocking); total_ms = elapsed time. */
+int mag_job_sweeps_timed(mag_job *job, int n, double *total_ms);
+/* Measurement of the dominant kernel: n half-sweep launches back to back
+ * (parities alternating) between two events, avg_ms = elapsed / n.  The
+ * sweeps are real and move the links, but are not entered in the job's
+ * sweep history (no per-sweep reduction runs). */
+int mag_job_halfsweep_timed(mag_job *job, int n, double *avg_ms);
+
+/* ------------------------------------------------------------ spin form
+ * The reference's own algorithm (lib/cpu/src/mag.cpp): one unit spin s(x)
+ * per site, links read-only, fixed links produced at the end by
+ * gauge_tranformation_spins.  On the GPU the sweeps run in checkerboard
+ * order (the reference's in-place lexicographic OpenMP loop races and is
+ * only deterministic on one thread).  Spins: V*3 doubles, site order of the
+ * links.  A session keeps links and spins resident between calls. */
+typedef struct mag_spin_session mag_spin_session;
+
+int mag_spin_session_create(const int *dims, int device, mag_spin_session **out);
+int mag_spin_session_destroy(mag_spin_session *s);
+int mag_spin_session_upload_links(mag_spin_session *s, const double *links);
+int mag_spin_session_download_links(mag_spin_session *s, double *links);
+int mag_spin_session_set_spins(mag_spin_session *s, const double *spins);
+int mag_spin_session_get_spins(mag_spin_session *s, double *spins);
+/* generate_spins_uniform (mag.h:179-199) on a Philox stream keyed by seed */
+int mag_spin_session_generate_spins(mag_spin_session *s, uint64_t seed);
+/* MAG_functional_su2_spin (mag.cpp:925-953) */
+int mag_spin_session_functional_spin(mag_spin_session *s, double *F);
+/* MAG_functional_su2 (mag.cpp:993-1001) of the resident links */
+int mag_spin_session_functional(mag_spin_session *s, double *F);
+/* One checkerboard sweep: mode 0 reflection (overrelaxation_update,
+ * mag.cpp:1375-1401), 1 relaxation (relaxation_update, mag.cpp:1477-1512),
+ * 2 geodesic omega rule (param = omega), 3 heat bath (heat_bath_update_fast,
+ * mag.cpp:785-814; param = T).  dmax/davg: max / mean of 1 - s_old.h_hat. */
+int mag_spin_session_sweep(mag_spin_session *s, int mode, double param, uint64_t seed,
+                           double *dmax, double *davg);
+/* make_simulated_annealing (mag.cpp:1585-1603), heat bath on a Philox
+ * stream keyed by seed */
+int mag_spin_session_simulated_annealing(mag_spin_session *s, double T_init, double T_final,
+                                         double T_step, int OR_steps,
+                                         int thermalization_steps, uint64_t seed);
+/* make_maximization_final (mag.cpp:1800-1816), at most max_iter iterations */
+int mag_spin_session_maximize_final(mag_spin_session *s, int OR_steps, double tol_max,
+                                    double tol_avg, int max_iter, int *iters);
+/* make_maximization_approximate (mag.cpp:1733-1750) */
+int mag_spin_session_maximize_approximate(mag_spin_session *s, int OR_steps,
+                                          int tolerance_digits, int max_iter, int *iters);
+/* gauge_tranformation_spins (mag.cpp:1045-1068) on the resident links */
+int mag_spin_session_gauge_transform(mag_spin_session *s);
+
+/* One-shot forms on host arrays (current HIP device), mirroring mag.h */
+int mag_su2_generate_spins_uniform(const int *dims, uint64_t seed, double *spins);
+int mag_su2_functional_spin(const int *dims, const double *links, const double *spins,
+                            double *F);
+int mag_su2_gauge_transform_spins(const int *dims, double *links, const double *spins);
+int mag_su2_maximize_final(const int *dims, const double *links, double *spins, int OR_steps,
+                           double tol_max, double tol_avg, int max_iter, int *iters);
+int mag_su2_maximize_approximate(const int *dims, const double *links, double *spins,
+                                 int OR_steps, int tolerance_digits, int max_iter, int *iters);
+int mag_su2_simulated_annealing(const int *dims, const double *links, double *spins,
+                                double T_init, double T_final, double T_step, int OR_steps,
+                                int thermalization_steps, uint64_t seed);
 
 /* Measured HBM bandwidth of a plain streaming copy kernel (read bytes +
  * write bytes per second / 1e9) over buffers of the given size: the

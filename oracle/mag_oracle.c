@@ -581,7 +581,7 @@ void or_gen_spins_uniform(double *spins, const int *dims, uint64_t seed) {
       philox_draw(seed, (uint32_t)s, attempt++, TAG_SPIN, o);
       x1 = 2 * u53(o[0], o[1]) - 1;
       x2 = 2 * u53(o[2], o[3]) - 1;
-    } while (x1 * x1 + x2 * x2 >= 1);
+    } while (x1 * x1 + x2 * x2 >= 1 && attempt < 64u); /* bounded, (1-pi/4)^64 ~ 1e-43 */
     double a = 1 - x1 * x1 - x2 * x2;
     spins[3 * s] = 2 * x1 * sqrt(a);
     spins[3 * s + 1] = 2 * x2 * sqrt(a);
@@ -668,12 +668,14 @@ static void su2_link_halfsweep(const geom_t *g, double *links, int parity,
       coords(g, s, c);
       if (parity_of(c) != parity) continue;
       int64_t bn[4];
-      double h[3] = {0, 0, 0};
+      /* h = [f(0)+b(0)+f(1)+b(1)] + [f(2)+b(2)+f(3)+b(3)] */
+      double hp[2][3] = {{0, 0, 0}, {0, 0, 0}};
       for (int mu = 0; mu < 4; mu++) {
         bn[mu] = nb(g, s, c, mu, -1);
-        field_fwd(qload(links + 4 * (4 * s + mu)), h);
-        field_bwd(qload(links + 4 * (4 * bn[mu] + mu)), h);
+        field_fwd(qload(links + 4 * (4 * s + mu)), hp[mu >> 1]);
+        field_bwd(qload(links + 4 * (4 * bn[mu] + mu)), hp[mu >> 1]);
       }
+      double h[3] = {hp[0][0] + hp[1][0], hp[0][1] + hp[1][1], hp[0][2] + hp[1][2]};
       double d = resid_of(h);
       rsum[r] += d;
       if (d > rmax[r]) rmax[r] = d;
@@ -1007,11 +1009,13 @@ static void su3_link_halfsweep(const geom_t *g, double *links, int parity,
       double dsite = 0;
       for (int sg = 0; sg < 3; sg++) {
         int i = CM_I[sg], j = CM_J[sg];
-        double h[3] = {0, 0, 0};
+        /* h = [f(0)+b(0)+f(1)+b(1)] + [f(2)+b(2)+f(3)+b(3)] */
+        double hp[2][3] = {{0, 0, 0}, {0, 0, 0}};
         for (int mu = 0; mu < 4; mu++) {
-          blk_fwd(F[mu].m[i][i], F[mu].m[i][j], F[mu].m[j][i], F[mu].m[j][j], h);
-          blk_bwd(B[mu].m[i][i], B[mu].m[i][j], B[mu].m[j][i], B[mu].m[j][j], h);
+          blk_fwd(F[mu].m[i][i], F[mu].m[i][j], F[mu].m[j][i], F[mu].m[j][j], hp[mu >> 1]);
+          blk_bwd(B[mu].m[i][i], B[mu].m[i][j], B[mu].m[j][i], B[mu].m[j][j], hp[mu >> 1]);
         }
+        double h[3] = {hp[0][0] + hp[1][0], hp[0][1] + hp[1][1], hp[0][2] + hp[1][2]};
         dsite += resid_of(h);
         double gq[4];
         or_omega_gauge(h, omega, gq);

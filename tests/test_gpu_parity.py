@@ -41,7 +41,7 @@ def test_su3_generator_matches_oracle(mag, oracle):
 # omega = 2 (pure reflection) conserves F and is not contracting: rounding
 # differences grow about 3x per sweep, so the trajectory is compared over a
 # few sweeps only (F conservation is checked over many below).
-@pytest.mark.parametrize("omega,n", [(1.0, 40), (1.7, 40), (2.0, 6)])
+@pytest.mark.parametrize("omega,n", [(1.0, 40), (1.7, 40), (2.0, 3)])
 def test_su2_fp64_sweeps_match_oracle(mag, oracle, omega, n):
     dims = SU2_DIMS
     links = oracle.gen_su2_config(dims, 0.3, 1234)
@@ -196,7 +196,7 @@ def test_errors_are_reported(mag):
         mag.MagJob(group=mag.SU2, dims=(7, 8, 8, 8))
     assert e.value.code == -2
     with mag.MagJob(group=mag.SU2, dims=(4, 4, 4, 4)) as job:
-        bad = np.zeros((256, 4))
+        bad = np.zeros((1024, 4))
         bad[3, 1] = np.nan
         with pytest.raises(mag.MagError) as e:
             job.upload(bad)
