@@ -13,8 +13,11 @@ Roofline: the dominant kernel is the half-sweep kernel.  Its algorithmic
 bytes per launch are (V/2) site updates x B_site, with
 B_site = 8 links x reals/link x bytes x 2 (read + write), plus, in mixed
 precision, the site gauge read + write (2 x reals x 4 bytes).  Its mean
-launch duration is measured with HIP events around every launch on the
-job's stream (mag_job_sweeps_timed).  Peak HBM = 8.0 TB/s.
+launch duration is measured with HIP events on the job's stream around a
+run of back-to-back half-sweep launches (mag_job_halfsweep_timed).  Peak
+HBM = 8.0 TB/s.  roofline.traffic is the HBM bytes per launch measured with
+rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE, scaled by a copy kernel of
+known size) and recorded in profiles/halfsweep_hbm_traffic_pmc.md.
 """
 from __future__ import annotations
 
@@ -28,6 +31,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+
+
+def pmc_traffic(kernel: str, lattice: str):
+    """HBM bytes per launch of a half-sweep kernel from the committed PMC
+    summary, or None when it holds no entry for that kernel and lattice."""
+    path = os.path.join(ROOT, "profiles", "halfsweep_hbm_traffic_pmc.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block).get(f"{kernel}|{lattice}", {}).get("traffic_bytes")
+    except (OSError, IndexError, ValueError):
+        return None
 
 
 def b_site(group: int, mixed: bool) -> int:
@@ -57,6 +72,11 @@ def measure(G, group, dims, precision, eps, omega, steps, warmup, seed=2024):
     mixed = precision == G.MIXED
     bytes_launch = (V // 2) * b_site(group, mixed)
     achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
+    kname = (f"void mag::k_su{group}_half<{'float' if mixed else 'double'}, "
+             f"{'true' if mixed else 'false'}>")
+    lattice = (f"{dims[0]}^4" if len(set(dims)) == 1
+               else f"{dims[0]}^3x{dims[3]}" if len(set(dims[:3])) == 1 else "x".join(map(str, dims)))
+    traffic = pmc_traffic(kname, lattice)
     return {
         "value": V * steps / (total_ms * 1e-3),
         "ms_per_step": total_ms / steps,
@@ -67,7 +87,7 @@ def measure(G, group, dims, precision, eps, omega, steps, warmup, seed=2024):
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": None,
+            "traffic": None if traffic is None else int(round(traffic)),
             "bytes_per_launch": bytes_launch,
             "b_site": b_site(group, mixed),
         },
@@ -144,6 +164,8 @@ def main():
                 "kernel_ms": r["kernel_ms"],
                 "roofline_frac": r["roofline"]["frac"],
                 "achieved_GBs": r["roofline"]["achieved"],
+                "traffic_bytes_per_launch": r["roofline"]["traffic"],
+                "alg_bytes_per_launch": r["roofline"]["bytes_per_launch"],
                 "b_site": r["roofline"]["b_site"],
             }
 

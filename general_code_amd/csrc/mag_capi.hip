@@ -184,7 +184,7 @@ int write_ctl(mag_job* j) {
 // after the configuration is in U64 (SoA FP64): reset the job state
 int reset_after_upload(mag_job* j) {
   if (j->prm.precision == MAG_PREC_MIXED) {
-    mag::launch_f64_to_f32(j->U64, j->W32, j->nreal, j->st);
+    mag::launch_links_f64_to_f32(j->U64, j->W32, j->g, j->prm.group, j->st);
     mag::launch_gauge_identity(j->G32, j->g, j->prm.group, j->st);
   }
   j->phase = 0;

@@ -93,8 +93,8 @@ void launch_su3_aos_to_soa(const double* aos, T* U, const Geom& g, hipStream_t s
 template <typename T>
 void launch_su3_soa_to_aos(const T* U, double* aos, const Geom& g, hipStream_t st);
 
-// elementwise FP64 -> FP32 copy (n reals)
-void launch_f64_to_f32(const double* a, float* b, size_t n, hipStream_t st);
+// FP64 links -> FP32 links (both in the chunked SoA layout)
+void launch_links_f64_to_f32(const double* a, float* b, const Geom& g, int group, hipStream_t st);
 // site gauges set to the identity (NR reals per site, both parities)
 void launch_gauge_identity(float* G, const Geom& g, int group, hipStream_t st);
 

@@ -35,7 +35,7 @@ __global__ __launch_bounds__(kBlock) void k_su3_functional(const T* __restrict__
   double fs = 0.0, z1 = 0.0, z2 = 0.0;
   if (h < g.Vh) {
 #pragma unroll
-    for (int mu = 0; mu < 4; mu++) fs += su3_link_F(ldm<T>(link_ptr<12>(U, g, p, mu, h)));
+    for (int mu = 0; mu < 4; mu++) fs += su3_link_F(ldm(link_ptr<12>(U, g, p, mu, h)));
   }
   block_reduce3<kWaves>(fs, z1, z2);
   write_part(part, fs, z1, z2);
@@ -76,9 +76,7 @@ __global__ void k_su2_aos_to_soa(const double* __restrict__ aos, T* __restrict__
 #pragma unroll
   for (int mu = 0; mu < 4; mu++) {
     const double* a = aos + ((size_t)s * 4 + mu) * 4;
-    T* o = link_ptr<4>(U, g, p, mu, h);
-#pragma unroll
-    for (int c = 0; c < 4; c++) o[c] = (T)a[c];
+    stq(link_ptr<4>(U, g, p, mu, h), Q<T>{(T)a[0], (T)a[1], (T)a[2], (T)a[3]});
   }
 }
 template <typename T>
@@ -91,9 +89,11 @@ __global__ void k_su2_soa_to_aos(const T* __restrict__ U, double* __restrict__ a
 #pragma unroll
   for (int mu = 0; mu < 4; mu++) {
     double* a = aos + ((size_t)s * 4 + mu) * 4;
-    const T* o = link_ptr<4>(U, g, p, mu, h);
-#pragma unroll
-    for (int c = 0; c < 4; c++) a[c] = (double)o[c];
+    const Q<T> q = ldq(link_ptr<4>(U, g, p, mu, h));
+    a[0] = q.a0;
+    a[1] = q.a1;
+    a[2] = q.a2;
+    a[3] = q.a3;
   }
 }
 template <typename T>
@@ -106,9 +106,10 @@ __global__ void k_su3_aos_to_soa(const double* __restrict__ aos, T* __restrict__
 #pragma unroll
   for (int mu = 0; mu < 4; mu++) {
     const double* a = aos + ((size_t)s * 4 + mu) * 18;
-    T* o = link_ptr<12>(U, g, p, mu, h);
+    M2<T> m;
 #pragma unroll
-    for (int c = 0; c < 12; c++) o[c] = (T)a[c];
+    for (int k = 0; k < 6; k++) m.r[k / 3][k % 3] = {(T)a[2 * k], (T)a[2 * k + 1]};
+    stm(link_ptr<12>(U, g, p, mu, h), m);
   }
 }
 template <typename T>
@@ -121,16 +122,17 @@ __global__ void k_su3_soa_to_aos(const T* __restrict__ U, double* __restrict__ a
 #pragma unroll
   for (int mu = 0; mu < 4; mu++) {
     double* a = aos + ((size_t)s * 4 + mu) * 18;
-    const T* o = link_ptr<12>(U, g, p, mu, h);
+    const M2<T> o = ldm(link_ptr<12>(U, g, p, mu, h));
     M2<double> m;
 #pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < 3; j++) m.r[i][j] = {(double)o[6 * i + 2 * j], (double)o[6 * i + 2 * j + 1]};
+    for (int k = 0; k < 6; k++) m.r[k / 3][k % 3] = {(double)o.r[k / 3][k % 3].re, (double)o.r[k / 3][k % 3].im};
     C<double> r2[3];
     row2_of(m, r2);
 #pragma unroll
-    for (int c = 0; c < 12; c++) a[c] = (double)o[c];
+    for (int k = 0; k < 6; k++) {
+      a[2 * k] = m.r[k / 3][k % 3].re;
+      a[2 * k + 1] = m.r[k / 3][k % 3].im;
+    }
 #pragma unroll
     for (int j = 0; j < 3; j++) {
       a[12 + 2 * j] = r2[j].re;
@@ -166,39 +168,53 @@ template void launch_su3_aos_to_soa<double>(const double*, double*, const Geom&,
 template void launch_su3_soa_to_aos<float>(const float*, double*, const Geom&, hipStream_t);
 template void launch_su3_soa_to_aos<double>(const double*, double*, const Geom&, hipStream_t);
 
-__global__ void k_f64_to_f32(const double* __restrict__ a, float* __restrict__ b, size_t n) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) b[i] = (float)a[i];
+// FP64 links -> FP32 links, link by link (the two chunk layouts differ)
+template <int NR>
+__global__ void k_links_f64_to_f32(const double* __restrict__ a, float* __restrict__ b, Geom g) {
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= (size_t)8 * g.Vh) return;
+  const uint32_t pm = (uint32_t)(l / g.Vh), h = (uint32_t)(l % g.Vh);
+  if constexpr (NR == 4) {
+    const Q<double> q = ldq(link_ptr_pm<4>(a, g, pm, h));
+    stq(link_ptr_pm<4>(b, g, pm, h), Q<float>{(float)q.a0, (float)q.a1, (float)q.a2, (float)q.a3});
+  } else {
+    const M2<double> m = ldm(link_ptr_pm<12>(a, g, pm, h));
+    M2<float> f;
+#pragma unroll
+    for (int k = 0; k < 6; k++) f.r[k / 3][k % 3] = {(float)m.r[k / 3][k % 3].re, (float)m.r[k / 3][k % 3].im};
+    stm(link_ptr_pm<12>(b, g, pm, h), f);
+  }
 }
-void launch_f64_to_f32(const double* a, float* b, size_t n, hipStream_t st) {
-  size_t blocks = (n + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)blocks), dim3(256), 0, st, a, b, n);
+void launch_links_f64_to_f32(const double* a, float* b, const Geom& g, int group, hipStream_t st) {
+  const unsigned blocks = (unsigned)(((size_t)8 * g.Vh + 255) / 256);
+  if (group == 2) hipLaunchKernelGGL(k_links_f64_to_f32<4>, dim3(blocks), dim3(256), 0, st, a, b, g);
+  else hipLaunchKernelGGL(k_links_f64_to_f32<12>, dim3(blocks), dim3(256), 0, st, a, b, g);
 }
 
-__global__ void k_gauge_identity(float* __restrict__ G, uint32_t nsites, int nr) {
+__global__ void k_gauge_identity(float* __restrict__ G, Geom g, int group) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsites) return;
-  float* o = G + (size_t)s * nr;
-  for (int c = 0; c < nr; c++) o[c] = 0.f;
-  if (nr == 4) {
-    o[0] = 1.f;
+  if (s >= 2 * g.Vh) return;
+  const int p = (int)(s / g.Vh);
+  const uint32_t h = s % g.Vh;
+  if (group == 2) {
+    stq(site_ptr<4>(G, g, p, h), Q<float>{1.f, 0.f, 0.f, 0.f});
   } else {
-    o[0] = 1.f;  // row 0 = (1,0,0)
-    o[8] = 1.f;  // row 1 = (0,1,0): element (1,1) real part at 6 + 2
+    M2<float> m;
+#pragma unroll
+    for (int k = 0; k < 6; k++) m.r[k / 3][k % 3] = {(k == 0 || k == 4) ? 1.f : 0.f, 0.f};
+    stm(site_ptr<12>(G, g, p, h), m);
   }
 }
 void launch_gauge_identity(float* G, const Geom& g, int group, hipStream_t st) {
   const uint32_t n = 2 * g.Vh;
-  hipLaunchKernelGGL(k_gauge_identity, dim3((n + 255) / 256), dim3(256), 0, st, G, n,
-                     group == 2 ? 4 : 12);
+  hipLaunchKernelGGL(k_gauge_identity, dim3((n + 255) / 256), dim3(256), 0, st, G, g, group);
 }
 
 // ================================================================== rebuild
-__device__ __forceinline__ Q<double> norm_q(const float* p) {
-  Q<double> q{(double)p[0], (double)p[1], (double)p[2], (double)p[3]};
+template <typename P>
+__device__ __forceinline__ Q<double> norm_q(LRef<P> r) {
+  const Q<float> f = ldq(r);
+  Q<double> q{(double)f.a0, (double)f.a1, (double)f.a2, (double)f.a3};
   double n = 1.0 / sqrt(q.a0 * q.a0 + q.a1 * q.a1 + q.a2 * q.a2 + q.a3 * q.a3);
   q.a0 *= n; q.a1 *= n; q.a2 *= n; q.a3 *= n;
   return q;
@@ -250,11 +266,11 @@ __global__ __launch_bounds__(256) void k_su2_rebuild(const double* __restrict__ 
   const uint32_t h = (blockIdx.x % nb) * 256 + threadIdx.x;
   if (h >= g.Vh) return;
   SiteIdx s = decode(g, h, p);
-  Q<double> gx = norm_q(G + ((size_t)p * g.Vh + h) * 4);
+  Q<double> gx = norm_q(site_ptr<4>(G, g, p, h));
 #pragma unroll
   for (int mu = 0; mu < 4; mu++) {
     uint32_t hf = fwd_h(g, h, s, mu);
-    Q<double> gf = norm_q(G + ((size_t)(1 - p) * g.Vh + hf) * 4);
+    Q<double> gf = norm_q(site_ptr<4>(G, g, 1 - p, hf));
     Q<double> u = ldq(link_ptr<4>(U0, g, p, mu, h));
     Q<double> w = qmul_dag(qmul(gx, u), gf);
     if (out32) stq(link_ptr<4>(out32, g, p, mu, h), Q<float>{(float)w.a0, (float)w.a1, (float)w.a2, (float)w.a3});
@@ -281,12 +297,12 @@ __global__ __launch_bounds__(256) void k_su3_rebuild(const double* __restrict__ 
   const uint32_t h = (blockIdx.x % nb) * 256 + threadIdx.x;
   if (h >= g.Vh) return;
   SiteIdx s = decode(g, h, p);
-  M3<double> gx = gs3(to_d(ldm<float>(G + ((size_t)p * g.Vh + h) * 12)));
+  M3<double> gx = gs3(to_d(ldm(site_ptr<12>(G, g, p, h))));
 #pragma unroll 1
   for (int mu = 0; mu < 4; mu++) {
     uint32_t hf = fwd_h(g, h, s, mu);
-    M3<double> gf = gs3(to_d(ldm<float>(G + ((size_t)(1 - p) * g.Vh + hf) * 12)));
-    M3<double> u = full3(ldm<double>(link_ptr<12>(U0, g, p, mu, h)));
+    M3<double> gf = gs3(to_d(ldm(site_ptr<12>(G, g, 1 - p, hf))));
+    M3<double> u = full3(ldm(link_ptr<12>(U0, g, p, mu, h)));
     // V = u gf^+
     M3<double> v;
 #pragma unroll
@@ -332,16 +348,18 @@ void launch_su3_rebuild(const double* U0, const float* G, float* out32, double* 
   hipLaunchKernelGGL(k_su3_rebuild, dim3(2 * nb), dim3(256), 0, st, U0, G, out32, out64, g);
 }
 
-__global__ void k_gauge_normalize(float* __restrict__ G, uint32_t nsites, int group) {
+__global__ void k_gauge_normalize(float* __restrict__ G, Geom g, int group) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsites) return;
+  if (s >= 2 * g.Vh) return;
+  const int par = (int)(s / g.Vh);
+  const uint32_t h = s % g.Vh;
   if (group == 2) {
-    float* p = G + (size_t)s * 4;
+    const LRef<float> p = site_ptr<4>(G, g, par, h);
     Q<double> q = norm_q(p);
     stq(p, Q<float>{(float)q.a0, (float)q.a1, (float)q.a2, (float)q.a3});
   } else {
-    float* p = G + (size_t)s * 12;
-    M3<double> m = gs3(to_d(ldm<float>(p)));
+    const LRef<float> p = site_ptr<12>(G, g, par, h);
+    M3<double> m = gs3(to_d(ldm(p)));
     M2<float> o;
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -352,34 +370,36 @@ __global__ void k_gauge_normalize(float* __restrict__ G, uint32_t nsites, int gr
 }
 void launch_gauge_normalize(float* G, const Geom& g, int group, hipStream_t st) {
   const uint32_t n = 2 * g.Vh;
-  hipLaunchKernelGGL(k_gauge_normalize, dim3((n + 255) / 256), dim3(256), 0, st, G, n, group);
+  hipLaunchKernelGGL(k_gauge_normalize, dim3((n + 255) / 256), dim3(256), 0, st, G, g, group);
 }
 
-__global__ void k_su2_reunit64(double* __restrict__ U, size_t nlinks) {
-  size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= nlinks) return;
-  Q<double> q = ldq(U + l * 4);
+__global__ void k_su2_reunit64(double* __restrict__ U, Geom g) {
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= (size_t)8 * g.Vh) return;
+  const LRef<double> r = link_ptr_pm<4>(U, g, (uint32_t)(l / g.Vh), (uint32_t)(l % g.Vh));
+  Q<double> q = ldq(r);
   double n = 1.0 / sqrt(q.a0 * q.a0 + q.a1 * q.a1 + q.a2 * q.a2 + q.a3 * q.a3);
-  stq(U + l * 4, Q<double>{q.a0 * n, q.a1 * n, q.a2 * n, q.a3 * n});
+  stq(r, Q<double>{q.a0 * n, q.a1 * n, q.a2 * n, q.a3 * n});
 }
-__global__ void k_su3_reunit64(double* __restrict__ U, size_t nlinks) {
-  size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= nlinks) return;
-  M3<double> m = gs3(ldm<double>(U + l * 12));
+__global__ void k_su3_reunit64(double* __restrict__ U, Geom g) {
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= (size_t)8 * g.Vh) return;
+  const LRef<double> r = link_ptr_pm<12>(U, g, (uint32_t)(l / g.Vh), (uint32_t)(l % g.Vh));
+  M3<double> m = gs3(ldm(r));
   M2<double> o;
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int k = 0; k < 3; k++) o.r[i][k] = m.m[i][k];
-  stm(U + l * 12, o);
+  stm(r, o);
 }
 void launch_su2_reunit64(double* U, const Geom& g, hipStream_t st) {
   size_t n = (size_t)8 * g.Vh;
-  hipLaunchKernelGGL(k_su2_reunit64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, U, n);
+  hipLaunchKernelGGL(k_su2_reunit64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, U, g);
 }
 void launch_su3_reunit64(double* U, const Geom& g, hipStream_t st) {
   size_t n = (size_t)8 * g.Vh;
-  hipLaunchKernelGGL(k_su3_reunit64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, U, n);
+  hipLaunchKernelGGL(k_su3_reunit64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, U, g);
 }
 
 // ================================================================== plaquette
@@ -423,10 +443,10 @@ __global__ __launch_bounds__(kBlock) void k_su3_plaquette(const double* __restri
     for (int mu = 0; mu < 3; mu++)
 #pragma unroll 1
       for (int nu = mu + 1; nu < 4; nu++) {
-        M3<double> a = full3(ldm<double>(link_ptr<12>(U, g, p, mu, h)));
-        M3<double> b = full3(ldm<double>(link_ptr<12>(U, g, 1 - p, nu, hf[mu])));
-        M3<double> c = full3(ldm<double>(link_ptr<12>(U, g, 1 - p, mu, hf[nu])));
-        M3<double> d = full3(ldm<double>(link_ptr<12>(U, g, p, nu, h)));
+        M3<double> a = full3(ldm(link_ptr<12>(U, g, p, mu, h)));
+        M3<double> b = full3(ldm(link_ptr<12>(U, g, 1 - p, nu, hf[mu])));
+        M3<double> c = full3(ldm(link_ptr<12>(U, g, 1 - p, mu, hf[nu])));
+        M3<double> d = full3(ldm(link_ptr<12>(U, g, p, nu, h)));
         // Re tr(a b c^+ d^+) / 3
         M3<double> ab, dc;
 #pragma unroll

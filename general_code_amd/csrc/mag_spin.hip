@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_spin_gauge_transform(double* __restr
 #pragma unroll
   for (int mu = 0; mu < 4; mu++) {
     const Q<double> Gf = gauge_matrix(ld_spin(S, g, 1 - p, fwd_h(g, h, s, mu)));
-    double* l = link_ptr<4>(U, g, p, mu, h);
+    const LRef<double> l = link_ptr<4>(U, g, p, mu, h);
     Q<double> A = qmul(G, ldq(l));
     A = qmul(A, Q<double>{Gf.a0, -Gf.a1, -Gf.a2, -Gf.a3});
     stq(l, A);

@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_su2_half(T* __restrict__ U, T* __res
       stq(link_ptr<4>(U, g, 1 - p, mu, hb[mu]), B[mu]);
     }
     if (GA) {
-      T* gp = site_ptr<4>(G, g, p, h);
+      const LRef<T> gp = site_ptr<4>(G, g, p, h);
       stq(gp, gmul(gq, ldq(gp)));
     }
     if (p == 1) {
@@ -254,8 +254,8 @@ __global__ __launch_bounds__(kBlock) void k_su3_half(T* __restrict__ U, T* __res
     const SiteIdx s = decode(g, h, p);
     uint32_t hb[4];
     back_all(g, h, s, hb);
-    T* pf[2];
-    T* pb[2];
+    LRef<T> pf[2];
+    LRef<T> pb[2];
     M2<T> F[2], B[2];
 #pragma unroll
     for (int k = 0; k < 2; k++) {
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_su3_half(T* __restrict__ U, T* __res
       B[k] = ldm<T>(pb[k]);
     }
     M2<T> Gx;
-    T* gp = GA ? site_ptr<12>(G, g, p, h) : nullptr;
+    const LRef<T> gp = site_ptr<12>(G, g, p, h);
     if (GA) Gx = ldm<T>(gp);
     T res = cm_subgroup<T, 0, 1, GA>(F, B, Gx, omega);
     res += cm_subgroup<T, 1, 2, GA>(F, B, Gx, omega);
