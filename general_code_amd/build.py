@@ -54,7 +54,24 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_cli(verbose)
     return LIB
+
+
+CLI = os.path.join(PKG, "mag_fixation")
+
+
+def build_cli(verbose: bool = False) -> str:
+    """The mag_fixation command-line app (host C++ over the C ABI)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(PKG, "cpp"), os.path.join(PKG, "cpp", "mag_fixation.cpp"),
+           "-L", PKG, "-lmag_gpu", "-Wl,-rpath,$ORIGIN", "-o", CLI]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"mag_fixation build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {CLI}")
+    return CLI
 
 
 if __name__ == "__main__":

@@ -1,0 +1,394 @@
+// mag.h -- C++ drop-in for the MAG call surface of ilyakudrov/general_code
+// (lib/cpu/include/mag.h), running on the MI355X through the C ABI of
+// include/mag_gpu.h.
+//
+// The functions are templates over the lattice and spin types, so an app
+// can pass either the reference's own objects
+//   Data::LatticeData<DataPatternLexicographical, su2>   (data.h:78-237)
+//   std::vector<spin>                                     (matrix.h:773-787)
+// or the layout-identical mirror types below; only the memory layout is
+// used: su2 = 4 doubles {a0,a1,a2,a3}, spin = 3 doubles, links in the
+// order 4*site + mu, site = x + Lx*(y + Ly*(z + Lz*t)) (indexing.h:71-126).
+// The reference's su3 is an Eigen::Matrix3cd (column-major); the mirror su3
+// below is row-major (the file order of matrix.h:236-254); both are
+// accepted (see su3_row_major).
+//
+// Differences from the reference, by design:
+//   * sweeps run in checkerboard order on the GPU (the reference's in-place
+//     lexicographic OpenMP loops race, mag.cpp:1386, 1491);
+//   * random numbers come from a Philox stream keyed by an explicit seed
+//     (the reference seeds with time(NULL), mag.cpp:145, 764);
+//   * failures throw mag_amd::error instead of printing and continuing;
+//   * the maximisation loops have an iteration cap (max_iter).
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "mag_gpu.h"
+
+namespace mag_amd {
+
+struct error : std::runtime_error {
+  int code;
+  error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc) {
+  if (rc != MAG_OK) throw error(rc, mag_last_error());
+}
+
+// ---------------------------------------------------------------- value types
+// su2: a0 + i (a1 s1 + a2 s2 + a3 s3) (matrix.h:8-13)
+struct su2 {
+  static constexpr int data_size = 4;
+  double a0 = 1, a1 = 0, a2 = 0, a3 = 0;
+};
+// spin (matrix.h:773-787); the default is (0, 0, 1) as in the reference
+struct spin {
+  double a1 = 0, a2 = 0, a3 = 1;
+};
+// su3, row-major (re, im) (matrix.h:236-254)
+struct su3 {
+  static constexpr int data_size = 18;
+  double v[18] = {1, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 1, 0};
+};
+
+// link index 4*site + mu (indexing.h:71-126)
+class DataPatternLexicographical {
+ public:
+  std::array<int, 4> lat_dim;
+  explicit DataPatternLexicographical(const std::array<int, 4>& d) : lat_dim(d) {}
+  int get_lattice_size() const { return lat_dim[0] * lat_dim[1] * lat_dim[2] * lat_dim[3]; }
+  int get_data_size() const { return get_lattice_size() * 4; }
+};
+
+namespace Data {
+// data.h:78-237 (storage part)
+template <class DataPattern, class MatrixType>
+class LatticeData {
+ public:
+  typedef MatrixType matrix_type;
+  typedef DataPattern data_pattern_type;
+  std::vector<MatrixType> array;
+  std::array<int, 4> lat_dim;
+  explicit LatticeData(const std::array<int, 4>& d)
+      : array((size_t)d[0] * d[1] * d[2] * d[3] * 4), lat_dim(d) {}
+  const MatrixType& operator[](int i) const { return array[i]; }
+  MatrixType& operator[](int i) { return array[i]; }
+};
+}  // namespace Data
+
+// su3 storage order of a matrix type: the mirror su3 is row-major; any
+// other 18-double type is taken to be Eigen::Matrix3cd (column-major).
+template <class M>
+struct su3_row_major : std::is_same<M, su3> {};
+
+namespace detail {
+
+template <class Lattice>
+const double* su2_links(const Lattice& conf) {
+  using M = typename Lattice::matrix_type;
+  static_assert(sizeof(M) == 4 * sizeof(double), "su2 link must be 4 doubles");
+  return reinterpret_cast<const double*>(conf.array.data());
+}
+template <class Lattice>
+double* su2_links(Lattice& conf) {
+  using M = typename Lattice::matrix_type;
+  static_assert(sizeof(M) == 4 * sizeof(double), "su2 link must be 4 doubles");
+  return reinterpret_cast<double*>(conf.array.data());
+}
+template <class S>
+double* spin_data(std::vector<S>& s) {
+  static_assert(sizeof(S) == 3 * sizeof(double), "spin must be 3 doubles");
+  return reinterpret_cast<double*>(s.data());
+}
+template <class Lattice>
+void dims_of(const Lattice& conf, int* d) {
+  for (int i = 0; i < 4; i++) d[i] = conf.lat_dim[i];
+}
+// su3 links in the ABI's row-major order
+template <class Lattice>
+std::vector<double> su3_links(const Lattice& conf) {
+  using M = typename Lattice::matrix_type;
+  static_assert(sizeof(M) == 18 * sizeof(double), "su3 link must be 18 doubles");
+  std::vector<double> out(conf.array.size() * 18);
+  const double* in = reinterpret_cast<const double*>(conf.array.data());
+  for (size_t l = 0; l < conf.array.size(); l++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        const size_t src = su3_row_major<M>::value ? (size_t)(3 * i + j) : (size_t)(i + 3 * j);
+        out[18 * l + 2 * (3 * i + j)] = in[18 * l + 2 * src];
+        out[18 * l + 2 * (3 * i + j) + 1] = in[18 * l + 2 * src + 1];
+      }
+  return out;
+}
+template <class Lattice>
+void su3_store(Lattice& conf, const std::vector<double>& rm) {
+  using M = typename Lattice::matrix_type;
+  double* o = reinterpret_cast<double*>(conf.array.data());
+  for (size_t l = 0; l < conf.array.size(); l++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        const size_t dst = su3_row_major<M>::value ? (size_t)(3 * i + j) : (size_t)(i + 3 * j);
+        o[18 * l + 2 * dst] = rm[18 * l + 2 * (3 * i + j)];
+        o[18 * l + 2 * dst + 1] = rm[18 * l + 2 * (3 * i + j) + 1];
+      }
+}
+
+// A spin session for one call: links (and spins) uploaded, freed on exit.
+class Session {
+ public:
+  template <class Lattice>
+  explicit Session(const Lattice& conf) {
+    int d[4];
+    dims_of(conf, d);
+    int dev = 0;
+    check(mag_device_count(&dev));
+    check(mag_spin_session_create(d, 0, &s_));
+    int rc = mag_spin_session_upload_links(s_, su2_links(conf));
+    if (rc != MAG_OK) {
+      mag_spin_session_destroy(s_);
+      throw error(rc, mag_last_error());
+    }
+  }
+  ~Session() { mag_spin_session_destroy(s_); }
+  Session(const Session&) = delete;
+  Session& operator=(const Session&) = delete;
+  mag_spin_session* get() const { return s_; }
+
+ private:
+  mag_spin_session* s_ = nullptr;
+};
+
+}  // namespace detail
+
+// ---------------------------------------------------------------- mag.h surface
+// generate_spins_uniform (mag.h:179-199)
+template <class SpinT = spin, class DataPattern>
+std::vector<SpinT> generate_spins_uniform(DataPattern& pattern, uint64_t seed) {
+  int d[4];
+  for (int i = 0; i < 4; i++) d[i] = pattern.lat_dim[i];
+  std::vector<SpinT> s((size_t)d[0] * d[1] * d[2] * d[3]);
+  check(mag_su2_generate_spins_uniform(d, seed, detail::spin_data(s)));
+  return s;
+}
+
+// MAG_functional_su2 (mag.h:69-70, mag.cpp:993-1001)
+template <class Lattice>
+double MAG_functional_su2(const Lattice& conf) {
+  int d[4];
+  detail::dims_of(conf, d);
+  double F;
+  check(mag_su2_functional(d, detail::su2_links(conf), &F));
+  return F;
+}
+
+// MAG_functional_su2_spin (mag.h:58-60, mag.cpp:925-953)
+template <class Lattice, class SpinT>
+double MAG_functional_su2_spin(const Lattice& conf, std::vector<SpinT>& spins) {
+  int d[4];
+  detail::dims_of(conf, d);
+  double F;
+  check(mag_su2_functional_spin(d, detail::su2_links(conf), detail::spin_data(spins), &F));
+  return F;
+}
+
+// gauge_tranformation_spins (mag.h:80-82, mag.cpp:1045-1068)
+template <class Lattice, class SpinT>
+void gauge_tranformation_spins(Lattice& conf, std::vector<SpinT>& spins) {
+  int d[4];
+  detail::dims_of(conf, d);
+  check(mag_su2_gauge_transform_spins(d, detail::su2_links(conf), detail::spin_data(spins)));
+}
+
+// make_simulated_annealing (mag.h:119-122, mag.cpp:1585-1603)
+template <class Lattice, class SpinT>
+void make_simulated_annealing(const Lattice& conf, std::vector<SpinT>& spins, double T_init,
+                              double T_final, double T_step, int OR_steps,
+                              int thermalization_steps, uint64_t seed = 1) {
+  detail::Session s(conf);
+  check(mag_spin_session_set_spins(s.get(), detail::spin_data(spins)));
+  check(mag_spin_session_simulated_annealing(s.get(), T_init, T_final, T_step, OR_steps,
+                                             thermalization_steps, seed));
+  check(mag_spin_session_get_spins(s.get(), detail::spin_data(spins)));
+}
+
+// make_maximization_final (mag.h:166-169, mag.cpp:1800-1816)
+template <class Lattice, class SpinT>
+int make_maximization_final(const Lattice& conf, std::vector<SpinT>& spins, int OR_steps,
+                            double tolerance_maximal, double tolerance_average,
+                            int max_iter = 1000000) {
+  detail::Session s(conf);
+  check(mag_spin_session_set_spins(s.get(), detail::spin_data(spins)));
+  int it = 0;
+  const int rc = mag_spin_session_maximize_final(s.get(), OR_steps, tolerance_maximal,
+                                                 tolerance_average, max_iter, &it);
+  check(mag_spin_session_get_spins(s.get(), detail::spin_data(spins)));
+  check(rc);
+  return it;
+}
+
+// make_maximization_approximate (mag.h:151-153, mag.cpp:1733-1750)
+template <class Lattice, class SpinT>
+int make_maximization_approximate(const Lattice& conf, std::vector<SpinT>& spins, int OR_steps,
+                                  int tolerance_digits, int max_iter = 1000000) {
+  detail::Session s(conf);
+  check(mag_spin_session_set_spins(s.get(), detail::spin_data(spins)));
+  int it = 0;
+  const int rc = mag_spin_session_maximize_approximate(s.get(), OR_steps, tolerance_digits,
+                                                       max_iter, &it);
+  check(mag_spin_session_get_spins(s.get(), detail::spin_data(spins)));
+  check(rc);
+  return it;
+}
+
+// mag_functional_su3 (mag.h:177, mag.cpp:1837-1862)
+template <class Lattice>
+double mag_functional_su3(const Lattice& conf) {
+  int d[4];
+  detail::dims_of(conf, d);
+  const std::vector<double> rm = detail::su3_links(conf);
+  double F;
+  check(mag_su3_functional(d, rm.data(), &F));
+  return F;
+}
+
+// plaket (plaket.h:84-101) for su2 (4 doubles) or su3 (18 doubles) links
+template <class Lattice>
+double plaket(const Lattice& conf) {
+  int d[4];
+  detail::dims_of(conf, d);
+  double P;
+  if constexpr (sizeof(typename Lattice::matrix_type) == 4 * sizeof(double)) {
+    check(mag_plaquette(MAG_SU2, d, detail::su2_links(conf), &P));
+  } else {
+    const std::vector<double> rm = detail::su3_links(conf);
+    check(mag_plaquette(MAG_SU3, d, rm.data(), &P));
+  }
+  return P;
+}
+
+// ---------------------------------------------------------------- link form
+// The north-star path: checkerboard omega-overrelaxation on the links
+// (SU(3): Cabibbo-Marinari), FP64 or mixed precision, in place.  Returns
+// the number of sweeps; the functional history is in *history if given.
+template <class Lattice>
+int fix_mag_links(Lattice& conf, double omega, double tol, int max_sweeps, int precision,
+                  std::vector<double>* history = nullptr) {
+  mag_params p;
+  mag_params_default(&p);
+  constexpr bool is3 = sizeof(typename Lattice::matrix_type) == 18 * sizeof(double);
+  p.group = is3 ? MAG_SU3 : MAG_SU2;
+  detail::dims_of(conf, p.dims);
+  p.omega = omega;
+  p.tol = tol;
+  p.max_sweeps = max_sweeps;
+  p.precision = precision;
+  mag_job* job = nullptr;
+  check(mag_job_create(&p, &job));
+  struct Guard {
+    mag_job* j;
+    ~Guard() { mag_job_destroy(j); }
+  } guard{job};
+  std::vector<double> rm;
+  if constexpr (is3) {
+    rm = detail::su3_links(conf);
+    check(mag_job_upload_links(job, rm.data()));
+  } else {
+    check(mag_job_upload_links(job, detail::su2_links(conf)));
+  }
+  const int rc = mag_job_run(job);
+  if (rc != MAG_OK && rc != MAG_ERR_NOT_CONVERGED) check(rc);
+  int sweeps = 0, conv = 0;
+  double F = 0;
+  check(mag_job_status(job, &sweeps, &conv, &F));
+  if constexpr (is3) {
+    check(mag_job_download_links(job, rm.data()));
+    detail::su3_store(conf, rm);
+  } else {
+    check(mag_job_download_links(job, detail::su2_links(conf)));
+  }
+  if (history) {
+    history->assign(sweeps, 0.0);
+    int n = 0;
+    check(mag_job_history(job, history->data(), nullptr, nullptr, sweeps, &n));
+  }
+  check(rc);
+  return sweeps;
+}
+
+// ---------------------------------------------------------------- file formats
+// Readers of the reference's on-disk link formats (data.h:114-142,
+// indexing.h:190-323); precision "double" or "float".
+//   lexicographical: element e of link (site, mu) at (4*site + mu)*N + e
+//   qcdstag:         at (site + mu1*V)*N + e', mu1 = (mu + 1) % 4 (t first),
+//                    and for su2 the element order reversed: e' = 4 - e, e > 0
+template <class Lattice>
+void read_links(Lattice& conf, const std::string& path, const std::string& format,
+                long bytes_skip, const std::string& precision) {
+  using M = typename Lattice::matrix_type;
+  constexpr int N = sizeof(M) / sizeof(double);
+  const size_t V = (size_t)conf.lat_dim[0] * conf.lat_dim[1] * conf.lat_dim[2] * conf.lat_dim[3];
+  const size_t n = V * 4 * N;
+  std::ifstream in(path, std::ios::binary);
+  if (!in) throw error(MAG_ERR_ARG, "cannot open " + path);
+  in.ignore(bytes_skip);
+  std::vector<double> raw(n);
+  if (precision == "double") {
+    if (!in.read(reinterpret_cast<char*>(raw.data()), n * sizeof(double)))
+      throw error(MAG_ERR_ARG, "short read: " + path);
+  } else if (precision == "float") {
+    std::vector<float> f(n);
+    if (!in.read(reinterpret_cast<char*>(f.data()), n * sizeof(float)))
+      throw error(MAG_ERR_ARG, "short read: " + path);
+    for (size_t i = 0; i < n; i++) raw[i] = f[i];
+  } else {
+    throw error(MAG_ERR_ARG, "precision must be float or double");
+  }
+  std::vector<double> rm(n);  // row-major links in the ABI order
+  for (size_t s = 0; s < V; s++)
+    for (int mu = 0; mu < 4; mu++)
+      for (int e = 0; e < N; e++) {
+        size_t idx;
+        if (format == "lexicographical") {
+          idx = (s * 4 + mu) * N + e;
+        } else if (format == "qcdstag") {
+          const int mu1 = (mu == 3) ? 0 : mu + 1;
+          const int e1 = (N == 4 && e > 0) ? 4 - e : e;
+          idx = (s + (size_t)mu1 * V) * N + e1;
+        } else {
+          throw error(MAG_ERR_ARG, "unknown format " + format);
+        }
+        rm[(s * 4 + mu) * N + e] = raw[idx];
+      }
+  if constexpr (N == 4) {
+    std::memcpy(conf.array.data(), rm.data(), n * sizeof(double));
+  } else {
+    detail::su3_store(conf, rm);
+  }
+}
+
+// write_data with FilePatternLexicographical (data.h:193-205): doubles
+template <class Lattice>
+void write_links_lexicographical(const Lattice& conf, const std::string& path) {
+  using M = typename Lattice::matrix_type;
+  constexpr int N = sizeof(M) / sizeof(double);
+  std::ofstream out(path, std::ios::binary);
+  if (!out) throw error(MAG_ERR_ARG, "cannot open " + path);
+  if constexpr (N == 4) {
+    out.write(reinterpret_cast<const char*>(conf.array.data()), conf.array.size() * 4 * sizeof(double));
+  } else {
+    const std::vector<double> rm = detail::su3_links(conf);
+    out.write(reinterpret_cast<const char*>(rm.data()), rm.size() * sizeof(double));
+  }
+  if (!out) throw error(MAG_ERR_ARG, "write failed: " + path);
+}
+
+}  // namespace mag_amd

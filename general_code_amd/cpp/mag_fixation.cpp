@@ -1,0 +1,150 @@
+// mag_fixation -- the MAG fixing app on the MI355X, with the command-line
+// flags of ilyakudrov/general_code apps/mag_su2/mag_fixation.cpp:60-117.
+//
+// New trial (--is_new_trial 1): uniform random spins -> simulated annealing
+// -> final maximisation (--is_final 1) or approximate maximisation -> gauge
+// transformation -> MAG functional and plaquette -> lexicographic output,
+// as in mag_fixation.cpp:165-249.  Extra flags of this build:
+//   --seed N            Philox seed of the spins and the heat bath
+//   --synthetic_eps E   no --path_conf: generate exp(i E a.sigma) links with
+//                       a Haar gauge transformation from --seed
+//   --link_form 1       run the link-form checkerboard omega-overrelaxation
+//                       (--omega, --tol, --max_sweeps) instead of the spin
+//                       drivers
+//   --max_iter N        iteration cap of the maximisation loops
+#include <chrono>
+#include <cstdio>
+#include <iostream>
+#include <sstream>
+#include <string>
+
+#include "mag.h"
+
+using namespace mag_amd;
+
+int main(int argc, char** argv) {
+  std::string path_conf, conf_format = "lexicographical", file_precision = "double";
+  std::string path_conf_output, path_spins_output, path_previous, path_functional_output;
+  int bytes_skip = 0;
+  double T_step = 0.01, T_init = 2.5, T_final = 0.01;
+  int OR_steps = 6, thermalization_steps = 50;
+  double tolerance_maximal = 1e-10, tolerance_average = 1e-14;
+  int tolerance_digits = 7, gauge_copies = 1;
+  int x_size = 0, y_size = 0, z_size = 0, t_size = 0;
+  bool is_new_trial = true, is_final = true, is_functional_save = false;
+  uint64_t seed = 1;
+  double synthetic_eps = -1;
+  bool link_form = false;
+  double omega = 1.7, tol = 1e-12;
+  int max_sweeps = 1000, max_iter = 1000000;
+
+  for (int i = 1; i < argc; i++) {
+    const std::string a = argv[i];
+    auto next = [&]() -> std::string {
+      if (i + 1 >= argc) throw error(MAG_ERR_ARG, "missing value for " + a);
+      return argv[++i];
+    };
+    if (a == "--conf_format") conf_format = next();
+    else if (a == "--file_precision") file_precision = next();
+    else if (a == "--path_conf") path_conf = next();
+    else if (a == "--path_previous") path_previous = next();
+    else if (a == "--path_conf_output") path_conf_output = next();
+    else if (a == "--path_spins_output") path_spins_output = next();
+    else if (a == "--path_functional_output") path_functional_output = next();
+    else if (a == "--bytes_skip") bytes_skip = std::stoi(next());
+    else if (a == "--T_step") T_step = std::stod(next());
+    else if (a == "--T_init") T_init = std::stod(next());
+    else if (a == "--T_final") T_final = std::stod(next());
+    else if (a == "--OR_steps") OR_steps = std::stoi(next());
+    else if (a == "--tolerance_maximal") tolerance_maximal = std::stod(next());
+    else if (a == "--tolerance_average") tolerance_average = std::stod(next());
+    else if (a == "--thermalization_steps") thermalization_steps = std::stoi(next());
+    else if (a == "--tolerance_digits") tolerance_digits = std::stoi(next());
+    else if (a == "--gauge_copies") gauge_copies = std::stoi(next());
+    else if (a == "--x_size") x_size = std::stoi(next());
+    else if (a == "--y_size") y_size = std::stoi(next());
+    else if (a == "--z_size") z_size = std::stoi(next());
+    else if (a == "--t_size") t_size = std::stoi(next());
+    else if (a == "--is_new_trial") std::istringstream(next()) >> is_new_trial;
+    else if (a == "--is_final") std::istringstream(next()) >> is_final;
+    else if (a == "--is_functional_save") std::istringstream(next()) >> is_functional_save;
+    else if (a == "--is_compare" || a == "--is_compare_spins" || a == "--ml5_conf_num") next();
+    else if (a == "--seed") seed = std::stoull(next());
+    else if (a == "--synthetic_eps") synthetic_eps = std::stod(next());
+    else if (a == "--link_form") std::istringstream(next()) >> link_form;
+    else if (a == "--omega") omega = std::stod(next());
+    else if (a == "--tol") tol = std::stod(next());
+    else if (a == "--max_sweeps") max_sweeps = std::stoi(next());
+    else if (a == "--max_iter") max_iter = std::stoi(next());
+    else {
+      std::cerr << "unknown parameter " << a << std::endl;
+      return 2;
+    }
+  }
+  try {
+    const std::array<int, 4> dims = {x_size, y_size, z_size, t_size};
+    Data::LatticeData<DataPatternLexicographical, su2> conf(dims);
+    if (!path_conf.empty()) {
+      read_links(conf, path_conf, conf_format, bytes_skip, file_precision);
+    } else if (synthetic_eps >= 0) {
+      mag_params p;
+      mag_params_default(&p);
+      for (int k = 0; k < 4; k++) p.dims[k] = dims[k];
+      mag_job* job = nullptr;
+      check(mag_job_create(&p, &job));
+      int rc = mag_job_generate(job, synthetic_eps, seed);
+      if (rc == MAG_OK) rc = mag_job_download_links(job, detail::su2_links(conf));
+      mag_job_destroy(job);
+      check(rc);
+    } else {
+      std::cerr << "need --path_conf or --synthetic_eps" << std::endl;
+      return 2;
+    }
+    std::cout.precision(17);
+    std::cout << "plaquette " << plaket(conf) << std::endl;
+    const auto t0 = std::chrono::steady_clock::now();
+    if (link_form) {
+      std::vector<double> hist;
+      const int sweeps = fix_mag_links(conf, omega, tol, max_sweeps, MAG_PREC_F64, &hist);
+      std::cout << "link-form sweeps " << sweeps << std::endl;
+    } else {
+      DataPatternLexicographical pattern(dims);
+      std::vector<spin> spins;
+      for (int copy = 1; copy <= gauge_copies; copy++) {
+        if (is_new_trial) {
+          spins = generate_spins_uniform<spin>(pattern, seed + copy);
+          make_simulated_annealing(conf, spins, T_init, T_final, T_step, OR_steps,
+                                   thermalization_steps, seed + copy);
+        } else {
+          throw error(MAG_ERR_ARG, "--is_new_trial 0 needs spins from --path_previous "
+                                   "(spin files are not supported by this build)");
+        }
+        if (is_final) {
+          const int it = make_maximization_final(conf, spins, OR_steps, tolerance_maximal,
+                                                 tolerance_average, max_iter);
+          std::cout << "maximization final iterations " << it << std::endl;
+        } else {
+          const int it = make_maximization_approximate(conf, spins, OR_steps, tolerance_digits,
+                                                       max_iter);
+          std::cout << "maximization approximate iterations " << it << std::endl;
+        }
+        std::cout << "spin functional " << MAG_functional_su2_spin(conf, spins) << std::endl;
+        gauge_tranformation_spins(conf, spins);
+        break;
+      }
+    }
+    const double secs =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::cout << "fixing time " << secs << std::endl;
+    std::cout << "final functional is " << MAG_functional_su2(conf) << std::endl;
+    std::cout << "plaquette " << plaket(conf) << std::endl;
+    if (!path_conf_output.empty()) write_links_lexicographical(conf, path_conf_output);
+  } catch (const error& e) {
+    std::cerr << "mag_fixation: " << e.what() << " (status " << e.code << ")" << std::endl;
+    return 1;
+  } catch (const std::exception& e) {
+    std::cerr << "mag_fixation: " << e.what() << std::endl;
+    return 1;
+  }
+  return 0;
+}

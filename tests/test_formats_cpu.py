@@ -1,0 +1,68 @@
+"""On-disk link formats of the reference (data.h:114-142, indexing.h:190-323)
+read by the C++ drop-in header general_code_amd/cpp/mag.h (host only)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def reader(tmp_path_factory, mag):
+    exe = str(tmp_path_factory.mktemp("bin") / "read_links")
+    pkg = os.path.join(ROOT, "general_code_amd")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    "-I", os.path.join(pkg, "cpp"), os.path.join(ROOT, "tests", "cpp", "read_links_main.cpp"),
+                    "-L", pkg, "-lmag_gpu", f"-Wl,-rpath,{pkg}", "-o", exe], check=True)
+    return exe
+
+
+def _qcdstag_su2(links, dims):
+    """Write links in the QCDSTAG su2 order: direction t first, element
+    order (a0, a3, a2, a1) (FilePatternQCDSTAG<4, su2>, indexing.h:286-323)."""
+    V = int(np.prod(dims))
+    L = links.reshape(V, 4, 4)
+    out = np.zeros((4, V, 4))
+    for mu in range(4):
+        mu1 = 0 if mu == 3 else mu + 1
+        out[mu1] = L[:, mu, :][:, [0, 3, 2, 1]]
+    return out.reshape(-1)
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_su2_formats_roundtrip(oracle, reader, tmp_path, precision):
+    dims = (4, 2, 4, 6)
+    links = oracle.gen_su2_config(dims, 0.5, 3)
+    if precision == "float":
+        links = links.astype(np.float32).astype(np.float64)
+    dt = np.float64 if precision == "double" else np.float32
+    lex = tmp_path / "lex.bin"
+    links.astype(dt).tofile(lex)
+    stag = tmp_path / "stag.bin"
+    _qcdstag_su2(links, dims).astype(dt).tofile(stag)
+    for path, fmt in [(lex, "lexicographical"), (stag, "qcdstag")]:
+        out = tmp_path / f"out_{fmt}.bin"
+        subprocess.run([reader, "2", *map(str, dims), str(path), fmt, precision, str(out)], check=True)
+        got = np.fromfile(out, dtype=np.float64).reshape(-1, 4)
+        assert np.array_equal(got, links)
+
+
+def test_su3_lexicographical_roundtrip(oracle, reader, tmp_path):
+    dims = (2, 2, 2, 4)
+    links = oracle.gen_su3_config(dims, 0.4, 5)
+    p = tmp_path / "su3.bin"
+    links.tofile(p)
+    out = tmp_path / "out.bin"
+    subprocess.run([reader, "3", *map(str, dims), str(p), "lexicographical", "double", str(out)],
+                   check=True)
+    assert np.array_equal(np.fromfile(out).reshape(-1, 18), links)
+
+
+def test_bad_file_reports_error(reader, tmp_path):
+    p = tmp_path / "short.bin"
+    np.zeros(10).tofile(p)
+    r = subprocess.run([reader, "2", "2", "2", "2", "2", str(p), "lexicographical", "double",
+                        str(tmp_path / "o.bin")], capture_output=True, text=True)
+    assert r.returncode == 1 and "short read" in r.stderr

@@ -1,0 +1,63 @@
+"""The mag_fixation app (general_code_amd/cpp/mag_fixation.cpp) end to end on
+the GPU, with the flags of apps/mag_su2/mag_fixation.cpp:60-117."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "general_code_amd", "mag_fixation")
+
+
+def _run(args):
+    r = subprocess.run([CLI, *args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def _num(out, key):
+    return [float(m) for m in re.findall(rf"{key}\s+([-+0-9.eE]+)", out)]
+
+
+def test_spin_path_from_file(oracle, tmp_path):
+    dims = (8, 8, 8, 8)
+    links = oracle.gen_su2_config(dims, 0.3, 21)
+    conf = tmp_path / "conf.bin"
+    links.tofile(conf)
+    outp = tmp_path / "fixed.bin"
+    out = _run(["--path_conf", str(conf), "--conf_format", "lexicographical",
+                "--file_precision", "double", "--x_size", "8", "--y_size", "8", "--z_size", "8",
+                "--t_size", "8", "--T_init", "2.0", "--T_final", "0.5", "--T_step", "0.25",
+                "--OR_steps", "3", "--thermalization_steps", "3", "--tolerance_maximal", "1e-9",
+                "--tolerance_average", "1e-13", "--is_final", "1", "--seed", "4",
+                "--path_conf_output", str(outp)])
+    plaq = _num(out, "plaquette")
+    assert abs(plaq[0] - oracle.su2_plaquette(links, dims)) < 1e-13
+    assert abs(plaq[1] - plaq[0]) < 1e-12
+    fixed = np.fromfile(outp).reshape(-1, 4)
+    F = _num(out, "final functional is")[0]
+    assert abs(oracle.su2_functional(fixed, dims) - F) < 1e-13
+    assert abs(_num(out, "spin functional")[0] - F) < 1e-12
+    assert F > oracle.su2_functional(links, dims) + 0.3
+
+
+def test_link_form_synthetic(oracle, tmp_path):
+    outp = tmp_path / "fixed.bin"
+    out = _run(["--synthetic_eps", "0.3", "--seed", "42", "--x_size", "8", "--y_size", "8",
+                "--z_size", "8", "--t_size", "8", "--link_form", "1", "--omega", "1.7",
+                "--tol", "1e-12", "--max_sweeps", "1000", "--path_conf_output", str(outp)])
+    dims = (8, 8, 8, 8)
+    ref = oracle.gen_su2_config(dims, 0.3, 42)
+    F_prev = oracle.su2_functional(ref, dims)
+    for n in range(1000):
+        F, _, _ = oracle.su2_link_sweep(ref, dims, 1.7)
+        if abs(F - F_prev) < 1e-12 * abs(F):
+            break
+        F_prev = F
+    assert abs(_num(out, "link-form sweeps")[0] - (n + 1)) <= 1
+    fixed = np.fromfile(outp).reshape(-1, 4)
+    assert abs(oracle.su2_functional(fixed, dims) - F) < 1e-12
