@@ -9,6 +9,7 @@
 #include "mag_gpu.h"
 #include "mag_capi_util.h"
 #include "mag_internal.h"
+#include "mag_job_impl.h"
 
 #include <cmath>
 #include <cstdarg>
@@ -81,34 +82,6 @@ bool all_finite(const double* a, size_t n) {
 
 }  // namespace
 
-struct mag_job {
-  mag_params prm;
-  Geom g;
-  int64_t V = 0;
-  int nr = 0;      // reals per link in the SoA layout (4 or 12)
-  int nr_in = 0;   // reals per link in the reference layout (4 or 18)
-  size_t nreal = 0;  // reals in one link array (8 * Vh * nr)
-  int nb = 0;        // workgroups per half-sweep (two lanes per site)
-  int nf = 0;        // workgroups per parity of the functional / plaquette kernels
-  hipStream_t st = nullptr;
-  double* U64 = nullptr;  // FP64 links (plain) / FP64 master then output (mixed)
-  float* W32 = nullptr;   // mixed: FP32 working links
-  float* G32 = nullptr;   // mixed: accumulated site gauge
-  double* part = nullptr;
-  JobCtl* ctl = nullptr;
-  double* hF = nullptr;
-  double* hR = nullptr;
-  double* hA = nullptr;
-  double* out3 = nullptr;
-  JobCtl* ctl_h = nullptr;   // pinned host mirrors
-  double* out3_h = nullptr;
-  bool loaded = false;
-  int phase = 0;             // mixed: 0 = FP32 sweeps, 1 = FP64 after rebuild
-  int fp64_start = -1;
-  int since_rebuild = 0;
-  double invF = 0, invV = 0;
-};
-
 namespace {
 
 bool mixed32(const mag_job* j) { return j->prm.precision == MAG_PREC_MIXED && j->phase == 0; }
@@ -132,14 +105,17 @@ void launch_half(mag_job* j, int parity) {
   }
 }
 
+// periodic rebuild (skipped by the kernels once the job is done) or the
+// final FP64 rebuild (forced)
 void launch_rebuild(mag_job* j, bool final64) {
+  const JobCtl* c = final64 ? nullptr : j->ctl;
   if (j->prm.group == MAG_SU2)
     mag::launch_su2_rebuild(j->U64, j->G32, final64 ? nullptr : j->W32, final64 ? j->U64 : nullptr,
-                            j->g, j->st);
+                            j->g, c, j->st);
   else
     mag::launch_su3_rebuild(j->U64, j->G32, final64 ? nullptr : j->W32, final64 ? j->U64 : nullptr,
-                            j->g, j->st);
-  if (!final64) mag::launch_gauge_normalize(j->G32, j->g, j->prm.group, j->st);
+                            j->g, c, j->st);
+  if (!final64) mag::launch_gauge_normalize(j->G32, j->g, j->prm.group, j->ctl, j->st);
 }
 
 // sweep tail shared by the plain and timed paths
@@ -258,6 +234,10 @@ int run_until_done(mag_job* j) {
 
 }  // namespace
 
+int mag::job_read_ctl(mag_job* j) { return read_ctl(j); }
+int mag::job_finalize_mixed(mag_job* j) { return finalize_mixed(j); }
+bool mag::job_mixed32(const mag_job* j) { return mixed32(j); }
+
 // ====================================================================== API
 extern "C" {
 
@@ -362,6 +342,7 @@ int mag_job_create(const mag_params* p, mag_job** out) {
 
 int mag_job_destroy(mag_job* job) {
   if (!job) return MAG_OK;
+  if (job->batch) mag_batch_remove(job->batch, job);
   hipSetDevice(job->prm.device);
   hipStreamSynchronize(job->st);
   free_job(job);
@@ -376,6 +357,7 @@ int mag_job_stream(mag_job* job, void** stream) {
 
 int mag_job_upload_links(mag_job* j, const double* links) {
   if (!j || !links) return fail(MAG_ERR_ARG, "NULL argument");
+  if (j->batch) return fail(MAG_ERR_STATE, "job is in a batch: sweep it through the batch");
   CHK(set_device(j));
   const size_t n = (size_t)j->V * 4 * j->nr_in;
   if (!all_finite(links, n)) return fail(MAG_ERR_NONFINITE, "links hold NaN or Inf");
@@ -391,12 +373,18 @@ int mag_job_upload_links(mag_job* j, const double* links) {
 
 int mag_job_upload_links_device(mag_job* j, const double* dev_links) {
   if (!j || !dev_links) return fail(MAG_ERR_ARG, "NULL argument");
+  if (j->batch) return fail(MAG_ERR_STATE, "job is in a batch: sweep it through the batch");
   CHK(set_device(j));
   return upload_device_aos(j, dev_links);
 }
 
 int mag_job_generate(mag_job* j, double eps, uint64_t seed) {
+  return mag_job_generate_copy(j, eps, seed, 0);
+}
+
+int mag_job_generate_copy(mag_job* j, double eps, uint64_t seed, uint32_t copy) {
   if (!j) return fail(MAG_ERR_ARG, "NULL job");
+  if (j->batch) return fail(MAG_ERR_STATE, "job is in a batch: sweep it through the batch");
   if (!std::isfinite(eps)) return fail(MAG_ERR_ARG, "eps must be finite");
   CHK(set_device(j));
   const size_t n = (size_t)j->V * 4 * j->nr_in;
@@ -409,9 +397,9 @@ int mag_job_generate(mag_job* j, double eps, uint64_t seed) {
       hipFree(stage);
       return fail(MAG_ERR_NOMEM, "hipMalloc failed: %s", hipGetErrorString(e));
     }
-    mag::launch_gen_su3(stage, gs, j->g, eps, seed, j->st);
+    mag::launch_gen_su3(stage, gs, j->g, eps, seed, mag::gauge_copy_seed(seed, copy), j->st);
   } else {
-    mag::launch_gen_su2(stage, j->g, eps, seed, j->st);
+    mag::launch_gen_su2(stage, j->g, eps, seed, mag::gauge_copy_seed(seed, copy), j->st);
   }
   int r = upload_device_aos(j, stage);
   hipStreamSynchronize(j->st);
@@ -423,6 +411,7 @@ int mag_job_generate(mag_job* j, double eps, uint64_t seed) {
 int mag_job_sweeps(mag_job* j, int n) {
   if (!j || n < 0) return fail(MAG_ERR_ARG, "bad argument");
   if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  if (j->batch) return fail(MAG_ERR_STATE, "job is in a batch: sweep it through the batch");
   CHK(set_device(j));
   for (int i = 0; i < n; i++) enqueue_sweep(j);
   HIPCHK(hipGetLastError());
@@ -432,6 +421,7 @@ int mag_job_sweeps(mag_job* j, int n) {
 int mag_job_finalize(mag_job* j) {
   if (!j) return fail(MAG_ERR_ARG, "NULL job");
   if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  if (j->batch) return fail(MAG_ERR_STATE, "job is in a batch: sweep it through the batch");
   CHK(set_device(j));
   return finalize_mixed(j);
 }
@@ -439,6 +429,7 @@ int mag_job_finalize(mag_job* j) {
 int mag_job_run(mag_job* j) {
   if (!j) return fail(MAG_ERR_ARG, "NULL job");
   if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  if (j->batch) return fail(MAG_ERR_STATE, "job is in a batch: sweep it through the batch");
   CHK(set_device(j));
   CHK(run_until_done(j));
   if (mixed32(j)) {
@@ -544,6 +535,7 @@ int mag_job_plaquette(mag_job* j, double* P) {
 int mag_job_sweeps_timed(mag_job* j, int n, double* total_ms) {
   if (!j || n < 1 || !total_ms) return fail(MAG_ERR_ARG, "bad argument");
   if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  if (j->batch) return fail(MAG_ERR_STATE, "job is in a batch: sweep it through the batch");
   CHK(set_device(j));
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
@@ -564,6 +556,7 @@ int mag_job_sweeps_timed(mag_job* j, int n, double* total_ms) {
 int mag_job_halfsweep_timed(mag_job* j, int n, double* avg_ms) {
   if (!j || n < 1 || !avg_ms) return fail(MAG_ERR_ARG, "bad argument");
   if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  if (j->batch) return fail(MAG_ERR_STATE, "job is in a batch: sweep it through the batch");
   CHK(set_device(j));
   CHK(read_ctl(j));
   if (j->ctl_h->done) return fail(MAG_ERR_STATE, "job is done: its kernels would return at once");

@@ -11,6 +11,9 @@
 //   gauge:      SU(2) normalised N(0,1)^4; SU(3) Gram-Schmidt of a complex
 //               Gaussian 3x3 times det^(-1/3) (Haar, Mezzadri 2007)
 //   U'_x,mu = g(x) U_x,mu g(x+mu)^+ (the form of mag.cpp:1056-1062)
+// The gauge draws use their own key, gseed = seed ^ (copy * 0x9E3779B97F4A7C15):
+// copy 0 is the plain configuration, copies 1, 2, ... are gauge copies of the
+// same physical field (same link draws, different random gauge g).
 #include "mag_internal.h"
 #include "mag_math.h"
 
@@ -61,7 +64,8 @@ __device__ __forceinline__ Q<double> su2_gauge(uint64_t seed, uint32_t site) {
 }
 
 // thread per lexicographic site; writes the reference AoS layout
-__global__ void k_gen_su2(double* __restrict__ aos, Geom g, double eps, uint64_t seed) {
+__global__ void k_gen_su2(double* __restrict__ aos, Geom g, double eps, uint64_t seed,
+                          uint64_t gseed) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= 2 * g.Vh) return;
   uint32_t c[4];
@@ -71,7 +75,7 @@ __global__ void k_gen_su2(double* __restrict__ aos, Geom g, double eps, uint64_t
   c[2] = r % (uint32_t)g.L2; c[3] = r / (uint32_t)g.L2;
   const uint32_t L[4] = {(uint32_t)g.L0, (uint32_t)g.L1, (uint32_t)g.L2, (uint32_t)g.L3};
   const uint32_t str[4] = {1u, L[0], L[0] * L[1], L[0] * L[1] * L[2]};
-  Q<double> gx = su2_gauge(seed, s);
+  Q<double> gx = su2_gauge(gseed, s);
   for (int mu = 0; mu < 4; mu++) {
     uint32_t l = 4 * s + mu;
     double z[4];
@@ -84,7 +88,7 @@ __global__ void k_gen_su2(double* __restrict__ aos, Geom g, double eps, uint64_t
       U = Q<double>{cos(eps * an), sa * z[0], sa * z[1], sa * z[2]};
     }
     uint32_t f = (c[mu] + 1 == L[mu]) ? s - (L[mu] - 1) * str[mu] : s + str[mu];
-    Q<double> gf = su2_gauge(seed, f);
+    Q<double> gf = su2_gauge(gseed, f);
     Q<double> A = qmul_dag(qmul(gx, U), gf);
     double* o = aos + (size_t)l * 4;
     o[0] = A.a0; o[1] = A.a1; o[2] = A.a2; o[3] = A.a3;
@@ -273,15 +277,16 @@ __global__ void k_gen_su3_links(double* __restrict__ aos, const double* __restri
     }
 }
 
-void launch_gen_su2(double* aos, const Geom& g, double eps, uint64_t seed, hipStream_t st) {
+void launch_gen_su2(double* aos, const Geom& g, double eps, uint64_t seed, uint64_t gseed,
+                    hipStream_t st) {
   const uint32_t V = 2 * g.Vh;
-  hipLaunchKernelGGL(k_gen_su2, dim3((V + 127) / 128), dim3(128), 0, st, aos, g, eps, seed);
+  hipLaunchKernelGGL(k_gen_su2, dim3((V + 127) / 128), dim3(128), 0, st, aos, g, eps, seed, gseed);
 }
 void launch_gen_su3(double* aos, double* gauge_scratch, const Geom& g, double eps,
-                    uint64_t seed, hipStream_t st) {
+                    uint64_t seed, uint64_t gseed, hipStream_t st) {
   const uint32_t V = 2 * g.Vh;
   hipLaunchKernelGGL(k_gen_su3_gauge, dim3((V + 127) / 128), dim3(128), 0, st, gauge_scratch,
-                     V, seed);
+                     V, gseed);
   const uint32_t nl = 4 * V;
   hipLaunchKernelGGL(k_gen_su3_links, dim3((nl + 127) / 128), dim3(128), 0, st, aos,
                      gauge_scratch, g, eps, seed);

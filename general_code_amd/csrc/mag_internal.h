@@ -73,9 +73,44 @@ void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega,
                      double* part, const JobCtl* ctl, hipStream_t st);
 
 // per-sweep (or initial) reduction of the partials into the job history
+constexpr int kRedBlock = 1024;  // threads of the one-workgroup reduction
 void launch_reduce(const double* part, int nparts, int mode, double invF,
                    double invV, JobCtl* ctl, double* hist_F, double* hist_rmax,
                    double* hist_ravg, double* out3, hipStream_t st);
+
+// ---------------------------------------------------------------- batches
+// One job of a batched launch.  A batched half-sweep grid is the
+// concatenation of the member jobs' own grids (block0 = first workgroup of
+// the job); each workgroup runs exactly the code and the partition of the
+// job's single launch, so a job's results do not depend on its batch.
+struct BatchJob {
+  void* U;        // links of the sweep (FP32 working or FP64)
+  void* G;        // accumulated site gauge (FP32 phase) or nullptr
+  const double* U0;  // FP64 master (mixed) for the rebuild
+  Geom g;
+  double* part;
+  JobCtl* ctl;
+  double* hF;
+  double* hR;
+  double* hA;
+  double omega;
+  double invF;
+  double invV;
+  int nb;      // workgroups of one half-sweep of this job
+  int nparts;  // partial triples of a sweep (2*nb)
+  int block0;  // first workgroup of this job in a batched half-sweep grid
+  int rb0;     // first workgroup of this job in a batched rebuild grid
+};
+
+// batched half-sweep of one class (group, FP32 phase or FP64) table
+void launch_half_batch(int group, bool fp32, const BatchJob* jobs, int njobs,
+                       int nblocks_total, int parity, hipStream_t st);
+// batched end-of-sweep reduction: one workgroup per job
+void launch_reduce_batch(const BatchJob* jobs, int njobs, hipStream_t st);
+// batched FP64 rebuild of the FP32 working links + gauge normalisation of a
+// class table (mixed FP32 phase); jobs that are done are skipped
+void launch_rebuild_batch(int group, const BatchJob* jobs, int njobs, int nblocks_total,
+                          hipStream_t st);
 
 // functional partials of a whole configuration (both parities): 2*nblocks
 template <typename T>
@@ -101,21 +136,30 @@ void launch_gauge_identity(float* G, const Geom& g, int group, hipStream_t st);
 // Rebuild links from the FP64 master and the accumulated site gauge:
 // out = G(x) U0 G(x+mu)^+ with G normalised in FP64 (SU(2): q/|q|, SU(3):
 // Gram-Schmidt).  out32 (FP32 working links) or out64 (may alias U0).
+// Skipped when ctl->done (ctl = nullptr forces it: the final rebuild).
 void launch_su2_rebuild(const double* U0, const float* G, float* out32,
-                        double* out64, const Geom& g, hipStream_t st);
+                        double* out64, const Geom& g, const JobCtl* ctl, hipStream_t st);
 void launch_su3_rebuild(const double* U0, const float* G, float* out32,
-                        double* out64, const Geom& g, hipStream_t st);
-// G <- float(normalise_fp64(G))
-void launch_gauge_normalize(float* G, const Geom& g, int group, hipStream_t st);
+                        double* out64, const Geom& g, const JobCtl* ctl, hipStream_t st);
+// G <- float(normalise_fp64(G)); skipped when ctl->done
+void launch_gauge_normalize(float* G, const Geom& g, int group, const JobCtl* ctl,
+                            hipStream_t st);
+// workgroups of one rebuild / normalisation over both parities
+inline int rebuild_blocks_of(const Geom& g) { return 2 * (int)((g.Vh + 255) / 256); }
 // FP64 reunitarisation of links in place (SU(2) q/|q|, SU(3) Gram-Schmidt)
 void launch_su2_reunit64(double* U, const Geom& g, hipStream_t st);
 void launch_su3_reunit64(double* U, const Geom& g, hipStream_t st);
 
 // seeded synthetic configurations in the reference AoS layout (FP64);
 // SU(3) needs V*18 doubles of scratch for the site gauges
-void launch_gen_su2(double* aos, const Geom& g, double eps, uint64_t seed, hipStream_t st);
+void launch_gen_su2(double* aos, const Geom& g, double eps, uint64_t seed, uint64_t gseed,
+                    hipStream_t st);
 void launch_gen_su3(double* aos, double* gauge_scratch, const Geom& g, double eps,
-                    uint64_t seed, hipStream_t st);
+                    uint64_t seed, uint64_t gseed, hipStream_t st);
+// key of gauge copy `copy` of the configuration seeded by `seed` (copy 0 = seed)
+inline uint64_t gauge_copy_seed(uint64_t seed, uint32_t copy) {
+  return seed ^ ((uint64_t)copy * 0x9E3779B97F4A7C15ull);
+}
 
 // plaquette partials (sum over sites of the 6-plane mean), 2*nblocks triples
 void launch_su2_plaquette(const double* U, const Geom& g, double* part, hipStream_t st);

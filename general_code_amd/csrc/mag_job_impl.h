@@ -1,0 +1,44 @@
+// mag_job_impl.h -- host-side state of a MAG job (C ABI handle), shared by
+// the job API (mag_capi.hip) and the batch / serving scheduler
+// (mag_batch.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mag_gpu.h"
+#include "mag_internal.h"
+
+struct mag_job {
+  mag_params prm;
+  mag::Geom g;
+  int64_t V = 0;
+  int nr = 0;        // reals per link in the SoA layout (4 or 12)
+  int nr_in = 0;     // reals per link in the reference layout (4 or 18)
+  size_t nreal = 0;  // reals in one link array (8 * Vh * nr)
+  int nb = 0;        // workgroups per half-sweep
+  int nf = 0;        // workgroups per parity of the functional / plaquette kernels
+  hipStream_t st = nullptr;
+  double* U64 = nullptr;  // FP64 links (plain) / FP64 master then output (mixed)
+  float* W32 = nullptr;   // mixed: FP32 working links
+  float* G32 = nullptr;   // mixed: accumulated site gauge
+  double* part = nullptr;
+  mag::JobCtl* ctl = nullptr;
+  double* hF = nullptr;
+  double* hR = nullptr;
+  double* hA = nullptr;
+  double* out3 = nullptr;
+  mag::JobCtl* ctl_h = nullptr;  // pinned host mirrors
+  double* out3_h = nullptr;
+  bool loaded = false;
+  int phase = 0;  // mixed: 0 = FP32 sweeps, 1 = FP64 after rebuild
+  int fp64_start = -1;
+  int since_rebuild = 0;  // sweeps enqueued since the last FP64 rebuild
+  double invF = 0, invV = 0;
+  mag_batch* batch = nullptr;  // batch the job belongs to, if any
+};
+
+namespace mag {
+// job operations shared with the batch scheduler (mag_capi.hip)
+int job_read_ctl(mag_job* j);
+int job_finalize_mixed(mag_job* j);
+bool job_mixed32(const mag_job* j);
+}  // namespace mag

@@ -257,13 +257,14 @@ __device__ __forceinline__ M2<double> to_d(const M2<float>& m) {
   return r;
 }
 
-__global__ __launch_bounds__(256) void k_su2_rebuild(const double* __restrict__ U0,
-                                                     const float* __restrict__ G,
-                                                     float* __restrict__ out32,
-                                                     double* out64, Geom g) {
-  const int nb = (int)((g.Vh + 255) / 256);
-  const int p = blockIdx.x / nb;
-  const uint32_t h = (blockIdx.x % nb) * 256 + threadIdx.x;
+// workgroup lb in [0, 2*ceil(Vh/256)) of a rebuild over both parities
+__device__ __forceinline__ void su2_rebuild_block(const double* __restrict__ U0,
+                                                  const float* __restrict__ G,
+                                                  float* __restrict__ out32, double* out64,
+                                                  const Geom& g, uint32_t lb) {
+  const uint32_t nb = (g.Vh + 255) / 256;
+  const int p = (int)(lb / nb);
+  const uint32_t h = (lb % nb) * 256 + threadIdx.x;
   if (h >= g.Vh) return;
   SiteIdx s = decode(g, h, p);
   Q<double> gx = norm_q(site_ptr<4>(G, g, p, h));
@@ -288,13 +289,13 @@ __device__ __forceinline__ M3<double> full3(const M2<double>& m) {
   return M;
 }
 
-__global__ __launch_bounds__(256) void k_su3_rebuild(const double* __restrict__ U0,
-                                                     const float* __restrict__ G,
-                                                     float* __restrict__ out32,
-                                                     double* out64, Geom g) {
-  const int nb = (int)((g.Vh + 255) / 256);
-  const int p = blockIdx.x / nb;
-  const uint32_t h = (blockIdx.x % nb) * 256 + threadIdx.x;
+__device__ __forceinline__ void su3_rebuild_block(const double* __restrict__ U0,
+                                                  const float* __restrict__ G,
+                                                  float* __restrict__ out32, double* out64,
+                                                  const Geom& g, uint32_t lb) {
+  const uint32_t nb = (g.Vh + 255) / 256;
+  const int p = (int)(lb / nb);
+  const uint32_t h = (lb % nb) * 256 + threadIdx.x;
   if (h >= g.Vh) return;
   SiteIdx s = decode(g, h, p);
   M3<double> gx = gs3(to_d(ldm(site_ptr<12>(G, g, p, h))));
@@ -337,19 +338,38 @@ __global__ __launch_bounds__(256) void k_su3_rebuild(const double* __restrict__ 
   }
 }
 
-void launch_su2_rebuild(const double* U0, const float* G, float* out32, double* out64,
-                        const Geom& g, hipStream_t st) {
-  const int nb = (int)((g.Vh + 255) / 256);
-  hipLaunchKernelGGL(k_su2_rebuild, dim3(2 * nb), dim3(256), 0, st, U0, G, out32, out64, g);
+// ctl: skip when the job is done (the periodic rebuild of a finished job
+// must not move its links); nullptr forces the rebuild (final rebuild)
+__global__ __launch_bounds__(256) void k_su2_rebuild(const double* __restrict__ U0,
+                                                     const float* __restrict__ G,
+                                                     float* __restrict__ out32, double* out64,
+                                                     Geom g, const JobCtl* __restrict__ ctl) {
+  if (ctl && ctl->done) return;
+  su2_rebuild_block(U0, G, out32, out64, g, blockIdx.x);
 }
-void launch_su3_rebuild(const double* U0, const float* G, float* out32, double* out64,
-                        const Geom& g, hipStream_t st) {
-  const int nb = (int)((g.Vh + 255) / 256);
-  hipLaunchKernelGGL(k_su3_rebuild, dim3(2 * nb), dim3(256), 0, st, U0, G, out32, out64, g);
+__global__ __launch_bounds__(256) void k_su3_rebuild(const double* __restrict__ U0,
+                                                     const float* __restrict__ G,
+                                                     float* __restrict__ out32, double* out64,
+                                                     Geom g, const JobCtl* __restrict__ ctl) {
+  if (ctl && ctl->done) return;
+  su3_rebuild_block(U0, G, out32, out64, g, blockIdx.x);
 }
 
-__global__ void k_gauge_normalize(float* __restrict__ G, Geom g, int group) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+static inline int rebuild_blocks(const Geom& g) { return 2 * (int)((g.Vh + 255) / 256); }
+
+void launch_su2_rebuild(const double* U0, const float* G, float* out32, double* out64,
+                        const Geom& g, const JobCtl* ctl, hipStream_t st) {
+  hipLaunchKernelGGL(k_su2_rebuild, dim3(rebuild_blocks(g)), dim3(256), 0, st, U0, G, out32,
+                     out64, g, ctl);
+}
+void launch_su3_rebuild(const double* U0, const float* G, float* out32, double* out64,
+                        const Geom& g, const JobCtl* ctl, hipStream_t st) {
+  hipLaunchKernelGGL(k_su3_rebuild, dim3(rebuild_blocks(g)), dim3(256), 0, st, U0, G, out32,
+                     out64, g, ctl);
+}
+
+__device__ __forceinline__ void gauge_normalize_site(float* __restrict__ G, const Geom& g,
+                                                     int group, uint32_t s) {
   if (s >= 2 * g.Vh) return;
   const int par = (int)(s / g.Vh);
   const uint32_t h = s % g.Vh;
@@ -368,9 +388,53 @@ __global__ void k_gauge_normalize(float* __restrict__ G, Geom g, int group) {
     stm(p, o);
   }
 }
-void launch_gauge_normalize(float* G, const Geom& g, int group, hipStream_t st) {
-  const uint32_t n = 2 * g.Vh;
-  hipLaunchKernelGGL(k_gauge_normalize, dim3((n + 255) / 256), dim3(256), 0, st, G, g, group);
+__global__ void k_gauge_normalize(float* __restrict__ G, Geom g, int group,
+                                  const JobCtl* __restrict__ ctl) {
+  if (ctl && ctl->done) return;
+  gauge_normalize_site(G, g, group, blockIdx.x * blockDim.x + threadIdx.x);
+}
+void launch_gauge_normalize(float* G, const Geom& g, int group, const JobCtl* ctl,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_gauge_normalize, dim3(rebuild_blocks(g)), dim3(256), 0, st, G, g, group,
+                     ctl);
+}
+
+// batched rebuild + normalisation: workgroup b -> job via rb0 (ascending)
+__device__ __forceinline__ int batch_rjob(const BatchJob* __restrict__ jobs, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].rb0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+template <int GROUP>
+__global__ __launch_bounds__(256) void k_rebuild_batch(const BatchJob* __restrict__ jobs,
+                                                       int njobs) {
+  const BatchJob& j = jobs[batch_rjob(jobs, njobs, blockIdx.x)];
+  if (j.ctl->done) return;
+  if (GROUP == 2)
+    su2_rebuild_block(j.U0, (const float*)j.G, (float*)j.U, nullptr, j.g, blockIdx.x - j.rb0);
+  else
+    su3_rebuild_block(j.U0, (const float*)j.G, (float*)j.U, nullptr, j.g, blockIdx.x - j.rb0);
+}
+template <int GROUP>
+__global__ void k_normalize_batch(const BatchJob* __restrict__ jobs, int njobs) {
+  const BatchJob& j = jobs[batch_rjob(jobs, njobs, blockIdx.x)];
+  if (j.ctl->done) return;
+  gauge_normalize_site((float*)j.G, j.g, GROUP, (blockIdx.x - j.rb0) * blockDim.x + threadIdx.x);
+}
+void launch_rebuild_batch(int group, const BatchJob* jobs, int njobs, int nblocks_total,
+                          hipStream_t st) {
+  if (njobs == 0 || nblocks_total == 0) return;
+  if (group == 2) {
+    hipLaunchKernelGGL(k_rebuild_batch<2>, dim3(nblocks_total), dim3(256), 0, st, jobs, njobs);
+    hipLaunchKernelGGL(k_normalize_batch<2>, dim3(nblocks_total), dim3(256), 0, st, jobs, njobs);
+  } else {
+    hipLaunchKernelGGL(k_rebuild_batch<3>, dim3(nblocks_total), dim3(256), 0, st, jobs, njobs);
+    hipLaunchKernelGGL(k_normalize_batch<3>, dim3(nblocks_total), dim3(256), 0, st, jobs, njobs);
+  }
 }
 
 __global__ void k_su2_reunit64(double* __restrict__ U, Geom g) {

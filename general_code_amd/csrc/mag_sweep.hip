@@ -28,17 +28,14 @@
 namespace mag {
 
 // ================================================================== reduction
-constexpr int kRedBlock = 1024;
-
 // Reduce nparts partial triples (F sum, residual sum, residual max) and
-// apply the result to the job's control block.  Thread t combines partials
-// t, t + kRedBlock, ... in order, then the fixed-order workgroup tree, so
-// the result depends only on the job's partials.
-__global__ __launch_bounds__(kRedBlock) void k_reduce(const double* __restrict__ part,
-                                                      int nparts, int mode, double invF,
-                                                      double invV, JobCtl* ctl, double* hF,
-                                                      double* hR, double* hA, double* out3) {
-  if (mode == RED_SWEEP && ctl->done) return;
+// apply the result to the job's control block; one workgroup of kRedBlock
+// threads.  Thread t combines partials t, t + kRedBlock, ... in order, then
+// the fixed-order workgroup tree, so the result depends only on the job's
+// partials (alone or in a batch).
+__device__ void reduce_job(const double* __restrict__ part, int nparts, int mode, double invF,
+                           double invV, JobCtl* ctl, double* hF, double* hR, double* hA,
+                           double* out3) {
   double a = 0.0, b = 0.0, c = 0.0;
   int i = threadIdx.x;
   // four triples in flight per thread, combined in index order
@@ -90,11 +87,29 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce(const double* __restrict__
   }
 }
 
+__global__ __launch_bounds__(kRedBlock) void k_reduce(const double* __restrict__ part,
+                                                      int nparts, int mode, double invF,
+                                                      double invV, JobCtl* ctl, double* hF,
+                                                      double* hR, double* hA, double* out3) {
+  if (mode == RED_SWEEP && ctl->done) return;
+  reduce_job(part, nparts, mode, invF, invV, ctl, hF, hR, hA, out3);
+}
+
+// one workgroup per job of the table
+__global__ __launch_bounds__(kRedBlock) void k_reduce_batch(const BatchJob* __restrict__ jobs) {
+  const BatchJob& j = jobs[blockIdx.x];
+  if (j.ctl->done) return;
+  reduce_job(j.part, j.nparts, RED_SWEEP, j.invF, j.invV, j.ctl, j.hF, j.hR, j.hA, nullptr);
+}
+
 void launch_reduce(const double* part, int nparts, int mode, double invF, double invV,
                    JobCtl* ctl, double* hF, double* hR, double* hA, double* out3,
                    hipStream_t st) {
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedBlock), 0, st, part, nparts, mode, invF, invV,
                      ctl, hF, hR, hA, out3);
+}
+void launch_reduce_batch(const BatchJob* jobs, int njobs, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_batch, dim3(njobs), dim3(kRedBlock), 0, st, jobs);
 }
 
 // ================================================================== helpers
@@ -112,16 +127,27 @@ __device__ __forceinline__ void back_all(const Geom& g, uint32_t h, const SiteId
   for (int mu = 0; mu < 4; mu++) hb[mu] = back_h(g, h, s, mu);
 }
 
+// job of workgroup b in a batch table (block0 ascending)
+__device__ __forceinline__ int batch_job(const BatchJob* __restrict__ jobs, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // ================================================================== SU(2)
-// One site per lane: the 8 links of an su2 site fit comfortably in
-// registers (6 waves/SIMD in FP64), and measured faster than the lane pair.
+// One workgroup lb of a half-sweep over the sites of parity p, one site per
+// lane; the 8 links of an su2 site fit comfortably in registers (6
+// waves/SIMD in FP64), and measured faster than a lane pair.  The black
+// (p = 1) half also sums the functional of the links just written.
 template <typename T, bool GA>
-__global__ __launch_bounds__(kBlock) void k_su2_half(T* __restrict__ U, T* __restrict__ G,
-                                                     Geom g, int p, T omega,
-                                                     double* __restrict__ part,
-                                                     const JobCtl* __restrict__ ctl) {
-  if (ctl->done) return;
-  const uint32_t h = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void su2_half_block(T* __restrict__ U, T* __restrict__ G,
+                                               const Geom& g, int p, T omega,
+                                               double* __restrict__ part, uint32_t lb) {
+  const uint32_t h = lb * kBlock + threadIdx.x;
   double fs = 0.0, rs = 0.0, rm = 0.0;
   if (h < g.Vh) {
     const SiteIdx s = decode(g, h, p);
@@ -162,7 +188,29 @@ __global__ __launch_bounds__(kBlock) void k_su2_half(T* __restrict__ U, T* __res
     rm = (double)res;
   }
   block_reduce3<kWaves>(fs, rs, rm);
-  write_part(part, fs, rs, rm);
+  if (threadIdx.x == 0) {
+    part[3 * lb] = fs;
+    part[3 * lb + 1] = rs;
+    part[3 * lb + 2] = rm;
+  }
+}
+
+template <typename T, bool GA>
+__global__ __launch_bounds__(kBlock) void k_su2_half(T* __restrict__ U, T* __restrict__ G,
+                                                     Geom g, int p, T omega,
+                                                     double* __restrict__ part,
+                                                     const JobCtl* __restrict__ ctl) {
+  if (ctl->done) return;
+  su2_half_block<T, GA>(U, G, g, p, omega, part, blockIdx.x);
+}
+
+template <typename T, bool GA>
+__global__ __launch_bounds__(kBlock) void k_su2_half_batch(const BatchJob* __restrict__ jobs,
+                                                           int njobs, int p) {
+  const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
+  if (j.ctl->done) return;
+  su2_half_block<T, GA>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega, j.part + (size_t)3 * j.nb * p,
+                        blockIdx.x - j.block0);
 }
 
 template <typename T>
@@ -240,13 +288,13 @@ __device__ __forceinline__ T cm_subgroup(M2<T>* F, M2<T>* B, M2<T>& Gx, T omega)
   return res;
 }
 
+// One workgroup lb of an SU(3) half-sweep, two lanes per site: lane q of
+// the pair holds the forward and backward links of mu = 2q, 2q+1.
 template <typename T, bool GA>
-__global__ __launch_bounds__(kBlock) void k_su3_half(T* __restrict__ U, T* __restrict__ G,
-                                                     Geom g, int p, T omega,
-                                                     double* __restrict__ part,
-                                                     const JobCtl* __restrict__ ctl) {
-  if (ctl->done) return;
-  const uint32_t tid = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict__ G,
+                                               const Geom& g, int p, T omega,
+                                               double* __restrict__ part, uint32_t lb) {
+  const uint32_t tid = lb * kBlock + threadIdx.x;
   const uint32_t h = tid >> 1;
   const int q = (int)(tid & 1u);
   double fs = 0.0, rs = 0.0, rm = 0.0;
@@ -263,12 +311,12 @@ __global__ __launch_bounds__(kBlock) void k_su3_half(T* __restrict__ U, T* __res
       const uint32_t hbm = q ? hb[2 + k] : hb[k];
       pf[k] = link_ptr<12>(U, g, p, mu, h);
       pb[k] = link_ptr<12>(U, g, 1 - p, mu, hbm);
-      F[k] = ldm<T>(pf[k]);
-      B[k] = ldm<T>(pb[k]);
+      F[k] = ldm(pf[k]);
+      B[k] = ldm(pb[k]);
     }
     M2<T> Gx;
     const LRef<T> gp = site_ptr<12>(G, g, p, h);
-    if (GA) Gx = ldm<T>(gp);
+    if (GA) Gx = ldm(gp);
     T res = cm_subgroup<T, 0, 1, GA>(F, B, Gx, omega);
     res += cm_subgroup<T, 1, 2, GA>(F, B, Gx, omega);
     res += cm_subgroup<T, 0, 2, GA>(F, B, Gx, omega);
@@ -288,7 +336,29 @@ __global__ __launch_bounds__(kBlock) void k_su3_half(T* __restrict__ U, T* __res
     }
   }
   block_reduce3<kWaves>(fs, rs, rm);
-  write_part(part, fs, rs, rm);
+  if (threadIdx.x == 0) {
+    part[3 * lb] = fs;
+    part[3 * lb + 1] = rs;
+    part[3 * lb + 2] = rm;
+  }
+}
+
+template <typename T, bool GA>
+__global__ __launch_bounds__(kBlock) void k_su3_half(T* __restrict__ U, T* __restrict__ G,
+                                                     Geom g, int p, T omega,
+                                                     double* __restrict__ part,
+                                                     const JobCtl* __restrict__ ctl) {
+  if (ctl->done) return;
+  su3_half_block<T, GA>(U, G, g, p, omega, part, blockIdx.x);
+}
+
+template <typename T, bool GA>
+__global__ __launch_bounds__(kBlock) void k_su3_half_batch(const BatchJob* __restrict__ jobs,
+                                                           int njobs, int p) {
+  const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
+  if (j.ctl->done) return;
+  su3_half_block<T, GA>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega, j.part + (size_t)3 * j.nb * p,
+                        blockIdx.x - j.block0);
 }
 
 template <typename T>
@@ -307,5 +377,20 @@ template void launch_su3_half<float>(float*, float*, const Geom&, int, double, d
                                      const JobCtl*, hipStream_t);
 template void launch_su3_half<double>(double*, double*, const Geom&, int, double, double*,
                                       const JobCtl*, hipStream_t);
+
+// batched half-sweep over a class table (group, precision): every job of
+// the table has the same group and storage type; GA = mixed FP32 phase
+void launch_half_batch(int group, bool fp32, const BatchJob* jobs, int njobs, int nblocks_total,
+                       int parity, hipStream_t st) {
+  if (njobs == 0 || nblocks_total == 0) return;
+  const dim3 grid(nblocks_total), block(kBlock);
+  if (group == 2) {
+    if (fp32) hipLaunchKernelGGL((k_su2_half_batch<float, true>), grid, block, 0, st, jobs, njobs, parity);
+    else hipLaunchKernelGGL((k_su2_half_batch<double, false>), grid, block, 0, st, jobs, njobs, parity);
+  } else {
+    if (fp32) hipLaunchKernelGGL((k_su3_half_batch<float, true>), grid, block, 0, st, jobs, njobs, parity);
+    else hipLaunchKernelGGL((k_su3_half_batch<double, false>), grid, block, 0, st, jobs, njobs, parity);
+  }
+}
 
 }  // namespace mag

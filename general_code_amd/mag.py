@@ -73,6 +73,8 @@ SIGNATURES = {
     "mag_job_upload_links": (ctypes.c_int, [_vp, _dp]),
     "mag_job_upload_links_device": (ctypes.c_int, [_vp, _vp]),
     "mag_job_generate": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_uint64]),
+    "mag_job_generate_copy": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_uint64,
+                                             ctypes.c_uint32]),
     "mag_job_sweeps": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mag_job_run": (ctypes.c_int, [_vp]),
     "mag_job_finalize": (ctypes.c_int, [_vp]),
@@ -89,6 +91,18 @@ SIGNATURES = {
     "mag_su3_functional": (ctypes.c_int, [_ip, _dp, _dp]),
     "mag_plaquette": (ctypes.c_int, [ctypes.c_int, _ip, _dp, _dp]),
     "mag_hbm_copy_bandwidth": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, _dp]),
+    # batches
+    "mag_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "mag_batch_destroy": (ctypes.c_int, [_vp]),
+    "mag_batch_add": (ctypes.c_int, [_vp, _vp]),
+    "mag_batch_remove": (ctypes.c_int, [_vp, _vp]),
+    "mag_batch_size": (ctypes.c_int, [_vp, _ip]),
+    "mag_batch_replay_sweeps": (ctypes.c_int, [_vp, _ip]),
+    "mag_batch_replays": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mag_batch_poll": (ctypes.c_int, [_vp, _ip, _ip]),
+    "mag_batch_run": (ctypes.c_int, [_vp]),
+    "mag_batch_replays_timed": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "mag_batch_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     # spin form (reference mag.h surface)
     "mag_spin_session_create": (ctypes.c_int, [_ip, ctypes.c_int, ctypes.POINTER(_vp)]),
     "mag_spin_session_destroy": (ctypes.c_int, [_vp]),
@@ -251,8 +265,11 @@ class MagJob:
             raise ValueError(f"links must hold {4 * V * self.params.reals_in} reals, got {a.size}")
         _check(lib().mag_job_upload_links(self._h, _dptr(a)))
 
-    def generate(self, eps: float, seed: int):
-        _check(lib().mag_job_generate(self._h, float(eps), int(seed) & 0xFFFFFFFFFFFFFFFF))
+    def generate(self, eps: float, seed: int, copy: int = 0):
+        """Seeded synthetic configuration on the GPU; copy > 0 is a gauge copy
+        of it (same link draws, another random gauge)."""
+        _check(lib().mag_job_generate_copy(self._h, float(eps), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                           int(copy)))
 
     def download(self) -> np.ndarray:
         V = self.params.volume
@@ -470,3 +487,63 @@ def gauge_transform_spins(dims, links, spins) -> np.ndarray:
     _check(lib().mag_su2_gauge_transform_spins(_dims(dims), _dptr(out),
                                                _dptr(np.ascontiguousarray(spins, np.float64))))
     return out
+
+
+# batches -------------------------------------------------------------------
+class MagBatch:
+    """Many jobs swept together: batched launches per (group, storage) class,
+    the replay captured in a hipGraph (include/mag_gpu.h, mag_batch_*)."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(lib().mag_batch_create(int(device), ctypes.byref(h)))
+        self._h = h
+        self._jobs = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mag_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, job: "MagJob"):
+        _check(lib().mag_batch_add(self._h, job._h))
+        self._jobs.append(job)
+
+    def remove(self, job: "MagJob"):
+        _check(lib().mag_batch_remove(self._h, job._h))
+        self._jobs.remove(job)
+
+    @property
+    def replay_sweeps(self) -> int:
+        n = ctypes.c_int()
+        _check(lib().mag_batch_replay_sweeps(self._h, ctypes.byref(n)))
+        return n.value
+
+    def replays(self, n: int = 1):
+        _check(lib().mag_batch_replays(self._h, int(n)))
+
+    def poll(self):
+        r = ctypes.c_int()
+        f = ctypes.c_int()
+        _check(lib().mag_batch_poll(self._h, ctypes.byref(r), ctypes.byref(f)))
+        return r.value, f.value
+
+    def run(self):
+        _check(lib().mag_batch_run(self._h))
+
+    def replays_timed(self, n: int) -> float:
+        ms = ctypes.c_double()
+        _check(lib().mag_batch_replays_timed(self._h, int(n), ctypes.byref(ms)))
+        return ms.value

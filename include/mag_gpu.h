@@ -99,6 +99,12 @@ int mag_job_upload_links_device(mag_job *job, const double *dev_links);
  * sum_a c_a lambda_a / 2), c ~ N(0,1)^8; then a Haar random gauge
  * transformation, all from a Philox4x32-10 stream keyed by seed. */
 int mag_job_generate(mag_job *job, double eps, uint64_t seed);
+/* Gauge copy `copy` of that configuration: the same link draws under a
+ * different random gauge (gauge key seed ^ copy * 0x9E3779B97F4A7C15), so
+ * copies differ only by a gauge transformation; copy 0 = mag_job_generate.
+ * Stands for the reference's gauge-copy loop, where each copy starts the
+ * maximisation from new random spins (apps/mag_su2/mag_fixation.cpp:164-172). */
+int mag_job_generate_copy(mag_job *job, double eps, uint64_t seed, uint32_t copy);
 
 /* Enqueue exactly n sweeps (fewer take effect once the job is done). */
 int mag_job_sweeps(mag_job *job, int n);
@@ -135,6 +141,40 @@ int mag_job_sweeps_timed(mag_job *job, int n, double *total_ms);
  * sweeps are real and move the links, but are not entered in the job's
  * sweep history (no per-sweep reduction runs). */
 int mag_job_halfsweep_timed(mag_job *job, int n, double *avg_ms);
+
+/* ------------------------------------------------------------ batches
+ * No counterpart in the reference.  A batch sweeps many jobs (mixed SU(2) /
+ * SU(3), mixed sizes, FP64 or mixed precision) together: each half-sweep is
+ * one launch per (group, storage type) class whose grid concatenates the
+ * member jobs' grids, each sweep ends with one reduction launch (a workgroup
+ * per job), and the sequence of `mag_batch_replay_sweeps` sweeps is captured
+ * once in a hipGraph and replayed.  Every job runs exactly the code and the
+ * partition of its own single launch, so its results are bit-identical alone
+ * and in any batch.  Jobs that converge stop consuming work (their
+ * workgroups return at once); a mixed-precision job whose FP32 phase ends is
+ * rebuilt in FP64 at the next poll and continues in the FP64 class.
+ * Member jobs must not be swept through the mag_job_* calls meanwhile. */
+typedef struct mag_batch mag_batch;
+
+int mag_batch_create(int device, mag_batch **out);
+int mag_batch_destroy(mag_batch *b);
+/* Add a loaded job (same device; fresh, or at a multiple of its
+ * reunit_every sweeps).  All members share reunit_every. */
+int mag_batch_add(mag_batch *b, mag_job *job);
+int mag_batch_remove(mag_batch *b, mag_job *job);
+int mag_batch_size(mag_batch *b, int *n);
+/* sweeps per graph replay (default 20; a multiple of reunit_every) */
+int mag_batch_replay_sweeps(mag_batch *b, int *n);
+/* Enqueue n replays (n * replay_sweeps sweeps of every active member). */
+int mag_batch_replays(mag_batch *b, int n);
+/* Blocking: wait for the batch stream, move mixed jobs whose FP32 phase is
+ * done to FP64, report members still running and members finished. */
+int mag_batch_poll(mag_batch *b, int *n_running, int *n_finished);
+/* Blocking: replay until every member is finished. */
+int mag_batch_run(mag_batch *b);
+/* Measurement: n replays between HIP events on the batch stream. */
+int mag_batch_replays_timed(mag_batch *b, int n, double *ms);
+int mag_batch_stream(mag_batch *b, void **stream);
 
 /* ------------------------------------------------------------ spin form
  * The reference's own algorithm (lib/cpu/src/mag.cpp): one unit spin s(x)

@@ -527,15 +527,27 @@ static inline void gauss2(uint64_t seed, uint32_t index, uint32_t draw,
 /* U = exp(i eps a.sigma) = (cos(eps|a|), sin(eps|a|) a_hat), a ~ N(0,1)^3,
  * then U'_xmu = g(x) U_xmu g(x+mu)^+ with Haar g (normalised N(0,1)^4),
  * the transformation form of mag.cpp:1056-1062. */
+/* key of gauge copy `copy` of the configuration seeded by `seed`: the same
+ * link draws under a different random gauge (copy 0 = the configuration) */
+static uint64_t gauge_copy_seed(uint64_t seed, uint32_t copy) {
+  return seed ^ ((uint64_t)copy * 0x9E3779B97F4A7C15ull);
+}
+
 void or_gen_su2_config(double *links, const int *dims, double eps,
                        uint64_t seed) {
+  or_gen_su2_config_copy(links, dims, eps, seed, 0);
+}
+
+void or_gen_su2_config_copy(double *links, const int *dims, double eps,
+                            uint64_t seed, uint32_t copy) {
   geom_t g = geom(dims);
+  const uint64_t gseed = gauge_copy_seed(seed, copy);
   double *gauge = (double *)malloc(sizeof(double) * 4 * (size_t)g.V);
 #pragma omp parallel for schedule(static)
   for (int64_t s = 0; s < g.V; s++) {
     double z[4];
-    gauss2(seed, (uint32_t)s, 0, TAG_GAUGE, z);
-    gauss2(seed, (uint32_t)s, 1, TAG_GAUGE, z + 2);
+    gauss2(gseed, (uint32_t)s, 0, TAG_GAUGE, z);
+    gauss2(gseed, (uint32_t)s, 1, TAG_GAUGE, z + 2);
     double n = sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2] + z[3] * z[3]);
     for (int k = 0; k < 4; k++) gauge[4 * s + k] = z[k] / n;
   }
@@ -934,10 +946,16 @@ static void haar_su3(uint64_t seed, uint32_t site, m3 *G) {
 
 void or_gen_su3_config(double *links18, const int *dims, double eps,
                        uint64_t seed) {
+  or_gen_su3_config_copy(links18, dims, eps, seed, 0);
+}
+
+void or_gen_su3_config_copy(double *links18, const int *dims, double eps,
+                            uint64_t seed, uint32_t copy) {
   geom_t g = geom(dims);
+  const uint64_t gseed = gauge_copy_seed(seed, copy);
   m3 *gauge = (m3 *)malloc(sizeof(m3) * (size_t)g.V);
 #pragma omp parallel for schedule(static)
-  for (int64_t s = 0; s < g.V; s++) haar_su3(seed, (uint32_t)s, gauge + s);
+  for (int64_t s = 0; s < g.V; s++) haar_su3(gseed, (uint32_t)s, gauge + s);
 #pragma omp parallel for schedule(static)
   for (int64_t s = 0; s < g.V; s++) {
     int c[4];

@@ -111,6 +111,11 @@ double or_su3_wilson_loop(const double *links18, const int *dims, int R, int T);
 void or_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 void or_gen_su2_config(double *links, const int *dims, double eps, uint64_t seed);
 void or_gen_su3_config(double *links18, const int *dims, double eps, uint64_t seed);
+/* gauge copy `copy`: same link draws, gauge key seed ^ copy * 0x9E3779B97F4A7C15 */
+void or_gen_su2_config_copy(double *links, const int *dims, double eps, uint64_t seed,
+                            uint32_t copy);
+void or_gen_su3_config_copy(double *links18, const int *dims, double eps, uint64_t seed,
+                            uint32_t copy);
 void or_gen_su3_exp(const double *c8, double eps, double *m18);
 void or_gen_spins_uniform(double *spins, const int *dims, uint64_t seed);   /* mag.cpp:143-164,166-236 */
 

@@ -82,6 +82,10 @@ def lib():
             "or_philox4x32_10": (None, [_u32p, _u32p, _u32p]),
             "or_gen_su2_config": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64]),
             "or_gen_su3_config": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64]),
+            "or_gen_su2_config_copy": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64,
+                                              ctypes.c_uint32]),
+            "or_gen_su3_config_copy": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64,
+                                              ctypes.c_uint32]),
             "or_gen_su3_exp": (None, [_dp, ctypes.c_double, _dp]),
             "or_gen_spins_uniform": (None, [_dp, _ip, ctypes.c_uint64]),
             "or_set_threads": (None, [ctypes.c_int]),
@@ -316,17 +320,19 @@ def philox(ctr, key):
     return list(o)
 
 
-def gen_su2_config(dims, eps, seed):
+def gen_su2_config(dims, eps, seed, copy=0):
+    """Seeded synthetic SU(2) links; copy > 0 gives a gauge copy (same link
+    draws, another random gauge)."""
     V = volume(dims)
     links = np.zeros((4 * V, 4))
-    lib().or_gen_su2_config(_d(links), _dims(dims), float(eps), int(seed))
+    lib().or_gen_su2_config_copy(_d(links), _dims(dims), float(eps), int(seed), int(copy))
     return links
 
 
-def gen_su3_config(dims, eps, seed):
+def gen_su3_config(dims, eps, seed, copy=0):
     V = volume(dims)
     links = np.zeros((4 * V, 18))
-    lib().or_gen_su3_config(_d(links), _dims(dims), float(eps), int(seed))
+    lib().or_gen_su3_config_copy(_d(links), _dims(dims), float(eps), int(seed), int(copy))
     return links
 
 

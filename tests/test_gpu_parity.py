@@ -38,6 +38,17 @@ def test_su3_generator_matches_oracle(mag, oracle):
     assert np.max(np.abs(got - ref)) < 1e-13
 
 
+@pytest.mark.parametrize("group,copy", [(2, 1), (2, 3), (3, 2)])
+def test_gauge_copy_generator_matches_oracle(mag, oracle, group, copy):
+    dims = (4, 4, 6, 4)
+    gen = oracle.gen_su2_config if group == 2 else oracle.gen_su3_config
+    ref = gen(dims, 0.4, 5150, copy=copy)
+    with mag.MagJob(group=group, dims=dims) as job:
+        job.generate(0.4, 5150, copy=copy)
+        got = job.download()
+    assert np.max(np.abs(got - ref)) < 1e-13
+
+
 # omega = 2 (pure reflection) conserves F and is not contracting: rounding
 # differences grow about 3x per sweep, so the trajectory is compared over a
 # few sweeps only (F conservation is checked over many below).

@@ -224,3 +224,24 @@ def test_heat_bath_distribution(oracle):
     z = np.sum(x - lang) / np.sqrt(np.sum(var))
     assert abs(z) < 5
     assert np.max(np.abs(np.linalg.norm(spins, axis=1) - 1)) < 1e-13
+
+
+# ---------------------------------------------------------------- gauge copies
+@pytest.mark.parametrize("group", [2, 3])
+def test_gauge_copies_are_gauge_equivalent(oracle, group):
+    """Copy 0 is the plain configuration; copy k > 0 differs link by link but
+    holds the same gauge-invariant content (plaquette, Wilson loops)."""
+    dims = (4, 4, 4, 6)
+    gen = oracle.gen_su2_config if group == 2 else oracle.gen_su3_config
+    plaq = oracle.su2_plaquette if group == 2 else oracle.su3_plaquette
+    wl = oracle.su2_wilson_loop if group == 2 else oracle.su3_wilson_loop
+    base = gen(dims, 0.5, 91)
+    assert np.array_equal(gen(dims, 0.5, 91, copy=0), base)
+    c1 = gen(dims, 0.5, 91, copy=1)
+    c2 = gen(dims, 0.5, 91, copy=2)
+    assert np.max(np.abs(c1 - base)) > 0.1 and np.max(np.abs(c2 - c1)) > 0.1
+    for c in (c1, c2):
+        assert abs(plaq(c, dims) - plaq(base, dims)) < 1e-13
+        assert abs(wl(c, dims, 2, 3) - wl(base, dims, 2, 3)) < 1e-13
+    # a different seed is a different field, not a gauge copy
+    assert abs(plaq(gen(dims, 0.5, 92), dims) - plaq(base, dims)) > 1e-6
