@@ -1,9 +1,10 @@
 // mag_batch.hip -- batched, graph-captured sweeps over many MAG jobs.
 //
-// The active members are split into four classes (SU(2) / SU(3) x FP32
-// working links of the mixed phase / FP64 links).  One sweep of the batch is
+// The active members are split into five classes (SU(2) / SU(3) x FP32
+// working links of the mixed phase / FP64 links, and the spin-form gauge
+// copy sets).  One sweep of the batch is
 //   for parity 0, 1: one half-sweep launch per non-empty class
-//   one reduction launch (a workgroup per active job)
+//   one reduction launch (a workgroup per active job, per copy of a copy set)
 //   every reunit_every sweeps: one FP64 rebuild + one normalisation launch
 //   per FP32 class
 // and `replay` sweeps (default 20) are captured in one hipGraph that is
@@ -24,6 +25,9 @@ using mag::BatchJob;
 using mag::fail;
 
 namespace {
+
+constexpr int kClasses = 5;  // 0..3: link form (SU(2)/SU(3) x FP32/FP64), 4: spin form
+constexpr int kSpinClass = 4;
 
 struct ClassTable {
   int group = 2;
@@ -57,7 +61,7 @@ struct mag_batch {
   std::vector<mag_job*> active;  // members not finished
   int replay = 20;
   int reunit = 0;  // shared reunit_every of mixed members (0: none yet)
-  ClassTable cls[4];
+  ClassTable cls[kClasses];
   std::vector<BatchJob> all;
   BatchJob* d_all = nullptr;
   size_t cap_all = 0;
@@ -69,6 +73,7 @@ struct mag_batch {
 namespace {
 
 int class_of(const mag_job* j) {
+  if (j->spin) return kSpinClass;
   return (j->prm.group == MAG_SU2 ? 0 : 2) + (mag::job_mixed32(j) ? 0 : 1);
 }
 
@@ -91,6 +96,10 @@ BatchJob describe(mag_job* j) {
   b.nparts = 2 * j->nb;
   b.block0 = 0;
   b.rb0 = 0;
+  b.S = j->S;
+  b.sstride = j->sstride;
+  b.pstride = j->pstride;
+  b.ncopies = j->spin ? j->ncopies : 0;
   return b;
 }
 
@@ -106,9 +115,9 @@ int refresh(mag_batch* b) {
   if (!b->dirty) return MAG_OK;
   drop_graph(b);
   b->all.clear();
-  for (int c = 0; c < 4; c++) {
-    b->cls[c].group = c < 2 ? 2 : 3;
-    b->cls[c].fp32 = (c % 2) == 0;
+  for (int c = 0; c < kClasses; c++) {
+    b->cls[c].group = (c < 2 || c == kSpinClass) ? 2 : 3;
+    b->cls[c].fp32 = c != kSpinClass && (c % 2) == 0;
     b->cls[c].h.clear();
     b->cls[c].nblocks = 0;
     b->cls[c].nrblocks = 0;
@@ -121,19 +130,36 @@ int refresh(mag_batch* b) {
     t.nblocks += d.nb;
     t.nrblocks += mag::rebuild_blocks_of(j->g);
     t.h.push_back(d);
-    b->all.push_back(d);
+    if (!j->spin) {
+      b->all.push_back(d);
+    } else {
+      // one reduction entry per copy
+      for (int c = 0; c < j->ncopies; c++) {
+        BatchJob e = d;
+        const size_t hs = (size_t)c * j->prm.max_sweeps;
+        e.part = j->part + (size_t)c * j->pstride;
+        e.ctl = j->ctl + c;
+        e.hF = j->hF + hs;
+        e.hR = j->hR + hs;
+        e.hA = j->hA + hs;
+        b->all.push_back(e);
+      }
+    }
   }
-  for (int c = 0; c < 4; c++) CHK(upload_table(b->cls[c].h, &b->cls[c].d, &b->cls[c].cap));
+  for (int c = 0; c < kClasses; c++) CHK(upload_table(b->cls[c].h, &b->cls[c].d, &b->cls[c].cap));
   CHK(upload_table(b->all, &b->d_all, &b->cap_all));
   b->dirty = false;
   if (b->active.empty()) return MAG_OK;
   HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < b->replay; i++) {
-    for (int p = 0; p < 2; p++)
+    for (int p = 0; p < 2; p++) {
       for (int c = 0; c < 4; c++) {
         const ClassTable& t = b->cls[c];
         mag::launch_half_batch(t.group, t.fp32, t.d, (int)t.h.size(), t.nblocks, p, b->st);
       }
+      const ClassTable& t = b->cls[kSpinClass];
+      mag::launch_copies_half_batch(t.d, (int)t.h.size(), t.nblocks, p, b->st);
+    }
     mag::launch_reduce_batch(b->d_all, (int)b->all.size(), b->st);
     if (b->reunit > 0 && (i + 1) % b->reunit == 0)
       for (int c = 0; c < 4; c += 2) {
@@ -147,7 +173,7 @@ int refresh(mag_batch* b) {
   return MAG_OK;
 }
 
-bool finished(const mag_job* j) { return j->ctl_h->done && !mag::job_mixed32(j); }
+bool finished(const mag_job* j) { return mag::job_done(j) && !mag::job_mixed32(j); }
 
 }  // namespace
 
@@ -175,7 +201,7 @@ int mag_batch_destroy(mag_batch* b) {
   hipStreamSynchronize(b->st);
   drop_graph(b);
   for (mag_job* j : b->members) j->batch = nullptr;
-  for (int c = 0; c < 4; c++) hipFree(b->cls[c].d);
+  for (int c = 0; c < kClasses; c++) hipFree(b->cls[c].d);
   hipFree(b->d_all);
   hipStreamDestroy(b->st);
   delete b;
@@ -260,7 +286,8 @@ int mag_batch_poll(mag_batch* b, int* n_running, int* n_finished) {
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->st));
   for (mag_job* j : b->active)
-    HIPCHK(hipMemcpyAsync(j->ctl_h, j->ctl, sizeof(mag::JobCtl), hipMemcpyDeviceToHost, b->st));
+    HIPCHK(hipMemcpyAsync(j->ctl_h, j->ctl, j->ncopies * sizeof(mag::JobCtl),
+                          hipMemcpyDeviceToHost, b->st));
   HIPCHK(hipStreamSynchronize(b->st));
   std::vector<mag_job*> still;
   for (mag_job* j : b->active) {
@@ -268,6 +295,9 @@ int mag_batch_poll(mag_batch* b, int* n_running, int* n_finished) {
       CHK(mag::job_finalize_mixed(j));  // FP32 phase over: FP64 rebuild, FP64 class
       b->dirty = true;
     }
+    bool switched = false;
+    CHK(mag::job_maybe_start_fp32(j, &switched));  // FP64 warm-up over: FP32 class
+    if (switched) b->dirty = true;
     if (finished(j)) b->dirty = true;
     else still.push_back(j);
   }

@@ -5,7 +5,8 @@
 // device-resident flags (no host synchronisation per sweep), and the
 // mixed-precision schedule (FP32 sweeps with an accumulated site gauge,
 // FP64 rebuild from the FP64 master every reunit_every sweeps, FP64 rebuild
-// and FP64 polish at the end).
+// and FP64 polish at the end; optionally after an FP64 warm-up), and the
+// spin form with gauge copies (read-only links, one spin field per copy).
 #include "mag_gpu.h"
 #include "mag_capi_util.h"
 #include "mag_internal.h"
@@ -69,6 +70,18 @@ int check_params(const mag_params* p) {
   if (p->precision == MAG_PREC_MIXED) {
     if (p->reunit_every < 1) return fail(MAG_ERR_ARG, "reunit_every must be >= 1");
     if (!(p->tol_mixed >= 0.0)) return fail(MAG_ERR_ARG, "tol_mixed must be >= 0");
+    if (p->fp64_warmup < 0 || p->fp64_warmup % kChunk != 0)
+      return fail(MAG_ERR_ARG, "fp64_warmup must be a non-negative multiple of %d", kChunk);
+  }
+  if (p->form == MAG_FORM_LINK) {
+    if (p->copies > 1) return fail(MAG_ERR_ARG, "the link form holds one copy; use MAG_FORM_SPIN");
+  } else if (p->form == MAG_FORM_SPIN) {
+    if (p->group != MAG_SU2) return fail(MAG_ERR_ARG, "the spin form is SU(2) only");
+    if (p->precision != MAG_PREC_F64) return fail(MAG_ERR_ARG, "the spin form runs in FP64");
+    if (p->copies < 1 || p->copies > MAG_MAX_COPIES)
+      return fail(MAG_ERR_ARG, "copies must be in 1..%d, got %d", MAG_MAX_COPIES, p->copies);
+  } else {
+    return fail(MAG_ERR_ARG, "form must be MAG_FORM_LINK or MAG_FORM_SPIN");
   }
   int64_t V;
   return check_dims(p->dims, &V);
@@ -92,7 +105,10 @@ int set_device(const mag_job* j) {
 }
 
 void launch_half(mag_job* j, int parity) {
-  if (mixed32(j)) {
+  if (j->spin) {
+    mag::launch_su2_copies_half(j->U64, j->S, j->sstride, j->ncopies, j->g, parity,
+                                j->prm.omega, j->part, j->pstride, j->ctl, j->st);
+  } else if (mixed32(j)) {
     if (j->prm.group == MAG_SU2)
       mag::launch_su2_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, j->part, j->ctl, j->st);
     else
@@ -120,6 +136,11 @@ void launch_rebuild(mag_job* j, bool final64) {
 
 // sweep tail shared by the plain and timed paths
 void enqueue_after_sweep(mag_job* j) {
+  if (j->spin) {
+    mag::launch_reduce_copies(j->part, j->pstride, 2 * j->nb, j->ncopies, j->invF, j->invV,
+                              j->ctl, j->hF, j->hR, j->hA, (size_t)j->prm.max_sweeps, j->st);
+    return;
+  }
   mag::launch_reduce(j->part, 2 * j->nb, mag::RED_SWEEP, j->invF, j->invV, j->ctl, j->hF,
                      j->hR, j->hA, nullptr, j->st);
   if (mixed32(j) && ++j->since_rebuild >= j->prm.reunit_every) {
@@ -136,7 +157,9 @@ void enqueue_sweep(mag_job* j) {
 
 // functional partials of the current working links
 void launch_functional(mag_job* j) {
-  if (mixed32(j)) {
+  if (j->spin) {
+    mag::launch_spin_functional(j->U64, j->S, j->g, j->part, nullptr, j->st);
+  } else if (mixed32(j)) {
     if (j->prm.group == MAG_SU2) mag::launch_su2_functional<float>(j->W32, j->g, j->part, j->st);
     else mag::launch_su3_functional<float>(j->W32, j->g, j->part, j->st);
   } else {
@@ -146,50 +169,101 @@ void launch_functional(mag_job* j) {
 }
 
 int read_ctl(mag_job* j) {
-  HIPCHK(hipMemcpyAsync(j->ctl_h, j->ctl, sizeof(JobCtl), hipMemcpyDeviceToHost, j->st));
+  HIPCHK(hipMemcpyAsync(j->ctl_h, j->ctl, j->ncopies * sizeof(JobCtl), hipMemcpyDeviceToHost,
+                        j->st));
   HIPCHK(hipStreamSynchronize(j->st));
   return MAG_OK;
 }
 
 int write_ctl(mag_job* j) {
-  HIPCHK(hipMemcpyAsync(j->ctl, j->ctl_h, sizeof(JobCtl), hipMemcpyHostToDevice, j->st));
+  HIPCHK(hipMemcpyAsync(j->ctl, j->ctl_h, j->ncopies * sizeof(JobCtl), hipMemcpyHostToDevice,
+                        j->st));
   HIPCHK(hipStreamSynchronize(j->st));
   return MAG_OK;
 }
 
+bool all_done(const mag_job* j) {
+  for (int c = 0; c < j->ncopies; c++)
+    if (!j->ctl_h[c].done) return false;
+  return true;
+}
+bool all_converged(const mag_job* j) {
+  for (int c = 0; c < j->ncopies; c++)
+    if (!j->ctl_h[c].converged) return false;
+  return true;
+}
+
+// initial control block and functional F0 of copy c (spin form) or of the
+// job (link form); links (and spins) already in place
+void init_copy(mag_job* j, int c) {
+  JobCtl& h = j->ctl_h[c];
+  std::memset(&h, 0, sizeof(JobCtl));
+  h.max_sweeps = j->prm.max_sweeps;
+  h.tol = (j->prm.precision == MAG_PREC_MIXED && j->phase == 0) ? j->prm.tol_mixed : j->prm.tol;
+  hipMemcpyAsync(j->ctl + c, &h, sizeof(JobCtl), hipMemcpyHostToDevice, j->st);
+  double* part = j->part + (size_t)c * j->pstride;
+  if (j->spin) mag::launch_spin_functional(j->U64, j->S + (size_t)c * j->sstride, j->g, part,
+                                           nullptr, j->st);
+  else if (j->prm.group == MAG_SU2) mag::launch_su2_functional<double>(j->U64, j->g, part, j->st);
+  else mag::launch_su3_functional<double>(j->U64, j->g, part, j->st);
+  const size_t hs = (size_t)c * j->prm.max_sweeps;
+  mag::launch_reduce(part, 2 * j->nf, mag::RED_INIT, j->invF, j->invV, j->ctl + c, j->hF + hs,
+                     j->hR + hs, j->hA + hs, nullptr, j->st);
+}
+
+// mixed with FP64 warm-up: FP32 working links from the FP64 links, identity
+// gauge, FP32 tolerance (ctl_h current)
+int start_fp32_phase(mag_job* j) {
+  mag::launch_links_f64_to_f32(j->U64, j->W32, j->g, j->prm.group, j->st);
+  mag::launch_gauge_identity(j->G32, j->g, j->prm.group, j->st);
+  j->phase = 0;
+  j->since_rebuild = 0;
+  j->ctl_h->tol = j->prm.tol_mixed;
+  return write_ctl(j);
+}
+
+int maybe_start_fp32(mag_job* j, bool* switched) {
+  *switched = false;
+  if (j->prm.precision != MAG_PREC_MIXED || j->phase != -1 || j->ctl_h->done) return MAG_OK;
+  if (j->ctl_h->sweep < j->prm.fp64_warmup) return MAG_OK;
+  *switched = true;
+  return start_fp32_phase(j);
+}
+
 // after the configuration is in U64 (SoA FP64): reset the job state
 int reset_after_upload(mag_job* j) {
-  if (j->prm.precision == MAG_PREC_MIXED) {
+  const bool mixed = j->prm.precision == MAG_PREC_MIXED;
+  j->phase = (mixed && j->prm.fp64_warmup > 0) ? -1 : 0;
+  if (mixed && j->phase == 0) {
     mag::launch_links_f64_to_f32(j->U64, j->W32, j->g, j->prm.group, j->st);
     mag::launch_gauge_identity(j->G32, j->g, j->prm.group, j->st);
   }
-  j->phase = 0;
-  j->fp64_start = (j->prm.precision == MAG_PREC_MIXED) ? -1 : 0;
+  j->fp64_start = mixed ? -1 : 0;
   j->since_rebuild = 0;
-  std::memset(j->ctl_h, 0, sizeof(JobCtl));
-  j->ctl_h->max_sweeps = j->prm.max_sweeps;
-  j->ctl_h->tol = (j->prm.precision == MAG_PREC_MIXED) ? j->prm.tol_mixed : j->prm.tol;
-  HIPCHK(hipMemcpyAsync(j->ctl, j->ctl_h, sizeof(JobCtl), hipMemcpyHostToDevice, j->st));
-  // initial functional from the FP64 links
-  if (j->prm.group == MAG_SU2) mag::launch_su2_functional<double>(j->U64, j->g, j->part, j->st);
-  else mag::launch_su3_functional<double>(j->U64, j->g, j->part, j->st);
-  mag::launch_reduce(j->part, 2 * j->nf, mag::RED_INIT, j->invF, j->invV, j->ctl, j->hF, j->hR,
-                     j->hA, nullptr, j->st);
+  // initial functional from the FP64 links (and each copy's spins)
+  for (int c = 0; c < j->ncopies; c++) init_copy(j, c);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(j->st));
   j->loaded = true;
   return MAG_OK;
 }
 
-int upload_device_aos(mag_job* j, const double* dev) {
+// copy_seed: spin form, start the copies from the gauge copies of the
+// synthetic configuration of that seed (nullptr: every copy at e3)
+int upload_device_aos(mag_job* j, const double* dev, const uint64_t* copy_seed = nullptr) {
   if (j->prm.group == MAG_SU2) mag::launch_su2_aos_to_soa<double>(dev, j->U64, j->g, j->st);
   else mag::launch_su3_aos_to_soa<double>(dev, j->U64, j->g, j->st);
+  if (j->spin) {
+    if (copy_seed) mag::launch_gen_copy_spins(j->S, j->sstride, j->ncopies, j->g, *copy_seed, j->st);
+    else mag::launch_spins_e3(j->S, j->sstride * j->ncopies, j->g, j->st);
+  }
   return reset_after_upload(j);
 }
 
 void free_job(mag_job* j) {
   if (!j) return;
   hipFree(j->U64);
+  hipFree(j->S);
   hipFree(j->W32);
   hipFree(j->G32);
   hipFree(j->part);
@@ -226,7 +300,9 @@ int finalize_mixed(mag_job* j) {
 int run_until_done(mag_job* j) {
   for (;;) {
     CHK(read_ctl(j));
-    if (j->ctl_h->done) return MAG_OK;
+    if (all_done(j)) return MAG_OK;
+    bool sw;
+    CHK(maybe_start_fp32(j, &sw));
     for (int i = 0; i < kChunk; i++) enqueue_sweep(j);
     HIPCHK(hipGetLastError());
   }
@@ -237,6 +313,8 @@ int run_until_done(mag_job* j) {
 int mag::job_read_ctl(mag_job* j) { return read_ctl(j); }
 int mag::job_finalize_mixed(mag_job* j) { return finalize_mixed(j); }
 bool mag::job_mixed32(const mag_job* j) { return mixed32(j); }
+bool mag::job_done(const mag_job* j) { return all_done(j); }
+int mag::job_maybe_start_fp32(mag_job* j, bool* switched) { return maybe_start_fp32(j, switched); }
 
 // ====================================================================== API
 extern "C" {
@@ -253,6 +331,9 @@ void mag_params_default(mag_params* p) {
   p->max_sweeps = 1000;
   p->reunit_every = 10;
   p->tol_mixed = 1e-9;
+  p->form = MAG_FORM_LINK;
+  p->copies = 1;
+  p->fp64_warmup = 0;
 }
 
 const char* mag_last_error(void) { return g_err.c_str(); }
@@ -269,10 +350,12 @@ int mag_job_footprint(const mag_params* p, uint64_t* bytes) {
   int64_t V = 1;
   for (int i = 0; i < 4; i++) V *= p->dims[i];
   const uint64_t nr = p->group == MAG_SU2 ? 4 : 12;
+  const uint64_t nc = p->form == MAG_FORM_SPIN ? (uint64_t)p->copies : 1;
   uint64_t b = (uint64_t)V * 4 * nr * 8;  // FP64 links / master
   if (p->precision == MAG_PREC_MIXED) b += (uint64_t)V * 4 * nr * 4 + (uint64_t)V * nr * 4;
+  if (p->form == MAG_FORM_SPIN) b += nc * (uint64_t)V * 3 * 8;  // spins
   const uint64_t nb = (uint64_t)((V + mag::kBlock - 1) / mag::kBlock);  // sweep workgroups
-  b += nb * 2 * 3 * 8 + (uint64_t)p->max_sweeps * 3 * 8 + sizeof(JobCtl) + 3 * 8;
+  b += nc * (nb * 2 * 3 * 8 + (uint64_t)p->max_sweeps * 3 * 8 + sizeof(JobCtl)) + 3 * 8;
   *bytes = b;
   return MAG_OK;
 }
@@ -298,6 +381,10 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   j->nf = mag::nblocks(j->g);
   j->invF = 1.0 / (double)(j->V * (p->group == MAG_SU2 ? 4 : 12));
   j->invV = 1.0 / (double)j->V;
+  j->spin = p->form == MAG_FORM_SPIN;
+  j->ncopies = j->spin ? p->copies : 1;
+  j->sstride = (size_t)6 * j->g.Vh;
+  j->pstride = (size_t)2 * j->nb * 3;
 #define ALLOC(ptr, bytes)                                                          \
   do {                                                                             \
     hipError_t e_ = hipMalloc((void**)&(ptr), (bytes));                            \
@@ -317,21 +404,22 @@ int mag_job_create(const mag_params* p, mag_job** out) {
     ALLOC(j->W32, j->nreal * sizeof(float));
     ALLOC(j->G32, (size_t)2 * j->g.Vh * j->nr * sizeof(float));
   }
-  ALLOC(j->part, (size_t)2 * j->nb * 3 * sizeof(double));
-  ALLOC(j->ctl, sizeof(JobCtl));
-  ALLOC(j->hF, (size_t)p->max_sweeps * sizeof(double));
-  ALLOC(j->hR, (size_t)p->max_sweeps * sizeof(double));
-  ALLOC(j->hA, (size_t)p->max_sweeps * sizeof(double));
+  if (j->spin) ALLOC(j->S, j->sstride * j->ncopies * sizeof(double));
+  ALLOC(j->part, j->pstride * j->ncopies * sizeof(double));
+  ALLOC(j->ctl, j->ncopies * sizeof(JobCtl));
+  ALLOC(j->hF, (size_t)p->max_sweeps * j->ncopies * sizeof(double));
+  ALLOC(j->hR, (size_t)p->max_sweeps * j->ncopies * sizeof(double));
+  ALLOC(j->hA, (size_t)p->max_sweeps * j->ncopies * sizeof(double));
   ALLOC(j->out3, 3 * sizeof(double));
 #undef ALLOC
-  if (hipHostMalloc((void**)&j->ctl_h, sizeof(JobCtl)) != hipSuccess ||
+  if (hipHostMalloc((void**)&j->ctl_h, j->ncopies * sizeof(JobCtl)) != hipSuccess ||
       hipHostMalloc((void**)&j->out3_h, 3 * sizeof(double)) != hipSuccess) {
     free_job(j);
     return fail(MAG_ERR_NOMEM, "pinned host allocation failed");
   }
-  std::memset(j->ctl_h, 0, sizeof(JobCtl));
-  j->ctl_h->done = 1;  // nothing to sweep before an upload
-  e = hipMemcpy(j->ctl, j->ctl_h, sizeof(JobCtl), hipMemcpyHostToDevice);
+  std::memset(j->ctl_h, 0, j->ncopies * sizeof(JobCtl));
+  for (int c = 0; c < j->ncopies; c++) j->ctl_h[c].done = 1;  // nothing to sweep before an upload
+  e = hipMemcpy(j->ctl, j->ctl_h, j->ncopies * sizeof(JobCtl), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     free_job(j);
     return fail(MAG_ERR_HIP, "hipMemcpy failed: %s", hipGetErrorString(e));
@@ -384,6 +472,8 @@ int mag_job_generate(mag_job* j, double eps, uint64_t seed) {
 
 int mag_job_generate_copy(mag_job* j, double eps, uint64_t seed, uint32_t copy) {
   if (!j) return fail(MAG_ERR_ARG, "NULL job");
+  if (j->spin && copy != 0)
+    return fail(MAG_ERR_ARG, "a spin-form job holds copies 0..copies-1: generate copy 0");
   if (j->batch) return fail(MAG_ERR_STATE, "job is in a batch: sweep it through the batch");
   if (!std::isfinite(eps)) return fail(MAG_ERR_ARG, "eps must be finite");
   CHK(set_device(j));
@@ -401,7 +491,7 @@ int mag_job_generate_copy(mag_job* j, double eps, uint64_t seed, uint32_t copy) 
   } else {
     mag::launch_gen_su2(stage, j->g, eps, seed, mag::gauge_copy_seed(seed, copy), j->st);
   }
-  int r = upload_device_aos(j, stage);
+  int r = upload_device_aos(j, stage, j->spin ? &seed : nullptr);
   hipStreamSynchronize(j->st);
   hipFree(stage);
   hipFree(gs);
@@ -437,7 +527,7 @@ int mag_job_run(mag_job* j) {
     CHK(run_until_done(j));
   }
   CHK(read_ctl(j));
-  if (!j->ctl_h->converged)
+  if (!all_converged(j))
     return fail(MAG_ERR_NOT_CONVERGED, "sweep cap %d reached before tol", j->prm.max_sweeps);
   return MAG_OK;
 }
@@ -447,7 +537,7 @@ int mag_job_status(mag_job* j, int* sweeps, int* converged, double* F) {
   CHK(set_device(j));
   CHK(read_ctl(j));
   if (sweeps) *sweeps = j->ctl_h->sweep;
-  if (converged) *converged = j->ctl_h->converged;
+  if (converged) *converged = all_converged(j) ? 1 : 0;
   if (F) *F = j->ctl_h->F_prev;
   return MAG_OK;
 }
@@ -483,6 +573,7 @@ int mag_job_fp64_start(mag_job* j, int* sweep) {
 int mag_job_download_links(mag_job* j, double* links) {
   if (!j || !links) return fail(MAG_ERR_ARG, "NULL argument");
   if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  if (j->spin) return mag_job_download_copy_links(j, 0, links);
   CHK(set_device(j));
   const size_t n = (size_t)j->V * 4 * j->nr_in;
   double* stage = nullptr;
@@ -559,7 +650,7 @@ int mag_job_halfsweep_timed(mag_job* j, int n, double* avg_ms) {
   if (j->batch) return fail(MAG_ERR_STATE, "job is in a batch: sweep it through the batch");
   CHK(set_device(j));
   CHK(read_ctl(j));
-  if (j->ctl_h->done) return fail(MAG_ERR_STATE, "job is done: its kernels would return at once");
+  if (all_done(j)) return fail(MAG_ERR_STATE, "job is done: its kernels would return at once");
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -573,6 +664,119 @@ int mag_job_halfsweep_timed(mag_job* j, int n, double* avg_ms) {
   *avg_ms = ms / n;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  return MAG_OK;
+}
+
+// ------------------------------------------------------------ gauge copies
+static int check_copy(mag_job* j, int c) {
+  if (!j) return fail(MAG_ERR_ARG, "NULL job");
+  if (c < 0 || c >= j->ncopies) return fail(MAG_ERR_ARG, "copy %d out of range 0..%d", c, j->ncopies - 1);
+  return MAG_OK;
+}
+
+int mag_job_set_copy_spins(mag_job* j, int c, const double* spins) {
+  CHK(check_copy(j, c));
+  if (!spins) return fail(MAG_ERR_ARG, "NULL spins");
+  if (!j->spin) return fail(MAG_ERR_STATE, "not a spin-form job");
+  if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  if (j->batch) return fail(MAG_ERR_STATE, "job is in a batch: sweep it through the batch");
+  const size_t n = (size_t)j->V * 3;
+  if (!all_finite(spins, n)) return fail(MAG_ERR_NONFINITE, "spins hold NaN or Inf");
+  CHK(set_device(j));
+  double* stage = nullptr;
+  HIPCHK(hipMalloc((void**)&stage, n * sizeof(double)));
+  hipError_t e = hipMemcpyAsync(stage, spins, n * sizeof(double), hipMemcpyHostToDevice, j->st);
+  if (e == hipSuccess) {
+    mag::launch_spins_layout(stage, j->S + (size_t)c * j->sstride, j->g, 1, j->st);
+    init_copy(j, c);
+    e = hipStreamSynchronize(j->st);
+  }
+  hipFree(stage);
+  if (e != hipSuccess) return fail(MAG_ERR_HIP, "set_copy_spins failed: %s", hipGetErrorString(e));
+  return MAG_OK;
+}
+
+int mag_job_get_copy_spins(mag_job* j, int c, double* spins) {
+  CHK(check_copy(j, c));
+  if (!spins) return fail(MAG_ERR_ARG, "NULL spins");
+  if (!j->spin) return fail(MAG_ERR_STATE, "not a spin-form job");
+  CHK(set_device(j));
+  const size_t n = (size_t)j->V * 3;
+  double* stage = nullptr;
+  HIPCHK(hipMalloc((void**)&stage, n * sizeof(double)));
+  mag::launch_spins_layout(stage, j->S + (size_t)c * j->sstride, j->g, 0, j->st);
+  hipError_t e = hipMemcpyAsync(spins, stage, n * sizeof(double), hipMemcpyDeviceToHost, j->st);
+  hipError_t e2 = hipStreamSynchronize(j->st);
+  hipFree(stage);
+  if (e != hipSuccess || e2 != hipSuccess)
+    return fail(MAG_ERR_HIP, "get_copy_spins failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+  return MAG_OK;
+}
+
+int mag_job_copy_status(mag_job* j, int c, int* sweeps, int* converged, double* F) {
+  CHK(check_copy(j, c));
+  CHK(set_device(j));
+  CHK(read_ctl(j));
+  if (sweeps) *sweeps = j->ctl_h[c].sweep;
+  if (converged) *converged = j->ctl_h[c].converged;
+  if (F) *F = j->ctl_h[c].F_prev;
+  return MAG_OK;
+}
+
+int mag_job_copy_history(mag_job* j, int c, double* F, double* rmax, double* ravg, int cap,
+                         int* n) {
+  CHK(check_copy(j, c));
+  if (cap < 0) return fail(MAG_ERR_ARG, "bad argument");
+  CHK(set_device(j));
+  CHK(read_ctl(j));
+  const int m = j->ctl_h[c].sweep < cap ? j->ctl_h[c].sweep : cap;
+  const size_t hs = (size_t)c * j->prm.max_sweeps;
+  if (m > 0) {
+    if (F) HIPCHK(hipMemcpy(F, j->hF + hs, m * sizeof(double), hipMemcpyDeviceToHost));
+    if (rmax) HIPCHK(hipMemcpy(rmax, j->hR + hs, m * sizeof(double), hipMemcpyDeviceToHost));
+    if (ravg) HIPCHK(hipMemcpy(ravg, j->hA + hs, m * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  if (n) *n = m;
+  return MAG_OK;
+}
+
+int mag_job_best_copy(mag_job* j, int* copy, double* F) {
+  if (!j || !copy) return fail(MAG_ERR_ARG, "NULL argument");
+  CHK(set_device(j));
+  CHK(read_ctl(j));
+  int best = 0;
+  for (int c = 1; c < j->ncopies; c++)
+    if (j->ctl_h[c].F_prev > j->ctl_h[best].F_prev) best = c;
+  *copy = best;
+  if (F) *F = j->ctl_h[best].F_prev;
+  return MAG_OK;
+}
+
+int mag_job_download_copy_links(mag_job* j, int c, double* links) {
+  CHK(check_copy(j, c));
+  if (!links) return fail(MAG_ERR_ARG, "NULL links");
+  if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  if (!j->spin) {
+    if (c != 0) return fail(MAG_ERR_ARG, "a link-form job holds copy 0 only");
+    return mag_job_download_links(j, links);
+  }
+  CHK(set_device(j));
+  const size_t n = (size_t)j->V * 4 * 4;
+  double *soa = nullptr, *stage = nullptr;
+  HIPCHK(hipMalloc((void**)&soa, j->nreal * sizeof(double)));
+  hipError_t e = hipMalloc((void**)&stage, n * sizeof(double));
+  if (e != hipSuccess) {
+    hipFree(soa);
+    return fail(MAG_ERR_NOMEM, "hipMalloc failed: %s", hipGetErrorString(e));
+  }
+  mag::launch_spin_links(j->U64, j->S + (size_t)c * j->sstride, soa, j->g, j->st);
+  mag::launch_su2_soa_to_aos<double>(soa, stage, j->g, j->st);
+  e = hipMemcpyAsync(links, stage, n * sizeof(double), hipMemcpyDeviceToHost, j->st);
+  hipError_t e2 = hipStreamSynchronize(j->st);
+  hipFree(soa);
+  hipFree(stage);
+  if (e != hipSuccess || e2 != hipSuccess)
+    return fail(MAG_ERR_HIP, "download failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
   return MAG_OK;
 }
 

@@ -292,4 +292,45 @@ void launch_gen_su3(double* aos, double* gauge_scratch, const Geom& g, double ep
                      gauge_scratch, g, eps, seed);
 }
 
+// ------------------------------------------------------------------ gauge copies
+// Initial spins of gauge copy c (spin form) of the configuration of
+// launch_gen_su2(seed, gseed_0): copy c of that configuration is
+// g_c U g_c^+ = h U_0 h^+ with h = g_c g_0^+, so it starts from the spin
+// of h (row 3 of its rotation); copy 0 starts exactly at e3.  Thread per
+// lexicographic site; SoA spins per parity, copy c at S + c*sstride.
+__global__ void k_gen_copy_spins(double* __restrict__ S, size_t sstride, int ncopies, Geom g,
+                                 uint64_t seed) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= 2 * g.Vh) return;
+  uint32_t r = s;
+  const uint32_t x = r % (uint32_t)g.L0;
+  r /= (uint32_t)g.L0;
+  const uint32_t y = r % (uint32_t)g.L1;
+  r /= (uint32_t)g.L1;
+  const uint32_t z = r % (uint32_t)g.L2, t = r / (uint32_t)g.L2;
+  const int p = (int)((x + y + z + t) & 1u);
+  const size_t h = s >> 1;
+  const Q<double> g0 = su2_gauge(gauge_copy_seed(seed, 0), s);
+  for (int c = 0; c < ncopies; c++) {
+    double v[3] = {0.0, 0.0, 1.0};
+    if (c > 0) {
+      const Q<double> q = qmul_dag(su2_gauge(gauge_copy_seed(seed, (uint32_t)c), s), g0);
+      v[0] = 2 * (q.a1 * q.a3 + q.a0 * q.a2);
+      v[1] = 2 * (q.a2 * q.a3 - q.a0 * q.a1);
+      v[2] = q.a0 * q.a0 + q.a3 * q.a3 - q.a1 * q.a1 - q.a2 * q.a2;
+    }
+    double* b = S + (size_t)c * sstride + (size_t)p * 3 * g.Vh + h;
+    b[0] = v[0];
+    b[g.Vh] = v[1];
+    b[2 * (size_t)g.Vh] = v[2];
+  }
+}
+
+void launch_gen_copy_spins(double* S, size_t sstride, int ncopies, const Geom& g, uint64_t seed,
+                           hipStream_t st) {
+  const uint32_t V = 2 * g.Vh;
+  hipLaunchKernelGGL(k_gen_copy_spins, dim3((V + 127) / 128), dim3(128), 0, st, S, sstride,
+                     ncopies, g, seed);
+}
+
 }  // namespace mag

@@ -72,6 +72,20 @@ template <typename T>
 void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega,
                      double* part, const JobCtl* ctl, hipStream_t st);
 
+// Gauge-copy (spin form, SU(2), FP64) half-sweep over all copies of one
+// configuration: the links U (read-only) are loaded once per site and
+// every copy not yet done updates its spins, copy c at S + c*sstride; its
+// partial triples go to part + c*pstride + 3*nb*parity (nb = nblocks(g))
+// and it is skipped once ctl[c].done.
+void launch_su2_copies_half(const double* U, double* S, size_t sstride, int ncopies,
+                            const Geom& g, int parity, double omega, double* part,
+                            size_t pstride, const JobCtl* ctl, hipStream_t st);
+// end-of-sweep reduction of every copy (one workgroup per copy): copy c
+// reduces part + c*pstride into ctl[c] and the history at hF + c*hstride
+void launch_reduce_copies(const double* part, size_t pstride, int nparts, int ncopies,
+                          double invF, double invV, JobCtl* ctl, double* hF, double* hR,
+                          double* hA, size_t hstride, hipStream_t st);
+
 // per-sweep (or initial) reduction of the partials into the job history
 constexpr int kRedBlock = 1024;  // threads of the one-workgroup reduction
 void launch_reduce(const double* part, int nparts, int mode, double invF,
@@ -100,11 +114,20 @@ struct BatchJob {
   int nparts;  // partial triples of a sweep (2*nb)
   int block0;  // first workgroup of this job in a batched half-sweep grid
   int rb0;     // first workgroup of this job in a batched rebuild grid
+  // spin form (gauge copies): spins of copy c at S + c*sstride, partials
+  // at part + c*pstride, control block ctl + c; ncopies = 0 in the link form
+  double* S;
+  size_t sstride;
+  size_t pstride;
+  int ncopies;
 };
 
 // batched half-sweep of one class (group, FP32 phase or FP64) table
 void launch_half_batch(int group, bool fp32, const BatchJob* jobs, int njobs,
                        int nblocks_total, int parity, hipStream_t st);
+// batched spin-form (gauge-copy) half-sweep of a table of copy sets
+void launch_copies_half_batch(const BatchJob* jobs, int njobs, int nblocks_total, int parity,
+                              hipStream_t st);
 // batched end-of-sweep reduction: one workgroup per job
 void launch_reduce_batch(const BatchJob* jobs, int njobs, hipStream_t st);
 // batched FP64 rebuild of the FP32 working links + gauge normalisation of a
@@ -157,7 +180,7 @@ void launch_gen_su2(double* aos, const Geom& g, double eps, uint64_t seed, uint6
 void launch_gen_su3(double* aos, double* gauge_scratch, const Geom& g, double eps,
                     uint64_t seed, uint64_t gseed, hipStream_t st);
 // key of gauge copy `copy` of the configuration seeded by `seed` (copy 0 = seed)
-inline uint64_t gauge_copy_seed(uint64_t seed, uint32_t copy) {
+__host__ __device__ inline uint64_t gauge_copy_seed(uint64_t seed, uint32_t copy) {
   return seed ^ ((uint64_t)copy * 0x9E3779B97F4A7C15ull);
 }
 
@@ -211,4 +234,13 @@ void launch_spins_uniform(double* aos, const Geom& g, uint64_t seed, hipStream_t
 // to_soa = 1: AoS -> SoA, 0: SoA -> AoS
 void launch_spins_layout(const double* aos, double* S, const Geom& g, int to_soa,
                          hipStream_t st);
+// every spin of n copies (nreal = n*6*Vh reals) set to e3
+void launch_spins_e3(double* S, size_t nreal, const Geom& g, hipStream_t st);
+// initial spins of gauge copies 0..ncopies-1 of a synthetic configuration
+// (copy c at S + c*sstride): spin of g_c(x) g_0(x)^+, copy 0 exactly e3
+void launch_gen_copy_spins(double* S, size_t sstride, int ncopies, const Geom& g,
+                           uint64_t seed, hipStream_t st);
+// out = G(s(x)) U G(s(x+mu))^+ with G = gauge_of_spin (chunked FP64 links)
+void launch_spin_links(const double* U, const double* S, double* out, const Geom& g,
+                       hipStream_t st);
 }  // namespace mag

@@ -29,11 +29,19 @@ struct mag_job {
   mag::JobCtl* ctl_h = nullptr;  // pinned host mirrors
   double* out3_h = nullptr;
   bool loaded = false;
-  int phase = 0;  // mixed: 0 = FP32 sweeps, 1 = FP64 after rebuild
+  int phase = 0;  // mixed: -1 = FP64 warm-up, 0 = FP32 sweeps, 1 = FP64 after rebuild
   int fp64_start = -1;
   int since_rebuild = 0;  // sweeps enqueued since the last FP64 rebuild
   double invF = 0, invV = 0;
   mag_batch* batch = nullptr;  // batch the job belongs to, if any
+  // spin form (gauge copies): U64 stays read-only; copy c has its spins at
+  // S + c*sstride, partials at part + c*pstride, control ctl[c] / ctl_h[c]
+  // and history at hF/hR/hA + c*max_sweeps
+  bool spin = false;
+  int ncopies = 1;
+  double* S = nullptr;
+  size_t sstride = 0;
+  size_t pstride = 0;
 };
 
 namespace mag {
@@ -41,4 +49,9 @@ namespace mag {
 int job_read_ctl(mag_job* j);
 int job_finalize_mixed(mag_job* j);
 bool job_mixed32(const mag_job* j);
+// every copy done / converged (read_ctl first)
+bool job_done(const mag_job* j);
+// mixed with an FP64 warm-up: enter the FP32 phase once the warm-up sweeps
+// are done (read_ctl first); *switched = whether it did
+int job_maybe_start_fp32(mag_job* j, bool* switched);
 }  // namespace mag

@@ -21,93 +21,9 @@
 #include "mag_layout.h"
 #include "mag_math.h"
 #include "mag_reduce.h"
+#include "mag_spin_math.h"
 
 namespace mag {
-
-struct V3 {
-  double a1, a2, a3;
-};
-
-__device__ __forceinline__ V3 ld_spin(const double* S, const Geom& g, int p, uint32_t h) {
-  const double* b = S + (size_t)p * 3 * g.Vh + h;
-  return V3{b[0], b[g.Vh], b[2 * (size_t)g.Vh]};
-}
-__device__ __forceinline__ void st_spin(double* S, const Geom& g, int p, uint32_t h, const V3& s) {
-  double* b = S + (size_t)p * 3 * g.Vh + h;
-  b[0] = s.a1;
-  b[g.Vh] = s.a2;
-  b[2 * (size_t)g.Vh] = s.a3;
-}
-
-// h += M(A) b (spin::contribution1, matrix.h:909-926)
-__device__ __forceinline__ void contribution1(V3& h, const Q<double>& A, const V3& b) {
-  const double q1 = A.a1 * A.a1, q2 = A.a2 * A.a2, q3 = A.a3 * A.a3;
-  const double q12 = A.a1 * A.a2, q13 = A.a1 * A.a3, q23 = A.a2 * A.a3;
-  const double q01 = A.a0 * A.a1, q02 = A.a0 * A.a2, q03 = A.a0 * A.a3;
-  const double A5 = A.a0 * A.a0 - q1 - q2 - q3;
-  h.a1 += b.a1 * (A5 + 2 * q1) + 2 * b.a2 * (q12 + q03) + 2 * b.a3 * (q13 - q02);
-  h.a2 += 2 * b.a1 * (q12 - q03) + b.a2 * (A5 + 2 * q2) + 2 * b.a3 * (q23 + q01);
-  h.a3 += 2 * b.a1 * (q13 + q02) + 2 * b.a2 * (q23 - q01) + b.a3 * (A5 + 2 * q3);
-}
-// h += M(A)^T b (spin::contribution1_conj, matrix.h:928-946)
-__device__ __forceinline__ void contribution1_conj(V3& h, const Q<double>& A, const V3& b) {
-  const double q1 = A.a1 * A.a1, q2 = A.a2 * A.a2, q3 = A.a3 * A.a3;
-  const double q12 = A.a1 * A.a2, q13 = A.a1 * A.a3, q23 = A.a2 * A.a3;
-  const double q01 = A.a0 * A.a1, q02 = A.a0 * A.a2, q03 = A.a0 * A.a3;
-  const double A5 = A.a0 * A.a0 - q1 - q2 - q3;
-  h.a1 += b.a1 * (A5 + 2 * q1) + 2 * b.a2 * (q12 - q03) + 2 * b.a3 * (q13 + q02);
-  h.a2 += 2 * b.a1 * (q12 + q03) + b.a2 * (A5 + 2 * q2) + 2 * b.a3 * (q23 - q01);
-  h.a3 += 2 * b.a1 * (q13 - q02) + 2 * b.a2 * (q23 + q01) + b.a3 * (A5 + 2 * q3);
-}
-// M(A) s (spin::contribution, matrix.h:870-887)
-__device__ __forceinline__ V3 contribution(const V3& s, const Q<double>& A) {
-  const double q1 = A.a1 * A.a1, q2 = A.a2 * A.a2, q3 = A.a3 * A.a3;
-  const double q12 = 2. * A.a1 * A.a2, q13 = 2. * A.a1 * A.a3, q23 = 2. * A.a2 * A.a3;
-  const double q01 = 2. * A.a0 * A.a1, q02 = 2. * A.a0 * A.a2, q03 = 2. * A.a0 * A.a3;
-  const double A5 = A.a0 * A.a0 - q1 - q2 - q3;
-  return V3{s.a1 * (A5 + 2 * q1) + s.a2 * (q12 + q03) + s.a3 * (q13 - q02),
-            s.a1 * (q12 - q03) + s.a2 * (A5 + 2 * q2) + s.a3 * (q23 + q01),
-            s.a1 * (q13 + q02) + s.a2 * (q23 - q01) + s.a3 * (A5 + 2 * q3)};
-}
-
-// reflect_fast (matrix.h:825-840)
-__device__ __forceinline__ void reflect_fast(V3& s, const V3& V) {
-  const double t1 = (V.a1 * V.a1 - V.a2 * V.a2 - V.a3 * V.a3) * s.a1 +
-                    2 * V.a1 * (V.a2 * s.a2 + V.a3 * s.a3);
-  const double t2 = (-V.a1 * V.a1 + V.a2 * V.a2 - V.a3 * V.a3) * s.a2 +
-                    2 * V.a2 * (V.a1 * s.a1 + V.a3 * s.a3);
-  const double t3 = (-V.a1 * V.a1 - V.a2 * V.a2 + V.a3 * V.a3) * s.a3 +
-                    2 * V.a3 * (V.a1 * s.a1 + V.a2 * s.a2);
-  const double n = sqrt(V.a1 * V.a1 + V.a2 * V.a2 + V.a3 * V.a3);
-  const double n2 = n * n;
-  s = V3{t1 / n2, t2 / n2, t3 / n2};
-}
-// parallel (matrix.h:842-856)
-__device__ __forceinline__ void parallel(V3& s, const V3& V) {
-  const double vn = sqrt(V.a1 * V.a1 + V.a2 * V.a2 + V.a3 * V.a3);
-  s = V3{V.a1 / vn, V.a2 / vn, V.a3 / vn};
-}
-// geodesic omega step of s towards h_hat
-__device__ __forceinline__ void spin_omega(V3& s, const V3& h, double omega) {
-  const double hn = sqrt(h.a1 * h.a1 + h.a2 * h.a2 + h.a3 * h.a3);
-  if (hn == 0.0) return;
-  const V3 hh{h.a1 / hn, h.a2 / hn, h.a3 / hn};
-  const double c = s.a1 * hh.a1 + s.a2 * hh.a2 + s.a3 * hh.a3;
-  V3 t{hh.a1 - c * s.a1, hh.a2 - c * s.a2, hh.a3 - c * s.a3};
-  double tn = sqrt(t.a1 * t.a1 + t.a2 * t.a2 + t.a3 * t.a3);
-  if (tn == 0.0) {
-    if (c >= 0) return;
-    V3 e{1.0, 0.0, 0.0};
-    if (fabs(s.a1) > 0.9) e = V3{0.0, 1.0, 0.0};
-    const double d = e.a1 * s.a1 + e.a2 * s.a2 + e.a3 * s.a3;
-    t = V3{e.a1 - d * s.a1, e.a2 - d * s.a2, e.a3 - d * s.a3};
-    tn = sqrt(t.a1 * t.a1 + t.a2 * t.a2 + t.a3 * t.a3);
-  }
-  const double theta = atan2(tn, c);
-  double sa, ca;
-  sincos(omega * theta, &sa, &ca);
-  s = V3{ca * s.a1 + sa * t.a1 / tn, ca * s.a2 + sa * t.a2 / tn, ca * s.a3 + sa * t.a3 / tn};
-}
 
 // Philox4x32-10 and the uniform of the synthetic generators
 __device__ __forceinline__ void philox_s(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
@@ -387,6 +303,44 @@ void launch_spins_layout(const double* aos, double* S, const Geom& g, int to_soa
   const uint32_t V = 2 * g.Vh;
   hipLaunchKernelGGL(k_spins_aos_to_soa, dim3((V + 255) / 256), dim3(256), 0, st, aos, S, g,
                      to_soa);
+}
+
+// ------------------------------------------------------------------ gauge copies
+__global__ void k_spins_e3(double* __restrict__ S, size_t nreal, uint32_t Vh) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nreal) return;
+  // component 2 (a3) of each parity block of each copy is 1
+  S[i] = ((i / Vh) % 3 == 2) ? 1.0 : 0.0;
+}
+
+// out_x,mu = G(s(x)) U_x,mu G(s(x+mu))^+, G = gauge_of_spin; thread per
+// site of either parity (grid 2*nblocks)
+__global__ __launch_bounds__(kBlock) void k_spin_links(const double* __restrict__ U,
+                                                       const double* __restrict__ S,
+                                                       double* __restrict__ out, Geom g) {
+  const int nb = (int)((g.Vh + kBlock - 1) / kBlock);
+  const int p = blockIdx.x / nb;
+  const uint32_t h = (blockIdx.x % nb) * kBlock + threadIdx.x;
+  if (h >= g.Vh) return;
+  const SiteIdx s = decode(g, h, p);
+  const Q<double> G = gauge_of_spin(ld_spin(S, g, p, h));
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) {
+    const Q<double> Gf = gauge_of_spin(ld_spin(S, g, 1 - p, fwd_h(g, h, s, mu)));
+    const Q<double> A = qmul_dag(qmul(G, ldq(link_ptr<4>(U, g, p, mu, h))), Gf);
+    stq(link_ptr<4>(out, g, p, mu, h), A);
+  }
+}
+
+void launch_spins_e3(double* S, size_t nreal, const Geom& g, hipStream_t st) {
+  if (nreal == 0) return;
+  // nreal is a whole number of copies: ncopies * 2 parities * 3 * Vh
+  hipLaunchKernelGGL(k_spins_e3, dim3((unsigned)((nreal + 255) / 256)), dim3(256), 0, st, S,
+                     nreal, g.Vh);
+}
+void launch_spin_links(const double* U, const double* S, double* out, const Geom& g,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_spin_links, dim3(2 * nblocks(g)), dim3(kBlock), 0, st, U, S, out, g);
 }
 
 }  // namespace mag

@@ -31,6 +31,8 @@ MAG_ERR_STATE = -7
 
 SU2, SU3 = 2, 3
 F64, MIXED = 0, 1
+FORM_LINK, FORM_SPIN = 0, 1
+MAX_COPIES = 8
 
 
 class MagError(RuntimeError):
@@ -54,6 +56,9 @@ class _Params(ctypes.Structure):
         ("max_sweeps", ctypes.c_int),
         ("reunit_every", ctypes.c_int),
         ("tol_mixed", ctypes.c_double),
+        ("form", ctypes.c_int),
+        ("copies", ctypes.c_int),
+        ("fp64_warmup", ctypes.c_int),
     ]
 
 
@@ -82,6 +87,12 @@ SIGNATURES = {
     "mag_job_initial_functional": (ctypes.c_int, [_vp, _dp]),
     "mag_job_history": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, _ip]),
     "mag_job_fp64_start": (ctypes.c_int, [_vp, _ip]),
+    "mag_job_set_copy_spins": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "mag_job_get_copy_spins": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "mag_job_copy_status": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _dp]),
+    "mag_job_copy_history": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _ip]),
+    "mag_job_best_copy": (ctypes.c_int, [_vp, _ip, _dp]),
+    "mag_job_download_copy_links": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "mag_job_download_links": (ctypes.c_int, [_vp, _dp]),
     "mag_job_functional": (ctypes.c_int, [_vp, _dp]),
     "mag_job_plaquette": (ctypes.c_int, [_vp, _dp]),
@@ -197,6 +208,9 @@ class MagParams:
     max_sweeps: int = 1000
     reunit_every: int = 10
     tol_mixed: float = 1e-9
+    form: int = FORM_LINK
+    copies: int = 1
+    fp64_warmup: int = 0
 
     def to_c(self) -> _Params:
         p = _Params()
@@ -211,6 +225,9 @@ class MagParams:
         p.max_sweeps = int(self.max_sweeps)
         p.reunit_every = int(self.reunit_every)
         p.tol_mixed = float(self.tol_mixed)
+        p.form = int(self.form)
+        p.copies = int(self.copies)
+        p.fp64_warmup = int(self.fp64_warmup)
         return p
 
     @property
@@ -331,6 +348,48 @@ class MagJob:
         s = ctypes.c_int()
         _check(lib().mag_job_fp64_start(self._h, ctypes.byref(s)))
         return s.value
+
+    # gauge copies (spin form) ---------------------------------------------
+    def set_copy_spins(self, copy: int, spins: np.ndarray):
+        a = np.ascontiguousarray(spins, dtype=np.float64)
+        if a.size != 3 * self.params.volume:
+            raise ValueError("spins must hold 3*V reals")
+        _check(lib().mag_job_set_copy_spins(self._h, int(copy), _dptr(a)))
+
+    def copy_spins(self, copy: int) -> np.ndarray:
+        out = np.empty((self.params.volume, 3))
+        _check(lib().mag_job_get_copy_spins(self._h, int(copy), _dptr(out)))
+        return out
+
+    def copy_status(self, copy: int):
+        s = ctypes.c_int()
+        c = ctypes.c_int()
+        F = ctypes.c_double()
+        _check(lib().mag_job_copy_status(self._h, int(copy), ctypes.byref(s), ctypes.byref(c),
+                                         ctypes.byref(F)))
+        return s.value, bool(c.value), F.value
+
+    def copy_history(self, copy: int):
+        n_max = self.params.max_sweeps
+        F = np.zeros(n_max)
+        R = np.zeros(n_max)
+        A = np.zeros(n_max)
+        n = ctypes.c_int()
+        _check(lib().mag_job_copy_history(self._h, int(copy), _dptr(F), _dptr(R), _dptr(A),
+                                          n_max, ctypes.byref(n)))
+        m = n.value
+        return F[:m].copy(), R[:m].copy(), A[:m].copy()
+
+    def best_copy(self):
+        c = ctypes.c_int()
+        F = ctypes.c_double()
+        _check(lib().mag_job_best_copy(self._h, ctypes.byref(c), ctypes.byref(F)))
+        return c.value, F.value
+
+    def download_copy(self, copy: int) -> np.ndarray:
+        out = np.empty((4 * self.params.volume, self.params.reals_in), dtype=np.float64)
+        _check(lib().mag_job_download_copy_links(self._h, int(copy), _dptr(out)))
+        return out
 
     def functional(self) -> float:
         F = ctypes.c_double()

@@ -66,10 +66,28 @@ typedef struct mag_params {
                       /*        every reunit_every sweeps                      */
   double tol_mixed;   /* mixed: FP32 phase ends at this relative change;     */
                       /*        FP64 sweeps then run down to tol              */
+  int form;           /* MAG_FORM_LINK or MAG_FORM_SPIN                        */
+  int copies;         /* gauge copies of the configuration swept together    */
+                      /* (spin form, 1..MAG_MAX_COPIES; 1 in the link form)  */
+  int fp64_warmup;    /* mixed: FP64 sweeps before the FP32 phase (a multiple */
+                      /*        of 20); the early sweeps pick the maximum     */
+                      /*        the run converges to                          */
 } mag_params;
 
+/* Link form: the links carry the gauge transformation and are rewritten by
+ * every half-sweep.  Spin form (SU(2), FP64): the links stay read-only and
+ * each gauge copy is a field of unit 3-vectors s(x) (the reference's spin
+ * form, mag.cpp:569-626, 1375-1512); all copies of one configuration are
+ * swept together, so the links are read once per half-sweep for all of
+ * them.  Both forms run the same checkerboard omega step and give the same
+ * functional per sweep to rounding. */
+#define MAG_FORM_LINK 0
+#define MAG_FORM_SPIN 1
+#define MAG_MAX_COPIES 8
+
 /* Fill defaults: SU(2), omega 1.7, tol 1e-12, 1000 sweeps, FP64,
- * reunit_every 10, tol_mixed 1e-9, device 0. */
+ * reunit_every 10, tol_mixed 1e-9, link form, 1 copy, no FP64 warm-up,
+ * device 0. */
 void mag_params_default(mag_params *p);
 
 const char *mag_last_error(void);
@@ -132,6 +150,36 @@ int mag_job_download_links(mag_job *job, double *links);
 /* MAG functional / plaquette of the device-resident links (blocking) */
 int mag_job_functional(mag_job *job, double *F);
 int mag_job_plaquette(mag_job *job, double *P);
+
+/* ---- gauge copies (spin form)
+ * A spin-form job holds one SU(2) configuration and `copies` spin fields.
+ * mag_job_upload_links starts every copy at s = e3 (the identity gauge);
+ * mag_job_generate(eps, seed) uploads copy 0 of the synthetic configuration
+ * and starts copy k at the spins of the gauge g_k g_0^+ (g_k: the Haar
+ * gauge of mag_job_generate_copy(eps, seed, k)), so copy k follows the same
+ * trajectory as a link-form job of mag_job_generate_copy(eps, seed, k).
+ * Each copy converges on its own (its own history and status); the
+ * link-form calls above (status, history, functional, download) report
+ * copy 0, mag_job_status reports converged only when every copy has.
+ * Stands for the reference's gauge-copy loop with best-copy selection
+ * (apps/mag_su2/mag_fixation.cpp:164-249). */
+/* start copy `copy` from the spins s (reference layout [V][3]) */
+int mag_job_set_copy_spins(mag_job *job, int copy, const double *spins);
+int mag_job_get_copy_spins(mag_job *job, int copy, double *spins);
+int mag_job_copy_status(mag_job *job, int copy, int *sweeps, int *converged, double *F);
+int mag_job_copy_history(mag_job *job, int copy, double *F, double *rmax, double *ravg,
+                         int cap, int *n);
+/* copy with the largest functional (the reference keeps the spins of the
+ * higher functional, apps/mag_su2/mag_fixation.cpp:223-240) */
+int mag_job_best_copy(mag_job *job, int *copy, double *F);
+/* Links of copy `copy` gauge-transformed by its spins:
+ * U'_x,mu = G(s(x)) U_x,mu G(s(x+mu))^+ with G(s) the SU(2) element of
+ * smallest rotation that takes s to e3 (regular everywhere except s = -e3,
+ * where any G with row 3 of its rotation = -e3 is used).  Equal to the
+ * reference's gauge_tranformation_spins (mag.cpp:1045-1068) up to a
+ * diagonal U(1) gauge, which leaves the MAG functional and every
+ * a0^2 + a3^2 unchanged. */
+int mag_job_download_copy_links(mag_job *job, int copy, double *links);
 
 /* Measurement: n sweeps bracketed by HIP events on the job's stream
  * (blocking); total_ms = elapsed time. */

@@ -428,6 +428,11 @@ static void spin_omega(double *s, const double *h, double omega) {
   double a = omega * theta;
   double ca = cos(a), sa = sin(a);
   for (int k = 0; k < 3; k++) s[k] = ca * s[k] + sa * t[k] / tn;
+  /* back onto the sphere: for omega > 1 this step multiplies a spin's
+   * deviation from unit length by about 2 omega - 1 per update, so rounding
+   * would grow exponentially over the sweeps without it */
+  double sn2 = 1.0 / sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+  for (int k = 0; k < 3; k++) s[k] *= sn2;
 }
 
 /* ------------------------------------------------------------------ */
@@ -538,19 +543,22 @@ void or_gen_su2_config(double *links, const int *dims, double eps,
   or_gen_su2_config_copy(links, dims, eps, seed, 0);
 }
 
+/* Haar SU(2) site gauge: normalised N(0,1)^4 */
+static void su2_site_gauge(uint64_t gseed, uint32_t site, double *q) {
+  double z[4];
+  gauss2(gseed, site, 0, TAG_GAUGE, z);
+  gauss2(gseed, site, 1, TAG_GAUGE, z + 2);
+  double n = sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2] + z[3] * z[3]);
+  for (int k = 0; k < 4; k++) q[k] = z[k] / n;
+}
+
 void or_gen_su2_config_copy(double *links, const int *dims, double eps,
                             uint64_t seed, uint32_t copy) {
   geom_t g = geom(dims);
   const uint64_t gseed = gauge_copy_seed(seed, copy);
   double *gauge = (double *)malloc(sizeof(double) * 4 * (size_t)g.V);
 #pragma omp parallel for schedule(static)
-  for (int64_t s = 0; s < g.V; s++) {
-    double z[4];
-    gauss2(gseed, (uint32_t)s, 0, TAG_GAUGE, z);
-    gauss2(gseed, (uint32_t)s, 1, TAG_GAUGE, z + 2);
-    double n = sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2] + z[3] * z[3]);
-    for (int k = 0; k < 4; k++) gauge[4 * s + k] = z[k] / n;
-  }
+  for (int64_t s = 0; s < g.V; s++) su2_site_gauge(gseed, (uint32_t)s, gauge + 4 * s);
 #pragma omp parallel for schedule(static)
   for (int64_t s = 0; s < g.V; s++) {
     int c[4];
@@ -578,6 +586,33 @@ void or_gen_su2_config_copy(double *links, const int *dims, double eps,
     }
   }
   free(gauge);
+}
+
+/* Starting spins of gauge copy `copy` in the spin form, for the links of
+ * copy 0: copy c of the configuration is g_c U g_c^+ = h U_0 h^+ with
+ * h = g_c g_0^+ (U_0 = copy 0), so the spin form on U_0 follows it from the
+ * spin of h, row 3 of its rotation
+ *   (2(a1 a3 + a0 a2), 2(a2 a3 - a0 a1), a0^2 + a3^2 - a1^2 - a2^2);
+ * copy 0 starts exactly at e3. */
+void or_gen_copy_spins(double *spins, const int *dims, uint64_t seed, uint32_t copy) {
+  geom_t g = geom(dims);
+#pragma omp parallel for schedule(static)
+  for (int64_t s = 0; s < g.V; s++) {
+    double *o = spins + 3 * s;
+    if (copy == 0) {
+      o[0] = 0;
+      o[1] = 0;
+      o[2] = 1;
+      continue;
+    }
+    double gc[4], g0[4];
+    su2_site_gauge(gauge_copy_seed(seed, copy), (uint32_t)s, gc);
+    su2_site_gauge(gauge_copy_seed(seed, 0), (uint32_t)s, g0);
+    q_t h = qmul_dag(qload(gc), qload(g0));
+    o[0] = 2 * (h.a1 * h.a3 + h.a0 * h.a2);
+    o[1] = 2 * (h.a2 * h.a3 - h.a0 * h.a1);
+    o[2] = h.a0 * h.a0 + h.a3 * h.a3 - h.a1 * h.a1 - h.a2 * h.a2;
+  }
 }
 
 /* uniform spins on S^2 (mag.cpp:166-236 Marsaglia construction, on a

@@ -112,6 +112,8 @@ void or_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 void or_gen_su2_config(double *links, const int *dims, double eps, uint64_t seed);
 void or_gen_su3_config(double *links18, const int *dims, double eps, uint64_t seed);
 /* gauge copy `copy`: same link draws, gauge key seed ^ copy * 0x9E3779B97F4A7C15 */
+/* starting spins of gauge copy `copy` (spin form, links of copy 0) */
+void or_gen_copy_spins(double *spins, const int *dims, uint64_t seed, uint32_t copy);
 void or_gen_su2_config_copy(double *links, const int *dims, double eps, uint64_t seed,
                             uint32_t copy);
 void or_gen_su3_config_copy(double *links18, const int *dims, double eps, uint64_t seed,

@@ -82,6 +82,7 @@ def lib():
             "or_philox4x32_10": (None, [_u32p, _u32p, _u32p]),
             "or_gen_su2_config": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64]),
             "or_gen_su3_config": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64]),
+            "or_gen_copy_spins": (None, [_dp, _ip, ctypes.c_uint64, ctypes.c_uint32]),
             "or_gen_su2_config_copy": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64,
                                               ctypes.c_uint32]),
             "or_gen_su3_config_copy": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64,
@@ -327,6 +328,14 @@ def gen_su2_config(dims, eps, seed, copy=0):
     links = np.zeros((4 * V, 4))
     lib().or_gen_su2_config_copy(_d(links), _dims(dims), float(eps), int(seed), int(copy))
     return links
+
+
+def gen_copy_spins(dims, seed, copy):
+    """Starting spins (V, 3) of gauge copy `copy` in the spin form, on the
+    links gen_su2_config(dims, eps, seed) (copy 0)."""
+    out = np.zeros((volume(dims), 3))
+    lib().or_gen_copy_spins(_d(out), _dims(dims), int(seed), int(copy))
+    return out
 
 
 def gen_su3_config(dims, eps, seed, copy=0):

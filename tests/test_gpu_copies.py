@@ -1,0 +1,195 @@
+"""Gauge copies in the spin form: one SU(2) configuration, several spin
+fields swept together with the links read once per half-sweep.
+
+Parity: each copy against the oracle's spin-form checkerboard sweep
+(or_su2_spin_halfsweep, omega rule) and against the link-form job of the
+same gauge copy, per-sweep functional to 1e-12 relative; final links
+compared through the U(1)-invariant a0^2 + a3^2 of every link (the two
+forms' outputs differ by a diagonal gauge)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (8, 8, 4, 8)
+EPS, SEED, OMEGA = 0.5, 4242, 1.7
+
+
+def _rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
+def _spin_job(mag, copies, n_max=3000, tol=0.0, dims=DIMS):
+    return mag.MagJob(group=mag.SU2, dims=dims, form=mag.FORM_SPIN, copies=copies,
+                      omega=OMEGA, tol=tol, max_sweeps=n_max)
+
+
+def _u1_invariant(links):
+    return links[:, 0] ** 2 + links[:, 3] ** 2
+
+
+def test_copies_match_oracle_spin_sweeps(mag, oracle):
+    n = 30
+    with _spin_job(mag, 3, n_max=n) as job:
+        job.generate(EPS, SEED)
+        job.sweeps(n)
+        hist = [job.copy_history(c)[0] for c in range(3)]
+        spins = [job.copy_spins(c) for c in range(3)]
+    U0 = oracle.gen_su2_config(DIMS, EPS, SEED)
+    for c in range(3):
+        s = oracle.gen_copy_spins(DIMS, SEED, c)
+        ref = []
+        for _ in range(n):
+            oracle.su2_spin_halfsweep(U0, s, DIMS, 0, 2, OMEGA)
+            oracle.su2_spin_halfsweep(U0, s, DIMS, 1, 2, OMEGA)
+            ref.append(oracle.su2_functional_spin(U0, s, DIMS))
+        assert len(hist[c]) == n
+        assert _rel(hist[c], ref) < 1e-12, c
+        assert np.max(np.abs(spins[c] - s)) < 1e-11, c
+
+
+def test_copies_match_link_form_copies(mag):
+    n = 40
+    with _spin_job(mag, 3, n_max=n) as job:
+        job.generate(EPS, SEED)
+        F0 = job.initial_functional()
+        job.sweeps(n)
+        hist = [job.copy_history(c)[0] for c in range(3)]
+        links = [job.download_copy(c) for c in range(3)]
+        P0 = job.plaquette()
+    for c in range(3):
+        with mag.MagJob(group=mag.SU2, dims=DIMS, omega=OMEGA, tol=0.0, max_sweeps=n) as lj:
+            lj.generate(EPS, SEED, copy=c)
+            if c == 0:
+                assert abs(lj.initial_functional() - F0) < 1e-14
+            lj.sweeps(n)
+            F, _, _ = lj.history()
+            ref = lj.download()
+            assert abs(lj.plaquette() - P0) < 1e-13
+        assert _rel(hist[c], F) < 1e-12, c
+        assert np.max(np.abs(_u1_invariant(links[c]) - _u1_invariant(ref))) < 1e-11, c
+        assert np.max(np.abs(np.linalg.norm(links[c], axis=1) - 1)) < 1e-13
+
+
+def test_copy_trajectory_independent_of_set(mag, oracle):
+    """Copy 2 of a 3-copy set and the same spins alone in a 1-copy job run
+    the same arithmetic and partition: bit-identical histories."""
+    n = 25
+    with mag.MagJob(group=mag.SU2, dims=DIMS) as lj:
+        lj.generate(EPS, SEED)
+        U0 = lj.download()  # the same generator kernel as the spin job's links
+    with _spin_job(mag, 3, n_max=n) as job:
+        job.generate(EPS, SEED)
+        start = job.copy_spins(2)
+        assert np.max(np.abs(start - oracle.gen_copy_spins(DIMS, SEED, 2))) < 1e-13
+        job.sweeps(n)
+        F3, R3, A3 = job.copy_history(2)
+        s3 = job.copy_spins(2)
+    with _spin_job(mag, 1, n_max=n) as one:
+        one.upload(U0)
+        one.set_copy_spins(0, start)
+        one.sweeps(n)
+        F1, R1, A1 = one.copy_history(0)
+        s1 = one.copy_spins(0)
+    assert np.array_equal(F3, F1) and np.array_equal(R3, R1) and np.array_equal(A3, A1)
+    assert np.array_equal(s3, s1)
+
+
+def test_copies_converge_and_best_copy(mag):
+    dims = (8, 8, 8, 8)
+    with _spin_job(mag, 4, n_max=4000, tol=1e-12, dims=dims) as job:
+        job.generate(EPS, SEED)
+        assert job.run()
+        st = [job.copy_status(c) for c in range(4)]
+        best, Fb = job.best_copy()
+    assert all(conv for _, conv, _ in st)
+    assert Fb == max(F for _, _, F in st)
+    assert st[best][2] == Fb
+    # copy 1 alone in the link form reaches the same maximum
+    with mag.MagJob(group=mag.SU2, dims=dims, omega=OMEGA, tol=1e-12, max_sweeps=4000) as lj:
+        lj.generate(EPS, SEED, copy=1)
+        assert lj.run()
+        s, conv, F = lj.status()
+    assert abs(F - st[1][2]) / F < 1e-10
+
+
+def test_copy_sets_in_batch_equal_alone(mag):
+    specs = [dict(form=mag.FORM_SPIN, copies=2, dims=(8, 8, 8, 8), seed=5),
+             dict(form=mag.FORM_LINK, copies=1, dims=(8, 4, 8, 8), seed=6),
+             dict(form=mag.FORM_SPIN, copies=3, dims=(4, 8, 4, 8), seed=7)]
+
+    def make(sp):
+        j = mag.MagJob(group=mag.SU2, dims=sp["dims"], form=sp["form"], copies=sp["copies"],
+                       omega=OMEGA, tol=1e-12, max_sweeps=3000)
+        j.generate(EPS, sp["seed"])
+        return j
+
+    alone = []
+    for sp in specs:
+        with make(sp) as j:
+            j.run()
+            alone.append([j.copy_history(c) for c in range(sp["copies"])])
+    jobs = [make(sp) for sp in specs]
+    with mag.MagBatch() as b:
+        for j in jobs:
+            b.add(j)
+        b.run()
+        for j, sp, ref in zip(jobs, specs, alone):
+            for c in range(sp["copies"]):
+                got = j.copy_history(c)
+                for x, y in zip(got, ref[c]):
+                    assert np.array_equal(x, y)
+            b.remove(j)
+    for j in jobs:
+        j.close()
+
+
+def test_mixed_fp64_warmup(mag):
+    """The first fp64_warmup sweeps of a mixed job are the FP64 job's sweeps
+    (same kernels); the FP32 phase follows; alone and in a batch agree."""
+    dims = (8, 8, 8, 8)
+
+    def make(prec, warm):
+        j = mag.MagJob(group=mag.SU2, dims=dims, precision=prec, omega=OMEGA, tol=1e-12,
+                       max_sweeps=3000, reunit_every=10, tol_mixed=1e-8, fp64_warmup=warm)
+        j.generate(EPS, 31)
+        return j
+
+    with make(mag.F64, 0) as ref:
+        ref.sweeps(40)
+        Fr, _, _ = ref.history()
+    with make(mag.MIXED, 40) as j:
+        j.run()
+        F, R, A = j.history()
+        start = j.fp64_start()
+        st = j.status()
+    assert np.array_equal(F[:40], Fr[:40])
+    assert start > 40 and st[1]
+    with make(mag.MIXED, 40) as jb, mag.MagBatch() as b:
+        b.add(jb)
+        b.run()
+        Fb, Rb, Ab = jb.history()
+        assert np.array_equal(Fb, F) and np.array_equal(Ab, A)
+        assert jb.fp64_start() == start
+        b.remove(jb)
+
+
+def test_spin_form_argument_errors(mag):
+    with pytest.raises(mag.MagError):
+        mag.MagJob(group=mag.SU3, dims=(4, 4, 4, 4), form=mag.FORM_SPIN, copies=2)
+    with pytest.raises(mag.MagError):
+        mag.MagJob(group=mag.SU2, dims=(4, 4, 4, 4), form=mag.FORM_SPIN, copies=9)
+    with pytest.raises(mag.MagError):
+        mag.MagJob(group=mag.SU2, dims=(4, 4, 4, 4), form=mag.FORM_SPIN, precision=mag.MIXED)
+    with pytest.raises(mag.MagError):
+        mag.MagJob(group=mag.SU2, dims=(4, 4, 4, 4), copies=2)
+    with pytest.raises(mag.MagError):
+        mag.MagJob(group=mag.SU2, dims=(4, 4, 4, 4), precision=mag.MIXED, fp64_warmup=30)
+    with _spin_job(mag, 2, dims=(4, 4, 4, 4)) as j:
+        j.generate(0.3, 1)
+        with pytest.raises(mag.MagError):
+            j.copy_status(2)
+        with pytest.raises(mag.MagError):
+            j.generate(0.3, 1, copy=1)

@@ -47,10 +47,10 @@ def test_spin_at_pole_has_no_gauge_matrix(mag, oracle, cfg):
     assert e.value.code == -4
 
 
-# relaxation contracts (20 sweeps at 1e-12); the geodesic omega=1.6 rule
-# contracts slowly from random spins (differences of ~1e-11 after 20
-# sweeps), compared over 8; reflection does not contract (3 sweeps)
-@pytest.mark.parametrize("mode,param,n,tol", [(1, 0.0, 20, 1e-12), (2, 1.6, 8, 1e-11),
+# relaxation and the geodesic omega rule (which puts each spin back on the
+# sphere) over 20 / 40 sweeps at 1e-12; reflection conserves F and does not
+# contract, so rounding differences grow: 3 sweeps
+@pytest.mark.parametrize("mode,param,n,tol", [(1, 0.0, 20, 1e-12), (2, 1.6, 40, 1e-12),
                                               (0, 0.0, 3, 1e-12)])
 def test_checkerboard_sweeps_match_oracle(mag, oracle, cfg, mode, param, n, tol):
     links, spins = cfg

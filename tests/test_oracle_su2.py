@@ -222,16 +222,19 @@ def test_link_sweep_equals_spin_sweep(oracle, omega):
     """Two independent formulations of the same checkerboard sweep: spins
     with read-only links (reference primitives, geodesic rule) and the
     link form the GPU runs.  F per sweep must agree."""
-    dims = (4, 4, 4, 4)
-    links = _lattice(oracle, dims, 0.4, 21)
-    spins = np.tile([0.0, 0.0, 1.0], (256, 1))
+    dims = (8, 8, 4, 8)
+    links = _lattice(oracle, dims, 0.5, 4242)
+    spins = np.tile([0.0, 0.0, 1.0], (2048, 1))
     W = links.copy()
-    n = 4 if omega == 2.0 else 12
+    # long enough for a spin-length instability of the omega step (rounding
+    # grown by ~2 omega - 1 per update) to show: it reaches 1e-9 by sweep 20
+    n = 4 if omega == 2.0 else 60
     for _ in range(n):
         oracle.su2_spin_halfsweep(links, spins, dims, 0, 2, omega)
         oracle.su2_spin_halfsweep(links, spins, dims, 1, 2, omega)
         F, _, _ = oracle.su2_link_sweep(W, dims, omega)
         assert abs(oracle.su2_functional_spin(links, spins, dims) - F) < 1e-13
+        assert np.max(np.abs(np.linalg.norm(spins, axis=1) - 1)) < 1e-14
 
 
 def test_spin_modes_reduce_to_reference_updates(oracle):
@@ -276,3 +279,29 @@ def test_link_sweep_thread_independent(oracle):
     oracle.set_threads(n0)
     assert ra == rb
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("copy", [0, 1, 3])
+def test_copy_spins_follow_link_form_copy(oracle, copy):
+    """Spin form on the links of copy 0, started at gen_copy_spins(copy),
+    is gauge copy `copy` of the link form: the same F at the start (the
+    invariant F_s(U, s) = F(G(s) U G(s')^+), mag.cpp:925-953 vs 993-1001)
+    and after every omega sweep."""
+    dims = (4, 4, 4, 6)
+    U0 = oracle.gen_su2_config(dims, 0.5, 77)
+    Uc = oracle.gen_su2_config(dims, 0.5, 77, copy=copy)
+    spins = oracle.gen_copy_spins(dims, 77, copy)
+    assert np.max(np.abs(np.linalg.norm(spins, axis=1) - 1)) < 1e-14
+    if copy == 0:
+        assert np.array_equal(spins, np.tile([0.0, 0.0, 1.0], (len(spins), 1)))
+    assert abs(oracle.su2_functional_spin(U0, spins, dims) - oracle.su2_functional(Uc, dims)) < 1e-14
+    for _ in range(10):
+        oracle.su2_spin_halfsweep(U0, spins, dims, 0, 2, 1.7)
+        oracle.su2_spin_halfsweep(U0, spins, dims, 1, 2, 1.7)
+        F, _, _ = oracle.su2_link_sweep(Uc, dims, 1.7)
+        assert abs(oracle.su2_functional_spin(U0, spins, dims) - F) < 1e-13
+    # a wrong pairing (copy spins on another copy's links) does not match
+    if copy:
+        wrong = oracle.gen_copy_spins(dims, 77, copy + 1)
+        assert abs(oracle.su2_functional_spin(U0, wrong, dims) -
+                   oracle.su2_functional(Uc, dims)) > 1e-4
