@@ -1,8 +1,8 @@
 // mag_batch.hip -- batched, graph-captured sweeps over many MAG jobs.
 //
-// The active members are split into five classes (SU(2) / SU(3) x FP32
-// working links of the mixed phase / FP64 links, and the spin-form gauge
-// copy sets).  One sweep of the batch is
+// The active members are split into classes: SU(2) / SU(3) x FP32 working
+// links of the mixed phase / FP64 links, and spin-form gauge-copy sets by
+// copy count.  One sweep of the batch is
 //   for parity 0, 1: one half-sweep launch per non-empty class
 //   one reduction launch (a workgroup per active job, per copy of a copy set)
 //   every reunit_every sweeps: one FP64 rebuild + one normalisation launch
@@ -26,8 +26,10 @@ using mag::fail;
 
 namespace {
 
-constexpr int kClasses = 5;  // 0..3: link form (SU(2)/SU(3) x FP32/FP64), 4: spin form
+// 0..3: link form (SU(2)/SU(3) x FP32/FP64); kSpinClass + K - 1: spin-form
+// copy sets of K copies (one kernel instance per K)
 constexpr int kSpinClass = 4;
+constexpr int kClasses = kSpinClass + MAG_MAX_COPIES;
 
 struct ClassTable {
   int group = 2;
@@ -73,7 +75,7 @@ struct mag_batch {
 namespace {
 
 int class_of(const mag_job* j) {
-  if (j->spin) return kSpinClass;
+  if (j->spin) return kSpinClass + j->ncopies - 1;
   return (j->prm.group == MAG_SU2 ? 0 : 2) + (mag::job_mixed32(j) ? 0 : 1);
 }
 
@@ -116,8 +118,8 @@ int refresh(mag_batch* b) {
   drop_graph(b);
   b->all.clear();
   for (int c = 0; c < kClasses; c++) {
-    b->cls[c].group = (c < 2 || c == kSpinClass) ? 2 : 3;
-    b->cls[c].fp32 = c != kSpinClass && (c % 2) == 0;
+    b->cls[c].group = (c < 2 || c >= kSpinClass) ? 2 : 3;
+    b->cls[c].fp32 = c < kSpinClass && (c % 2) == 0;
     b->cls[c].h.clear();
     b->cls[c].nblocks = 0;
     b->cls[c].nrblocks = 0;
@@ -157,8 +159,11 @@ int refresh(mag_batch* b) {
         const ClassTable& t = b->cls[c];
         mag::launch_half_batch(t.group, t.fp32, t.d, (int)t.h.size(), t.nblocks, p, b->st);
       }
-      const ClassTable& t = b->cls[kSpinClass];
-      mag::launch_copies_half_batch(t.d, (int)t.h.size(), t.nblocks, p, b->st);
+      for (int c = kSpinClass; c < kClasses; c++) {
+        const ClassTable& t = b->cls[c];
+        mag::launch_copies_half_batch(c - kSpinClass + 1, t.d, (int)t.h.size(), t.nblocks, p,
+                                      b->st);
+      }
     }
     mag::launch_reduce_batch(b->d_all, (int)b->all.size(), b->st);
     if (b->reunit > 0 && (i + 1) % b->reunit == 0)

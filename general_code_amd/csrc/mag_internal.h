@@ -125,9 +125,10 @@ struct BatchJob {
 // batched half-sweep of one class (group, FP32 phase or FP64) table
 void launch_half_batch(int group, bool fp32, const BatchJob* jobs, int njobs,
                        int nblocks_total, int parity, hipStream_t st);
-// batched spin-form (gauge-copy) half-sweep of a table of copy sets
-void launch_copies_half_batch(const BatchJob* jobs, int njobs, int nblocks_total, int parity,
-                              hipStream_t st);
+// batched spin-form (gauge-copy) half-sweep of a table of copy sets that
+// all hold ncopies copies
+void launch_copies_half_batch(int ncopies, const BatchJob* jobs, int njobs, int nblocks_total,
+                              int parity, hipStream_t st);
 // batched end-of-sweep reduction: one workgroup per job
 void launch_reduce_batch(const BatchJob* jobs, int njobs, hipStream_t st);
 // batched FP64 rebuild of the FP32 working links + gauge normalisation of a

@@ -75,4 +75,40 @@ __device__ __forceinline__ void block_reduce3(double& a, double& b, double& c) {
   }
 }
 
+// K independent triples with one barrier; for each triple the operations
+// and their order are those of block_reduce3, so the results are identical.
+template <int NW, int K>
+__device__ __forceinline__ void block_reduce3k(double* a, double* b, double* c) {
+  __shared__ double s[3 * K][NW];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    a[k] = wave_sum(a[k]);
+    b[k] = wave_sum(b[k]);
+    c[k] = wave_max(c[k]);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      s[3 * k][w] = a[k];
+      s[3 * k + 1][w] = b[k];
+      s[3 * k + 2][w] = c[k];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      a[k] = s[3 * k][0];
+      b[k] = s[3 * k + 1][0];
+      c[k] = s[3 * k + 2][0];
+      for (int i = 1; i < NW; i++) {
+        a[k] += s[3 * k][i];
+        b[k] += s[3 * k + 1][i];
+        c[k] = fmax(c[k], s[3 * k + 2][i]);
+      }
+    }
+  }
+}
+
 }  // namespace mag

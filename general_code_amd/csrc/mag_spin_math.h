@@ -2,6 +2,10 @@
 // kernels (mag_spin.hip) and the gauge-copy half-sweep (mag_sweep.hip).
 // Spins live in HBM as SoA per parity and component:
 //   s_c(p, h) = S[(p*3 + c)*Vh + h]
+// No multiply-add contraction in these helpers: every product and sum is
+// rounded on its own, as in the oracle (gcc -ffp-contract=off), and a
+// copy's arithmetic cannot change with the code around it (the gauge-copy
+// kernel instances for different copy counts give identical results).
 #pragma once
 #include "mag_internal.h"
 #include "mag_math.h"
@@ -25,6 +29,7 @@ __device__ __forceinline__ void st_spin(double* S, const Geom& g, int p, uint32_
 
 // h += M(A) b (spin::contribution1, matrix.h:909-926)
 __device__ __forceinline__ void contribution1(V3& h, const Q<double>& A, const V3& b) {
+#pragma clang fp contract(off)
   const double q1 = A.a1 * A.a1, q2 = A.a2 * A.a2, q3 = A.a3 * A.a3;
   const double q12 = A.a1 * A.a2, q13 = A.a1 * A.a3, q23 = A.a2 * A.a3;
   const double q01 = A.a0 * A.a1, q02 = A.a0 * A.a2, q03 = A.a0 * A.a3;
@@ -35,6 +40,7 @@ __device__ __forceinline__ void contribution1(V3& h, const Q<double>& A, const V
 }
 // h += M(A)^T b (spin::contribution1_conj, matrix.h:928-946)
 __device__ __forceinline__ void contribution1_conj(V3& h, const Q<double>& A, const V3& b) {
+#pragma clang fp contract(off)
   const double q1 = A.a1 * A.a1, q2 = A.a2 * A.a2, q3 = A.a3 * A.a3;
   const double q12 = A.a1 * A.a2, q13 = A.a1 * A.a3, q23 = A.a2 * A.a3;
   const double q01 = A.a0 * A.a1, q02 = A.a0 * A.a2, q03 = A.a0 * A.a3;
@@ -45,6 +51,7 @@ __device__ __forceinline__ void contribution1_conj(V3& h, const Q<double>& A, co
 }
 // M(A) s (spin::contribution, matrix.h:870-887)
 __device__ __forceinline__ V3 contribution(const V3& s, const Q<double>& A) {
+#pragma clang fp contract(off)
   const double q1 = A.a1 * A.a1, q2 = A.a2 * A.a2, q3 = A.a3 * A.a3;
   const double q12 = 2. * A.a1 * A.a2, q13 = 2. * A.a1 * A.a3, q23 = 2. * A.a2 * A.a3;
   const double q01 = 2. * A.a0 * A.a1, q02 = 2. * A.a0 * A.a2, q03 = 2. * A.a0 * A.a3;
@@ -56,6 +63,7 @@ __device__ __forceinline__ V3 contribution(const V3& s, const Q<double>& A) {
 
 // reflect_fast (matrix.h:825-840)
 __device__ __forceinline__ void reflect_fast(V3& s, const V3& V) {
+#pragma clang fp contract(off)
   const double t1 = (V.a1 * V.a1 - V.a2 * V.a2 - V.a3 * V.a3) * s.a1 +
                     2 * V.a1 * (V.a2 * s.a2 + V.a3 * s.a3);
   const double t2 = (-V.a1 * V.a1 + V.a2 * V.a2 - V.a3 * V.a3) * s.a2 +
@@ -68,11 +76,13 @@ __device__ __forceinline__ void reflect_fast(V3& s, const V3& V) {
 }
 // parallel (matrix.h:842-856)
 __device__ __forceinline__ void parallel(V3& s, const V3& V) {
+#pragma clang fp contract(off)
   const double vn = sqrt(V.a1 * V.a1 + V.a2 * V.a2 + V.a3 * V.a3);
   s = V3{V.a1 / vn, V.a2 / vn, V.a3 / vn};
 }
 // geodesic omega step of s towards h_hat
 __device__ __forceinline__ void spin_omega(V3& s, const V3& h, double omega) {
+#pragma clang fp contract(off)
   const double hn = sqrt(h.a1 * h.a1 + h.a2 * h.a2 + h.a3 * h.a3);
   if (hn == 0.0) return;
   const V3 hh{h.a1 / hn, h.a2 / hn, h.a3 / hn};
@@ -103,6 +113,7 @@ __device__ __forceinline__ void spin_omega(V3& s, const V3& h, double omega) {
 // a diagonal U(1) factor on the left (the link functional is invariant
 // under the same U(1)).
 __device__ __forceinline__ V3 spin_of_gauge(const Q<double>& q) {
+#pragma clang fp contract(off)
   return V3{2 * (q.a1 * q.a3 + q.a0 * q.a2), 2 * (q.a2 * q.a3 - q.a0 * q.a1),
             q.a0 * q.a0 + q.a3 * q.a3 - q.a1 * q.a1 - q.a2 * q.a2};
 }
@@ -113,12 +124,45 @@ __device__ __forceinline__ V3 spin_of_gauge(const Q<double>& q) {
 // G = (0, 1, 0, 0).  Normalised, so a spin off the unit sphere by rounding
 // still gives an SU(2) element.
 __device__ __forceinline__ Q<double> gauge_of_spin(const V3& s) {
+#pragma clang fp contract(off)
   const double t = s.a1 * s.a1 + s.a2 * s.a2;
   const double c2 = s.a3 >= 0 ? 0.5 * (1 + s.a3) : 0.5 * t / (1 - s.a3);
   if (!(c2 > 1e-300)) return Q<double>{0.0, 1.0, 0.0, 0.0};
   const double c = sqrt(c2), r = 0.5 / c;
   const double n = 1.0 / sqrt(c2 + t * r * r);
   return Q<double>{c * n, -s.a2 * r * n, s.a1 * r * n, 0.0};
+}
+
+// The adjoint 3x3 matrix M(A) of a link, entries as in contribution1 with
+// the off-diagonal factor 2 left out: computed once per link and applied to
+// the neighbour spin of every gauge copy.  add_M / add_MT evaluate the
+// expressions of contribution1 / contribution1_conj term for term.
+struct Adj {
+  double d1, d2, d3;             // A5 + 2 q_i
+  double p12, m12, p13, m13, p23, m23;  // q12 +- q03, q13 -+ q02, q23 +- q01
+};
+__device__ __forceinline__ Adj adj_of(const Q<double>& A) {
+#pragma clang fp contract(off)
+  const double q1 = A.a1 * A.a1, q2 = A.a2 * A.a2, q3 = A.a3 * A.a3;
+  const double q12 = A.a1 * A.a2, q13 = A.a1 * A.a3, q23 = A.a2 * A.a3;
+  const double q01 = A.a0 * A.a1, q02 = A.a0 * A.a2, q03 = A.a0 * A.a3;
+  const double A5 = A.a0 * A.a0 - q1 - q2 - q3;
+  return Adj{A5 + 2 * q1, A5 + 2 * q2, A5 + 2 * q3, q12 + q03, q12 - q03,
+             q13 - q02,   q13 + q02,   q23 + q01,   q23 - q01};
+}
+// h += M(A) b
+__device__ __forceinline__ void add_M(V3& h, const Adj& M, const V3& b) {
+#pragma clang fp contract(off)
+  h.a1 += b.a1 * M.d1 + 2 * b.a2 * M.p12 + 2 * b.a3 * M.p13;
+  h.a2 += 2 * b.a1 * M.m12 + b.a2 * M.d2 + 2 * b.a3 * M.p23;
+  h.a3 += 2 * b.a1 * M.m13 + 2 * b.a2 * M.m23 + b.a3 * M.d3;
+}
+// h += M(A)^T b
+__device__ __forceinline__ void add_MT(V3& h, const Adj& M, const V3& b) {
+#pragma clang fp contract(off)
+  h.a1 += b.a1 * M.d1 + 2 * b.a2 * M.m12 + 2 * b.a3 * M.m13;
+  h.a2 += 2 * b.a1 * M.p12 + b.a2 * M.d2 + 2 * b.a3 * M.m23;
+  h.a3 += 2 * b.a1 * M.p13 + 2 * b.a2 * M.p23 + b.a3 * M.d3;
 }
 
 }  // namespace mag

@@ -397,87 +397,102 @@ void launch_half_batch(int group, bool fp32, const BatchJob* jobs, int njobs, in
 // ================================================================== gauge copies
 // Spin-form half-sweep over the gauge copies of one SU(2) configuration,
 // one site per lane (the reference's spin form, mag.cpp:569-626, with the
-// omega step of the link form).  The 8 links of the site are loaded once
-// and stay in registers while every copy that is not done gathers its 8
-// neighbour spins, forms
+// omega step of the link form).  K (the set's copy count) is a template
+// parameter so the K field accumulators live in registers.  Each of the 8
+// links of the site is loaded once, its adjoint matrix M formed once, and
+// applied to the neighbour spin of every copy that is not done:
 //   h = sum_mu [ M(U_x,mu) s(x+mu) + M(U_x-mu,mu)^T s(x-mu) ]
-// (in the order f0, b0, f1, b1, f2, b2, f3, b3 of contribution_site) and
-// rotates s(x) towards h_hat by omega*theta.  Per copy and workgroup the
+// (in the order f0, b0, f1, b1, f2, b2, f3, b3 of contribution_site); then
+// each copy rotates s(x) towards h_hat by omega*theta.  Per copy and workgroup the
 // partial triple is (sum s'.h on the black half, sum d, max d) with
 // d = 1 - s.h_hat before the step: every link has exactly one black end,
 // so the black half's sum of s'.h is 4V times the spin functional
 // (mag.cpp:925-953) after the sweep.  A copy's arithmetic and partition do
 // not depend on the other copies, so its trajectory is the same in a set
 // of any size.
+template <int K>
 __device__ __forceinline__ void su2_copies_block(const double* __restrict__ U,
                                                  double* __restrict__ S, size_t sstride,
-                                                 int ncopies, const Geom& g, int p,
-                                                 double omega, double* __restrict__ part,
-                                                 size_t pstride,
+                                                 const Geom& g, int p, double omega,
+                                                 double* __restrict__ part, size_t pstride,
                                                  const JobCtl* __restrict__ ctl, uint32_t lb) {
+#pragma clang fp contract(off)
+  bool act[K];
   bool live = false;
-  for (int c = 0; c < ncopies; c++) live |= !ctl[c].done;
-  if (!live) return;  // uniform over the workgroup
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    act[k] = !ctl[k].done;  // uniform over the workgroup
+    live |= act[k];
+  }
+  if (!live) return;
   const uint32_t h = lb * kBlock + threadIdx.x;
-  const bool in = h < g.Vh;
-  Q<double> F[4], B[4];
-  uint32_t hf[4], hb[4];
-  if (in) {
+  double fs[K], ds[K], dm[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) fs[k] = ds[k] = dm[k] = 0.0;
+  if (h < g.Vh) {
     const SiteIdx s = decode(g, h, p);
+    V3 H[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) H[k] = V3{0.0, 0.0, 0.0};
 #pragma unroll
     for (int mu = 0; mu < 4; mu++) {
-      hf[mu] = fwd_h(g, h, s, mu);
-      hb[mu] = back_h(g, h, s, mu);
-      F[mu] = ldq(link_ptr<4>(U, g, p, mu, h));
-      B[mu] = ldq(link_ptr<4>(U, g, 1 - p, mu, hb[mu]));
+      const uint32_t hf = fwd_h(g, h, s, mu);
+      const Adj Mf = adj_of(ldq(link_ptr<4>(U, g, p, mu, h)));
+#pragma unroll
+      for (int k = 0; k < K; k++)
+        if (act[k]) add_M(H[k], Mf, ld_spin(S + k * sstride, g, 1 - p, hf));
+      const uint32_t hb = back_h(g, h, s, mu);
+      const Adj Mb = adj_of(ldq(link_ptr<4>(U, g, 1 - p, mu, hb)));
+#pragma unroll
+      for (int k = 0; k < K; k++)
+        if (act[k]) add_MT(H[k], Mb, ld_spin(S + k * sstride, g, 1 - p, hb));
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      if (!act[k]) continue;
+      double* Sk = S + k * sstride;
+      V3 sp = ld_spin(Sk, g, p, h);
+      const V3& Hk = H[k];
+      const double hn = sqrt(Hk.a1 * Hk.a1 + Hk.a2 * Hk.a2 + Hk.a3 * Hk.a3);
+      const double d = hn > 0 ? 1 - (sp.a1 * Hk.a1 + sp.a2 * Hk.a2 + sp.a3 * Hk.a3) / hn : 0.0;
+      spin_omega(sp, Hk, omega);
+      st_spin(Sk, g, p, h, sp);
+      if (p == 1) fs[k] = sp.a1 * Hk.a1 + sp.a2 * Hk.a2 + sp.a3 * Hk.a3;
+      ds[k] = d;
+      dm[k] = d;
     }
   }
-  for (int c = 0; c < ncopies; c++) {
-    if (ctl[c].done) continue;  // uniform over the workgroup
-    double* Sc = S + (size_t)c * sstride;
-    double fs = 0.0, ds = 0.0, dm = 0.0;
-    if (in) {
-      V3 H{0.0, 0.0, 0.0};
+  block_reduce3k<kWaves, K>(fs, ds, dm);
+  if (threadIdx.x == 0) {
 #pragma unroll
-      for (int mu = 0; mu < 4; mu++) {
-        contribution1(H, F[mu], ld_spin(Sc, g, 1 - p, hf[mu]));
-        contribution1_conj(H, B[mu], ld_spin(Sc, g, 1 - p, hb[mu]));
-      }
-      V3 sp = ld_spin(Sc, g, p, h);
-      const double hn = sqrt(H.a1 * H.a1 + H.a2 * H.a2 + H.a3 * H.a3);
-      const double d = hn > 0 ? 1 - (sp.a1 * H.a1 + sp.a2 * H.a2 + sp.a3 * H.a3) / hn : 0.0;
-      spin_omega(sp, H, omega);
-      st_spin(Sc, g, p, h, sp);
-      if (p == 1) fs = sp.a1 * H.a1 + sp.a2 * H.a2 + sp.a3 * H.a3;
-      ds = d;
-      dm = d;
+    for (int k = 0; k < K; k++) {
+      if (!act[k]) continue;
+      double* o = part + k * pstride + 3 * (size_t)lb;
+      o[0] = fs[k];
+      o[1] = ds[k];
+      o[2] = dm[k];
     }
-    block_reduce3<kWaves>(fs, ds, dm);
-    if (threadIdx.x == 0) {
-      double* o = part + (size_t)c * pstride + 3 * (size_t)lb;
-      o[0] = fs;
-      o[1] = ds;
-      o[2] = dm;
-    }
-    __syncthreads();  // the reduction's LDS is reused by the next copy
   }
 }
 
+template <int K>
 __global__ __launch_bounds__(kBlock) void k_su2_copies_half(const double* __restrict__ U,
                                                             double* __restrict__ S,
-                                                            size_t sstride, int ncopies, Geom g,
-                                                            int p, double omega,
+                                                            size_t sstride, Geom g, int p,
+                                                            double omega,
                                                             double* __restrict__ part,
                                                             size_t pstride,
                                                             const JobCtl* __restrict__ ctl) {
-  su2_copies_block(U, S, sstride, ncopies, g, p, omega, part, pstride, ctl, blockIdx.x);
+  su2_copies_block<K>(U, S, sstride, g, p, omega, part, pstride, ctl, blockIdx.x);
 }
 
+// every copy set of the table has K copies
+template <int K>
 __global__ __launch_bounds__(kBlock) void k_copies_half_batch(const BatchJob* __restrict__ jobs,
                                                               int njobs, int p) {
   const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
-  su2_copies_block((const double*)j.U, j.S, j.sstride, j.ncopies, j.g, p, j.omega,
-                   j.part + (size_t)3 * j.nb * p, j.pstride, j.ctl, blockIdx.x - j.block0);
+  su2_copies_block<K>((const double*)j.U, j.S, j.sstride, j.g, p, j.omega,
+                      j.part + (size_t)3 * j.nb * p, j.pstride, j.ctl, blockIdx.x - j.block0);
 }
 
 // one workgroup per copy
@@ -497,8 +512,21 @@ void launch_su2_copies_half(const double* U, double* S, size_t sstride, int ncop
                             const Geom& g, int parity, double omega, double* part,
                             size_t pstride, const JobCtl* ctl, hipStream_t st) {
   const int nb = nblocks(g);
-  hipLaunchKernelGGL(k_su2_copies_half, dim3(nb), dim3(kBlock), 0, st, U, S, sstride, ncopies,
-                     g, parity, omega, part + (size_t)3 * nb * parity, pstride, ctl);
+  double* pp = part + (size_t)3 * nb * parity;
+#define L_(K)                                                                                \
+  hipLaunchKernelGGL(k_su2_copies_half<K>, dim3(nb), dim3(kBlock), 0, st, U, S, sstride, g,  \
+                     parity, omega, pp, pstride, ctl)
+  switch (ncopies) {
+    case 1: L_(1); break;
+    case 2: L_(2); break;
+    case 3: L_(3); break;
+    case 4: L_(4); break;
+    case 5: L_(5); break;
+    case 6: L_(6); break;
+    case 7: L_(7); break;
+    default: L_(8); break;
+  }
+#undef L_
 }
 void launch_reduce_copies(const double* part, size_t pstride, int nparts, int ncopies,
                           double invF, double invV, JobCtl* ctl, double* hF, double* hR,
@@ -506,11 +534,23 @@ void launch_reduce_copies(const double* part, size_t pstride, int nparts, int nc
   hipLaunchKernelGGL(k_reduce_copies, dim3(ncopies), dim3(kRedBlock), 0, st, part, pstride,
                      nparts, invF, invV, ctl, hF, hR, hA, hstride);
 }
-void launch_copies_half_batch(const BatchJob* jobs, int njobs, int nblocks_total, int parity,
-                              hipStream_t st) {
+void launch_copies_half_batch(int ncopies, const BatchJob* jobs, int njobs, int nblocks_total,
+                              int parity, hipStream_t st) {
   if (njobs == 0 || nblocks_total == 0) return;
-  hipLaunchKernelGGL(k_copies_half_batch, dim3(nblocks_total), dim3(kBlock), 0, st, jobs, njobs,
-                     parity);
+#define L_(K)                                                                                 \
+  hipLaunchKernelGGL(k_copies_half_batch<K>, dim3(nblocks_total), dim3(kBlock), 0, st, jobs, \
+                     njobs, parity)
+  switch (ncopies) {
+    case 1: L_(1); break;
+    case 2: L_(2); break;
+    case 3: L_(3); break;
+    case 4: L_(4); break;
+    case 5: L_(5); break;
+    case 6: L_(6); break;
+    case 7: L_(7); break;
+    default: L_(8); break;
+  }
+#undef L_
 }
 
 }  // namespace mag
