@@ -229,6 +229,81 @@ def batch_bench(G, max_sweeps=8000, su3_precision=None, su3_warmup=100):
     }
 
 
+SERVE_SIZES = [(16,) * 4, (24,) * 4, (32,) * 4, (24, 24, 24, 48), (48,) * 4]
+
+
+def serve_workload(n=40, rate=20.0, seed=4):
+    """Poisson arrivals (mean `rate` per second, seeded): 60% SU(2) (eps 0.5,
+    omega 1.7, tol 1e-12) over the five sizes, 40% SU(3) (eps 0.4, omega
+    1.6, tol 1e-11) over the four that are not SU(2)-only (48^4)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    reqs = []
+    for i in range(n):
+        su2 = bool(rng.random() < 0.6)
+        dims = SERVE_SIZES[int(rng.integers(0, 5 if su2 else 4))]
+        reqs.append(dict(t=float(t[i]), group=2 if su2 else 3, dims=dims, eps=0.5 if su2 else 0.4,
+                         omega=1.7 if su2 else 1.6, tol=1e-12 if su2 else 1e-11, seed=5000 + i))
+    return reqs
+
+
+def serve(G, reqs, plain, max_sweeps, su3_tol_mixed=1e-10, su3_warmup=100):
+    """Replay the arrival stream against a MagServer.  plain: one job at a
+    time, FP64 link form (the reference's one-configuration-per-run use);
+    else continuous batching with SU(2) in the spin form (FP64) and SU(3)
+    in mixed precision after an FP64 warm-up."""
+    def params(r):
+        p = G.MagParams(group=r["group"], dims=r["dims"], omega=r["omega"], tol=r["tol"],
+                        max_sweeps=max_sweeps)
+        if not plain:
+            if r["group"] == 2:
+                p.form, p.copies = G.FORM_SPIN, 1
+            else:
+                p.precision, p.tol_mixed, p.fp64_warmup = G.MIXED, su3_tol_mixed, su3_warmup
+        return p
+
+    with G.MagServer(max_active=1 if plain else 0) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for r in reqs:
+            dt = r["t"] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(params(r), eps=r["eps"], seed=r["seed"]))
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    lat = sorted(r["t_done"] - r["t_submit"] for r in res)
+    span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+    pct = lambda q: lat[min(len(lat) - 1, int(round(q * (len(lat) - 1))))]  # noqa: E731
+    return res, {
+        "p50_ms": round(1e3 * pct(0.50), 2),
+        "p99_ms": round(1e3 * pct(0.99), 2),
+        "configs_per_s": len(res) / span,
+        "makespan_s": round(span, 3),
+        "converged": sum(1 for r in res if r["converged"]),
+        "peak_active": st["peak_active"],
+        "peak_hbm_MiB": round(st["peak_hbm"] / 2 ** 20, 1),
+    }
+
+
+def serve_bench(G, n=40, max_sweeps=4000):
+    reqs = serve_workload(n)
+    plain_res, plain = serve(G, reqs, True, max_sweeps)
+    res, served = serve(G, reqs, False, max_sweeps)
+    dF = [abs(a["F"] - b["F"]) / abs(b["F"]) for a, b in zip(res, plain_res)]
+    return {
+        "workload": f"{n} requests, Poisson 20/s (seeded), 60% SU(2) / 40% SU(3), sizes "
+                    "16^4, 24^4, 32^4, 24^3x48, 48^4 (SU(2) only); sweep cap "
+                    f"{max_sweeps}",
+        "plain_fp64_one_at_a_time": plain,
+        "continuous_batching": served,
+        "throughput_ratio": served["configs_per_s"] / plain["configs_per_s"],
+        "max_rel_dF_vs_plain": max(dF),
+        "requests_rel_dF_above_1e-9": sum(1 for d in dF if d > 1e-9),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)

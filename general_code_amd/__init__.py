@@ -4,7 +4,7 @@ The compute path is libmag_gpu.so (hand-written gfx950 HIP kernels behind the
 C ABI of include/mag_gpu.h); this package is its Python front end.
 """
 from .mag import (  # noqa: F401
-    F64, FORM_LINK, FORM_SPIN, MAX_COPIES, MIXED, SU2, SU3, MAG_OK, MagBatch, MagError, MagJob, MagParams, NotConverged,
+    F64, FORM_LINK, FORM_SPIN, MAX_COPIES, MIXED, SU2, SU3, MAG_OK, MagBatch, MagError, MagJob, MagParams, MagServer, NotConverged,
     SPIN_HEATBATH, SPIN_OMEGA, SPIN_OR, SPIN_RELAX, SpinSession, device_count,
     functional_spin, gauge_transform_spins, generate_spins_uniform, hbm_copy_bandwidth, lib, plaquette, su2_functional, su3_functional,
 )

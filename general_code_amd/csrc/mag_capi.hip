@@ -199,7 +199,9 @@ void init_copy(mag_job* j, int c) {
   JobCtl& h = j->ctl_h[c];
   std::memset(&h, 0, sizeof(JobCtl));
   h.max_sweeps = j->prm.max_sweeps;
-  h.tol = (j->prm.precision == MAG_PREC_MIXED && j->phase == 0) ? j->prm.tol_mixed : j->prm.tol;
+  const bool fp32 = j->prm.precision == MAG_PREC_MIXED && j->phase == 0;
+  h.tol = fp32 ? j->prm.tol_mixed : j->prm.tol;
+  h.window = fp32 ? j->prm.reunit_every : 0;
   hipMemcpyAsync(j->ctl + c, &h, sizeof(JobCtl), hipMemcpyHostToDevice, j->st);
   double* part = j->part + (size_t)c * j->pstride;
   if (j->spin) mag::launch_spin_functional(j->U64, j->S + (size_t)c * j->sstride, j->g, part,
@@ -219,6 +221,7 @@ int start_fp32_phase(mag_job* j) {
   j->phase = 0;
   j->since_rebuild = 0;
   j->ctl_h->tol = j->prm.tol_mixed;
+  j->ctl_h->window = j->prm.reunit_every;
   return write_ctl(j);
 }
 
@@ -292,6 +295,7 @@ int finalize_mixed(mag_job* j) {
   HIPCHK(hipGetLastError());
   CHK(read_ctl(j));
   j->ctl_h->tol = j->prm.tol;
+  j->ctl_h->window = 0;
   j->ctl_h->converged = 0;
   j->ctl_h->done = (j->ctl_h->sweep >= j->ctl_h->max_sweeps) ? 1 : 0;
   return write_ctl(j);

@@ -43,7 +43,10 @@ struct JobCtl {
   double tol;      // relative functional change for convergence
   double F_prev;   // functional after the last sweep (initial value first)
   double F0;       // functional of the uploaded configuration
-  double pad;
+  int window;      // > 1: convergence on the mean change per sweep over the
+                   // last `window` sweeps (the FP32 phase: a whole period
+                   // between two FP64 rebuilds); else on the last sweep
+  int pad;
 };
 
 // reduction kernel modes

@@ -71,7 +71,9 @@ __device__ void reduce_job(const double* __restrict__ part, int nparts, int mode
       hF[sw] = F;
       hR[sw] = c;
       hA[sw] = b * invV;
-      const double rel = fabs(F - ctl->F_prev) / fabs(F);
+      const int w = ctl->window;
+      double rel = fabs(F - ctl->F_prev) / fabs(F);
+      if (w > 1) rel = sw >= w ? fabs(F - hF[sw - w]) / (w * fabs(F)) : INFINITY;
       ctl->F_prev = F;
       ctl->sweep = sw + 1;
       if (rel < ctl->tol) {
@@ -232,6 +234,20 @@ template void launch_su2_half<double>(double*, double*, const Geom&, int, double
                                       const JobCtl*, hipStream_t);
 
 // ================================================================== SU(3)
+// rows 0, 1 scaled by (3 - |row|^2) / 2, the Newton step of 1/|row| at 1
+template <typename T>
+__device__ __forceinline__ void renorm_rows(M2<T>& m) {
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    T n = T(0);
+#pragma unroll
+    for (int c = 0; c < 3; c++) n += m.r[i][c].re * m.r[i][c].re + m.r[i][c].im * m.r[i][c].im;
+    const T f = T(1.5) - T(0.5) * n;
+#pragma unroll
+    for (int c = 0; c < 3; c++) m.r[i][c] = C<T>{m.r[i][c].re * f, m.r[i][c].im * f};
+  }
+}
+
 // Cabibbo-Marinari subgroup (I,J) update of the 4 links a lane holds
 // (forward F[k], backward B[k], mu = 2q + k), rows 0,1 in registers, row 2
 // rebuilt where the subgroup touches it.  Returns 1 - cos(theta).
@@ -321,6 +337,16 @@ __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict_
     T res = cm_subgroup<T, 0, 1, GA>(F, B, Gx, omega);
     res += cm_subgroup<T, 1, 2, GA>(F, B, Gx, omega);
     res += cm_subgroup<T, 0, 2, GA>(F, B, Gx, omega);
+    if constexpr (sizeof(T) == 4) {
+      // FP32: the stored rows lose norm steadily (measured: |row|^2 - 1
+      // about -1e-7 per sweep, lowering F between FP64 rebuilds); one Newton
+      // step of 1/sqrt(|row|^2) per update keeps them on the sphere
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        renorm_rows(F[k]);
+        renorm_rows(B[k]);
+      }
+    }
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       stm(pf[k], F[k]);

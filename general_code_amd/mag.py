@@ -66,6 +66,43 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 
+
+class _Request(ctypes.Structure):
+    _fields_ = [
+        ("params", _Params),
+        ("links", ctypes.POINTER(ctypes.c_double)),
+        ("eps", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("copy", ctypes.c_uint32),
+    ]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64),
+        ("status", ctypes.c_int),
+        ("sweeps", ctypes.c_int),
+        ("converged", ctypes.c_int),
+        ("best_copy", ctypes.c_int),
+        ("fp64_start", ctypes.c_int),
+        ("F", ctypes.c_double),
+        ("t_submit", ctypes.c_double),
+        ("t_admit", ctypes.c_double),
+        ("t_done", ctypes.c_double),
+    ]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("budget", ctypes.c_uint64),
+        ("peak_hbm", ctypes.c_uint64),
+        ("submitted", ctypes.c_uint64),
+        ("finished", ctypes.c_uint64),
+        ("admitted", ctypes.c_uint64),
+        ("replays", ctypes.c_uint64),
+        ("peak_active", ctypes.c_int),
+    ]
+
 # name -> (restype, argtypes); every entry point of include/mag_gpu.h
 SIGNATURES = {
     "mag_params_default": (None, [ctypes.POINTER(_Params)]),
@@ -114,6 +151,15 @@ SIGNATURES = {
     "mag_batch_run": (ctypes.c_int, [_vp]),
     "mag_batch_replays_timed": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "mag_batch_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "mag_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_int,
+                                         ctypes.POINTER(_vp)]),
+    "mag_server_destroy": (ctypes.c_int, [_vp]),
+    "mag_server_submit": (ctypes.c_int, [_vp, ctypes.POINTER(_Request),
+                                         ctypes.POINTER(ctypes.c_uint64)]),
+    "mag_server_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_Result)]),
+    "mag_server_drain": (ctypes.c_int, [_vp]),
+    "mag_server_clock": (ctypes.c_int, [_vp, _dp]),
+    "mag_server_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(_ServerStats)]),
     # spin form (reference mag.h surface)
     "mag_spin_session_create": (ctypes.c_int, [_ip, ctypes.c_int, ctypes.POINTER(_vp)]),
     "mag_spin_session_destroy": (ctypes.c_int, [_vp]),
@@ -606,3 +652,69 @@ class MagBatch:
         ms = ctypes.c_double()
         _check(lib().mag_batch_replays_timed(self._h, int(n), ctypes.byref(ms)))
         return ms.value
+
+
+# serving -------------------------------------------------------------------
+class MagServer:
+    """Continuous-batching server (mag_server_* of include/mag_gpu.h): a
+    native scheduler thread admits requests by HBM footprint and sweeps every
+    admitted job in one graph-captured batch until it converges."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, max_active: int = 0):
+        h = _vp()
+        _check(lib().mag_server_create(int(device), int(hbm_budget), int(max_active),
+                                       ctypes.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mag_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, params: MagParams | None = None, eps: float = 0.0, seed: int = 0,
+               copy: int = 0, links: np.ndarray | None = None, **kw) -> int:
+        p = params if params is not None else MagParams(**kw)
+        r = _Request()
+        r.params = p.to_c()
+        r.eps = float(eps)
+        r.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        r.copy = int(copy)
+        if links is not None:
+            a = np.ascontiguousarray(links, dtype=np.float64)
+            if a.size != 4 * p.volume * p.reals_in:
+                raise ValueError(f"links must hold {4 * p.volume * p.reals_in} reals")
+            r.links = _dptr(a)
+        rid = ctypes.c_uint64()
+        _check(lib().mag_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int) -> dict:
+        r = _Result()
+        _check(lib().mag_server_wait(self._h, int(rid), ctypes.byref(r)))
+        return {f: getattr(r, f) for f, _ in _Result._fields_}
+
+    def drain(self):
+        _check(lib().mag_server_drain(self._h))
+
+    def clock(self) -> float:
+        t = ctypes.c_double()
+        _check(lib().mag_server_clock(self._h, ctypes.byref(t)))
+        return t.value
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        _check(lib().mag_server_get_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in _ServerStats._fields_}

@@ -64,8 +64,12 @@ typedef struct mag_params {
   int max_sweeps;     /* sweep cap (all phases together)                       */
   int reunit_every;   /* mixed: rebuild FP32 links from the FP64 master      */
                       /*        every reunit_every sweeps                      */
-  double tol_mixed;   /* mixed: FP32 phase ends at this relative change;     */
-                      /*        FP64 sweeps then run down to tol              */
+  double tol_mixed;   /* mixed: FP32 phase ends when the mean relative     */
+                      /*        change per sweep over the last reunit_every   */
+                      /*        sweeps drops below it (the FP32 functional    */
+                      /*        steps at each rebuild; a whole period between */
+                      /*        two rebuilds averages that out); FP64 sweeps  */
+                      /*        then run down to tol                          */
   int form;           /* MAG_FORM_LINK or MAG_FORM_SPIN                        */
   int copies;         /* gauge copies of the configuration swept together    */
                       /* (spin form, 1..MAG_MAX_COPIES; 1 in the link form)  */
@@ -223,6 +227,62 @@ int mag_batch_run(mag_batch *b);
 /* Measurement: n replays between HIP events on the batch stream. */
 int mag_batch_replays_timed(mag_batch *b, int n, double *ms);
 int mag_batch_stream(mag_batch *b, void **stream);
+
+/* ------------------------------------------------------------ serving
+ * No counterpart in the reference (one configuration per process,
+ * apps/mag_su2/mag_fixation.cpp).  A server owns one device, a scheduler
+ * thread and a batch.  Requests are submitted from any thread and admitted
+ * in arrival order while the HBM footprint of the admitted jobs
+ * (mag_job_footprint) stays within the budget and their count within
+ * max_active (0: no limit); an admitted job joins the batch at the next
+ * replay (continuous batching), keeps its links in HBM until it has
+ * converged (or hit its sweep cap), and leaves the batch at once, freeing
+ * its memory.  Each result is the one the job gives when run alone.
+ * Mixed-precision requests served together must share reunit_every. */
+typedef struct mag_server mag_server;
+typedef struct mag_request {
+  mag_params params;
+  const double *links; /* links in the reference layout, copied at submit, */
+                       /* or NULL: synthetic input (mag_job_generate_copy;  */
+                       /* the spin form generates its copies 0..copies-1)   */
+  double eps;
+  uint64_t seed;
+  uint32_t copy;
+} mag_request;
+typedef struct mag_result {
+  uint64_t id;
+  int status;       /* MAG_OK, MAG_ERR_NOT_CONVERGED, or the job's error     */
+  int sweeps;       /* sweeps of the reported copy                          */
+  int converged;    /* every copy converged                                 */
+  int best_copy;    /* copy with the largest functional (spin form)         */
+  int fp64_start;   /* mixed: sweep of the switch to the FP64 polish        */
+  double F;         /* final functional of best_copy                        */
+  double t_submit;  /* seconds on the server clock: submitted, admitted,    */
+  double t_admit;   /* finished                                             */
+  double t_done;
+} mag_result;
+typedef struct mag_server_stats {
+  uint64_t budget;     /* HBM budget in bytes                               */
+  uint64_t peak_hbm;   /* largest footprint admitted at once                */
+  uint64_t submitted;
+  uint64_t finished;
+  uint64_t admitted;
+  uint64_t replays;    /* batch replays launched                            */
+  int peak_active;     /* most jobs on the GPU at once                      */
+} mag_server_stats;
+
+/* hbm_budget 0: 80% of the device's free memory */
+int mag_server_create(int device, uint64_t hbm_budget, int max_active, mag_server **out);
+/* finishes every submitted request, then stops the scheduler */
+int mag_server_destroy(mag_server *s);
+/* non-blocking; a request larger than the whole budget is refused */
+int mag_server_submit(mag_server *s, const mag_request *r, uint64_t *id);
+/* blocks until request id is finished */
+int mag_server_wait(mag_server *s, uint64_t id, mag_result *out);
+/* blocks until every request submitted so far is finished */
+int mag_server_drain(mag_server *s);
+int mag_server_clock(mag_server *s, double *seconds);
+int mag_server_get_stats(mag_server *s, mag_server_stats *st);
 
 /* ------------------------------------------------------------ spin form
  * The reference's own algorithm (lib/cpu/src/mag.cpp): one unit spin s(x)

@@ -1,0 +1,108 @@
+"""Continuous-batching server: results equal to each job run alone,
+admission by HBM footprint, requests overlapping on the device."""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REQS = [
+    dict(group=2, dims=(8, 8, 8, 8), form=1, copies=2, omega=1.7, tol=1e-12, eps=0.5, seed=1),
+    dict(group=2, dims=(8, 8, 4, 8), omega=1.7, tol=1e-12, eps=0.5, seed=2, copy=1),
+    dict(group=3, dims=(4, 4, 4, 8), precision=1, fp64_warmup=20, omega=1.6, tol=1e-11,
+         eps=0.4, seed=3),
+    dict(group=3, dims=(4, 4, 4, 4), omega=1.6, tol=1e-11, eps=0.4, seed=4),
+    dict(group=2, dims=(8, 8, 8, 8), precision=1, omega=1.7, tol=1e-12, eps=0.5, seed=5),
+]
+
+
+def _params(mag, r):
+    kw = {k: v for k, v in r.items() if k not in ("eps", "seed", "copy")}
+    return mag.MagParams(max_sweeps=4000, reunit_every=10, tol_mixed=1e-8, **kw)
+
+
+def _alone(mag, r):
+    p = _params(mag, r)
+    with mag.MagJob(p) as j:
+        if p.form == mag.FORM_SPIN:
+            j.generate(r["eps"], r["seed"])
+        else:
+            j.generate(r["eps"], r["seed"], copy=r.get("copy", 0))
+        j.run()
+        best, F = j.best_copy()
+        sw, conv, _ = j.copy_status(best)
+        _, allconv, _ = j.status()
+        return dict(F=F, sweeps=sw, best_copy=best, converged=allconv, fp64_start=j.fp64_start())
+
+
+def test_served_results_equal_alone(mag):
+    ref = [_alone(mag, r) for r in REQS]
+    with mag.MagServer() as srv:
+        ids = [srv.submit(_params(mag, r), eps=r["eps"], seed=r["seed"], copy=r.get("copy", 0))
+               for r in REQS]
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    for r, a in zip(res, ref):
+        assert r["status"] == 0
+        assert r["F"] == a["F"]
+        assert r["sweeps"] == a["sweeps"]
+        assert r["best_copy"] == a["best_copy"]
+        assert bool(r["converged"]) == a["converged"]
+        assert r["fp64_start"] == a["fp64_start"]
+        assert r["t_submit"] <= r["t_admit"] <= r["t_done"]
+    assert st["finished"] == len(REQS) and st["admitted"] == len(REQS)
+    assert st["peak_active"] >= 2  # the requests overlapped on the device
+
+
+def test_admission_by_footprint(mag):
+    big = dict(group=2, dims=(16, 16, 16, 16), omega=1.7, tol=1e-12, eps=0.5, seed=7)
+    fp = _params(mag, big).footprint()
+    with mag.MagServer(hbm_budget=int(1.5 * fp)) as srv:
+        ids = [srv.submit(_params(mag, big), eps=0.5, seed=7 + k) for k in range(3)]
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    assert st["peak_hbm"] <= int(1.5 * fp)
+    assert st["peak_active"] == 1
+    # one at a time, in arrival order
+    for a, b in zip(res, res[1:]):
+        assert b["t_admit"] >= a["t_done"] - 1e-3
+    assert all(r["status"] == 0 for r in res)
+
+
+def test_refuses_request_larger_than_budget(mag):
+    with mag.MagServer(hbm_budget=1 << 20) as srv:
+        with pytest.raises(mag.MagError):
+            srv.submit(mag.MagParams(group=2, dims=(16, 16, 16, 16)), eps=0.5, seed=1)
+
+
+def test_late_short_request_overtakes(mag):
+    """Continuous batching: a small request that arrives while a large one
+    runs is admitted at once and finishes first."""
+    long = mag.MagParams(group=2, dims=(32, 32, 32, 32), omega=1.7, tol=1e-12, max_sweeps=4000)
+    short = mag.MagParams(group=2, dims=(8, 8, 8, 8), omega=1.7, tol=1e-12, max_sweeps=4000)
+    with mag.MagServer() as srv:
+        a = srv.submit(long, eps=0.5, seed=11)
+        time.sleep(0.02)
+        b = srv.submit(short, eps=0.5, seed=12)
+        rb = srv.wait(b)
+        ra = srv.wait(a)
+    assert rb["t_admit"] < ra["t_done"]
+    assert rb["t_done"] < ra["t_done"]
+    assert ra["status"] == 0 and rb["status"] == 0
+    alone = _alone(mag, dict(group=2, dims=(8, 8, 8, 8), omega=1.7, tol=1e-12, eps=0.5, seed=12))
+    assert rb["F"] == alone["F"] and rb["sweeps"] == alone["sweeps"]
+
+
+def test_served_from_host_links(mag, oracle):
+    dims = (8, 8, 4, 8)
+    links = oracle.gen_su2_config(dims, 0.4, 99)
+    p = mag.MagParams(group=2, dims=dims, omega=1.7, tol=1e-12, max_sweeps=4000)
+    with mag.MagJob(p) as j:
+        j.upload(links)
+        j.run()
+        s, c, F = j.status()
+    with mag.MagServer() as srv:
+        r = srv.wait(srv.submit(p, links=links))
+    assert r["F"] == F and r["sweeps"] == s and r["status"] == 0
+    assert np.isfinite(r["F"])
