@@ -167,21 +167,23 @@ def run_plain(G, specs, max_sweeps):
 
 def run_batched(G, specs, max_sweeps, su3_precision, su3_warmup):
     """All configurations in one graph-captured batch: each SU(2)
-    configuration is one spin-form copy set (FP64, links read once for all
-    its copies), each SU(3) copy a link-form job (mixed precision with an
-    FP64 warm-up, or FP64)."""
+    configuration is one spin-form copy set (links read once for all its
+    copies; mixed: FP32-rounded links after the FP64 warm-up, spins and
+    arithmetic in FP64), each SU(3) copy a link-form job (mixed: FP32 sweeps
+    after the warm-up); `su3_precision` applies to both."""
     jobs = []
     for sp in specs:
         if sp["group"] == 2:
             j = G.MagJob(group=2, dims=sp["dims"], omega=sp["omega"], tol=sp["tol"],
-                         max_sweeps=max_sweeps, form=G.FORM_SPIN, copies=sp["copies"])
+                         max_sweeps=max_sweeps, form=G.FORM_SPIN, copies=sp["copies"],
+                         precision=su3_precision, tol_mixed=sp["tol"], fp64_warmup=su3_warmup)
             j.generate(sp["eps"], sp["seed"])
             jobs.append((j, list(range(sp["copies"]))))
         else:
             for c in range(sp["copies"]):
                 j = G.MagJob(group=3, dims=sp["dims"], omega=sp["omega"], tol=sp["tol"],
                              max_sweeps=max_sweeps, precision=su3_precision, reunit_every=10,
-                             tol_mixed=1e-9, fp64_warmup=su3_warmup)
+                             tol_mixed=1e-11, fp64_warmup=su3_warmup)
                 j.generate(sp["eps"], sp["seed"], copy=c)
                 jobs.append((j, [None]))
     batch = G.MagBatch()
@@ -208,7 +210,7 @@ def batch_bench(G, max_sweeps=8000, su3_precision=None, su3_warmup=100):
     su3_precision = G.MIXED if su3_precision is None else su3_precision
     t_plain, plain = run_plain(G, specs, max_sweeps)
     t_batch, batched = run_batched(G, specs, max_sweeps, su3_precision, su3_warmup)
-    dF = [abs(b[2] - p[2]) / abs(p[2]) for p, b in zip(plain, batched)]
+    dF = [abs(b[2] - p[2]) / abs(p[2]) for p, b in zip(plain, batched) if p[1] and b[1]]
     return {
         "workload": "16 configurations (4x16^4, 4x24^4, 4x32^4 SU(2), 4x16^3x32 SU(3)), "
                     f"{n} gauge copies, each run to convergence",
@@ -222,7 +224,7 @@ def batch_bench(G, max_sweeps=8000, su3_precision=None, su3_warmup=100):
         "su3_fp64_warmup": su3_warmup,
         "converged_plain": sum(1 for p in plain if p[1]),
         "converged_batched": sum(1 for b in batched if b[1]),
-        "max_rel_dF_vs_plain": max(dF),
+        "max_rel_dF_vs_plain_converged": max(dF) if dF else None,
         "copies_rel_dF_above_1e-9": sum(1 for d in dF if d > 1e-9),
         "sweeps_plain": [p[0] for p in plain],
         "sweeps_batched": [b[0] for b in batched],
@@ -248,19 +250,21 @@ def serve_workload(n=40, rate=20.0, seed=4):
     return reqs
 
 
-def serve(G, reqs, plain, max_sweeps, su3_tol_mixed=1e-10, su3_warmup=100):
+def serve(G, reqs, plain, max_sweeps, su3_tol_mixed=1e-11, su3_warmup=100):
     """Replay the arrival stream against a MagServer.  plain: one job at a
     time, FP64 link form (the reference's one-configuration-per-run use);
-    else continuous batching with SU(2) in the spin form (FP64) and SU(3)
-    in mixed precision after an FP64 warm-up."""
+    else continuous batching, mixed precision after an FP64 warm-up: SU(2)
+    in the spin form (FP32-rounded links, FP64 spins), SU(3) in the link
+    form (FP32 sweeps)."""
     def params(r):
         p = G.MagParams(group=r["group"], dims=r["dims"], omega=r["omega"], tol=r["tol"],
                         max_sweeps=max_sweeps)
         if not plain:
+            p.precision, p.fp64_warmup = G.MIXED, su3_warmup
             if r["group"] == 2:
-                p.form, p.copies = G.FORM_SPIN, 1
+                p.form, p.copies, p.tol_mixed = G.FORM_SPIN, 1, r["tol"]
             else:
-                p.precision, p.tol_mixed, p.fp64_warmup = G.MIXED, su3_tol_mixed, su3_warmup
+                p.tol_mixed = su3_tol_mixed
         return p
 
     with G.MagServer(max_active=1 if plain else 0) as srv:
@@ -291,7 +295,9 @@ def serve_bench(G, n=40, max_sweeps=4000):
     reqs = serve_workload(n)
     plain_res, plain = serve(G, reqs, True, max_sweeps)
     res, served = serve(G, reqs, False, max_sweeps)
-    dF = [abs(a["F"] - b["F"]) / abs(b["F"]) for a, b in zip(res, plain_res)]
+    # functional agreement over the requests that converged in both runs
+    dF = [abs(a["F"] - b["F"]) / abs(b["F"]) for a, b in zip(res, plain_res)
+          if a["converged"] and b["converged"]]
     return {
         "workload": f"{n} requests, Poisson 20/s (seeded), 60% SU(2) / 40% SU(3), sizes "
                     "16^4, 24^4, 32^4, 24^3x48, 48^4 (SU(2) only); sweep cap "
@@ -299,7 +305,8 @@ def serve_bench(G, n=40, max_sweeps=4000):
         "plain_fp64_one_at_a_time": plain,
         "continuous_batching": served,
         "throughput_ratio": served["configs_per_s"] / plain["configs_per_s"],
-        "max_rel_dF_vs_plain": max(dF),
+        "max_rel_dF_vs_plain_converged": max(dF) if dF else None,
+        "requests_compared": len(dF),
         "requests_rel_dF_above_1e-9": sum(1 for d in dF if d > 1e-9),
     }
 
@@ -312,6 +319,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -348,6 +358,15 @@ def main():
                 "b_site": r["roofline"]["b_site"],
             }
 
+    batched = None
+    if not args.no_batch:
+        print("[bench] batched gauge copies", file=sys.stderr, flush=True)
+        batched = batch_bench(G)
+    served = None
+    if not args.no_serve:
+        print("[bench] serving", file=sys.stderr, flush=True)
+        served = serve_bench(G, args.serve_requests)
+
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
@@ -382,6 +401,8 @@ def main():
         "frac_of_copy": round(head["roofline"]["achieved"] / copy_gbs, 4),
         "cpu_baseline": cpu,
         "extra": extra,
+        "batched": batched,
+        "serving": served,
     }
     print(json.dumps(line), flush=True)
 

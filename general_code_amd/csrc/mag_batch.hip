@@ -18,6 +18,7 @@
 #include "mag_job_impl.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <new>
 #include <vector>
 
@@ -26,10 +27,11 @@ using mag::fail;
 
 namespace {
 
-// 0..3: link form (SU(2)/SU(3) x FP32/FP64); kSpinClass + K - 1: spin-form
-// copy sets of K copies (one kernel instance per K)
+// 0..3: link form (SU(2)/SU(3) x FP32/FP64); kSpinClass + K - 1 (+
+// MAG_MAX_COPIES with FP32 links): spin-form copy sets of K copies (one
+// kernel instance per K and link type)
 constexpr int kSpinClass = 4;
-constexpr int kClasses = kSpinClass + MAG_MAX_COPIES;
+constexpr int kClasses = kSpinClass + 2 * MAG_MAX_COPIES;
 
 struct ClassTable {
   int group = 2;
@@ -70,12 +72,16 @@ struct mag_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool dirty = true;
+  // MAG_BATCH_DIRECT=1 in the environment: issue each replay's launches on
+  // the stream instead of replaying a graph (for tools that do not follow
+  // graph launches); the launches and results are the same
+  bool direct = false;
 };
 
 namespace {
 
 int class_of(const mag_job* j) {
-  if (j->spin) return kSpinClass + j->ncopies - 1;
+  if (j->spin) return kSpinClass + j->ncopies - 1 + (mag::job_mixed32(j) ? MAG_MAX_COPIES : 0);
   return (j->prm.group == MAG_SU2 ? 0 : 2) + (mag::job_mixed32(j) ? 0 : 1);
 }
 
@@ -111,6 +117,8 @@ void drop_graph(mag_batch* b) {
   b->exec = nullptr;
   b->graph = nullptr;
 }
+
+void enqueue_replay(mag_batch* b);
 
 // rebuild the device tables and re-capture the replay graph
 int refresh(mag_batch* b) {
@@ -151,8 +159,17 @@ int refresh(mag_batch* b) {
   for (int c = 0; c < kClasses; c++) CHK(upload_table(b->cls[c].h, &b->cls[c].d, &b->cls[c].cap));
   CHK(upload_table(b->all, &b->d_all, &b->cap_all));
   b->dirty = false;
-  if (b->active.empty()) return MAG_OK;
+  if (b->active.empty() || b->direct) return MAG_OK;
   HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+  enqueue_replay(b);
+  hipError_t e = hipStreamEndCapture(b->st, &b->graph);
+  if (e != hipSuccess) return fail(MAG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return MAG_OK;
+}
+
+// the launches of one replay (captured into the graph, or issued directly)
+void enqueue_replay(mag_batch* b) {
   for (int i = 0; i < b->replay; i++) {
     for (int p = 0; p < 2; p++) {
       for (int c = 0; c < 4; c++) {
@@ -161,8 +178,9 @@ int refresh(mag_batch* b) {
       }
       for (int c = kSpinClass; c < kClasses; c++) {
         const ClassTable& t = b->cls[c];
-        mag::launch_copies_half_batch(c - kSpinClass + 1, t.d, (int)t.h.size(), t.nblocks, p,
-                                      b->st);
+        const int k = (c - kSpinClass) % MAG_MAX_COPIES + 1;
+        const bool fp32 = c >= kSpinClass + MAG_MAX_COPIES;
+        mag::launch_copies_half_batch(k, fp32, t.d, (int)t.h.size(), t.nblocks, p, b->st);
       }
     }
     mag::launch_reduce_batch(b->d_all, (int)b->all.size(), b->st);
@@ -172,10 +190,6 @@ int refresh(mag_batch* b) {
         mag::launch_rebuild_batch(t.group, t.d, (int)t.h.size(), t.nrblocks, b->st);
       }
   }
-  hipError_t e = hipStreamEndCapture(b->st, &b->graph);
-  if (e != hipSuccess) return fail(MAG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-  HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-  return MAG_OK;
 }
 
 bool finished(const mag_job* j) { return mag::job_done(j) && !mag::job_mixed32(j); }
@@ -190,6 +204,8 @@ int mag_batch_create(int device, mag_batch** out) {
   mag_batch* b = new (std::nothrow) mag_batch();
   if (!b) return fail(MAG_ERR_NOMEM, "host allocation failed");
   b->device = device;
+  const char* dv = std::getenv("MAG_BATCH_DIRECT");
+  b->direct = dv && dv[0] == '1';
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -218,7 +234,7 @@ int mag_batch_add(mag_batch* b, mag_job* j) {
   if (j->batch) return fail(MAG_ERR_STATE, "job is already in a batch");
   if (!j->loaded) return fail(MAG_ERR_STATE, "job has no configuration");
   if (j->prm.device != b->device) return fail(MAG_ERR_ARG, "job and batch on different devices");
-  if (j->prm.precision == MAG_PREC_MIXED) {
+  if (j->prm.precision == MAG_PREC_MIXED && !j->spin) {
     if (b->reunit == 0) {
       if (b->replay % j->prm.reunit_every != 0)
         return fail(MAG_ERR_ARG, "reunit_every %d does not divide the replay length %d",
@@ -279,8 +295,14 @@ int mag_batch_replays(mag_batch* b, int n) {
   if (!b || n < 0) return fail(MAG_ERR_ARG, "bad argument");
   HIPCHK(hipSetDevice(b->device));
   CHK(refresh(b));
-  if (!b->exec) return MAG_OK;
-  for (int i = 0; i < n; i++) HIPCHK(hipGraphLaunch(b->exec, b->st));
+  if (b->direct) {
+    if (b->active.empty()) return MAG_OK;
+    for (int i = 0; i < n; i++) enqueue_replay(b);
+    HIPCHK(hipGetLastError());
+  } else {
+    if (!b->exec) return MAG_OK;
+    for (int i = 0; i < n; i++) HIPCHK(hipGraphLaunch(b->exec, b->st));
+  }
   for (mag_job* j : b->active)
     if (mag::job_mixed32(j)) j->since_rebuild = (j->since_rebuild + n * b->replay) % j->prm.reunit_every;
   return MAG_OK;
@@ -296,7 +318,7 @@ int mag_batch_poll(mag_batch* b, int* n_running, int* n_finished) {
   HIPCHK(hipStreamSynchronize(b->st));
   std::vector<mag_job*> still;
   for (mag_job* j : b->active) {
-    if (j->ctl_h->done && mag::job_mixed32(j)) {
+    if (mag::job_done(j) && mag::job_mixed32(j)) {
       CHK(mag::job_finalize_mixed(j));  // FP32 phase over: FP64 rebuild, FP64 class
       b->dirty = true;
     }

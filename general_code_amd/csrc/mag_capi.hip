@@ -77,7 +77,6 @@ int check_params(const mag_params* p) {
     if (p->copies > 1) return fail(MAG_ERR_ARG, "the link form holds one copy; use MAG_FORM_SPIN");
   } else if (p->form == MAG_FORM_SPIN) {
     if (p->group != MAG_SU2) return fail(MAG_ERR_ARG, "the spin form is SU(2) only");
-    if (p->precision != MAG_PREC_F64) return fail(MAG_ERR_ARG, "the spin form runs in FP64");
     if (p->copies < 1 || p->copies > MAG_MAX_COPIES)
       return fail(MAG_ERR_ARG, "copies must be in 1..%d, got %d", MAG_MAX_COPIES, p->copies);
   } else {
@@ -106,8 +105,12 @@ int set_device(const mag_job* j) {
 
 void launch_half(mag_job* j, int parity) {
   if (j->spin) {
-    mag::launch_su2_copies_half(j->U64, j->S, j->sstride, j->ncopies, j->g, parity,
-                                j->prm.omega, j->part, j->pstride, j->ctl, j->st);
+    if (mixed32(j))
+      mag::launch_su2_copies_half<float>(j->W32, j->S, j->sstride, j->ncopies, j->g, parity,
+                                         j->prm.omega, j->part, j->pstride, j->ctl, j->st);
+    else
+      mag::launch_su2_copies_half<double>(j->U64, j->S, j->sstride, j->ncopies, j->g, parity,
+                                          j->prm.omega, j->part, j->pstride, j->ctl, j->st);
   } else if (mixed32(j)) {
     if (j->prm.group == MAG_SU2)
       mag::launch_su2_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, j->part, j->ctl, j->st);
@@ -201,7 +204,7 @@ void init_copy(mag_job* j, int c) {
   h.max_sweeps = j->prm.max_sweeps;
   const bool fp32 = j->prm.precision == MAG_PREC_MIXED && j->phase == 0;
   h.tol = fp32 ? j->prm.tol_mixed : j->prm.tol;
-  h.window = fp32 ? j->prm.reunit_every : 0;
+  h.window = (fp32 && !j->spin) ? j->prm.reunit_every : 0;
   hipMemcpyAsync(j->ctl + c, &h, sizeof(JobCtl), hipMemcpyHostToDevice, j->st);
   double* part = j->part + (size_t)c * j->pstride;
   if (j->spin) mag::launch_spin_functional(j->U64, j->S + (size_t)c * j->sstride, j->g, part,
@@ -217,18 +220,26 @@ void init_copy(mag_job* j, int c) {
 // gauge, FP32 tolerance (ctl_h current)
 int start_fp32_phase(mag_job* j) {
   mag::launch_links_f64_to_f32(j->U64, j->W32, j->g, j->prm.group, j->st);
-  mag::launch_gauge_identity(j->G32, j->g, j->prm.group, j->st);
+  if (!j->spin) mag::launch_gauge_identity(j->G32, j->g, j->prm.group, j->st);
   j->phase = 0;
   j->since_rebuild = 0;
-  j->ctl_h->tol = j->prm.tol_mixed;
-  j->ctl_h->window = j->prm.reunit_every;
+  for (int c = 0; c < j->ncopies; c++) {
+    j->ctl_h[c].tol = j->prm.tol_mixed;
+    j->ctl_h[c].window = j->spin ? 0 : j->prm.reunit_every;
+  }
   return write_ctl(j);
+}
+
+int max_sweep(const mag_job* j) {
+  int m = 0;
+  for (int c = 0; c < j->ncopies; c++) m = j->ctl_h[c].sweep > m ? j->ctl_h[c].sweep : m;
+  return m;
 }
 
 int maybe_start_fp32(mag_job* j, bool* switched) {
   *switched = false;
-  if (j->prm.precision != MAG_PREC_MIXED || j->phase != -1 || j->ctl_h->done) return MAG_OK;
-  if (j->ctl_h->sweep < j->prm.fp64_warmup) return MAG_OK;
+  if (j->prm.precision != MAG_PREC_MIXED || j->phase != -1 || all_done(j)) return MAG_OK;
+  if (max_sweep(j) < j->prm.fp64_warmup) return MAG_OK;
   *switched = true;
   return start_fp32_phase(j);
 }
@@ -239,7 +250,7 @@ int reset_after_upload(mag_job* j) {
   j->phase = (mixed && j->prm.fp64_warmup > 0) ? -1 : 0;
   if (mixed && j->phase == 0) {
     mag::launch_links_f64_to_f32(j->U64, j->W32, j->g, j->prm.group, j->st);
-    mag::launch_gauge_identity(j->G32, j->g, j->prm.group, j->st);
+    if (!j->spin) mag::launch_gauge_identity(j->G32, j->g, j->prm.group, j->st);
   }
   j->fp64_start = mixed ? -1 : 0;
   j->since_rebuild = 0;
@@ -282,8 +293,34 @@ void free_job(mag_job* j) {
 }
 
 // finish the FP32 phase: links <- G U0 G^+ in FP64, F_prev <- F(links)
+// spin form: the FP32 links were only read; FP64 sweeps go on from the
+// spins with the FP64 links
+int finalize_mixed_spin(mag_job* j) {
+  CHK(read_ctl(j));
+  j->phase = 1;
+  j->fp64_start = max_sweep(j);
+  for (int c = 0; c < j->ncopies; c++) {
+    double* part = j->part + (size_t)c * j->pstride;
+    const size_t hs = (size_t)c * j->prm.max_sweeps;
+    mag::launch_spin_functional(j->U64, j->S + (size_t)c * j->sstride, j->g, part, nullptr, j->st);
+    mag::launch_reduce(part, 2 * j->nf, mag::RED_RESET, j->invF, j->invV, j->ctl + c,
+                       j->hF + hs, j->hR + hs, j->hA + hs, nullptr, j->st);
+  }
+  HIPCHK(hipGetLastError());
+  CHK(read_ctl(j));
+  for (int c = 0; c < j->ncopies; c++) {
+    JobCtl& h = j->ctl_h[c];
+    h.tol = j->prm.tol;
+    h.window = 0;
+    h.converged = 0;
+    h.done = (h.sweep >= h.max_sweeps) ? 1 : 0;
+  }
+  return write_ctl(j);
+}
+
 int finalize_mixed(mag_job* j) {
   if (!mixed32(j)) return MAG_OK;
+  if (j->spin) return finalize_mixed_spin(j);
   CHK(read_ctl(j));
   launch_rebuild(j, true);
   j->phase = 1;
@@ -357,7 +394,10 @@ int mag_job_footprint(const mag_params* p, uint64_t* bytes) {
   const uint64_t nc = p->form == MAG_FORM_SPIN ? (uint64_t)p->copies : 1;
   uint64_t b = (uint64_t)V * 4 * nr * 8;  // FP64 links / master
   if (p->precision == MAG_PREC_MIXED) b += (uint64_t)V * 4 * nr * 4 + (uint64_t)V * nr * 4;
-  if (p->form == MAG_FORM_SPIN) b += nc * (uint64_t)V * 3 * 8;  // spins
+  if (p->form == MAG_FORM_SPIN) {
+    b += nc * (uint64_t)V * 3 * 8;  // spins
+    if (p->precision == MAG_PREC_MIXED) b -= (uint64_t)V * nr * 4;  // no site gauge
+  }
   const uint64_t nb = (uint64_t)((V + mag::kBlock - 1) / mag::kBlock);  // sweep workgroups
   b += nc * (nb * 2 * 3 * 8 + (uint64_t)p->max_sweeps * 3 * 8 + sizeof(JobCtl)) + 3 * 8;
   *bytes = b;
@@ -406,7 +446,7 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   ALLOC(j->U64, j->nreal * sizeof(double));
   if (p->precision == MAG_PREC_MIXED) {
     ALLOC(j->W32, j->nreal * sizeof(float));
-    ALLOC(j->G32, (size_t)2 * j->g.Vh * j->nr * sizeof(float));
+    if (!j->spin) ALLOC(j->G32, (size_t)2 * j->g.Vh * j->nr * sizeof(float));
   }
   if (j->spin) ALLOC(j->S, j->sstride * j->ncopies * sizeof(double));
   ALLOC(j->part, j->pstride * j->ncopies * sizeof(double));

@@ -80,7 +80,10 @@ void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega,
 // every copy not yet done updates its spins, copy c at S + c*sstride; its
 // partial triples go to part + c*pstride + 3*nb*parity (nb = nblocks(g))
 // and it is skipped once ctl[c].done.
-void launch_su2_copies_half(const double* U, double* S, size_t sstride, int ncopies,
+// TL = float: the FP32-rounded links of the mixed form (converted to FP64
+// on load; spins and arithmetic stay FP64).
+template <typename TL>
+void launch_su2_copies_half(const TL* U, double* S, size_t sstride, int ncopies,
                             const Geom& g, int parity, double omega, double* part,
                             size_t pstride, const JobCtl* ctl, hipStream_t st);
 // end-of-sweep reduction of every copy (one workgroup per copy): copy c
@@ -130,8 +133,8 @@ void launch_half_batch(int group, bool fp32, const BatchJob* jobs, int njobs,
                        int nblocks_total, int parity, hipStream_t st);
 // batched spin-form (gauge-copy) half-sweep of a table of copy sets that
 // all hold ncopies copies
-void launch_copies_half_batch(int ncopies, const BatchJob* jobs, int njobs, int nblocks_total,
-                              int parity, hipStream_t st);
+void launch_copies_half_batch(int ncopies, bool fp32, const BatchJob* jobs, int njobs,
+                              int nblocks_total, int parity, hipStream_t st);
 // batched end-of-sweep reduction: one workgroup per job
 void launch_reduce_batch(const BatchJob* jobs, int njobs, hipStream_t st);
 // batched FP64 rebuild of the FP32 working links + gauge normalisation of a

@@ -436,8 +436,15 @@ void launch_half_batch(int group, bool fp32, const BatchJob* jobs, int njobs, in
 // (mag.cpp:925-953) after the sweep.  A copy's arithmetic and partition do
 // not depend on the other copies, so its trajectory is the same in a set
 // of any size.
-template <int K>
-__device__ __forceinline__ void su2_copies_block(const double* __restrict__ U,
+template <typename TL>
+__device__ __forceinline__ Q<double> ld_link64(const TL* U, const Geom& g, int p, int mu,
+                                               uint32_t h) {
+  const Q<TL> q = ldq(link_ptr<4>(U, g, p, mu, h));
+  return Q<double>{(double)q.a0, (double)q.a1, (double)q.a2, (double)q.a3};
+}
+
+template <int K, typename TL>
+__device__ __forceinline__ void su2_copies_block(const TL* __restrict__ U,
                                                  double* __restrict__ S, size_t sstride,
                                                  const Geom& g, int p, double omega,
                                                  double* __restrict__ part, size_t pstride,
@@ -463,12 +470,12 @@ __device__ __forceinline__ void su2_copies_block(const double* __restrict__ U,
 #pragma unroll
     for (int mu = 0; mu < 4; mu++) {
       const uint32_t hf = fwd_h(g, h, s, mu);
-      const Adj Mf = adj_of(ldq(link_ptr<4>(U, g, p, mu, h)));
+      const Adj Mf = adj_of(ld_link64(U, g, p, mu, h));
 #pragma unroll
       for (int k = 0; k < K; k++)
         if (act[k]) add_M(H[k], Mf, ld_spin(S + k * sstride, g, 1 - p, hf));
       const uint32_t hb = back_h(g, h, s, mu);
-      const Adj Mb = adj_of(ldq(link_ptr<4>(U, g, 1 - p, mu, hb)));
+      const Adj Mb = adj_of(ld_link64(U, g, 1 - p, mu, hb));
 #pragma unroll
       for (int k = 0; k < K; k++)
         if (act[k]) add_MT(H[k], Mb, ld_spin(S + k * sstride, g, 1 - p, hb));
@@ -501,24 +508,25 @@ __device__ __forceinline__ void su2_copies_block(const double* __restrict__ U,
   }
 }
 
-template <int K>
-__global__ __launch_bounds__(kBlock) void k_su2_copies_half(const double* __restrict__ U,
+template <int K, typename TL>
+__global__ __launch_bounds__(kBlock) void k_su2_copies_half(const TL* __restrict__ U,
                                                             double* __restrict__ S,
                                                             size_t sstride, Geom g, int p,
                                                             double omega,
                                                             double* __restrict__ part,
                                                             size_t pstride,
                                                             const JobCtl* __restrict__ ctl) {
-  su2_copies_block<K>(U, S, sstride, g, p, omega, part, pstride, ctl, blockIdx.x);
+  su2_copies_block<K, TL>(U, S, sstride, g, p, omega, part, pstride, ctl, blockIdx.x);
 }
 
-// every copy set of the table has K copies
-template <int K>
+// every copy set of the table has K copies and links of type TL
+template <int K, typename TL>
 __global__ __launch_bounds__(kBlock) void k_copies_half_batch(const BatchJob* __restrict__ jobs,
                                                               int njobs, int p) {
   const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
-  su2_copies_block<K>((const double*)j.U, j.S, j.sstride, j.g, p, j.omega,
-                      j.part + (size_t)3 * j.nb * p, j.pstride, j.ctl, blockIdx.x - j.block0);
+  su2_copies_block<K, TL>((const TL*)j.U, j.S, j.sstride, j.g, p, j.omega,
+                          j.part + (size_t)3 * j.nb * p, j.pstride, j.ctl,
+                          blockIdx.x - j.block0);
 }
 
 // one workgroup per copy
@@ -534,14 +542,15 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_copies(const double* __res
              nullptr);
 }
 
-void launch_su2_copies_half(const double* U, double* S, size_t sstride, int ncopies,
-                            const Geom& g, int parity, double omega, double* part,
-                            size_t pstride, const JobCtl* ctl, hipStream_t st) {
+template <typename TL>
+void launch_su2_copies_half(const TL* U, double* S, size_t sstride, int ncopies, const Geom& g,
+                            int parity, double omega, double* part, size_t pstride,
+                            const JobCtl* ctl, hipStream_t st) {
   const int nb = nblocks(g);
   double* pp = part + (size_t)3 * nb * parity;
-#define L_(K)                                                                                \
-  hipLaunchKernelGGL(k_su2_copies_half<K>, dim3(nb), dim3(kBlock), 0, st, U, S, sstride, g,  \
-                     parity, omega, pp, pstride, ctl)
+#define L_(K)                                                                                   \
+  hipLaunchKernelGGL((k_su2_copies_half<K, TL>), dim3(nb), dim3(kBlock), 0, st, U, S, sstride, \
+                     g, parity, omega, pp, pstride, ctl)
   switch (ncopies) {
     case 1: L_(1); break;
     case 2: L_(2); break;
@@ -554,18 +563,30 @@ void launch_su2_copies_half(const double* U, double* S, size_t sstride, int ncop
   }
 #undef L_
 }
+template void launch_su2_copies_half<double>(const double*, double*, size_t, int, const Geom&,
+                                             int, double, double*, size_t, const JobCtl*,
+                                             hipStream_t);
+template void launch_su2_copies_half<float>(const float*, double*, size_t, int, const Geom&, int,
+                                            double, double*, size_t, const JobCtl*,
+                                            hipStream_t);
 void launch_reduce_copies(const double* part, size_t pstride, int nparts, int ncopies,
                           double invF, double invV, JobCtl* ctl, double* hF, double* hR,
                           double* hA, size_t hstride, hipStream_t st) {
   hipLaunchKernelGGL(k_reduce_copies, dim3(ncopies), dim3(kRedBlock), 0, st, part, pstride,
                      nparts, invF, invV, ctl, hF, hR, hA, hstride);
 }
-void launch_copies_half_batch(int ncopies, const BatchJob* jobs, int njobs, int nblocks_total,
-                              int parity, hipStream_t st) {
+void launch_copies_half_batch(int ncopies, bool fp32, const BatchJob* jobs, int njobs,
+                              int nblocks_total, int parity, hipStream_t st) {
   if (njobs == 0 || nblocks_total == 0) return;
 #define L_(K)                                                                                 \
-  hipLaunchKernelGGL(k_copies_half_batch<K>, dim3(nblocks_total), dim3(kBlock), 0, st, jobs, \
-                     njobs, parity)
+  do {                                                                                        \
+    if (fp32)                                                                                 \
+      hipLaunchKernelGGL((k_copies_half_batch<K, float>), dim3(nblocks_total), dim3(kBlock), \
+                         0, st, jobs, njobs, parity);                                         \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_copies_half_batch<K, double>), dim3(nblocks_total), dim3(kBlock),\
+                         0, st, jobs, njobs, parity);                                         \
+  } while (0)
   switch (ncopies) {
     case 1: L_(1); break;
     case 2: L_(2); break;

@@ -182,7 +182,8 @@ def test_spin_form_argument_errors(mag):
     with pytest.raises(mag.MagError):
         mag.MagJob(group=mag.SU2, dims=(4, 4, 4, 4), form=mag.FORM_SPIN, copies=9)
     with pytest.raises(mag.MagError):
-        mag.MagJob(group=mag.SU2, dims=(4, 4, 4, 4), form=mag.FORM_SPIN, precision=mag.MIXED)
+        mag.MagJob(group=mag.SU2, dims=(4, 4, 4, 4), form=mag.FORM_SPIN, precision=mag.MIXED,
+                   fp64_warmup=50)
     with pytest.raises(mag.MagError):
         mag.MagJob(group=mag.SU2, dims=(4, 4, 4, 4), copies=2)
     with pytest.raises(mag.MagError):
@@ -193,3 +194,38 @@ def test_spin_form_argument_errors(mag):
             j.copy_status(2)
         with pytest.raises(mag.MagError):
             j.generate(0.3, 1, copy=1)
+
+
+def test_spin_form_mixed_reaches_fp64_maximum(mag):
+    """Mixed spin form: FP64 warm-up, then sweeps that read FP32-rounded
+    links (spins and arithmetic FP64), then FP64 polish.  Every copy ends at
+    the FP64 run's maximum; alone and in a batch agree bit for bit."""
+    dims = (16, 16, 8, 8)
+
+    def make(prec):
+        j = mag.MagJob(group=mag.SU2, dims=dims, form=mag.FORM_SPIN, copies=3, precision=prec,
+                       omega=OMEGA, tol=1e-12, max_sweeps=4000, tol_mixed=1e-12,
+                       fp64_warmup=100)
+        j.generate(EPS, 77)
+        return j
+
+    with make(mag.F64) as ref:
+        assert ref.run()
+        Fr = [ref.copy_status(c)[2] for c in range(3)]
+    with make(mag.MIXED) as j:
+        assert j.run()
+        st = [j.copy_status(c) for c in range(3)]
+        hist = [j.copy_history(c) for c in range(3)]
+        start = j.fp64_start()
+    assert start > 100
+    for c in range(3):
+        assert st[c][1]
+        assert abs(st[c][2] - Fr[c]) / Fr[c] < 1e-10, c
+    with make(mag.MIXED) as jb, mag.MagBatch() as b:
+        b.add(jb)
+        b.run()
+        for c in range(3):
+            for x, y in zip(jb.copy_history(c), hist[c]):
+                assert np.array_equal(x, y)
+        assert jb.fp64_start() == start
+        b.remove(jb)
