@@ -30,8 +30,10 @@ namespace {
 // 0..3: link form (SU(2)/SU(3) x FP32/FP64); kSpinClass + K - 1 (+
 // MAG_MAX_COPIES with FP32 links): spin-form copy sets of K copies (one
 // kernel instance per K and link type)
+// kSiteClass3 (+1 with FP32 links): SU(3) site-form copy sets
 constexpr int kSpinClass = 4;
-constexpr int kClasses = kSpinClass + 2 * MAG_MAX_COPIES;
+constexpr int kSiteClass3 = kSpinClass + 2 * MAG_MAX_COPIES;
+constexpr int kClasses = kSiteClass3 + 2;
 
 struct ClassTable {
   int group = 2;
@@ -39,6 +41,9 @@ struct ClassTable {
   std::vector<BatchJob> h;
   BatchJob* d = nullptr;
   size_t cap = 0;
+  std::vector<BatchJob> red;  // reduction entries (one per job, per copy)
+  BatchJob* d_red = nullptr;
+  size_t cap_red = 0;
   int nblocks = 0;   // half-sweep grid
   int nrblocks = 0;  // rebuild grid (FP32 classes)
 };
@@ -76,11 +81,21 @@ struct mag_batch {
   // the stream instead of replaying a graph (for tools that do not follow
   // graph launches); the launches and results are the same
   bool direct = false;
+  // Jobs of different classes never share data, so each class's sweeps form
+  // an independent chain: with `fork` the chains run on streams of their own
+  // (graph branches) and overlap on the device; without it
+  // (MAG_BATCH_SERIAL=1) every sweep launches all classes in turn on one
+  // stream.  Per-job results are the same either way.
+  bool fork = true;
+  hipStream_t fs[kClasses] = {};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[kClasses] = {};
 };
 
 namespace {
 
 int class_of(const mag_job* j) {
+  if (j->spin && j->prm.group == MAG_SU3) return kSiteClass3 + (mag::job_mixed32(j) ? 1 : 0);
   if (j->spin) return kSpinClass + j->ncopies - 1 + (mag::job_mixed32(j) ? MAG_MAX_COPIES : 0);
   return (j->prm.group == MAG_SU2 ? 0 : 2) + (mag::job_mixed32(j) ? 0 : 1);
 }
@@ -104,7 +119,7 @@ BatchJob describe(mag_job* j) {
   b.nparts = 2 * j->nb;
   b.block0 = 0;
   b.rb0 = 0;
-  b.S = j->S;
+  b.S = (j->spin && j->prm.group == MAG_SU3 && fp32) ? (void*)j->S32 : (void*)j->S;
   b.sstride = j->sstride;
   b.pstride = j->pstride;
   b.ncopies = j->spin ? j->ncopies : 0;
@@ -112,13 +127,14 @@ BatchJob describe(mag_job* j) {
 }
 
 void drop_graph(mag_batch* b) {
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
 }
 
-void enqueue_replay(mag_batch* b);
+hipError_t enqueue_replay(mag_batch* b);
+int ensure_streams(mag_batch* b);
 
 // rebuild the device tables and re-capture the replay graph
 int refresh(mag_batch* b) {
@@ -126,9 +142,10 @@ int refresh(mag_batch* b) {
   drop_graph(b);
   b->all.clear();
   for (int c = 0; c < kClasses; c++) {
-    b->cls[c].group = (c < 2 || c >= kSpinClass) ? 2 : 3;
+    b->cls[c].group = (c < 2 || (c >= kSpinClass && c < kSiteClass3)) ? 2 : 3;
     b->cls[c].fp32 = c < kSpinClass && (c % 2) == 0;
     b->cls[c].h.clear();
+    b->cls[c].red.clear();
     b->cls[c].nblocks = 0;
     b->cls[c].nrblocks = 0;
   }
@@ -142,6 +159,7 @@ int refresh(mag_batch* b) {
     t.h.push_back(d);
     if (!j->spin) {
       b->all.push_back(d);
+      t.red.push_back(d);
     } else {
       // one reduction entry per copy
       for (int c = 0; c < j->ncopies; c++) {
@@ -153,43 +171,102 @@ int refresh(mag_batch* b) {
         e.hR = j->hR + hs;
         e.hA = j->hA + hs;
         b->all.push_back(e);
+        t.red.push_back(e);
       }
     }
   }
-  for (int c = 0; c < kClasses; c++) CHK(upload_table(b->cls[c].h, &b->cls[c].d, &b->cls[c].cap));
+  for (int c = 0; c < kClasses; c++) {
+    ClassTable& t = b->cls[c];
+    CHK(upload_table(t.h, &t.d, &t.cap));
+    CHK(upload_table(t.red, &t.d_red, &t.cap_red));
+  }
   CHK(upload_table(b->all, &b->d_all, &b->cap_all));
   b->dirty = false;
+  CHK(ensure_streams(b));
   if (b->active.empty() || b->direct) return MAG_OK;
   HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-  enqueue_replay(b);
+  const hipError_t er = enqueue_replay(b);
   hipError_t e = hipStreamEndCapture(b->st, &b->graph);
+  if (er != hipSuccess) return fail(MAG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(er));
   if (e != hipSuccess) return fail(MAG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
   HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   return MAG_OK;
 }
 
 // the launches of one replay (captured into the graph, or issued directly)
-void enqueue_replay(mag_batch* b) {
-  for (int i = 0; i < b->replay; i++) {
-    for (int p = 0; p < 2; p++) {
-      for (int c = 0; c < 4; c++) {
-        const ClassTable& t = b->cls[c];
-        mag::launch_half_batch(t.group, t.fp32, t.d, (int)t.h.size(), t.nblocks, p, b->st);
-      }
-      for (int c = kSpinClass; c < kClasses; c++) {
-        const ClassTable& t = b->cls[c];
-        const int k = (c - kSpinClass) % MAG_MAX_COPIES + 1;
-        const bool fp32 = c >= kSpinClass + MAG_MAX_COPIES;
-        mag::launch_copies_half_batch(k, fp32, t.d, (int)t.h.size(), t.nblocks, p, b->st);
-      }
-    }
-    mag::launch_reduce_batch(b->d_all, (int)b->all.size(), b->st);
-    if (b->reunit > 0 && (i + 1) % b->reunit == 0)
-      for (int c = 0; c < 4; c += 2) {
-        const ClassTable& t = b->cls[c];
-        mag::launch_rebuild_batch(t.group, t.d, (int)t.h.size(), t.nrblocks, b->st);
-      }
+// half-sweep of one class
+void launch_class_half(mag_batch* b, int c, int p, hipStream_t st) {
+  const ClassTable& t = b->cls[c];
+  if (c < kSpinClass) {
+    mag::launch_half_batch(t.group, t.fp32, t.d, (int)t.h.size(), t.nblocks, p, st);
+  } else if (c >= kSiteClass3) {
+    mag::launch_su3_site_copies_batch(c == kSiteClass3 + 1, t.d, (int)t.h.size(), t.nblocks, p,
+                                      st);
+  } else {
+    const int k = (c - kSpinClass) % MAG_MAX_COPIES + 1;
+    const bool fp32 = c >= kSpinClass + MAG_MAX_COPIES;
+    mag::launch_copies_half_batch(k, fp32, t.d, (int)t.h.size(), t.nblocks, p, st);
   }
+}
+
+// sweep i's rebuild of a link-form FP32 class
+void launch_class_rebuild(mag_batch* b, int c, int i, hipStream_t st) {
+  const ClassTable& t = b->cls[c];
+  if (c < kSpinClass && t.fp32 && b->reunit > 0 && (i + 1) % b->reunit == 0)
+    mag::launch_rebuild_batch(t.group, t.d, (int)t.h.size(), t.nrblocks, st);
+}
+
+int live_classes(const mag_batch* b, std::vector<int>& live) {
+  live.clear();
+  for (int c = 0; c < kClasses; c++)
+    if (!b->cls[c].h.empty()) live.push_back(c);
+  return (int)live.size();
+}
+
+// the side streams and join events of the forked replay (created on first
+// use, outside any capture)
+int ensure_streams(mag_batch* b) {
+  std::vector<int> live;
+  const int n = live_classes(b, live);
+  if (!b->fork || n < 2) return MAG_OK;
+  for (int k = 0; k < n; k++) {
+    if (!b->fs[k]) HIPCHK(hipStreamCreateWithFlags(&b->fs[k], hipStreamNonBlocking));
+    if (!b->ev_join[k]) HIPCHK(hipEventCreateWithFlags(&b->ev_join[k], hipEventDisableTiming));
+  }
+  return MAG_OK;
+}
+
+// the launches of one replay (captured into the graph, or issued directly);
+// returns the first error of the stream-ordering calls
+hipError_t enqueue_replay(mag_batch* b) {
+  std::vector<int> live;
+  live_classes(b, live);
+  if (!b->fork || live.size() < 2) {
+    for (int i = 0; i < b->replay; i++) {
+      for (int p = 0; p < 2; p++)
+        for (int c : live) launch_class_half(b, c, p, b->st);
+      mag::launch_reduce_batch(b->d_all, (int)b->all.size(), b->st);
+      for (int c : live) launch_class_rebuild(b, c, i, b->st);
+    }
+    return hipSuccess;
+  }
+  hipError_t e = hipEventRecord(b->ev_fork, b->st);
+  for (size_t k = 0; k < live.size() && e == hipSuccess; k++) {
+    const int c = live[k];
+    hipStream_t s = b->fs[k];
+    e = hipStreamWaitEvent(s, b->ev_fork, 0);
+    if (e != hipSuccess) break;
+    const ClassTable& t = b->cls[c];
+    for (int i = 0; i < b->replay; i++) {
+      launch_class_half(b, c, 0, s);
+      launch_class_half(b, c, 1, s);
+      mag::launch_reduce_batch(t.d_red, (int)t.red.size(), s);
+      launch_class_rebuild(b, c, i, s);
+    }
+    e = hipEventRecord(b->ev_join[k], s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->st, b->ev_join[k], 0);
+  }
+  return e;
 }
 
 bool finished(const mag_job* j) { return mag::job_done(j) && !mag::job_mixed32(j); }
@@ -206,9 +283,14 @@ int mag_batch_create(int device, mag_batch** out) {
   b->device = device;
   const char* dv = std::getenv("MAG_BATCH_DIRECT");
   b->direct = dv && dv[0] == '1';
+  const char* sv = std::getenv("MAG_BATCH_SERIAL");
+  b->fork = !(sv && sv[0] == '1');
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
   if (e != hipSuccess) {
+    if (b->st) (void)hipStreamDestroy(b->st);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     delete b;
     return fail(MAG_ERR_HIP, "batch setup failed: %s", hipGetErrorString(e));
   }
@@ -218,13 +300,20 @@ int mag_batch_create(int device, mag_batch** out) {
 
 int mag_batch_destroy(mag_batch* b) {
   if (!b) return MAG_OK;
-  hipSetDevice(b->device);
-  hipStreamSynchronize(b->st);
+  // best-effort teardown: errors here have nowhere to go
+  (void)hipSetDevice(b->device);
+  (void)hipStreamSynchronize(b->st);
   drop_graph(b);
   for (mag_job* j : b->members) j->batch = nullptr;
-  for (int c = 0; c < kClasses; c++) hipFree(b->cls[c].d);
-  hipFree(b->d_all);
-  hipStreamDestroy(b->st);
+  for (int c = 0; c < kClasses; c++) {
+    (void)hipFree(b->cls[c].d);
+    (void)hipFree(b->cls[c].d_red);
+    if (b->fs[c]) (void)hipStreamDestroy(b->fs[c]);
+    if (b->ev_join[c]) (void)hipEventDestroy(b->ev_join[c]);
+  }
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  (void)hipFree(b->d_all);
+  (void)hipStreamDestroy(b->st);
   delete b;
   return MAG_OK;
 }
@@ -297,7 +386,7 @@ int mag_batch_replays(mag_batch* b, int n) {
   CHK(refresh(b));
   if (b->direct) {
     if (b->active.empty()) return MAG_OK;
-    for (int i = 0; i < n; i++) enqueue_replay(b);
+    for (int i = 0; i < n; i++) HIPCHK(enqueue_replay(b));
     HIPCHK(hipGetLastError());
   } else {
     if (!b->exec) return MAG_OK;

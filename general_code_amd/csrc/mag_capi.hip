@@ -76,7 +76,6 @@ int check_params(const mag_params* p) {
   if (p->form == MAG_FORM_LINK) {
     if (p->copies > 1) return fail(MAG_ERR_ARG, "the link form holds one copy; use MAG_FORM_SPIN");
   } else if (p->form == MAG_FORM_SPIN) {
-    if (p->group != MAG_SU2) return fail(MAG_ERR_ARG, "the spin form is SU(2) only");
     if (p->copies < 1 || p->copies > MAG_MAX_COPIES)
       return fail(MAG_ERR_ARG, "copies must be in 1..%d, got %d", MAG_MAX_COPIES, p->copies);
   } else {
@@ -104,7 +103,14 @@ int set_device(const mag_job* j) {
 }
 
 void launch_half(mag_job* j, int parity) {
-  if (j->spin) {
+  if (j->spin && j->prm.group == MAG_SU3) {
+    if (mixed32(j))
+      mag::launch_su3_site_copies<float>(j->W32, j->S32, j->sstride, j->ncopies, j->g, parity,
+                                         j->prm.omega, j->part, j->pstride, j->ctl, j->st);
+    else
+      mag::launch_su3_site_copies<double>(j->U64, j->S, j->sstride, j->ncopies, j->g, parity,
+                                          j->prm.omega, j->part, j->pstride, j->ctl, j->st);
+  } else if (j->spin) {
     if (mixed32(j))
       mag::launch_su2_copies_half<float>(j->W32, j->S, j->sstride, j->ncopies, j->g, parity,
                                          j->prm.omega, j->part, j->pstride, j->ctl, j->st);
@@ -160,9 +166,9 @@ void enqueue_sweep(mag_job* j) {
 
 // functional partials of the current working links
 void launch_functional(mag_job* j) {
-  if (j->spin) {
+  if (j->spin && j->prm.group == MAG_SU2) {
     mag::launch_spin_functional(j->U64, j->S, j->g, j->part, nullptr, j->st);
-  } else if (mixed32(j)) {
+  } else if (mixed32(j) && !j->spin) {
     if (j->prm.group == MAG_SU2) mag::launch_su2_functional<float>(j->W32, j->g, j->part, j->st);
     else mag::launch_su3_functional<float>(j->W32, j->g, j->part, j->st);
   } else {
@@ -196,9 +202,29 @@ bool all_converged(const mag_job* j) {
   return true;
 }
 
-// initial control block and functional F0 of copy c (spin form) or of the
-// job (link form); links (and spins) already in place
-void init_copy(mag_job* j, int c) {
+bool su3_site(const mag_job* j) { return j->spin && j->prm.group == MAG_SU3; }
+
+// SU(3) site form: the FP64 links of copy c, G_c U G_c^+, into tmp (FP64
+// chunked links); in the FP32 phase the FP64 gauges are refreshed first
+void su3_copy_links(mag_job* j, int c, double* tmp) {
+  double* Gc = j->S + (size_t)c * j->sstride;
+  if (j->prm.precision == MAG_PREC_MIXED && j->phase == 0)
+    mag::launch_su3_sites_to64(j->S32 + (size_t)c * j->sstride, Gc, j->g, j->st);
+  mag::launch_su3_site_links(j->U64, Gc, tmp, j->g, j->st);
+}
+
+// FP64 link buffer for the SU(3) site form (nullptr otherwise)
+int alloc_tmp(mag_job* j, double** tmp) {
+  *tmp = nullptr;
+  if (!su3_site(j)) return MAG_OK;
+  HIPCHK(hipMalloc((void**)tmp, j->nreal * sizeof(double)));
+  return MAG_OK;
+}
+
+// initial control block and functional F0 of copy c (site form) or of the
+// job (link form); links (and site variables) already in place; tmp: an
+// FP64 link buffer for the SU(3) site form
+void init_copy(mag_job* j, int c, double* tmp) {
   JobCtl& h = j->ctl_h[c];
   std::memset(&h, 0, sizeof(JobCtl));
   h.max_sweeps = j->prm.max_sweeps;
@@ -207,9 +233,12 @@ void init_copy(mag_job* j, int c) {
   h.window = (fp32 && !j->spin) ? j->prm.reunit_every : 0;
   hipMemcpyAsync(j->ctl + c, &h, sizeof(JobCtl), hipMemcpyHostToDevice, j->st);
   double* part = j->part + (size_t)c * j->pstride;
-  if (j->spin) mag::launch_spin_functional(j->U64, j->S + (size_t)c * j->sstride, j->g, part,
-                                           nullptr, j->st);
-  else if (j->prm.group == MAG_SU2) mag::launch_su2_functional<double>(j->U64, j->g, part, j->st);
+  if (su3_site(j)) {
+    su3_copy_links(j, c, tmp);
+    mag::launch_su3_functional<double>(tmp, j->g, part, j->st);
+  } else if (j->spin) {
+    mag::launch_spin_functional(j->U64, j->S + (size_t)c * j->sstride, j->g, part, nullptr, j->st);
+  } else if (j->prm.group == MAG_SU2) mag::launch_su2_functional<double>(j->U64, j->g, part, j->st);
   else mag::launch_su3_functional<double>(j->U64, j->g, part, j->st);
   const size_t hs = (size_t)c * j->prm.max_sweeps;
   mag::launch_reduce(part, 2 * j->nf, mag::RED_INIT, j->invF, j->invV, j->ctl + c, j->hF + hs,
@@ -221,6 +250,10 @@ void init_copy(mag_job* j, int c) {
 int start_fp32_phase(mag_job* j) {
   mag::launch_links_f64_to_f32(j->U64, j->W32, j->g, j->prm.group, j->st);
   if (!j->spin) mag::launch_gauge_identity(j->G32, j->g, j->prm.group, j->st);
+  if (su3_site(j))
+    for (int c = 0; c < j->ncopies; c++)
+      mag::launch_su3_sites_to32(j->S + (size_t)c * j->sstride, j->S32 + (size_t)c * j->sstride,
+                                 j->g, j->st);
   j->phase = 0;
   j->since_rebuild = 0;
   for (int c = 0; c < j->ncopies; c++) {
@@ -251,11 +284,21 @@ int reset_after_upload(mag_job* j) {
   if (mixed && j->phase == 0) {
     mag::launch_links_f64_to_f32(j->U64, j->W32, j->g, j->prm.group, j->st);
     if (!j->spin) mag::launch_gauge_identity(j->G32, j->g, j->prm.group, j->st);
+    if (su3_site(j))
+      for (int c = 0; c < j->ncopies; c++)
+        mag::launch_su3_sites_to32(j->S + (size_t)c * j->sstride,
+                                   j->S32 + (size_t)c * j->sstride, j->g, j->st);
   }
   j->fp64_start = mixed ? -1 : 0;
   j->since_rebuild = 0;
-  // initial functional from the FP64 links (and each copy's spins)
-  for (int c = 0; c < j->ncopies; c++) init_copy(j, c);
+  // initial functional from the FP64 links (and each copy's site variable)
+  double* tmp = nullptr;
+  CHK(alloc_tmp(j, &tmp));
+  for (int c = 0; c < j->ncopies; c++) init_copy(j, c, tmp);
+  if (tmp) {
+    (void)hipStreamSynchronize(j->st);
+    (void)hipFree(tmp);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(j->st));
   j->loaded = true;
@@ -267,7 +310,14 @@ int reset_after_upload(mag_job* j) {
 int upload_device_aos(mag_job* j, const double* dev, const uint64_t* copy_seed = nullptr) {
   if (j->prm.group == MAG_SU2) mag::launch_su2_aos_to_soa<double>(dev, j->U64, j->g, j->st);
   else mag::launch_su3_aos_to_soa<double>(dev, j->U64, j->g, j->st);
-  if (j->spin) {
+  if (su3_site(j)) {
+    if (copy_seed) {
+      mag::launch_gen_su3_copy_gauges(j->S, j->sstride, j->ncopies, j->g, *copy_seed, j->st);
+    } else {
+      for (int c = 0; c < j->ncopies; c++)
+        mag::launch_su3_sites_identity(j->S + (size_t)c * j->sstride, j->g, j->st);
+    }
+  } else if (j->spin) {
     if (copy_seed) mag::launch_gen_copy_spins(j->S, j->sstride, j->ncopies, j->g, *copy_seed, j->st);
     else mag::launch_spins_e3(j->S, j->sstride * j->ncopies, j->g, j->st);
   }
@@ -278,6 +328,7 @@ void free_job(mag_job* j) {
   if (!j) return;
   hipFree(j->U64);
   hipFree(j->S);
+  hipFree(j->S32);
   hipFree(j->W32);
   hipFree(j->G32);
   hipFree(j->part);
@@ -297,15 +348,27 @@ void free_job(mag_job* j) {
 // spins with the FP64 links
 int finalize_mixed_spin(mag_job* j) {
   CHK(read_ctl(j));
-  j->phase = 1;
-  j->fp64_start = max_sweep(j);
+  double* tmp = nullptr;
+  CHK(alloc_tmp(j, &tmp));
   for (int c = 0; c < j->ncopies; c++) {
     double* part = j->part + (size_t)c * j->pstride;
     const size_t hs = (size_t)c * j->prm.max_sweeps;
-    mag::launch_spin_functional(j->U64, j->S + (size_t)c * j->sstride, j->g, part, nullptr, j->st);
+    if (su3_site(j)) {
+      su3_copy_links(j, c, tmp);  // FP32 gauges -> FP64 (Gram-Schmidt), then the links
+      mag::launch_su3_functional<double>(tmp, j->g, part, j->st);
+    } else {
+      mag::launch_spin_functional(j->U64, j->S + (size_t)c * j->sstride, j->g, part, nullptr,
+                                  j->st);
+    }
     mag::launch_reduce(part, 2 * j->nf, mag::RED_RESET, j->invF, j->invV, j->ctl + c,
                        j->hF + hs, j->hR + hs, j->hA + hs, nullptr, j->st);
   }
+  if (tmp) {
+    (void)hipStreamSynchronize(j->st);
+    (void)hipFree(tmp);
+  }
+  j->phase = 1;
+  j->fp64_start = max_sweep(j);
   HIPCHK(hipGetLastError());
   CHK(read_ctl(j));
   for (int c = 0; c < j->ncopies; c++) {
@@ -395,8 +458,13 @@ int mag_job_footprint(const mag_params* p, uint64_t* bytes) {
   uint64_t b = (uint64_t)V * 4 * nr * 8;  // FP64 links / master
   if (p->precision == MAG_PREC_MIXED) b += (uint64_t)V * 4 * nr * 4 + (uint64_t)V * nr * 4;
   if (p->form == MAG_FORM_SPIN) {
-    b += nc * (uint64_t)V * 3 * 8;  // spins
-    if (p->precision == MAG_PREC_MIXED) b -= (uint64_t)V * nr * 4;  // no site gauge
+    const uint64_t sr = p->group == MAG_SU2 ? 3 : 12;  // reals per site per copy
+    b += nc * (uint64_t)V * sr * 8;
+    if (p->precision == MAG_PREC_MIXED) {
+      b -= (uint64_t)V * nr * 4;  // no accumulated link-form gauge
+      if (p->group == MAG_SU3) b += nc * (uint64_t)V * sr * 4;
+    }
+    if (p->group == MAG_SU3) b += (uint64_t)V * 4 * nr * 8;  // transient FP64 links
   }
   const uint64_t nb = (uint64_t)((V + mag::kBlock - 1) / mag::kBlock);  // sweep workgroups
   b += nc * (nb * 2 * 3 * 8 + (uint64_t)p->max_sweeps * 3 * 8 + sizeof(JobCtl)) + 3 * 8;
@@ -427,7 +495,7 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   j->invV = 1.0 / (double)j->V;
   j->spin = p->form == MAG_FORM_SPIN;
   j->ncopies = j->spin ? p->copies : 1;
-  j->sstride = (size_t)6 * j->g.Vh;
+  j->sstride = (size_t)2 * j->g.Vh * (p->group == MAG_SU2 ? 3 : 12);
   j->pstride = (size_t)2 * j->nb * 3;
 #define ALLOC(ptr, bytes)                                                          \
   do {                                                                             \
@@ -447,6 +515,7 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   if (p->precision == MAG_PREC_MIXED) {
     ALLOC(j->W32, j->nreal * sizeof(float));
     if (!j->spin) ALLOC(j->G32, (size_t)2 * j->g.Vh * j->nr * sizeof(float));
+    if (j->spin && p->group == MAG_SU3) ALLOC(j->S32, j->sstride * j->ncopies * sizeof(float));
   }
   if (j->spin) ALLOC(j->S, j->sstride * j->ncopies * sizeof(double));
   ALLOC(j->part, j->pstride * j->ncopies * sizeof(double));
@@ -642,9 +711,20 @@ int mag_job_functional(mag_job* j, double* F) {
   if (!j || !F) return fail(MAG_ERR_ARG, "NULL argument");
   if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
   CHK(set_device(j));
-  launch_functional(j);
+  double* tmp = nullptr;
+  CHK(alloc_tmp(j, &tmp));
+  if (tmp) {
+    su3_copy_links(j, 0, tmp);
+    mag::launch_su3_functional<double>(tmp, j->g, j->part, j->st);
+  } else {
+    launch_functional(j);
+  }
   mag::launch_reduce(j->part, 2 * j->nf, mag::RED_PLAIN, j->invF, j->invV, j->ctl, j->hF, j->hR,
                      j->hA, j->out3, j->st);
+  if (tmp) {
+    (void)hipStreamSynchronize(j->st);
+    (void)hipFree(tmp);
+  }
   HIPCHK(hipMemcpyAsync(j->out3_h, j->out3, 3 * sizeof(double), hipMemcpyDeviceToHost, j->st));
   HIPCHK(hipStreamSynchronize(j->st));
   *F = j->out3_h[0];
@@ -721,7 +801,7 @@ static int check_copy(mag_job* j, int c) {
 int mag_job_set_copy_spins(mag_job* j, int c, const double* spins) {
   CHK(check_copy(j, c));
   if (!spins) return fail(MAG_ERR_ARG, "NULL spins");
-  if (!j->spin) return fail(MAG_ERR_STATE, "not a spin-form job");
+  if (!j->spin || j->prm.group != MAG_SU2) return fail(MAG_ERR_STATE, "not an SU(2) spin-form job");
   if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
   if (j->batch) return fail(MAG_ERR_STATE, "job is in a batch: sweep it through the batch");
   const size_t n = (size_t)j->V * 3;
@@ -732,7 +812,7 @@ int mag_job_set_copy_spins(mag_job* j, int c, const double* spins) {
   hipError_t e = hipMemcpyAsync(stage, spins, n * sizeof(double), hipMemcpyHostToDevice, j->st);
   if (e == hipSuccess) {
     mag::launch_spins_layout(stage, j->S + (size_t)c * j->sstride, j->g, 1, j->st);
-    init_copy(j, c);
+    init_copy(j, c, nullptr);
     e = hipStreamSynchronize(j->st);
   }
   hipFree(stage);
@@ -743,7 +823,7 @@ int mag_job_set_copy_spins(mag_job* j, int c, const double* spins) {
 int mag_job_get_copy_spins(mag_job* j, int c, double* spins) {
   CHK(check_copy(j, c));
   if (!spins) return fail(MAG_ERR_ARG, "NULL spins");
-  if (!j->spin) return fail(MAG_ERR_STATE, "not a spin-form job");
+  if (!j->spin || j->prm.group != MAG_SU2) return fail(MAG_ERR_STATE, "not an SU(2) spin-form job");
   CHK(set_device(j));
   const size_t n = (size_t)j->V * 3;
   double* stage = nullptr;
@@ -805,7 +885,7 @@ int mag_job_download_copy_links(mag_job* j, int c, double* links) {
     return mag_job_download_links(j, links);
   }
   CHK(set_device(j));
-  const size_t n = (size_t)j->V * 4 * 4;
+  const size_t n = (size_t)j->V * 4 * j->nr_in;
   double *soa = nullptr, *stage = nullptr;
   HIPCHK(hipMalloc((void**)&soa, j->nreal * sizeof(double)));
   hipError_t e = hipMalloc((void**)&stage, n * sizeof(double));
@@ -813,8 +893,13 @@ int mag_job_download_copy_links(mag_job* j, int c, double* links) {
     hipFree(soa);
     return fail(MAG_ERR_NOMEM, "hipMalloc failed: %s", hipGetErrorString(e));
   }
-  mag::launch_spin_links(j->U64, j->S + (size_t)c * j->sstride, soa, j->g, j->st);
-  mag::launch_su2_soa_to_aos<double>(soa, stage, j->g, j->st);
+  if (su3_site(j)) {
+    su3_copy_links(j, c, soa);
+    mag::launch_su3_soa_to_aos<double>(soa, stage, j->g, j->st);
+  } else {
+    mag::launch_spin_links(j->U64, j->S + (size_t)c * j->sstride, soa, j->g, j->st);
+    mag::launch_su2_soa_to_aos<double>(soa, stage, j->g, j->st);
+  }
   e = hipMemcpyAsync(links, stage, n * sizeof(double), hipMemcpyDeviceToHost, j->st);
   hipError_t e2 = hipStreamSynchronize(j->st);
   hipFree(soa);

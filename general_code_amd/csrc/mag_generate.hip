@@ -15,6 +15,7 @@
 // copy 0 is the plain configuration, copies 1, 2, ... are gauge copies of the
 // same physical field (same link draws, different random gauge g).
 #include "mag_internal.h"
+#include "mag_layout.h"
 #include "mag_math.h"
 
 namespace mag {
@@ -330,6 +331,47 @@ void launch_gen_copy_spins(double* S, size_t sstride, int ncopies, const Geom& g
                            hipStream_t st) {
   const uint32_t V = 2 * g.Vh;
   hipLaunchKernelGGL(k_gen_copy_spins, dim3((V + 127) / 128), dim3(128), 0, st, S, sstride,
+                     ncopies, g, seed);
+}
+
+// Starting site gauges of SU(3) gauge copies 0..ncopies-1 (site form, on
+// the links of copy 0): h_c(x) = g_c(x) g_0(x)^+ with the Haar gauges of
+// launch_gen_su3 (so G_c U_0 G_c^+ is copy c), copy 0 the identity.  Rows
+// 0, 1 in the chunked site layout, copy c at G + c*gstride.
+__global__ void k_gen_su3_copy_gauges(double* __restrict__ G, size_t gstride, int ncopies,
+                                      Geom g, uint64_t seed) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= 2 * g.Vh) return;
+  uint32_t r = s;
+  const uint32_t x = r % (uint32_t)g.L0;
+  r /= (uint32_t)g.L0;
+  const uint32_t y = r % (uint32_t)g.L1;
+  r /= (uint32_t)g.L1;
+  const uint32_t z = r % (uint32_t)g.L2, t = r / (uint32_t)g.L2;
+  const int p = (int)((x + y + z + t) & 1u);
+  const uint32_t h = s >> 1;
+  const M3<double> g0 = su3_haar(gauge_copy_seed(seed, 0), s);
+  for (int c = 0; c < ncopies; c++) {
+    M2<double> m;
+    if (c == 0) {
+#pragma unroll
+      for (int k = 0; k < 6; k++) m.r[k / 3][k % 3] = {(k == 0 || k == 4) ? 1.0 : 0.0, 0.0};
+    } else {
+      M3<double> q = m3mul_dag(su3_haar(gauge_copy_seed(seed, (uint32_t)c), s), g0);
+      reunit3(q);
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) m.r[i][k] = q.m[i][k];
+    }
+    stm(site_ptr<12>(G + (size_t)c * gstride, g, p, h), m);
+  }
+}
+
+void launch_gen_su3_copy_gauges(double* G, size_t gstride, int ncopies, const Geom& g,
+                                uint64_t seed, hipStream_t st) {
+  const uint32_t V = 2 * g.Vh;
+  hipLaunchKernelGGL(k_gen_su3_copy_gauges, dim3((V + 127) / 128), dim3(128), 0, st, G, gstride,
                      ncopies, g, seed);
 }
 

@@ -86,6 +86,13 @@ template <typename TL>
 void launch_su2_copies_half(const TL* U, double* S, size_t sstride, int ncopies,
                             const Geom& g, int parity, double omega, double* part,
                             size_t pstride, const JobCtl* ctl, hipStream_t st);
+// SU(3) site form: links U read-only, site gauge of copy c at G + c*gstride
+// (2 rows, chunked site layout, the links' type); partials and control as
+// for the spin form, nb = sweep_blocks(g, 3)
+template <typename T>
+void launch_su3_site_copies(const T* U, T* G, size_t gstride, int ncopies, const Geom& g,
+                            int parity, double omega, double* part, size_t pstride,
+                            const JobCtl* ctl, hipStream_t st);
 // end-of-sweep reduction of every copy (one workgroup per copy): copy c
 // reduces part + c*pstride into ctl[c] and the history at hF + c*hstride
 void launch_reduce_copies(const double* part, size_t pstride, int nparts, int ncopies,
@@ -122,7 +129,7 @@ struct BatchJob {
   int rb0;     // first workgroup of this job in a batched rebuild grid
   // spin form (gauge copies): spins of copy c at S + c*sstride, partials
   // at part + c*pstride, control block ctl + c; ncopies = 0 in the link form
-  double* S;
+  void* S;  // spins (double) / site gauges (the link type)
   size_t sstride;
   size_t pstride;
   int ncopies;
@@ -135,6 +142,9 @@ void launch_half_batch(int group, bool fp32, const BatchJob* jobs, int njobs,
 // all hold ncopies copies
 void launch_copies_half_batch(int ncopies, bool fp32, const BatchJob* jobs, int njobs,
                               int nblocks_total, int parity, hipStream_t st);
+// batched SU(3) site-form half-sweep (FP32 or FP64 class)
+void launch_su3_site_copies_batch(bool fp32, const BatchJob* jobs, int njobs, int nblocks_total,
+                                  int parity, hipStream_t st);
 // batched end-of-sweep reduction: one workgroup per job
 void launch_reduce_batch(const BatchJob* jobs, int njobs, hipStream_t st);
 // batched FP64 rebuild of the FP32 working links + gauge normalisation of a
@@ -247,6 +257,16 @@ void launch_spins_e3(double* S, size_t nreal, const Geom& g, hipStream_t st);
 // (copy c at S + c*sstride): spin of g_c(x) g_0(x)^+, copy 0 exactly e3
 void launch_gen_copy_spins(double* S, size_t sstride, int ncopies, const Geom& g,
                            uint64_t seed, hipStream_t st);
+// SU(3) site form: links of a copy out = G(x) U0 G(x+mu)^+ (FP64); site
+// gauge arrays FP64 <-> FP32 (to FP64 with Gram-Schmidt); identity gauges;
+// starting gauges of the synthetic gauge copies (copy c at G + c*gstride)
+void launch_su3_site_links(const double* U0, const double* G, double* out, const Geom& g,
+                           hipStream_t st);
+void launch_su3_sites_to32(const double* a, float* b, const Geom& g, hipStream_t st);
+void launch_su3_sites_to64(const float* a, double* b, const Geom& g, hipStream_t st);
+void launch_su3_sites_identity(double* G, const Geom& g, hipStream_t st);
+void launch_gen_su3_copy_gauges(double* G, size_t gstride, int ncopies, const Geom& g,
+                                uint64_t seed, hipStream_t st);
 // out = G(s(x)) U G(s(x+mu))^+ with G = gauge_of_spin (chunked FP64 links)
 void launch_spin_links(const double* U, const double* S, double* out, const Geom& g,
                        hipStream_t st);

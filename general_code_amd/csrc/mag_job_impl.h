@@ -34,12 +34,16 @@ struct mag_job {
   int since_rebuild = 0;  // sweeps enqueued since the last FP64 rebuild
   double invF = 0, invV = 0;
   mag_batch* batch = nullptr;  // batch the job belongs to, if any
-  // spin form (gauge copies): U64 stays read-only; copy c has its spins at
-  // S + c*sstride, partials at part + c*pstride, control ctl[c] / ctl_h[c]
-  // and history at hF/hR/hA + c*max_sweeps
+  // site form (gauge copies): U64 stays read-only; copy c has its site
+  // variable (SU(2) spins, SU(3) site gauges) at S + c*sstride, partials at
+  // part + c*pstride, control ctl[c] / ctl_h[c] and history at
+  // hF/hR/hA + c*max_sweeps
+  // SU(3) in the site form: S holds the FP64 site gauges of the copies
+  // (2 rows, chunked site layout), S32 their FP32 working copies (mixed)
   bool spin = false;
   int ncopies = 1;
   double* S = nullptr;
+  float* S32 = nullptr;
   size_t sstride = 0;
   size_t pstride = 0;
 };

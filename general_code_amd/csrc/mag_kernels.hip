@@ -248,6 +248,7 @@ __device__ __forceinline__ M3<double> gs3(const M2<double>& in) {
   row2_of(m, M.m[2]);
   return M;
 }
+__device__ __forceinline__ M2<double> to_d(const M2<double>& m) { return m; }
 __device__ __forceinline__ M2<double> to_d(const M2<float>& m) {
   M2<double> r;
 #pragma unroll
@@ -289,8 +290,9 @@ __device__ __forceinline__ M3<double> full3(const M2<double>& m) {
   return M;
 }
 
+template <typename TG>
 __device__ __forceinline__ void su3_rebuild_block(const double* __restrict__ U0,
-                                                  const float* __restrict__ G,
+                                                  const TG* __restrict__ G,
                                                   float* __restrict__ out32, double* out64,
                                                   const Geom& g, uint32_t lb) {
   const uint32_t nb = (g.Vh + 255) / 256;
@@ -564,4 +566,61 @@ void launch_copy(const void* a, void* b, size_t bytes, hipStream_t st) {
   hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, st, (const float4*)a,
                      (float4*)b, n);
 }
+// ================================================================== SU(3) site form
+// out = G(x) U0 G(x+mu)^+ (FP64) from FP64 site gauges: the links of a copy
+__global__ __launch_bounds__(256) void k_su3_site_links(const double* __restrict__ U0,
+                                                        const double* __restrict__ G,
+                                                        double* __restrict__ out, Geom g) {
+  su3_rebuild_block<double>(U0, G, nullptr, out, g, blockIdx.x);
+}
+void launch_su3_site_links(const double* U0, const double* G, double* out, const Geom& g,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_su3_site_links, dim3(2 * ((g.Vh + 255) / 256)), dim3(256), 0, st, U0, G,
+                     out, g);
+}
+
+// site gauges (2 rows, chunked site layout): FP64 -> FP32, and FP32 -> FP64
+// with the rows put back on SU(3) in FP64 (Gram-Schmidt, row 2 implied)
+__global__ void k_su3_sites_to32(const double* __restrict__ a, float* __restrict__ b, Geom g) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= 2 * g.Vh) return;
+  const int p = (int)(s / g.Vh);
+  const uint32_t h = s % g.Vh;
+  const M2<double> m = ldm(site_ptr<12>(a, g, p, h));
+  M2<float> f;
+#pragma unroll
+  for (int k = 0; k < 6; k++) f.r[k / 3][k % 3] = {(float)m.r[k / 3][k % 3].re, (float)m.r[k / 3][k % 3].im};
+  stm(site_ptr<12>(b, g, p, h), f);
+}
+__global__ void k_su3_sites_to64(const float* __restrict__ a, double* __restrict__ b, Geom g) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= 2 * g.Vh) return;
+  const int p = (int)(s / g.Vh);
+  const uint32_t h = s % g.Vh;
+  const M3<double> m = gs3(to_d(ldm(site_ptr<12>(a, g, p, h))));
+  M2<double> r;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) r.r[i][k] = m.m[i][k];
+  stm(site_ptr<12>(b, g, p, h), r);
+}
+__global__ void k_su3_sites_identity(double* __restrict__ G, Geom g) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= 2 * g.Vh) return;
+  M2<double> m;
+#pragma unroll
+  for (int k = 0; k < 6; k++) m.r[k / 3][k % 3] = {(k == 0 || k == 4) ? 1.0 : 0.0, 0.0};
+  stm(site_ptr<12>(G, g, (int)(s / g.Vh), s % g.Vh), m);
+}
+void launch_su3_sites_to32(const double* a, float* b, const Geom& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_su3_sites_to32, dim3((2 * g.Vh + 255) / 256), dim3(256), 0, st, a, b, g);
+}
+void launch_su3_sites_to64(const float* a, double* b, const Geom& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_su3_sites_to64, dim3((2 * g.Vh + 255) / 256), dim3(256), 0, st, a, b, g);
+}
+void launch_su3_sites_identity(double* G, const Geom& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_su3_sites_identity, dim3((2 * g.Vh + 255) / 256), dim3(256), 0, st, G, g);
+}
+
 }  // namespace mag

@@ -524,7 +524,7 @@ template <int K, typename TL>
 __global__ __launch_bounds__(kBlock) void k_copies_half_batch(const BatchJob* __restrict__ jobs,
                                                               int njobs, int p) {
   const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
-  su2_copies_block<K, TL>((const TL*)j.U, j.S, j.sstride, j.g, p, j.omega,
+  su2_copies_block<K, TL>((const TL*)j.U, (double*)j.S, j.sstride, j.g, p, j.omega,
                           j.part + (size_t)3 * j.nb * p, j.pstride, j.ctl,
                           blockIdx.x - j.block0);
 }
@@ -598,6 +598,186 @@ void launch_copies_half_batch(int ncopies, bool fp32, const BatchJob* jobs, int 
     default: L_(8); break;
   }
 #undef L_
+}
+
+// ================================================================== SU(3) gauge copies
+// Site form of SU(3) (the SU(3) counterpart of the spin form): the links U
+// stay read-only and each gauge copy c holds a site gauge field G_c.  A
+// site's effective links
+//   forward  G_c(x) U_x,mu G_c(x+mu)^+,   backward  G_c(x-mu) U_x-mu,mu G_c(x)^+
+// are formed in registers and updated by the Cabibbo-Marinari code of the
+// link form (cm_subgroup, accumulating r into G_c(x)); only G_c(x) is
+// written.  Copy c therefore follows the link-form trajectory of the links
+// G_c U G_c^+, and all copies of a configuration share the link reads
+// (copy 0 reads U from HBM, the others mostly from cache).  Two lanes per
+// site as in the link form.
+template <typename T>
+__device__ __forceinline__ M3<T> full3t(const M2<T>& m) {
+  M3<T> M;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) M.m[i][k] = m.r[i][k];
+  row2_of(m, M.m[2]);
+  return M;
+}
+// A B^+
+template <typename T>
+__device__ __forceinline__ M3<T> mul3_dag(const M3<T>& A, const M3<T>& B) {
+  M3<T> R;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C<T> acc = cmulc(A.m[i][0], B.m[j][0]);
+      acc = cadd(acc, cmulc(A.m[i][1], B.m[j][1]));
+      R.m[i][j] = cadd(acc, cmulc(A.m[i][2], B.m[j][2]));
+    }
+  return R;
+}
+// rows 0, 1 of A B
+template <typename T>
+__device__ __forceinline__ M2<T> rows_mul(const M3<T>& A, const M3<T>& B) {
+  M2<T> R;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C<T> acc = cmul(A.m[i][0], B.m[0][j]);
+      acc = cadd(acc, cmul(A.m[i][1], B.m[1][j]));
+      R.r[i][j] = cadd(acc, cmul(A.m[i][2], B.m[2][j]));
+    }
+  return R;
+}
+// (2x3 R) B^+
+template <typename T>
+__device__ __forceinline__ M2<T> rows_mul_dag(const M2<T>& A, const M3<T>& B) {
+  M2<T> R;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C<T> acc = cmulc(A.r[i][0], B.m[j][0]);
+      acc = cadd(acc, cmulc(A.r[i][1], B.m[j][1]));
+      R.r[i][j] = cadd(acc, cmulc(A.r[i][2], B.m[j][2]));
+    }
+  return R;
+}
+
+template <typename T>
+__device__ __forceinline__ void su3_site_copies_block(const T* __restrict__ U,
+                                                      T* __restrict__ G, size_t gstride,
+                                                      int ncopies, const Geom& g, int p,
+                                                      T omega, double* __restrict__ part,
+                                                      size_t pstride,
+                                                      const JobCtl* __restrict__ ctl,
+                                                      uint32_t lb) {
+  bool live = false;
+  for (int c = 0; c < ncopies; c++) live |= !ctl[c].done;
+  if (!live) return;  // uniform over the workgroup
+  const uint32_t tid = lb * kBlock + threadIdx.x;
+  const uint32_t h = tid >> 1;
+  const int q = (int)(tid & 1u);
+  const bool in = h < g.Vh;
+  SiteIdx s{};
+  uint32_t hb[4] = {0, 0, 0, 0};
+  if (in) {
+    s = decode(g, h, p);
+    back_all(g, h, s, hb);
+  }
+  for (int c = 0; c < ncopies; c++) {
+    if (ctl[c].done) continue;  // uniform over the workgroup
+    T* Gc = G + (size_t)c * gstride;
+    double fs = 0.0, rs = 0.0, rm = 0.0;
+    if (in) {
+      const LRef<T> gp = site_ptr<12>(Gc, g, p, h);
+      M2<T> Gx2 = ldm(gp);
+      const M3<T> Gx = full3t(Gx2);
+      M2<T> F[2], B[2];
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        const int mu = 2 * q + k;
+        const uint32_t hfm = fwd_h(g, h, s, mu);
+        const uint32_t hbm = q ? hb[2 + k] : hb[k];
+        {
+          const M3<T> Uf = full3t(ldm(link_ptr<12>(U, g, p, mu, h)));
+          const M3<T> Gf = full3t(ldm(site_ptr<12>(Gc, g, 1 - p, hfm)));
+          F[k] = rows_mul(Gx, mul3_dag(Uf, Gf));
+        }
+        {
+          const M3<T> Ub = full3t(ldm(link_ptr<12>(U, g, 1 - p, mu, hbm)));
+          const M3<T> Gb = full3t(ldm(site_ptr<12>(Gc, g, 1 - p, hbm)));
+          B[k] = rows_mul_dag(rows_mul(Gb, Ub), Gx);
+        }
+      }
+      T res = cm_subgroup<T, 0, 1, true>(F, B, Gx2, omega);
+      res += cm_subgroup<T, 1, 2, true>(F, B, Gx2, omega);
+      res += cm_subgroup<T, 0, 2, true>(F, B, Gx2, omega);
+      if constexpr (sizeof(T) == 4) renorm_rows(Gx2);
+      if (q == 0) stm(gp, Gx2);
+      if (p == 1) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) fs += su3_link_F(F[k]) + su3_link_F(B[k]);
+      }
+      if (q == 0) {
+        rs = (double)res;
+        rm = (double)res;
+      }
+    }
+    block_reduce3<kWaves>(fs, rs, rm);
+    if (threadIdx.x == 0) {
+      double* o = part + (size_t)c * pstride + 3 * (size_t)lb;
+      o[0] = fs;
+      o[1] = rs;
+      o[2] = rm;
+    }
+    __syncthreads();  // the reduction's LDS is reused by the next copy
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_su3_site_copies(const T* __restrict__ U,
+                                                            T* __restrict__ G, size_t gstride,
+                                                            int ncopies, Geom g, int p,
+                                                            T omega, double* __restrict__ part,
+                                                            size_t pstride,
+                                                            const JobCtl* __restrict__ ctl) {
+  su3_site_copies_block<T>(U, G, gstride, ncopies, g, p, omega, part, pstride, ctl, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_su3_site_copies_batch(
+    const BatchJob* __restrict__ jobs, int njobs, int p) {
+  const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
+  su3_site_copies_block<T>((const T*)j.U, (T*)j.S, j.sstride, j.ncopies, j.g, p, (T)j.omega,
+                           j.part + (size_t)3 * j.nb * p, j.pstride, j.ctl,
+                           blockIdx.x - j.block0);
+}
+
+template <typename T>
+void launch_su3_site_copies(const T* U, T* G, size_t gstride, int ncopies, const Geom& g,
+                            int parity, double omega, double* part, size_t pstride,
+                            const JobCtl* ctl, hipStream_t st) {
+  const int nb = sweep_blocks(g, 3);
+  hipLaunchKernelGGL((k_su3_site_copies<T>), dim3(nb), dim3(kBlock), 0, st, U, G, gstride,
+                     ncopies, g, parity, (T)omega, part + (size_t)3 * nb * parity, pstride, ctl);
+}
+template void launch_su3_site_copies<float>(const float*, float*, size_t, int, const Geom&, int,
+                                            double, double*, size_t, const JobCtl*,
+                                            hipStream_t);
+template void launch_su3_site_copies<double>(const double*, double*, size_t, int, const Geom&,
+                                             int, double, double*, size_t, const JobCtl*,
+                                             hipStream_t);
+
+void launch_su3_site_copies_batch(bool fp32, const BatchJob* jobs, int njobs, int nblocks_total,
+                                  int parity, hipStream_t st) {
+  if (njobs == 0 || nblocks_total == 0) return;
+  if (fp32)
+    hipLaunchKernelGGL((k_su3_site_copies_batch<float>), dim3(nblocks_total), dim3(kBlock), 0,
+                       st, jobs, njobs, parity);
+  else
+    hipLaunchKernelGGL((k_su3_site_copies_batch<double>), dim3(nblocks_total), dim3(kBlock), 0,
+                       st, jobs, njobs, parity);
 }
 
 }  // namespace mag

@@ -177,8 +177,10 @@ def test_mixed_fp64_warmup(mag):
 
 
 def test_spin_form_argument_errors(mag):
-    with pytest.raises(mag.MagError):
-        mag.MagJob(group=mag.SU3, dims=(4, 4, 4, 4), form=mag.FORM_SPIN, copies=2)
+    with mag.MagJob(group=mag.SU3, dims=(4, 4, 4, 4), form=mag.FORM_SPIN, copies=2) as j3:
+        j3.generate(0.3, 1)
+        with pytest.raises(mag.MagError):  # SU(3) site variables are gauges, not spins
+            j3.copy_spins(0)
     with pytest.raises(mag.MagError):
         mag.MagJob(group=mag.SU2, dims=(4, 4, 4, 4), form=mag.FORM_SPIN, copies=9)
     with pytest.raises(mag.MagError):
@@ -229,3 +231,72 @@ def test_spin_form_mixed_reaches_fp64_maximum(mag):
                 assert np.array_equal(x, y)
         assert jb.fp64_start() == start
         b.remove(jb)
+
+
+# ------------------------------------------------------------ SU(3) site form
+SU3_DIMS = (4, 4, 4, 8)
+
+
+def test_su3_site_copies_match_link_form(mag, oracle):
+    """SU(3) site form: copy c (site gauges G_c on the links of copy 0)
+    follows the link-form job of gauge copy c: F per sweep to 1e-12, links
+    G_c U G_c^+ to 1e-11; and the link-form job itself follows the oracle."""
+    n = 30
+    with mag.MagJob(group=mag.SU3, dims=SU3_DIMS, form=mag.FORM_SPIN, copies=3, omega=1.6,
+                    tol=0.0, max_sweeps=n) as job:
+        job.generate(0.4, 606)
+        F0 = [job.copy_history(c) for c in range(3)]
+        assert all(len(h[0]) == 0 for h in F0)
+        job.sweeps(n)
+        hist = [job.copy_history(c)[0] for c in range(3)]
+        links = [job.download_copy(c) for c in range(3)]
+    for c in range(3):
+        with mag.MagJob(group=mag.SU3, dims=SU3_DIMS, omega=1.6, tol=0.0, max_sweeps=n) as lj:
+            lj.generate(0.4, 606, copy=c)
+            lj.sweeps(n)
+            F, _, _ = lj.history()
+            ref = lj.download()
+        assert _rel(hist[c], F) < 1e-12, c
+        assert np.max(np.abs(links[c] - ref)) < 1e-11, c
+    # oracle on copy 2's links
+    W = oracle.gen_su3_config(SU3_DIMS, 0.4, 606, copy=2)
+    Fo = [oracle.su3_link_sweep(W, SU3_DIMS, 1.6)[0] for _ in range(n)]
+    assert _rel(hist[2], Fo) < 1e-12
+    assert np.max(np.abs(links[2] - W)) < 1e-11
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_su3_site_copies_batch_equals_alone(mag, prec):
+    def make():
+        j = mag.MagJob(group=mag.SU3, dims=SU3_DIMS, form=mag.FORM_SPIN, copies=2, omega=1.6,
+                       tol=1e-11, max_sweeps=3000, precision=prec, tol_mixed=1e-11,
+                       fp64_warmup=40)
+        j.generate(0.4, 707)
+        return j
+
+    with make() as a:
+        a.run()
+        ref = [a.copy_history(c) for c in range(2)]
+        st = [a.copy_status(c) for c in range(2)]
+        start = a.fp64_start()
+    other = mag.MagJob(group=mag.SU2, dims=(8, 8, 8, 8), omega=1.7, tol=1e-12, max_sweeps=3000)
+    other.generate(0.5, 3)
+    with make() as b, mag.MagBatch() as bt:
+        bt.add(b)
+        bt.add(other)
+        bt.run()
+        for c in range(2):
+            for x, y in zip(b.copy_history(c), ref[c]):
+                assert np.array_equal(x, y)
+        assert b.fp64_start() == start
+        bt.remove(b)
+        bt.remove(other)
+    other.close()
+    if prec == 1:
+        with mag.MagJob(group=mag.SU3, dims=SU3_DIMS, form=mag.FORM_SPIN, copies=2, omega=1.6,
+                        tol=1e-11, max_sweeps=3000) as f:
+            f.generate(0.4, 707)
+            f.run()
+            for c in range(2):
+                Ff = f.copy_status(c)[2]
+                assert abs(st[c][2] - Ff) / Ff < 1e-9
