@@ -165,7 +165,7 @@ def run_plain(G, specs, max_sweeps):
     return total, out
 
 
-def run_batched(G, specs, max_sweeps, su3_precision, su3_warmup):
+def run_batched(G, specs, max_sweeps, su3_precision, su3_warmup, su3_site=False):
     """All configurations in one graph-captured batch: each SU(2)
     configuration is one spin-form copy set (links read once for all its
     copies; mixed: FP32-rounded links after the FP64 warm-up, spins and
@@ -177,6 +177,12 @@ def run_batched(G, specs, max_sweeps, su3_precision, su3_warmup):
             j = G.MagJob(group=2, dims=sp["dims"], omega=sp["omega"], tol=sp["tol"],
                          max_sweeps=max_sweeps, form=G.FORM_SPIN, copies=sp["copies"],
                          precision=su3_precision, tol_mixed=sp["tol"], fp64_warmup=su3_warmup)
+            j.generate(sp["eps"], sp["seed"])
+            jobs.append((j, list(range(sp["copies"]))))
+        elif su3_site:
+            j = G.MagJob(group=3, dims=sp["dims"], omega=sp["omega"], tol=sp["tol"],
+                         max_sweeps=max_sweeps, form=G.FORM_SPIN, copies=sp["copies"],
+                         precision=su3_precision, tol_mixed=1e-11, fp64_warmup=su3_warmup)
             j.generate(sp["eps"], sp["seed"])
             jobs.append((j, list(range(sp["copies"]))))
         else:

@@ -117,6 +117,7 @@ BatchJob describe(mag_job* j) {
   b.invV = j->invV;
   b.nb = j->nb;
   b.nparts = 2 * j->nb;
+  b.nrparts = mag::rebuild_blocks_of(j->g);
   b.block0 = 0;
   b.rb0 = 0;
   b.S = (j->spin && j->prm.group == MAG_SU3 && fp32) ? (void*)j->S32 : (void*)j->S;
@@ -212,8 +213,10 @@ void launch_class_half(mag_batch* b, int c, int p, hipStream_t st) {
 // sweep i's rebuild of a link-form FP32 class
 void launch_class_rebuild(mag_batch* b, int c, int i, hipStream_t st) {
   const ClassTable& t = b->cls[c];
-  if (c < kSpinClass && t.fp32 && b->reunit > 0 && (i + 1) % b->reunit == 0)
+  if (c < kSpinClass && t.fp32 && b->reunit > 0 && (i + 1) % b->reunit == 0) {
     mag::launch_rebuild_batch(t.group, t.d, (int)t.h.size(), t.nrblocks, st);
+    mag::launch_reduce_batch(t.d_red, (int)t.red.size(), mag::RED_REBUILD, st);
+  }
 }
 
 int live_classes(const mag_batch* b, std::vector<int>& live) {
@@ -245,7 +248,7 @@ hipError_t enqueue_replay(mag_batch* b) {
     for (int i = 0; i < b->replay; i++) {
       for (int p = 0; p < 2; p++)
         for (int c : live) launch_class_half(b, c, p, b->st);
-      mag::launch_reduce_batch(b->d_all, (int)b->all.size(), b->st);
+      mag::launch_reduce_batch(b->d_all, (int)b->all.size(), mag::RED_SWEEP, b->st);
       for (int c : live) launch_class_rebuild(b, c, i, b->st);
     }
     return hipSuccess;
@@ -260,7 +263,7 @@ hipError_t enqueue_replay(mag_batch* b) {
     for (int i = 0; i < b->replay; i++) {
       launch_class_half(b, c, 0, s);
       launch_class_half(b, c, 1, s);
-      mag::launch_reduce_batch(t.d_red, (int)t.red.size(), s);
+      mag::launch_reduce_batch(t.d_red, (int)t.red.size(), mag::RED_SWEEP, s);
       launch_class_rebuild(b, c, i, s);
     }
     e = hipEventRecord(b->ev_join[k], s);

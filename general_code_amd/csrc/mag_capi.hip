@@ -132,15 +132,22 @@ void launch_half(mag_job* j, int parity) {
 
 // periodic rebuild (skipped by the kernels once the job is done) or the
 // final FP64 rebuild (forced)
+// (the periodic one also sums F of the rebuilt links and tests convergence
+// on it: RED_REBUILD)
 void launch_rebuild(mag_job* j, bool final64) {
   const JobCtl* c = final64 ? nullptr : j->ctl;
+  double* part = final64 ? nullptr : j->part;
   if (j->prm.group == MAG_SU2)
     mag::launch_su2_rebuild(j->U64, j->G32, final64 ? nullptr : j->W32, final64 ? j->U64 : nullptr,
-                            j->g, c, j->st);
+                            j->g, c, part, j->st);
   else
     mag::launch_su3_rebuild(j->U64, j->G32, final64 ? nullptr : j->W32, final64 ? j->U64 : nullptr,
-                            j->g, c, j->st);
-  if (!final64) mag::launch_gauge_normalize(j->G32, j->g, j->prm.group, j->ctl, j->st);
+                            j->g, c, part, j->st);
+  if (!final64) {
+    mag::launch_gauge_normalize(j->G32, j->g, j->prm.group, j->ctl, j->st);
+    mag::launch_reduce(j->part, mag::rebuild_blocks_of(j->g), mag::RED_REBUILD, j->invF, j->invV,
+                       j->ctl, j->hF, j->hR, j->hA, nullptr, j->st);
+  }
 }
 
 // sweep tail shared by the plain and timed paths
@@ -230,7 +237,8 @@ void init_copy(mag_job* j, int c, double* tmp) {
   h.max_sweeps = j->prm.max_sweeps;
   const bool fp32 = j->prm.precision == MAG_PREC_MIXED && j->phase == 0;
   h.tol = fp32 ? j->prm.tol_mixed : j->prm.tol;
-  h.window = (fp32 && !j->spin) ? j->prm.reunit_every : 0;
+  h.window = (fp32 && !j->spin) ? -1 : 0;
+  h.rb_every = j->prm.reunit_every;
   hipMemcpyAsync(j->ctl + c, &h, sizeof(JobCtl), hipMemcpyHostToDevice, j->st);
   double* part = j->part + (size_t)c * j->pstride;
   if (su3_site(j)) {
@@ -258,7 +266,9 @@ int start_fp32_phase(mag_job* j) {
   j->since_rebuild = 0;
   for (int c = 0; c < j->ncopies; c++) {
     j->ctl_h[c].tol = j->prm.tol_mixed;
-    j->ctl_h[c].window = j->spin ? 0 : j->prm.reunit_every;
+    j->ctl_h[c].window = j->spin ? 0 : -1;
+    j->ctl_h[c].rb_every = j->prm.reunit_every;
+    j->ctl_h[c].F_ref = 0.0;
   }
   return write_ctl(j);
 }

@@ -44,15 +44,19 @@ struct JobCtl {
   double F_prev;   // functional after the last sweep (initial value first)
   double F0;       // functional of the uploaded configuration
   int window;      // > 1: convergence on the mean change per sweep over the
-                   // last `window` sweeps (the FP32 phase: a whole period
-                   // between two FP64 rebuilds); else on the last sweep
-  int pad;
+                   // last `window` sweeps; -1: no test per sweep (tested
+                   // at the FP64 rebuilds instead); else on the last sweep
+  int rb_every;    // sweeps between two FP64 rebuilds (mixed link form)
+  double F_ref;    // F of the links of the previous rebuild (0: none yet)
+  double pad;
 };
 
 // reduction kernel modes
 //   RED_INIT : F0 = F_prev = F      RED_SWEEP: append a sweep to the history
 //   RED_PLAIN: out3 only            RED_RESET: F_prev = F (phase change)
-enum { RED_INIT = 0, RED_SWEEP = 1, RED_PLAIN = 2, RED_RESET = 3 };
+//   RED_REBUILD: F of freshly rebuilt links (FP64); converged when the mean
+//                change per sweep since the previous rebuild is below tol
+enum { RED_INIT = 0, RED_SWEEP = 1, RED_PLAIN = 2, RED_RESET = 3, RED_REBUILD = 4 };
 
 // ---------------------------------------------------------------- launchers
 // Partial buffers hold 3 doubles per workgroup: (F sum, residual sum,
@@ -125,6 +129,7 @@ struct BatchJob {
   double invV;
   int nb;      // workgroups of one half-sweep of this job
   int nparts;  // partial triples of a sweep (2*nb)
+  int nrparts; // partial triples of a rebuild
   int block0;  // first workgroup of this job in a batched half-sweep grid
   int rb0;     // first workgroup of this job in a batched rebuild grid
   // spin form (gauge copies): spins of copy c at S + c*sstride, partials
@@ -145,8 +150,9 @@ void launch_copies_half_batch(int ncopies, bool fp32, const BatchJob* jobs, int 
 // batched SU(3) site-form half-sweep (FP32 or FP64 class)
 void launch_su3_site_copies_batch(bool fp32, const BatchJob* jobs, int njobs, int nblocks_total,
                                   int parity, hipStream_t st);
-// batched end-of-sweep reduction: one workgroup per job
-void launch_reduce_batch(const BatchJob* jobs, int njobs, hipStream_t st);
+// batched reduction, one workgroup per job: mode RED_SWEEP (end of sweep)
+// or RED_REBUILD (after a batched rebuild)
+void launch_reduce_batch(const BatchJob* jobs, int njobs, int mode, hipStream_t st);
 // batched FP64 rebuild of the FP32 working links + gauge normalisation of a
 // class table (mixed FP32 phase); jobs that are done are skipped
 void launch_rebuild_batch(int group, const BatchJob* jobs, int njobs, int nblocks_total,
@@ -177,10 +183,14 @@ void launch_gauge_identity(float* G, const Geom& g, int group, hipStream_t st);
 // out = G(x) U0 G(x+mu)^+ with G normalised in FP64 (SU(2): q/|q|, SU(3):
 // Gram-Schmidt).  out32 (FP32 working links) or out64 (may alias U0).
 // Skipped when ctl->done (ctl = nullptr forces it: the final rebuild).
+// part != nullptr: the F partials of the rebuilt links (rebuild_blocks_of(g)
+// triples) for a RED_REBUILD reduction
 void launch_su2_rebuild(const double* U0, const float* G, float* out32,
-                        double* out64, const Geom& g, const JobCtl* ctl, hipStream_t st);
+                        double* out64, const Geom& g, const JobCtl* ctl, double* part,
+                        hipStream_t st);
 void launch_su3_rebuild(const double* U0, const float* G, float* out32,
-                        double* out64, const Geom& g, const JobCtl* ctl, hipStream_t st);
+                        double* out64, const Geom& g, const JobCtl* ctl, double* part,
+                        hipStream_t st);
 // G <- float(normalise_fp64(G)); skipped when ctl->done
 void launch_gauge_normalize(float* G, const Geom& g, int group, const JobCtl* ctl,
                             hipStream_t st);

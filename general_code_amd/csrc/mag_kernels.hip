@@ -258,26 +258,43 @@ __device__ __forceinline__ M2<double> to_d(const M2<float>& m) {
   return r;
 }
 
+// the functional partial of a rebuild workgroup (F of the links it wrote,
+// in FP64) to part[3*lb]; part = nullptr: none
+__device__ __forceinline__ void rebuild_partial(double* part, double fs, uint32_t lb) {
+  if (!part) return;  // uniform
+  double z1 = 0.0, z2 = 0.0;
+  block_reduce3<4>(fs, z1, z2);
+  if (threadIdx.x == 0) {
+    part[3 * lb] = fs;
+    part[3 * lb + 1] = 0.0;
+    part[3 * lb + 2] = 0.0;
+  }
+}
+
 // workgroup lb in [0, 2*ceil(Vh/256)) of a rebuild over both parities
 __device__ __forceinline__ void su2_rebuild_block(const double* __restrict__ U0,
                                                   const float* __restrict__ G,
                                                   float* __restrict__ out32, double* out64,
-                                                  const Geom& g, uint32_t lb) {
+                                                  const Geom& g, uint32_t lb, double* part) {
   const uint32_t nb = (g.Vh + 255) / 256;
   const int p = (int)(lb / nb);
   const uint32_t h = (lb % nb) * 256 + threadIdx.x;
-  if (h >= g.Vh) return;
-  SiteIdx s = decode(g, h, p);
-  Q<double> gx = norm_q(site_ptr<4>(G, g, p, h));
+  double fs = 0.0;
+  if (h < g.Vh) {
+    SiteIdx s = decode(g, h, p);
+    Q<double> gx = norm_q(site_ptr<4>(G, g, p, h));
 #pragma unroll
-  for (int mu = 0; mu < 4; mu++) {
-    uint32_t hf = fwd_h(g, h, s, mu);
-    Q<double> gf = norm_q(site_ptr<4>(G, g, 1 - p, hf));
-    Q<double> u = ldq(link_ptr<4>(U0, g, p, mu, h));
-    Q<double> w = qmul_dag(qmul(gx, u), gf);
-    if (out32) stq(link_ptr<4>(out32, g, p, mu, h), Q<float>{(float)w.a0, (float)w.a1, (float)w.a2, (float)w.a3});
-    else stq(link_ptr<4>(out64, g, p, mu, h), w);
+    for (int mu = 0; mu < 4; mu++) {
+      uint32_t hf = fwd_h(g, h, s, mu);
+      Q<double> gf = norm_q(site_ptr<4>(G, g, 1 - p, hf));
+      Q<double> u = ldq(link_ptr<4>(U0, g, p, mu, h));
+      Q<double> w = qmul_dag(qmul(gx, u), gf);
+      if (part) fs += link_F(w);
+      if (out32) stq(link_ptr<4>(out32, g, p, mu, h), Q<float>{(float)w.a0, (float)w.a1, (float)w.a2, (float)w.a3});
+      else stq(link_ptr<4>(out64, g, p, mu, h), w);
+    }
   }
+  rebuild_partial(part, fs, lb);
 }
 
 __device__ __forceinline__ M3<double> full3(const M2<double>& m) {
@@ -294,50 +311,54 @@ template <typename TG>
 __device__ __forceinline__ void su3_rebuild_block(const double* __restrict__ U0,
                                                   const TG* __restrict__ G,
                                                   float* __restrict__ out32, double* out64,
-                                                  const Geom& g, uint32_t lb) {
+                                                  const Geom& g, uint32_t lb, double* part) {
   const uint32_t nb = (g.Vh + 255) / 256;
   const int p = (int)(lb / nb);
   const uint32_t h = (lb % nb) * 256 + threadIdx.x;
-  if (h >= g.Vh) return;
-  SiteIdx s = decode(g, h, p);
-  M3<double> gx = gs3(to_d(ldm(site_ptr<12>(G, g, p, h))));
+  double fs = 0.0;
+  if (h < g.Vh) {
+    SiteIdx s = decode(g, h, p);
+    M3<double> gx = gs3(to_d(ldm(site_ptr<12>(G, g, p, h))));
 #pragma unroll 1
-  for (int mu = 0; mu < 4; mu++) {
-    uint32_t hf = fwd_h(g, h, s, mu);
-    M3<double> gf = gs3(to_d(ldm(site_ptr<12>(G, g, 1 - p, hf))));
-    M3<double> u = full3(ldm(link_ptr<12>(U0, g, p, mu, h)));
-    // V = u gf^+
-    M3<double> v;
+    for (int mu = 0; mu < 4; mu++) {
+      uint32_t hf = fwd_h(g, h, s, mu);
+      M3<double> gf = gs3(to_d(ldm(site_ptr<12>(G, g, 1 - p, hf))));
+      M3<double> u = full3(ldm(link_ptr<12>(U0, g, p, mu, h)));
+      // V = u gf^+
+      M3<double> v;
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+      for (int i = 0; i < 3; i++)
 #pragma unroll
-      for (int j = 0; j < 3; j++) {
-        C<double> acc{0.0, 0.0};
+        for (int j = 0; j < 3; j++) {
+          C<double> acc{0.0, 0.0};
 #pragma unroll
-        for (int k = 0; k < 3; k++) acc = cadd(acc, cmulc(u.m[i][k], gf.m[j][k]));
-        v.m[i][j] = acc;
-      }
-    M2<double> w;
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        C<double> acc{0.0, 0.0};
-#pragma unroll
-        for (int k = 0; k < 3; k++) acc = cadd(acc, cmul(gx.m[i][k], v.m[k][j]));
-        w.r[i][j] = acc;
-      }
-    if (out32) {
-      M2<float> wf;
+          for (int k = 0; k < 3; k++) acc = cadd(acc, cmulc(u.m[i][k], gf.m[j][k]));
+          v.m[i][j] = acc;
+        }
+      M2<double> w;
 #pragma unroll
       for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) wf.r[i][j] = {(float)w.r[i][j].re, (float)w.r[i][j].im};
-      stm(link_ptr<12>(out32, g, p, mu, h), wf);
-    } else {
-      stm(link_ptr<12>(out64, g, p, mu, h), w);
+        for (int j = 0; j < 3; j++) {
+          C<double> acc{0.0, 0.0};
+#pragma unroll
+          for (int k = 0; k < 3; k++) acc = cadd(acc, cmul(gx.m[i][k], v.m[k][j]));
+          w.r[i][j] = acc;
+        }
+      if (part) fs += su3_link_F(w);
+      if (out32) {
+        M2<float> wf;
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int j = 0; j < 3; j++) wf.r[i][j] = {(float)w.r[i][j].re, (float)w.r[i][j].im};
+        stm(link_ptr<12>(out32, g, p, mu, h), wf);
+      } else {
+        stm(link_ptr<12>(out64, g, p, mu, h), w);
+      }
     }
   }
+  rebuild_partial(part, fs, lb);
 }
 
 // ctl: skip when the job is done (the periodic rebuild of a finished job
@@ -345,29 +366,31 @@ __device__ __forceinline__ void su3_rebuild_block(const double* __restrict__ U0,
 __global__ __launch_bounds__(256) void k_su2_rebuild(const double* __restrict__ U0,
                                                      const float* __restrict__ G,
                                                      float* __restrict__ out32, double* out64,
-                                                     Geom g, const JobCtl* __restrict__ ctl) {
+                                                     Geom g, const JobCtl* __restrict__ ctl,
+                                                     double* part) {
   if (ctl && ctl->done) return;
-  su2_rebuild_block(U0, G, out32, out64, g, blockIdx.x);
+  su2_rebuild_block(U0, G, out32, out64, g, blockIdx.x, part);
 }
 __global__ __launch_bounds__(256) void k_su3_rebuild(const double* __restrict__ U0,
                                                      const float* __restrict__ G,
                                                      float* __restrict__ out32, double* out64,
-                                                     Geom g, const JobCtl* __restrict__ ctl) {
+                                                     Geom g, const JobCtl* __restrict__ ctl,
+                                                     double* part) {
   if (ctl && ctl->done) return;
-  su3_rebuild_block(U0, G, out32, out64, g, blockIdx.x);
+  su3_rebuild_block(U0, G, out32, out64, g, blockIdx.x, part);
 }
 
 static inline int rebuild_blocks(const Geom& g) { return 2 * (int)((g.Vh + 255) / 256); }
 
 void launch_su2_rebuild(const double* U0, const float* G, float* out32, double* out64,
-                        const Geom& g, const JobCtl* ctl, hipStream_t st) {
+                        const Geom& g, const JobCtl* ctl, double* part, hipStream_t st) {
   hipLaunchKernelGGL(k_su2_rebuild, dim3(rebuild_blocks(g)), dim3(256), 0, st, U0, G, out32,
-                     out64, g, ctl);
+                     out64, g, ctl, part);
 }
 void launch_su3_rebuild(const double* U0, const float* G, float* out32, double* out64,
-                        const Geom& g, const JobCtl* ctl, hipStream_t st) {
+                        const Geom& g, const JobCtl* ctl, double* part, hipStream_t st) {
   hipLaunchKernelGGL(k_su3_rebuild, dim3(rebuild_blocks(g)), dim3(256), 0, st, U0, G, out32,
-                     out64, g, ctl);
+                     out64, g, ctl, part);
 }
 
 __device__ __forceinline__ void gauge_normalize_site(float* __restrict__ G, const Geom& g,
@@ -417,9 +440,11 @@ __global__ __launch_bounds__(256) void k_rebuild_batch(const BatchJob* __restric
   const BatchJob& j = jobs[batch_rjob(jobs, njobs, blockIdx.x)];
   if (j.ctl->done) return;
   if (GROUP == 2)
-    su2_rebuild_block(j.U0, (const float*)j.G, (float*)j.U, nullptr, j.g, blockIdx.x - j.rb0);
+    su2_rebuild_block(j.U0, (const float*)j.G, (float*)j.U, nullptr, j.g, blockIdx.x - j.rb0,
+                      j.part);
   else
-    su3_rebuild_block(j.U0, (const float*)j.G, (float*)j.U, nullptr, j.g, blockIdx.x - j.rb0);
+    su3_rebuild_block(j.U0, (const float*)j.G, (float*)j.U, nullptr, j.g, blockIdx.x - j.rb0,
+                      j.part);
 }
 template <int GROUP>
 __global__ void k_normalize_batch(const BatchJob* __restrict__ jobs, int njobs) {
@@ -571,7 +596,7 @@ void launch_copy(const void* a, void* b, size_t bytes, hipStream_t st) {
 __global__ __launch_bounds__(256) void k_su3_site_links(const double* __restrict__ U0,
                                                         const double* __restrict__ G,
                                                         double* __restrict__ out, Geom g) {
-  su3_rebuild_block<double>(U0, G, nullptr, out, g, blockIdx.x);
+  su3_rebuild_block<double>(U0, G, nullptr, out, g, blockIdx.x, nullptr);
 }
 void launch_su3_site_links(const double* U0, const double* G, double* out, const Geom& g,
                            hipStream_t st) {

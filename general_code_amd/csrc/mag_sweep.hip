@@ -74,6 +74,7 @@ __device__ void reduce_job(const double* __restrict__ part, int nparts, int mode
       const int w = ctl->window;
       double rel = fabs(F - ctl->F_prev) / fabs(F);
       if (w > 1) rel = sw >= w ? fabs(F - hF[sw - w]) / (w * fabs(F)) : INFINITY;
+      if (w < 0) rel = INFINITY;
       ctl->F_prev = F;
       ctl->sweep = sw + 1;
       if (rel < ctl->tol) {
@@ -81,6 +82,16 @@ __device__ void reduce_job(const double* __restrict__ part, int nparts, int mode
         ctl->done = 1;
       }
       if (sw + 1 >= ctl->max_sweeps) ctl->done = 1;
+    } else if (mode == RED_REBUILD) {
+      // FP64 links rebuilt from the master: no FP32 rounding in this F
+      if (ctl->F_ref != 0.0) {
+        const double rel = fabs(F - ctl->F_ref) / (ctl->rb_every * fabs(F));
+        if (rel < ctl->tol) {
+          ctl->converged = 1;
+          ctl->done = 1;
+        }
+      }
+      ctl->F_ref = F;
     }
     if (out3) {
       out3[0] = F;
@@ -94,15 +105,17 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce(const double* __restrict__
                                                       int nparts, int mode, double invF,
                                                       double invV, JobCtl* ctl, double* hF,
                                                       double* hR, double* hA, double* out3) {
-  if (mode == RED_SWEEP && ctl->done) return;
+  if ((mode == RED_SWEEP || mode == RED_REBUILD) && ctl->done) return;
   reduce_job(part, nparts, mode, invF, invV, ctl, hF, hR, hA, out3);
 }
 
 // one workgroup per job of the table
-__global__ __launch_bounds__(kRedBlock) void k_reduce_batch(const BatchJob* __restrict__ jobs) {
+__global__ __launch_bounds__(kRedBlock) void k_reduce_batch(const BatchJob* __restrict__ jobs,
+                                                            int mode) {
   const BatchJob& j = jobs[blockIdx.x];
   if (j.ctl->done) return;
-  reduce_job(j.part, j.nparts, RED_SWEEP, j.invF, j.invV, j.ctl, j.hF, j.hR, j.hA, nullptr);
+  reduce_job(j.part, mode == RED_REBUILD ? j.nrparts : j.nparts, mode, j.invF, j.invV, j.ctl,
+             j.hF, j.hR, j.hA, nullptr);
 }
 
 void launch_reduce(const double* part, int nparts, int mode, double invF, double invV,
@@ -111,8 +124,9 @@ void launch_reduce(const double* part, int nparts, int mode, double invF, double
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedBlock), 0, st, part, nparts, mode, invF, invV,
                      ctl, hF, hR, hA, out3);
 }
-void launch_reduce_batch(const BatchJob* jobs, int njobs, hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce_batch, dim3(njobs), dim3(kRedBlock), 0, st, jobs);
+void launch_reduce_batch(const BatchJob* jobs, int njobs, int mode, hipStream_t st) {
+  if (njobs == 0) return;
+  hipLaunchKernelGGL(k_reduce_batch, dim3(njobs), dim3(kRedBlock), 0, st, jobs, mode);
 }
 
 // ================================================================== helpers

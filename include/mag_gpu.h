@@ -65,11 +65,10 @@ typedef struct mag_params {
   int reunit_every;   /* mixed: rebuild FP32 links from the FP64 master      */
                       /*        every reunit_every sweeps                      */
   double tol_mixed;   /* mixed: FP32 phase ends when the mean relative     */
-                      /*        change per sweep over the last reunit_every   */
-                      /*        sweeps drops below it (the FP32 functional    */
-                      /*        steps at each rebuild; a whole period between */
-                      /*        two rebuilds averages that out); FP64 sweeps  */
-                      /*        then run down to tol                          */
+                      /*        change per sweep drops below it; link form:   */
+                      /*        measured on the FP64 links of two successive  */
+                      /*        rebuilds (free of FP32 rounding), site form:  */
+                      /*        per sweep.  FP64 sweeps then run down to tol  */
   int form;           /* MAG_FORM_LINK or MAG_FORM_SPIN                        */
   int copies;         /* gauge copies of the configuration swept together    */
                       /* (spin form, 1..MAG_MAX_COPIES; 1 in the link form)  */
