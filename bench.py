@@ -96,32 +96,62 @@ def measure(G, group, dims, precision, eps, omega, steps, warmup, seed=2024):
     }
 
 
+def _median_rate(fn, V, seconds, min_runs=5):
+    """site-updates/s of fn() (one sweep): median over runs after one
+    warm-up, for about `seconds` (at least min_runs runs)."""
+    fn()
+    ts = []
+    t_end = time.perf_counter() + seconds
+    while len(ts) < min_runs or (time.perf_counter() < t_end and len(ts) < 2000):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return V / ts[len(ts) // 2], len(ts)
+
+
 def cpu_baseline(dims, eps, omega, seconds):
-    """The oracle's checkerboard link sweep (FP64, OpenMP over the host's
-    cores) on the same lattice, for a bounded number of sweeps."""
+    """CPU timings on the host's cores, same lattice and configuration:
+    (i) the reference's own sweep as it runs there -- the spin form,
+    lexicographic, in place, OpenMP collapse(4) (mag.cpp:1375-1401, racy
+    as in the reference; the reported value, the faster of the two) -- and
+    (ii) the oracle's checkerboard omega link sweep, the algorithm the GPU
+    runs; each with all threads and with one thread; median of >= 5 sweeps
+    after a warm-up."""
     from oracle import oracle as O
     O.build()
     threads = O.get_threads()
     links = O.gen_su2_config(dims, eps, 2024)
+    spins = O.gen_spins_uniform(dims, 2024)
     V = 1
     for d in dims:
         V *= d
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        O.su2_link_sweep(links, dims, omega)
-        n += 1
-        el = time.perf_counter() - t0
-        if (el >= seconds and n >= 3) or n >= 2000:
-            break
+    q = seconds / 4.0
+    link_all, n_link = _median_rate(lambda: O.su2_link_sweep(links, dims, omega), V, q)
+    spin_all, n_spin = _median_rate(lambda: O.su2_or_sweep_lexi_omp(links, spins, dims), V, q)
+    O.set_threads(1)
+    try:
+        link_one, _ = _median_rate(lambda: O.su2_link_sweep(links, dims, omega), V, q)
+        spin_one, _ = _median_rate(lambda: O.su2_or_sweep_lexi_omp(links, spins, dims), V, q)
+    finally:
+        O.set_threads(threads)
+    lat = "x".join(map(str, dims))
     return {
-        "value": V * n / el,
+        "value": spin_all,
         "unit": "site-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"SU(2) {'x'.join(map(str, dims))} FP64 checkerboard omega={omega} link "
-                  f"sweeps of the oracle (C, gcc -O3, OpenMP {threads} threads): {n} sweeps "
-                  f"in {el:.1f} s",
+        "sample": f"SU(2) {lat} FP64: the reference's spin overrelaxation sweep as it runs "
+                  "there (lexicographic, in place, OpenMP collapse(4), mag.cpp:1375-1401), "
+                  f"restated in C (gcc -O3 -fopenmp), {threads} threads, median of {n_spin} "
+                  "sweeps after a warm-up",
+        "single_thread": spin_one,
+        "algorithm_matched": {
+            "value": link_all,
+            "single_thread": link_one,
+            "sample": f"SU(2) {lat} checkerboard omega={omega} link sweeps of the oracle "
+                      f"(the GPU's algorithm), {n_link} sweeps",
+        },
     }
 
 

@@ -311,6 +311,26 @@ void or_su2_or_sweep_lexi(const double *links, double *spins, const int *dims) {
   }
 }
 
+/* Timing baseline only: the reference's overrelaxation_update
+ * (mag.cpp:1375-1401) as it runs there -- an OpenMP loop over all sites in
+ * lexicographic order, collapse(4), spins updated in place.  Neighbouring
+ * sites updated by different threads race, exactly as in the reference, so
+ * the result depends on the thread schedule; it is never used as a check. */
+void or_su2_or_sweep_lexi_omp(const double *links, double *spins, const int *dims) {
+  geom_t g = geom(dims);
+  const int L0 = g.L[0], L1 = g.L[1], L2 = g.L[2], L3 = g.L[3];
+#pragma omp parallel for collapse(4) schedule(static)
+  for (int t = 0; t < L3; t++)
+    for (int z = 0; z < L2; z++)
+      for (int y = 0; y < L1; y++)
+        for (int x = 0; x < L0; x++) {
+          const int64_t s = x + (int64_t)L0 * (y + (int64_t)L1 * (z + (int64_t)L2 * t));
+          double h[3];
+          contribution_site_g(&g, links, spins, s, h);
+          or_spin_reflect_fast(spins + 3 * s, h);
+        }
+}
+
 /* relaxation_update (mag.cpp:1477-1512) */
 void or_su2_relax_sweep_lexi(const double *links, double *spins,
                              const int *dims, double *diff_max,

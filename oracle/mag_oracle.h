@@ -55,6 +55,8 @@ double or_su2_functional_spin(const double *links, const double *spins,
 void or_su2_gauge_transform_spins(double *links, const double *spins,
                                   const int *dims);                      /* mag.cpp:1045-1068 */
 void or_su2_or_sweep_lexi(const double *links, double *spins, const int *dims);   /* mag.cpp:1375-1401 */
+/* timing baseline only: the reference's racy in-place OpenMP sweep */
+void or_su2_or_sweep_lexi_omp(const double *links, double *spins, const int *dims);
 void or_su2_relax_sweep_lexi(const double *links, double *spins, const int *dims,
                              double *diff_max, double *diff_avg);                 /* mag.cpp:1477-1512 */
 int or_su2_maximize_final(const double *links, double *spins, const int *dims,

@@ -54,6 +54,7 @@ def lib():
             "or_su2_functional_spin": (ctypes.c_double, [_dp, _dp, _ip]),
             "or_su2_gauge_transform_spins": (None, [_dp, _dp, _ip]),
             "or_su2_or_sweep_lexi": (None, [_dp, _dp, _ip]),
+            "or_su2_or_sweep_lexi_omp": (None, [_dp, _dp, _ip]),
             "or_su2_relax_sweep_lexi": (None, [_dp, _dp, _ip, _dp, _dp]),
             "or_su2_maximize_final": (ctypes.c_int, [_dp, _dp, _ip, ctypes.c_int,
                                                      ctypes.c_double, ctypes.c_double,
@@ -213,6 +214,11 @@ def su2_gauge_transform_spins(links, spins, dims):
 
 def su2_or_sweep_lexi(links, spins, dims):
     lib().or_su2_or_sweep_lexi(_d(links), _d(spins), _dims(dims))
+
+
+def su2_or_sweep_lexi_omp(links, spins, dims):
+    """Timing baseline: the reference's in-place OpenMP sweep (racy)."""
+    lib().or_su2_or_sweep_lexi_omp(_d(links), _d(spins), _dims(dims))
 
 
 def su2_relax_sweep_lexi(links, spins, dims):
