@@ -334,6 +334,10 @@ def serve_bench(G, n=40, max_sweeps=4000):
     # functional agreement over the requests that converged in both runs
     dF = [abs(a["F"] - b["F"]) / abs(b["F"]) for a, b in zip(res, plain_res)
           if a["converged"] and b["converged"]]
+    off = [dict(group=r["group"], dims=r["dims"], F_plain=b["F"], F_served=a["F"],
+                sweeps_plain=b["sweeps"], sweeps_served=a["sweeps"])
+           for r, a, b in zip(reqs, res, plain_res)
+           if a["converged"] and b["converged"] and abs(a["F"] - b["F"]) > 1e-9 * abs(b["F"])]
     return {
         "workload": f"{n} requests, Poisson 20/s (seeded), 60% SU(2) / 40% SU(3), sizes "
                     "16^4, 24^4, 32^4, 24^3x48, 48^4 (SU(2) only); sweep cap "
@@ -344,6 +348,7 @@ def serve_bench(G, n=40, max_sweeps=4000):
         "max_rel_dF_vs_plain_converged": max(dF) if dF else None,
         "requests_compared": len(dF),
         "requests_rel_dF_above_1e-9": sum(1 for d in dF if d > 1e-9),
+        "requests_off": off,
     }
 
 

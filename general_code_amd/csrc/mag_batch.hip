@@ -116,7 +116,8 @@ BatchJob describe(mag_job* j) {
   b.invF = j->invF;
   b.invV = j->invV;
   b.nb = j->nb;
-  b.nparts = 2 * j->nb;
+  b.nw = j->nw;
+  b.nparts = 2 * j->nw;
   b.nrparts = mag::rebuild_blocks_of(j->g);
   b.block0 = 0;
   b.rb0 = 0;

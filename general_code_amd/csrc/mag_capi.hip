@@ -102,6 +102,20 @@ int set_device(const mag_job* j) {
   return MAG_OK;
 }
 
+// the job's launch shape for its FP64 / FP32 link-form kernel (tuned on
+// first use; the default shape if tuning fails -- every shape gives the
+// same results)
+mag::Shape link_shape(mag_job* j, bool fp32) {
+  const int k = fp32 ? 1 : 0;
+  if (!j->shape_ok[k]) {
+    mag::Shape sh;
+    if (mag::job_shape(j, fp32, &sh) != MAG_OK) sh = mag::Shape{};
+    j->shape[k] = sh;
+    j->shape_ok[k] = true;
+  }
+  return j->shape[k];
+}
+
 void launch_half(mag_job* j, int parity) {
   if (j->spin && j->prm.group == MAG_SU3) {
     if (mixed32(j))
@@ -118,15 +132,17 @@ void launch_half(mag_job* j, int parity) {
       mag::launch_su2_copies_half<double>(j->U64, j->S, j->sstride, j->ncopies, j->g, parity,
                                           j->prm.omega, j->part, j->pstride, j->ctl, j->st);
   } else if (mixed32(j)) {
+    const mag::Shape sh = link_shape(j, true);
     if (j->prm.group == MAG_SU2)
-      mag::launch_su2_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, j->part, j->ctl, j->st);
+      mag::launch_su2_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, j->part, j->ctl, sh, j->st);
     else
-      mag::launch_su3_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, j->part, j->ctl, j->st);
+      mag::launch_su3_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, j->part, j->ctl, sh, j->st);
   } else {
+    const mag::Shape sh = link_shape(j, false);
     if (j->prm.group == MAG_SU2)
-      mag::launch_su2_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, j->part, j->ctl, j->st);
+      mag::launch_su2_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, j->part, j->ctl, sh, j->st);
     else
-      mag::launch_su3_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, j->part, j->ctl, j->st);
+      mag::launch_su3_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, j->part, j->ctl, sh, j->st);
   }
 }
 
@@ -153,11 +169,11 @@ void launch_rebuild(mag_job* j, bool final64) {
 // sweep tail shared by the plain and timed paths
 void enqueue_after_sweep(mag_job* j) {
   if (j->spin) {
-    mag::launch_reduce_copies(j->part, j->pstride, 2 * j->nb, j->ncopies, j->invF, j->invV,
+    mag::launch_reduce_copies(j->part, j->pstride, 2 * j->nw, j->ncopies, j->invF, j->invV,
                               j->ctl, j->hF, j->hR, j->hA, (size_t)j->prm.max_sweeps, j->st);
     return;
   }
-  mag::launch_reduce(j->part, 2 * j->nb, mag::RED_SWEEP, j->invF, j->invV, j->ctl, j->hF,
+  mag::launch_reduce(j->part, 2 * j->nw, mag::RED_SWEEP, j->invF, j->invV, j->ctl, j->hF,
                      j->hR, j->hA, nullptr, j->st);
   if (mixed32(j) && ++j->since_rebuild >= j->prm.reunit_every) {
     launch_rebuild(j, false);
@@ -476,8 +492,9 @@ int mag_job_footprint(const mag_params* p, uint64_t* bytes) {
     }
     if (p->group == MAG_SU3) b += (uint64_t)V * 4 * nr * 8;  // transient FP64 links
   }
-  const uint64_t nb = (uint64_t)((V + mag::kBlock - 1) / mag::kBlock);  // sweep workgroups
-  b += nc * (nb * 2 * 3 * 8 + (uint64_t)p->max_sweeps * 3 * 8 + sizeof(JobCtl)) + 3 * 8;
+  const uint64_t lanes = (uint64_t)(V / 2) * (p->group == MAG_SU2 ? 1 : 2);
+  const uint64_t nw = (lanes + 63) / 64;  // partial triples per half-sweep
+  b += nc * (nw * 2 * 3 * 8 + (uint64_t)p->max_sweeps * 3 * 8 + sizeof(JobCtl)) + 3 * 8;
   *bytes = b;
   return MAG_OK;
 }
@@ -500,13 +517,14 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   j->nr_in = p->group == MAG_SU2 ? 4 : 18;
   j->nreal = (size_t)8 * j->g.Vh * j->nr;
   j->nb = mag::sweep_blocks(j->g, p->group);
+  j->nw = mag::sweep_waves(j->g, p->group);
   j->nf = mag::nblocks(j->g);
   j->invF = 1.0 / (double)(j->V * (p->group == MAG_SU2 ? 4 : 12));
   j->invV = 1.0 / (double)j->V;
   j->spin = p->form == MAG_FORM_SPIN;
   j->ncopies = j->spin ? p->copies : 1;
   j->sstride = (size_t)2 * j->g.Vh * (p->group == MAG_SU2 ? 3 : 12);
-  j->pstride = (size_t)2 * j->nb * 3;
+  j->pstride = (size_t)2 * j->nw * 3;
 #define ALLOC(ptr, bytes)                                                          \
   do {                                                                             \
     hipError_t e_ = hipMalloc((void**)&(ptr), (bytes));                            \
@@ -798,6 +816,14 @@ int mag_job_halfsweep_timed(mag_job* j, int n, double* avg_ms) {
   *avg_ms = ms / n;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  return MAG_OK;
+}
+
+int mag_job_shape(mag_job* j, int fp32, int* nt, int* spt) {
+  if (!j || !nt || !spt) return fail(MAG_ERR_ARG, "NULL argument");
+  const int k = fp32 ? 1 : 0;
+  *nt = j->shape_ok[k] ? j->shape[k].nt : 0;
+  *spt = j->shape_ok[k] ? j->shape[k].spt : 0;
   return MAG_OK;
 }
 

@@ -62,22 +62,35 @@ enum { RED_INIT = 0, RED_SWEEP = 1, RED_PLAIN = 2, RED_RESET = 3, RED_REBUILD = 
 // Partial buffers hold 3 doubles per workgroup: (F sum, residual sum,
 // residual max).  A sweep uses 2*nblocks(g) partial triples (red, black).
 inline int nblocks(const Geom& g) { return (int)((g.Vh + kBlock - 1) / kBlock); }
-// workgroups of a half-sweep: SU(2) one lane per site, SU(3) two
+// workgroups of a half-sweep at the default shape: SU(2) one lane per
+// site, SU(3) two
 inline int sweep_blocks(const Geom& g, int group) {
   const size_t lanes = (size_t)g.Vh * (group == 2 ? 1 : 2);
   return (int)((lanes + kBlock - 1) / kBlock);
 }
+// partial triples of a half-sweep: one per wave of 64 lanes
+inline int sweep_waves(const Geom& g, int group) {
+  const size_t lanes = (size_t)g.Vh * (group == 2 ? 1 : 2);
+  return (int)((lanes + 63) / 64);
+}
+// Launch shape of a half-sweep kernel: nt threads per workgroup (64, 128,
+// 256 or 512) and spt tiles of nt lanes per thread (1 or 2).  Partials are
+// per wave, so the shape never changes a result.
+struct Shape {
+  int nt = 256;
+  int spt = 1;
+};
 
 // One half-sweep of parity `parity`; partial triples go to
-// part[3*nb*parity ...], nb = sweep_blocks(g); G (mixed precision) is the
-// accumulated site gauge or nullptr.  The kernels return at once when
+// part[3*nw*parity ...], nw = sweep_waves(g, group); G (mixed precision) is
+// the accumulated site gauge or nullptr.  The kernels return at once when
 // ctl->done is set.
 template <typename T>
 void launch_su2_half(T* U, T* G, const Geom& g, int parity, double omega,
-                     double* part, const JobCtl* ctl, hipStream_t st);
+                     double* part, const JobCtl* ctl, Shape sh, hipStream_t st);
 template <typename T>
 void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega,
-                     double* part, const JobCtl* ctl, hipStream_t st);
+                     double* part, const JobCtl* ctl, Shape sh, hipStream_t st);
 
 // Gauge-copy (spin form, SU(2), FP64) half-sweep over all copies of one
 // configuration: the links U (read-only) are loaded once per site and
@@ -127,8 +140,9 @@ struct BatchJob {
   double omega;
   double invF;
   double invV;
-  int nb;      // workgroups of one half-sweep of this job
-  int nparts;  // partial triples of a sweep (2*nb)
+  int nb;      // workgroups of one half-sweep of this job (default shape)
+  int nw;      // partial triples of one half-sweep (one per wave)
+  int nparts;  // partial triples of a sweep (2*nw)
   int nrparts; // partial triples of a rebuild
   int block0;  // first workgroup of this job in a batched half-sweep grid
   int rb0;     // first workgroup of this job in a batched rebuild grid

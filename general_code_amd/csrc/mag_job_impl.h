@@ -14,7 +14,10 @@ struct mag_job {
   int nr = 0;        // reals per link in the SoA layout (4 or 12)
   int nr_in = 0;     // reals per link in the reference layout (4 or 18)
   size_t nreal = 0;  // reals in one link array (8 * Vh * nr)
-  int nb = 0;        // workgroups per half-sweep
+  int nb = 0;        // workgroups per half-sweep at the default shape
+  int nw = 0;        // partial triples per half-sweep (one per wave)
+  mag::Shape shape[2];           // link-form launch shape, FP64 / FP32 kernel
+  bool shape_ok[2] = {false, false};
   int nf = 0;        // workgroups per parity of the functional / plaquette kernels
   hipStream_t st = nullptr;
   double* U64 = nullptr;  // FP64 links (plain) / FP64 master then output (mixed)
@@ -58,4 +61,7 @@ bool job_done(const mag_job* j);
 // mixed with an FP64 warm-up: enter the FP32 phase once the warm-up sweeps
 // are done (read_ctl first); *switched = whether it did
 int job_maybe_start_fp32(mag_job* j, bool* switched);
+// launch shape of the job's link-form half-sweep (mag_tune.hip): cached per
+// (group, precision, device, lattice), tuned on first use
+int job_shape(const mag_job* j, bool fp32, Shape* out);
 }  // namespace mag

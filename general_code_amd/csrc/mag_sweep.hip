@@ -155,16 +155,40 @@ __device__ __forceinline__ int batch_job(const BatchJob* __restrict__ jobs, int 
   return lo;
 }
 
+// ================================================================== launch shapes
+// The partial triples of a half-sweep are per wave: wave w of a job covers
+// its lanes 64w .. 64w+63 (64 SU(2) sites, 32 SU(3) sites) and its triple
+// goes to part[3w].  A launch shape -- NT threads per workgroup, SPT
+// 64-lane-aligned tiles of NT lanes per thread (workgroup lb, tile k covers
+// lanes (lb*SPT + k)*NT ...) -- only decides which thread computes which
+// wave, never the arithmetic or the order of a partial, so every shape
+// gives bit-identical results and the shape is free for the autotuner.
+__device__ __forceinline__ void wave_partial(double* __restrict__ part, uint32_t lane0,
+                                             uint32_t nlanes, double f, double r, double m) {
+  f = wave_sum(f);
+  r = wave_sum(r);
+  m = wave_max(m);
+  if ((threadIdx.x & 63u) == 0 && lane0 < nlanes) {
+    const uint32_t w = lane0 >> 6;
+    part[3 * w] = f;
+    part[3 * w + 1] = r;
+    part[3 * w + 2] = m;
+  }
+}
+
 // ================================================================== SU(2)
-// One workgroup lb of a half-sweep over the sites of parity p, one site per
-// lane; the 8 links of an su2 site fit comfortably in registers (6
+// Tile k of workgroup lb of a half-sweep over the sites of parity p, one
+// site per lane; the 8 links of an su2 site fit comfortably in registers (6
 // waves/SIMD in FP64), and measured faster than a lane pair.  The black
 // (p = 1) half also sums the functional of the links just written.
-template <typename T, bool GA>
+template <typename T, bool GA, int NT, int SPT>
 __device__ __forceinline__ void su2_half_block(T* __restrict__ U, T* __restrict__ G,
                                                const Geom& g, int p, T omega,
                                                double* __restrict__ part, uint32_t lb) {
-  const uint32_t h = lb * kBlock + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < SPT; k++) {
+  const uint32_t base = (lb * SPT + k) * NT;
+  const uint32_t h = base + threadIdx.x;
   double fs = 0.0, rs = 0.0, rm = 0.0;
   if (h < g.Vh) {
     const SiteIdx s = decode(g, h, p);
@@ -204,21 +228,16 @@ __device__ __forceinline__ void su2_half_block(T* __restrict__ U, T* __restrict_
     rs = (double)res;
     rm = (double)res;
   }
-  block_reduce3<kWaves>(fs, rs, rm);
-  if (threadIdx.x == 0) {
-    part[3 * lb] = fs;
-    part[3 * lb + 1] = rs;
-    part[3 * lb + 2] = rm;
+  wave_partial(part, base + (threadIdx.x & ~63u), g.Vh, fs, rs, rm);
   }
 }
 
-template <typename T, bool GA>
-__global__ __launch_bounds__(kBlock) void k_su2_half(T* __restrict__ U, T* __restrict__ G,
-                                                     Geom g, int p, T omega,
-                                                     double* __restrict__ part,
-                                                     const JobCtl* __restrict__ ctl) {
+template <typename T, bool GA, int NT, int SPT>
+__global__ __launch_bounds__(NT) void k_su2_half(T* __restrict__ U, T* __restrict__ G, Geom g,
+                                                 int p, T omega, double* __restrict__ part,
+                                                 const JobCtl* __restrict__ ctl) {
   if (ctl->done) return;
-  su2_half_block<T, GA>(U, G, g, p, omega, part, blockIdx.x);
+  su2_half_block<T, GA, NT, SPT>(U, G, g, p, omega, part, blockIdx.x);
 }
 
 template <typename T, bool GA>
@@ -226,29 +245,50 @@ __global__ __launch_bounds__(kBlock) void k_su2_half_batch(const BatchJob* __res
                                                            int njobs, int p) {
   const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
   if (j.ctl->done) return;
-  su2_half_block<T, GA>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega, j.part + (size_t)3 * j.nb * p,
-                        blockIdx.x - j.block0);
+  su2_half_block<T, GA, kBlock, 1>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega,
+                                   j.part + (size_t)3 * j.nw * p, blockIdx.x - j.block0);
 }
+
+// launch of a shape (NT, SPT) of the supported set
+#define MAG_SHAPE_SWITCH(sh, L_)                  \
+  switch (sh.nt * 4 + sh.spt) {                   \
+    case 64 * 4 + 1: L_(64, 1); break;            \
+    case 64 * 4 + 2: L_(64, 2); break;            \
+    case 128 * 4 + 1: L_(128, 1); break;          \
+    case 128 * 4 + 2: L_(128, 2); break;          \
+    case 256 * 4 + 2: L_(256, 2); break;          \
+    case 512 * 4 + 1: L_(512, 1); break;          \
+    case 512 * 4 + 2: L_(512, 2); break;          \
+    default: L_(256, 1); break;                   \
+  }
 
 template <typename T>
 void launch_su2_half(T* U, T* G, const Geom& g, int parity, double omega, double* part,
-                     const JobCtl* ctl, hipStream_t st) {
-  const int nb = sweep_blocks(g, 2);
-  double* pp = part + (size_t)3 * nb * parity;
-  if (G)
-    hipLaunchKernelGGL((k_su2_half<T, true>), dim3(nb), dim3(kBlock), 0, st, U, G, g, parity,
-                       (T)omega, pp, ctl);
-  else
-    hipLaunchKernelGGL((k_su2_half<T, false>), dim3(nb), dim3(kBlock), 0, st, U, G, g, parity,
-                       (T)omega, pp, ctl);
+                     const JobCtl* ctl, Shape sh, hipStream_t st) {
+  double* pp = part + (size_t)3 * sweep_waves(g, 2) * parity;
+  const uint32_t lanes = g.Vh;
+#define L_(NT, SPT)                                                                        \
+  do {                                                                                     \
+    const dim3 grid((lanes + NT * SPT - 1) / (NT * SPT));                                  \
+    if (G)                                                                                 \
+      hipLaunchKernelGGL((k_su2_half<T, true, NT, SPT>), grid, dim3(NT), 0, st, U, G, g,   \
+                         parity, (T)omega, pp, ctl);                                       \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_su2_half<T, false, NT, SPT>), grid, dim3(NT), 0, st, U, G, g,  \
+                         parity, (T)omega, pp, ctl);                                       \
+  } while (0)
+  MAG_SHAPE_SWITCH(sh, L_)
+#undef L_
 }
 template void launch_su2_half<float>(float*, float*, const Geom&, int, double, double*,
-                                     const JobCtl*, hipStream_t);
+                                     const JobCtl*, Shape, hipStream_t);
 template void launch_su2_half<double>(double*, double*, const Geom&, int, double, double*,
-                                      const JobCtl*, hipStream_t);
+                                      const JobCtl*, Shape, hipStream_t);
 
 // ================================================================== SU(3)
 // rows 0, 1 scaled by (3 - |row|^2) / 2, the Newton step of 1/|row| at 1
+// (the FP32 site gauges of the site form, which are never rebuilt in FP64
+// during their FP32 phase, lose about 1e-7 of row norm per sweep without it)
 template <typename T>
 __device__ __forceinline__ void renorm_rows(M2<T>& m) {
 #pragma unroll
@@ -321,11 +361,14 @@ __device__ __forceinline__ T cm_subgroup(M2<T>* F, M2<T>* B, M2<T>& Gx, T omega)
 
 // One workgroup lb of an SU(3) half-sweep, two lanes per site: lane q of
 // the pair holds the forward and backward links of mu = 2q, 2q+1.
-template <typename T, bool GA>
+template <typename T, bool GA, int NT, int SPT>
 __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict__ G,
                                                const Geom& g, int p, T omega,
                                                double* __restrict__ part, uint32_t lb) {
-  const uint32_t tid = lb * kBlock + threadIdx.x;
+#pragma unroll
+  for (int kk = 0; kk < SPT; kk++) {
+  const uint32_t base = (lb * SPT + kk) * NT;
+  const uint32_t tid = base + threadIdx.x;
   const uint32_t h = tid >> 1;
   const int q = (int)(tid & 1u);
   double fs = 0.0, rs = 0.0, rm = 0.0;
@@ -351,16 +394,6 @@ __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict_
     T res = cm_subgroup<T, 0, 1, GA>(F, B, Gx, omega);
     res += cm_subgroup<T, 1, 2, GA>(F, B, Gx, omega);
     res += cm_subgroup<T, 0, 2, GA>(F, B, Gx, omega);
-    if constexpr (sizeof(T) == 4) {
-      // FP32: the stored rows lose norm steadily (measured: |row|^2 - 1
-      // about -1e-7 per sweep, lowering F between FP64 rebuilds); one Newton
-      // step of 1/sqrt(|row|^2) per update keeps them on the sphere
-#pragma unroll
-      for (int k = 0; k < 2; k++) {
-        renorm_rows(F[k]);
-        renorm_rows(B[k]);
-      }
-    }
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       stm(pf[k], F[k]);
@@ -376,21 +409,16 @@ __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict_
       rm = (double)res;
     }
   }
-  block_reduce3<kWaves>(fs, rs, rm);
-  if (threadIdx.x == 0) {
-    part[3 * lb] = fs;
-    part[3 * lb + 1] = rs;
-    part[3 * lb + 2] = rm;
+  wave_partial(part, base + (threadIdx.x & ~63u), 2 * g.Vh, fs, rs, rm);
   }
 }
 
-template <typename T, bool GA>
-__global__ __launch_bounds__(kBlock) void k_su3_half(T* __restrict__ U, T* __restrict__ G,
-                                                     Geom g, int p, T omega,
-                                                     double* __restrict__ part,
-                                                     const JobCtl* __restrict__ ctl) {
+template <typename T, bool GA, int NT, int SPT>
+__global__ __launch_bounds__(NT) void k_su3_half(T* __restrict__ U, T* __restrict__ G, Geom g,
+                                                 int p, T omega, double* __restrict__ part,
+                                                 const JobCtl* __restrict__ ctl) {
   if (ctl->done) return;
-  su3_half_block<T, GA>(U, G, g, p, omega, part, blockIdx.x);
+  su3_half_block<T, GA, NT, SPT>(U, G, g, p, omega, part, blockIdx.x);
 }
 
 template <typename T, bool GA>
@@ -398,26 +426,32 @@ __global__ __launch_bounds__(kBlock) void k_su3_half_batch(const BatchJob* __res
                                                            int njobs, int p) {
   const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
   if (j.ctl->done) return;
-  su3_half_block<T, GA>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega, j.part + (size_t)3 * j.nb * p,
-                        blockIdx.x - j.block0);
+  su3_half_block<T, GA, kBlock, 1>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega,
+                                   j.part + (size_t)3 * j.nw * p, blockIdx.x - j.block0);
 }
 
 template <typename T>
 void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega, double* part,
-                     const JobCtl* ctl, hipStream_t st) {
-  const int nb = sweep_blocks(g, 3);
-  double* pp = part + (size_t)3 * nb * parity;
-  if (G)
-    hipLaunchKernelGGL((k_su3_half<T, true>), dim3(nb), dim3(kBlock), 0, st, U, G, g, parity,
-                       (T)omega, pp, ctl);
-  else
-    hipLaunchKernelGGL((k_su3_half<T, false>), dim3(nb), dim3(kBlock), 0, st, U, G, g, parity,
-                       (T)omega, pp, ctl);
+                     const JobCtl* ctl, Shape sh, hipStream_t st) {
+  double* pp = part + (size_t)3 * sweep_waves(g, 3) * parity;
+  const uint32_t lanes = 2 * g.Vh;
+#define L_(NT, SPT)                                                                        \
+  do {                                                                                     \
+    const dim3 grid((lanes + NT * SPT - 1) / (NT * SPT));                                  \
+    if (G)                                                                                 \
+      hipLaunchKernelGGL((k_su3_half<T, true, NT, SPT>), grid, dim3(NT), 0, st, U, G, g,   \
+                         parity, (T)omega, pp, ctl);                                       \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_su3_half<T, false, NT, SPT>), grid, dim3(NT), 0, st, U, G, g,  \
+                         parity, (T)omega, pp, ctl);                                       \
+  } while (0)
+  MAG_SHAPE_SWITCH(sh, L_)
+#undef L_
 }
 template void launch_su3_half<float>(float*, float*, const Geom&, int, double, double*,
-                                     const JobCtl*, hipStream_t);
+                                     const JobCtl*, Shape, hipStream_t);
 template void launch_su3_half<double>(double*, double*, const Geom&, int, double, double*,
-                                      const JobCtl*, hipStream_t);
+                                      const JobCtl*, Shape, hipStream_t);
 
 // batched half-sweep over a class table (group, precision): every job of
 // the table has the same group and storage type; GA = mixed FP32 phase
@@ -509,17 +543,11 @@ __device__ __forceinline__ void su2_copies_block(const TL* __restrict__ U,
       dm[k] = d;
     }
   }
-  block_reduce3k<kWaves, K>(fs, ds, dm);
-  if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-      if (!act[k]) continue;
-      double* o = part + k * pstride + 3 * (size_t)lb;
-      o[0] = fs[k];
-      o[1] = ds[k];
-      o[2] = dm[k];
-    }
-  }
+  for (int k = 0; k < K; k++)
+    if (act[k])  // uniform
+      wave_partial(part + k * pstride, lb * kBlock + (threadIdx.x & ~63u), g.Vh, fs[k], ds[k],
+                   dm[k]);
 }
 
 template <int K, typename TL>
@@ -539,7 +567,7 @@ __global__ __launch_bounds__(kBlock) void k_copies_half_batch(const BatchJob* __
                                                               int njobs, int p) {
   const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
   su2_copies_block<K, TL>((const TL*)j.U, (double*)j.S, j.sstride, j.g, p, j.omega,
-                          j.part + (size_t)3 * j.nb * p, j.pstride, j.ctl,
+                          j.part + (size_t)3 * j.nw * p, j.pstride, j.ctl,
                           blockIdx.x - j.block0);
 }
 
@@ -561,7 +589,7 @@ void launch_su2_copies_half(const TL* U, double* S, size_t sstride, int ncopies,
                             int parity, double omega, double* part, size_t pstride,
                             const JobCtl* ctl, hipStream_t st) {
   const int nb = nblocks(g);
-  double* pp = part + (size_t)3 * nb * parity;
+  double* pp = part + (size_t)3 * sweep_waves(g, 2) * parity;
 #define L_(K)                                                                                   \
   hipLaunchKernelGGL((k_su2_copies_half<K, TL>), dim3(nb), dim3(kBlock), 0, st, U, S, sstride, \
                      g, parity, omega, pp, pstride, ctl)
@@ -738,14 +766,8 @@ __device__ __forceinline__ void su3_site_copies_block(const T* __restrict__ U,
         rm = (double)res;
       }
     }
-    block_reduce3<kWaves>(fs, rs, rm);
-    if (threadIdx.x == 0) {
-      double* o = part + (size_t)c * pstride + 3 * (size_t)lb;
-      o[0] = fs;
-      o[1] = rs;
-      o[2] = rm;
-    }
-    __syncthreads();  // the reduction's LDS is reused by the next copy
+    wave_partial(part + (size_t)c * pstride, lb * kBlock + (threadIdx.x & ~63u), 2 * g.Vh, fs,
+                 rs, rm);
   }
 }
 
@@ -764,7 +786,7 @@ __global__ __launch_bounds__(kBlock) void k_su3_site_copies_batch(
     const BatchJob* __restrict__ jobs, int njobs, int p) {
   const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
   su3_site_copies_block<T>((const T*)j.U, (T*)j.S, j.sstride, j.ncopies, j.g, p, (T)j.omega,
-                           j.part + (size_t)3 * j.nb * p, j.pstride, j.ctl,
+                           j.part + (size_t)3 * j.nw * p, j.pstride, j.ctl,
                            blockIdx.x - j.block0);
 }
 
@@ -774,7 +796,8 @@ void launch_su3_site_copies(const T* U, T* G, size_t gstride, int ncopies, const
                             const JobCtl* ctl, hipStream_t st) {
   const int nb = sweep_blocks(g, 3);
   hipLaunchKernelGGL((k_su3_site_copies<T>), dim3(nb), dim3(kBlock), 0, st, U, G, gstride,
-                     ncopies, g, parity, (T)omega, part + (size_t)3 * nb * parity, pstride, ctl);
+                     ncopies, g, parity, (T)omega, part + (size_t)3 * sweep_waves(g, 3) * parity,
+                     pstride, ctl);
 }
 template void launch_su3_site_copies<float>(const float*, float*, size_t, int, const Geom&, int,
                                             double, double*, size_t, const JobCtl*,

@@ -33,6 +33,8 @@ SU2, SU3 = 2, 3
 F64, MIXED = 0, 1
 FORM_LINK, FORM_SPIN = 0, 1
 MAX_COPIES = 8
+TUNE_AUTO, TUNE_FIXED, TUNE_DEFAULT = 0, 1, 2
+SHAPES = [(64, 1), (64, 2), (128, 1), (128, 2), (256, 1), (256, 2), (512, 1), (512, 2)]
 
 
 class MagError(RuntimeError):
@@ -130,6 +132,9 @@ SIGNATURES = {
     "mag_job_copy_history": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _ip]),
     "mag_job_best_copy": (ctypes.c_int, [_vp, _ip, _dp]),
     "mag_job_download_copy_links": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "mag_tune_set": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "mag_tune_clear": (ctypes.c_int, []),
+    "mag_job_shape": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
     "mag_job_download_links": (ctypes.c_int, [_vp, _dp]),
     "mag_job_functional": (ctypes.c_int, [_vp, _dp]),
     "mag_job_plaquette": (ctypes.c_int, [_vp, _dp]),
@@ -394,6 +399,14 @@ class MagJob:
         s = ctypes.c_int()
         _check(lib().mag_job_fp64_start(self._h, ctypes.byref(s)))
         return s.value
+
+    def shape(self, fp32: bool = False):
+        """(threads per workgroup, tiles per thread) of the job's FP64 or FP32
+        link-form half-sweep; (0, 0) before its first sweep."""
+        nt = ctypes.c_int()
+        spt = ctypes.c_int()
+        _check(lib().mag_job_shape(self._h, int(bool(fp32)), ctypes.byref(nt), ctypes.byref(spt)))
+        return nt.value, spt.value
 
     # gauge copies (spin form) ---------------------------------------------
     def set_copy_spins(self, copy: int, spins: np.ndarray):
@@ -718,3 +731,14 @@ class MagServer:
         st = _ServerStats()
         _check(lib().mag_server_get_stats(self._h, ctypes.byref(st)))
         return {f: getattr(st, f) for f, _ in _ServerStats._fields_}
+
+
+# launch shapes --------------------------------------------------------------
+def tune_set(mode: int, nt: int = 256, spt: int = 1):
+    """TUNE_AUTO (time every shape on first use), TUNE_FIXED (nt, spt) or
+    TUNE_DEFAULT (256, 1); every shape gives bit-identical results."""
+    _check(lib().mag_tune_set(int(mode), int(nt), int(spt)))
+
+
+def tune_clear():
+    _check(lib().mag_tune_clear())

@@ -154,6 +154,25 @@ int mag_job_download_links(mag_job *job, double *links);
 int mag_job_functional(mag_job *job, double *F);
 int mag_job_plaquette(mag_job *job, double *P);
 
+/* ---- launch shapes
+ * The link-form half-sweep kernels run with 64, 128, 256 or 512 threads per
+ * workgroup and 1 or 2 tiles per thread; their partial sums are per wave,
+ * so every shape gives bit-identical results.  By default the first sweep
+ * on a new (group, precision, lattice) times every shape on a scratch copy
+ * of the job's links and keeps the fastest for the process (MAG_TUNE_AUTO);
+ * MAG_TUNE_FIXED uses (nt, spt) everywhere, MAG_TUNE_DEFAULT uses (256, 1)
+ * without tuning.  Batches always use (256, 1).  No counterpart in the
+ * reference. */
+#define MAG_TUNE_AUTO 0
+#define MAG_TUNE_FIXED 1
+#define MAG_TUNE_DEFAULT 2
+int mag_tune_set(int mode, int nt, int spt);
+/* forget the tuned shapes */
+int mag_tune_clear(void);
+/* shape the job's FP64 (fp32 = 0) or FP32 (fp32 = 1) link-form kernel uses;
+ * 0, 0 before its first sweep */
+int mag_job_shape(mag_job *job, int fp32, int *nt, int *spt);
+
 /* ---- gauge copies (spin form)
  * A spin-form job holds one SU(2) configuration and `copies` spin fields.
  * mag_job_upload_links starts every copy at s = e3 (the identity gauge);
