@@ -195,7 +195,15 @@ def run_plain(G, specs, max_sweeps):
     return total, out
 
 
-def run_batched(G, specs, max_sweeps, su3_precision, su3_warmup, su3_site=False):
+# FP64 warm-up sweeps before the mixed phase: the spin form's FP32-rounded
+# links are a fixed perturbation of the whole problem and need a longer
+# FP64 start on large lattices to keep the FP64 run's maximum (48^4: 100
+# sweeps were not enough for one seed, 300 were)
+SU2_WARMUP, SU3_WARMUP = 300, 100
+
+
+def run_batched(G, specs, max_sweeps, su3_precision, su3_warmup, su3_site=False,
+                su2_warmup=SU2_WARMUP):
     """All configurations in one graph-captured batch: each SU(2)
     configuration is one spin-form copy set (links read once for all its
     copies; mixed: FP32-rounded links after the FP64 warm-up, spins and
@@ -206,7 +214,7 @@ def run_batched(G, specs, max_sweeps, su3_precision, su3_warmup, su3_site=False)
         if sp["group"] == 2:
             j = G.MagJob(group=2, dims=sp["dims"], omega=sp["omega"], tol=sp["tol"],
                          max_sweeps=max_sweeps, form=G.FORM_SPIN, copies=sp["copies"],
-                         precision=su3_precision, tol_mixed=sp["tol"], fp64_warmup=su3_warmup)
+                         precision=su3_precision, tol_mixed=sp["tol"], fp64_warmup=su2_warmup)
             j.generate(sp["eps"], sp["seed"])
             jobs.append((j, list(range(sp["copies"]))))
         elif su3_site:
@@ -240,7 +248,7 @@ def run_batched(G, specs, max_sweeps, su3_precision, su3_warmup, su3_site=False)
     return el, out
 
 
-def batch_bench(G, max_sweeps=8000, su3_precision=None, su3_warmup=100):
+def batch_bench(G, max_sweeps=8000, su3_precision=None, su3_warmup=SU3_WARMUP):
     specs = batch_workload()
     n = sum(sp["copies"] for sp in specs)
     su3_precision = G.MIXED if su3_precision is None else su3_precision
@@ -257,7 +265,7 @@ def batch_bench(G, max_sweeps=8000, su3_precision=None, su3_warmup=100):
         "batched_configs_per_s": n / t_batch,
         "speedup": t_plain / t_batch,
         "su3_precision": "mixed" if su3_precision == G.MIXED else "fp64",
-        "su3_fp64_warmup": su3_warmup,
+        "fp64_warmup": {"su2": SU2_WARMUP, "su3": su3_warmup},
         "converged_plain": sum(1 for p in plain if p[1]),
         "converged_batched": sum(1 for b in batched if b[1]),
         "max_rel_dF_vs_plain_converged": max(dF) if dF else None,
@@ -286,7 +294,8 @@ def serve_workload(n=40, rate=20.0, seed=4):
     return reqs
 
 
-def serve(G, reqs, plain, max_sweeps, su3_tol_mixed=1e-11, su3_warmup=100):
+def serve(G, reqs, plain, max_sweeps, su3_tol_mixed=1e-11, su3_warmup=SU3_WARMUP,
+          su2_warmup=SU2_WARMUP):
     """Replay the arrival stream against a MagServer.  plain: one job at a
     time, FP64 link form (the reference's one-configuration-per-run use);
     else continuous batching, mixed precision after an FP64 warm-up: SU(2)
@@ -296,11 +305,12 @@ def serve(G, reqs, plain, max_sweeps, su3_tol_mixed=1e-11, su3_warmup=100):
         p = G.MagParams(group=r["group"], dims=r["dims"], omega=r["omega"], tol=r["tol"],
                         max_sweeps=max_sweeps)
         if not plain:
-            p.precision, p.fp64_warmup = G.MIXED, su3_warmup
+            p.precision = G.MIXED
             if r["group"] == 2:
                 p.form, p.copies, p.tol_mixed = G.FORM_SPIN, 1, r["tol"]
+                p.fp64_warmup = su2_warmup
             else:
-                p.tol_mixed = su3_tol_mixed
+                p.tol_mixed, p.fp64_warmup = su3_tol_mixed, su3_warmup
         return p
 
     with G.MagServer(max_active=1 if plain else 0) as srv:
