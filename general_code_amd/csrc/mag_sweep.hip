@@ -155,6 +155,18 @@ __device__ __forceinline__ int batch_job(const BatchJob* __restrict__ jobs, int 
   return lo;
 }
 
+// ================================================================== XCD-aware order
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8), each
+// with its own L2.  The site-form kernels reuse neighbour spins / gauges
+// (read by up to 8 sites), so workgroup b processes the logical block that
+// gives XCD b % 8 one contiguous range of blocks: neighbouring sites then
+// mostly run on the same XCD and share its L2.  A bijection of [0, n); it
+// only reorders which workgroup computes which block.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+  const uint32_t q = n / 8, r = n % 8, x = b % 8, i = b / 8;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // ================================================================== launch shapes
 // The partial triples of a half-sweep are per wave: wave w of a job covers
 // its lanes 64w .. 64w+63 (64 SU(2) sites, 32 SU(3) sites) and its triple
@@ -558,17 +570,19 @@ __global__ __launch_bounds__(kBlock) void k_su2_copies_half(const TL* __restrict
                                                             double* __restrict__ part,
                                                             size_t pstride,
                                                             const JobCtl* __restrict__ ctl) {
-  su2_copies_block<K, TL>(U, S, sstride, g, p, omega, part, pstride, ctl, blockIdx.x);
+  su2_copies_block<K, TL>(U, S, sstride, g, p, omega, part, pstride, ctl,
+                          xcd_block(blockIdx.x, gridDim.x));
 }
 
 // every copy set of the table has K copies and links of type TL
 template <int K, typename TL>
 __global__ __launch_bounds__(kBlock) void k_copies_half_batch(const BatchJob* __restrict__ jobs,
                                                               int njobs, int p) {
-  const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
+  const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const BatchJob& j = jobs[batch_job(jobs, njobs, (int)lb)];
   su2_copies_block<K, TL>((const TL*)j.U, (double*)j.S, j.sstride, j.g, p, j.omega,
                           j.part + (size_t)3 * j.nw * p, j.pstride, j.ctl,
-                          blockIdx.x - j.block0);
+                          lb - j.block0);
 }
 
 // one workgroup per copy
@@ -778,16 +792,18 @@ __global__ __launch_bounds__(kBlock) void k_su3_site_copies(const T* __restrict_
                                                             T omega, double* __restrict__ part,
                                                             size_t pstride,
                                                             const JobCtl* __restrict__ ctl) {
-  su3_site_copies_block<T>(U, G, gstride, ncopies, g, p, omega, part, pstride, ctl, blockIdx.x);
+  su3_site_copies_block<T>(U, G, gstride, ncopies, g, p, omega, part, pstride, ctl,
+                           xcd_block(blockIdx.x, gridDim.x));
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_su3_site_copies_batch(
     const BatchJob* __restrict__ jobs, int njobs, int p) {
-  const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
+  const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const BatchJob& j = jobs[batch_job(jobs, njobs, (int)lb)];
   su3_site_copies_block<T>((const T*)j.U, (T*)j.S, j.sstride, j.ncopies, j.g, p, (T)j.omega,
                            j.part + (size_t)3 * j.nw * p, j.pstride, j.ctl,
-                           blockIdx.x - j.block0);
+                           lb - j.block0);
 }
 
 template <typename T>
