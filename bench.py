@@ -62,11 +62,14 @@ def measure(G, group, dims, precision, eps, omega, steps, warmup, seed=2024):
         job.generate(eps, seed)
         if warmup:
             job.sweeps_timed(warmup)
+        else:
+            job.sweeps(1)  # the autotuner times the launch shapes on a job's first sweep
         # whole steps (sweep = 2 half-sweeps + reduction), then the dominant
         # kernel's mean duration over back-to-back half-sweep launches
         total_ms = job.sweeps_timed(steps)
         kern_ms = job.halfsweep_timed(2 * steps)
         sweeps, _, F = job.status()
+        shape = job.shape(fp32=precision == G.MIXED)  # chosen by the autotuner in the warm-up
     finally:
         job.close()
     mixed = precision == G.MIXED
@@ -93,6 +96,7 @@ def measure(G, group, dims, precision, eps, omega, steps, warmup, seed=2024):
         },
         "F_after": F,
         "sweeps": sweeps,
+        "shape": list(shape),
     }
 
 
@@ -407,6 +411,7 @@ def main():
                 "traffic_bytes_per_launch": r["roofline"]["traffic"],
                 "alg_bytes_per_launch": r["roofline"]["bytes_per_launch"],
                 "b_site": r["roofline"]["b_site"],
+                "launch_shape": r["shape"],
             }
 
     batched = None
@@ -447,6 +452,7 @@ def main():
                      ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "kernel": "k_su2_half<double> (one checkerboard half-sweep)",
         "kernel_ms": head["kernel_ms"],
+        "launch_shape": head["shape"],
         "b_site": head["roofline"]["b_site"],
         "copy_kernel_GBs": round(copy_gbs, 1),
         "frac_of_copy": round(head["roofline"]["achieved"] / copy_gbs, 4),
