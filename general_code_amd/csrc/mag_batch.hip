@@ -1,17 +1,26 @@
 // mag_batch.hip -- batched, graph-captured sweeps over many MAG jobs.
 //
 // The active members are split into classes: SU(2) / SU(3) x FP32 working
-// links of the mixed phase / FP64 links, and spin-form gauge-copy sets by
-// copy count.  One sweep of the batch is
-//   for parity 0, 1: one half-sweep launch per non-empty class
-//   one reduction launch (a workgroup per active job, per copy of a copy set)
-//   every reunit_every sweeps: one FP64 rebuild + one normalisation launch
-//   per FP32 class
-// and `replay` sweeps (default 20) are captured in one hipGraph that is
-// replayed until the membership or a job's phase changes.  The device
-// tables (BatchJob) map a workgroup to its job; a job's workgroups run the
-// code and partition of its own single launch, so per-job results do not
-// depend on the batch.
+// links of the mixed phase / FP64 links, spin-form gauge-copy sets by copy
+// count and link precision, SU(3) site-form sets by precision.  One sweep of
+// a class is
+//   for parity 0, 1: one half-sweep launch whose grid concatenates the
+//                    member jobs' grids
+//   one reduction launch (a workgroup per job, per copy of a copy set)
+//   every reunit_every sweeps (link-form FP32 classes): one FP64 rebuild,
+//   one normalisation and one rebuild reduction
+// and `replay` sweeps (default 20) plus a gather of every member's control
+// block into pinned host memory are captured in one hipGraph per class.
+//
+// Classes share no data, so each class is an independent chain on a stream
+// of its own: the scheduler (mag_batch_step) relaunches a class's graph as
+// soon as that class's previous replay has finished and its members have
+// been examined, without waiting for the other classes.  A class's tables
+// and graph are rebuilt only while it is idle; a job that changes class
+// (FP64 warm-up over, FP32 phase over) joins its new class at that class's
+// next launch.  The device tables (BatchJob) map a workgroup to its job; a
+// job's workgroups run the code and partition of its own single launch, so
+// per-job results do not depend on the batch.
 #include "mag_capi_util.h"
 #include "mag_gpu.h"
 #include "mag_internal.h"
@@ -20,10 +29,13 @@
 #include <algorithm>
 #include <cstdlib>
 #include <new>
+#include <thread>
+#include <utility>
 #include <vector>
 
 using mag::BatchJob;
 using mag::fail;
+using mag::JobCtl;
 
 namespace {
 
@@ -35,22 +47,34 @@ constexpr int kSpinClass = 4;
 constexpr int kSiteClass3 = kSpinClass + 2 * MAG_MAX_COPIES;
 constexpr int kClasses = kSiteClass3 + 2;
 
-struct ClassTable {
+struct ClassRun {
   int group = 2;
   bool fp32 = false;
-  std::vector<BatchJob> h;
+  std::vector<mag_job*> jobs;  // current members (host view)
+  bool dirty = true;           // jobs changed since the tables were built
+  // what the device tables and the captured graph reference
+  std::vector<BatchJob> h;                      // half-sweep table
+  std::vector<BatchJob> red;                    // reduction entries
+  std::vector<std::pair<mag_job*, int>> owner;  // red[i] -> (job, copy)
+  std::vector<mag_job*> cap_jobs;
   BatchJob* d = nullptr;
   size_t cap = 0;
-  std::vector<BatchJob> red;  // reduction entries (one per job, per copy)
   BatchJob* d_red = nullptr;
   size_t cap_red = 0;
+  JobCtl* ctl_out = nullptr;  // pinned host memory, one entry per red[i]
+  size_t cap_out = 0;
   int nblocks = 0;   // half-sweep grid
-  int nrblocks = 0;  // rebuild grid (FP32 classes)
+  int nrblocks = 0;  // rebuild grid (FP32 link-form classes)
+  hipStream_t s = nullptr;
+  hipEvent_t ev = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool inflight = false;  // replays launched and not yet examined
 };
 
 int upload_table(std::vector<BatchJob>& h, BatchJob** d, size_t* cap) {
   if (h.size() > *cap) {
-    hipFree(*d);
+    (void)hipFree(*d);
     *d = nullptr;
     *cap = 0;
     const size_t c = std::max<size_t>(h.size(), 16);
@@ -65,31 +89,19 @@ int upload_table(std::vector<BatchJob>& h, BatchJob** d, size_t* cap) {
 
 struct mag_batch {
   int device = 0;
-  hipStream_t st = nullptr;
+  hipStream_t st = nullptr;  // the batch stream: joins every class after replays()
   std::vector<mag_job*> members;
   std::vector<mag_job*> active;  // members not finished
   int replay = 20;
-  int reunit = 0;  // shared reunit_every of mixed members (0: none yet)
-  ClassTable cls[kClasses];
-  std::vector<BatchJob> all;
-  BatchJob* d_all = nullptr;
-  size_t cap_all = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  bool dirty = true;
+  int reunit = 0;  // shared reunit_every of mixed link-form members (0: none yet)
+  ClassRun cls[kClasses];
   // MAG_BATCH_DIRECT=1 in the environment: issue each replay's launches on
   // the stream instead of replaying a graph (for tools that do not follow
   // graph launches); the launches and results are the same
   bool direct = false;
-  // Jobs of different classes never share data, so each class's sweeps form
-  // an independent chain: with `fork` the chains run on streams of their own
-  // (graph branches) and overlap on the device; without it
-  // (MAG_BATCH_SERIAL=1) every sweep launches all classes in turn on one
-  // stream.  Per-job results are the same either way.
+  // MAG_BATCH_SERIAL=1: every class on the batch stream, one after the
+  // other, instead of a stream per class; per-job results are the same
   bool fork = true;
-  hipStream_t fs[kClasses] = {};
-  hipEvent_t ev_fork = nullptr;
-  hipEvent_t ev_join[kClasses] = {};
 };
 
 namespace {
@@ -99,6 +111,8 @@ int class_of(const mag_job* j) {
   if (j->spin) return kSpinClass + j->ncopies - 1 + (mag::job_mixed32(j) ? MAG_MAX_COPIES : 0);
   return (j->prm.group == MAG_SU2 ? 0 : 2) + (mag::job_mixed32(j) ? 0 : 1);
 }
+
+bool finished(const mag_job* j) { return mag::job_done(j) && !mag::job_mixed32(j); }
 
 BatchJob describe(mag_job* j) {
   BatchJob b;
@@ -128,77 +142,16 @@ BatchJob describe(mag_job* j) {
   return b;
 }
 
-void drop_graph(mag_batch* b) {
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->exec = nullptr;
-  b->graph = nullptr;
+void drop_graph(ClassRun& t) {
+  if (t.exec) (void)hipGraphExecDestroy(t.exec);
+  if (t.graph) (void)hipGraphDestroy(t.graph);
+  t.exec = nullptr;
+  t.graph = nullptr;
 }
 
-hipError_t enqueue_replay(mag_batch* b);
-int ensure_streams(mag_batch* b);
-
-// rebuild the device tables and re-capture the replay graph
-int refresh(mag_batch* b) {
-  if (!b->dirty) return MAG_OK;
-  drop_graph(b);
-  b->all.clear();
-  for (int c = 0; c < kClasses; c++) {
-    b->cls[c].group = (c < 2 || (c >= kSpinClass && c < kSiteClass3)) ? 2 : 3;
-    b->cls[c].fp32 = c < kSpinClass && (c % 2) == 0;
-    b->cls[c].h.clear();
-    b->cls[c].red.clear();
-    b->cls[c].nblocks = 0;
-    b->cls[c].nrblocks = 0;
-  }
-  for (mag_job* j : b->active) {
-    ClassTable& t = b->cls[class_of(j)];
-    BatchJob d = describe(j);
-    d.block0 = t.nblocks;
-    d.rb0 = t.nrblocks;
-    t.nblocks += d.nb;
-    t.nrblocks += mag::rebuild_blocks_of(j->g);
-    t.h.push_back(d);
-    if (!j->spin) {
-      b->all.push_back(d);
-      t.red.push_back(d);
-    } else {
-      // one reduction entry per copy
-      for (int c = 0; c < j->ncopies; c++) {
-        BatchJob e = d;
-        const size_t hs = (size_t)c * j->prm.max_sweeps;
-        e.part = j->part + (size_t)c * j->pstride;
-        e.ctl = j->ctl + c;
-        e.hF = j->hF + hs;
-        e.hR = j->hR + hs;
-        e.hA = j->hA + hs;
-        b->all.push_back(e);
-        t.red.push_back(e);
-      }
-    }
-  }
-  for (int c = 0; c < kClasses; c++) {
-    ClassTable& t = b->cls[c];
-    CHK(upload_table(t.h, &t.d, &t.cap));
-    CHK(upload_table(t.red, &t.d_red, &t.cap_red));
-  }
-  CHK(upload_table(b->all, &b->d_all, &b->cap_all));
-  b->dirty = false;
-  CHK(ensure_streams(b));
-  if (b->active.empty() || b->direct) return MAG_OK;
-  HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-  const hipError_t er = enqueue_replay(b);
-  hipError_t e = hipStreamEndCapture(b->st, &b->graph);
-  if (er != hipSuccess) return fail(MAG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(er));
-  if (e != hipSuccess) return fail(MAG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-  HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-  return MAG_OK;
-}
-
-// the launches of one replay (captured into the graph, or issued directly)
 // half-sweep of one class
-void launch_class_half(mag_batch* b, int c, int p, hipStream_t st) {
-  const ClassTable& t = b->cls[c];
+void launch_class_half(const mag_batch* b, int c, int p, hipStream_t st) {
+  const ClassRun& t = b->cls[c];
   if (c < kSpinClass) {
     mag::launch_half_batch(t.group, t.fp32, t.d, (int)t.h.size(), t.nblocks, p, st);
   } else if (c >= kSiteClass3) {
@@ -211,69 +164,167 @@ void launch_class_half(mag_batch* b, int c, int p, hipStream_t st) {
   }
 }
 
-// sweep i's rebuild of a link-form FP32 class
-void launch_class_rebuild(mag_batch* b, int c, int i, hipStream_t st) {
-  const ClassTable& t = b->cls[c];
-  if (c < kSpinClass && t.fp32 && b->reunit > 0 && (i + 1) % b->reunit == 0) {
-    mag::launch_rebuild_batch(t.group, t.d, (int)t.h.size(), t.nrblocks, st);
-    mag::launch_reduce_batch(t.d_red, (int)t.red.size(), mag::RED_REBUILD, st);
+// one replay of class c: `replay` sweeps, then the control-block gather
+void enqueue_class_replay(const mag_batch* b, int c, hipStream_t s) {
+  const ClassRun& t = b->cls[c];
+  for (int i = 0; i < b->replay; i++) {
+    launch_class_half(b, c, 0, s);
+    launch_class_half(b, c, 1, s);
+    mag::launch_reduce_batch(t.d_red, (int)t.red.size(), mag::RED_SWEEP, s);
+    if (c < kSpinClass && t.fp32 && b->reunit > 0 && (i + 1) % b->reunit == 0) {
+      mag::launch_rebuild_batch(t.group, t.d, (int)t.h.size(), t.nrblocks, s);
+      mag::launch_reduce_batch(t.d_red, (int)t.red.size(), mag::RED_REBUILD, s);
+    }
   }
+  mag::launch_gather_ctl(t.d_red, (int)t.red.size(), t.ctl_out, s);
 }
 
-int live_classes(const mag_batch* b, std::vector<int>& live) {
-  live.clear();
-  for (int c = 0; c < kClasses; c++)
-    if (!b->cls[c].h.empty()) live.push_back(c);
-  return (int)live.size();
+// the class's stream and completion event (created on first use, outside
+// any capture)
+int ensure_stream(mag_batch* b, int c) {
+  ClassRun& t = b->cls[c];
+  if (!t.s) {
+    if (b->fork) HIPCHK(hipStreamCreateWithFlags(&t.s, hipStreamNonBlocking));
+    else t.s = b->st;
+  }
+  if (!t.ev) HIPCHK(hipEventCreateWithFlags(&t.ev, hipEventDisableTiming));
+  return MAG_OK;
 }
 
-// the side streams and join events of the forked replay (created on first
-// use, outside any capture)
-int ensure_streams(mag_batch* b) {
-  std::vector<int> live;
-  const int n = live_classes(b, live);
-  if (!b->fork || n < 2) return MAG_OK;
-  for (int k = 0; k < n; k++) {
-    if (!b->fs[k]) HIPCHK(hipStreamCreateWithFlags(&b->fs[k], hipStreamNonBlocking));
-    if (!b->ev_join[k]) HIPCHK(hipEventCreateWithFlags(&b->ev_join[k], hipEventDisableTiming));
+// rebuild class c's device tables and graph from its members; c is idle
+int refresh_class(mag_batch* b, int c) {
+  ClassRun& t = b->cls[c];
+  if (!t.dirty) return MAG_OK;
+  drop_graph(t);
+  t.group = (c < 2 || (c >= kSpinClass && c < kSiteClass3)) ? 2 : 3;
+  t.fp32 = c < kSpinClass && (c % 2) == 0;
+  t.h.clear();
+  t.red.clear();
+  t.owner.clear();
+  t.cap_jobs = t.jobs;
+  t.nblocks = 0;
+  t.nrblocks = 0;
+  for (mag_job* j : t.jobs) {
+    BatchJob d = describe(j);
+    d.block0 = t.nblocks;
+    d.rb0 = t.nrblocks;
+    t.nblocks += d.nb;
+    t.nrblocks += mag::rebuild_blocks_of(j->g);
+    t.h.push_back(d);
+    if (!j->spin) {
+      t.red.push_back(d);
+      t.owner.emplace_back(j, 0);
+    } else {
+      // one reduction entry per copy
+      for (int k = 0; k < j->ncopies; k++) {
+        BatchJob e = d;
+        const size_t hs = (size_t)k * j->prm.max_sweeps;
+        e.part = j->part + (size_t)k * j->pstride;
+        e.ctl = j->ctl + k;
+        e.hF = j->hF + hs;
+        e.hR = j->hR + hs;
+        e.hA = j->hA + hs;
+        t.red.push_back(e);
+        t.owner.emplace_back(j, k);
+      }
+    }
+  }
+  CHK(upload_table(t.h, &t.d, &t.cap));
+  CHK(upload_table(t.red, &t.d_red, &t.cap_red));
+  if (t.red.size() > t.cap_out) {
+    if (t.ctl_out) (void)hipHostFree(t.ctl_out);
+    t.ctl_out = nullptr;
+    t.cap_out = 0;
+    const size_t n = std::max<size_t>(t.red.size(), 16);
+    HIPCHK(hipHostMalloc((void**)&t.ctl_out, n * sizeof(JobCtl), hipHostMallocMapped | hipHostMallocCoherent));
+    t.cap_out = n;
+  }
+  t.dirty = false;
+  CHK(ensure_stream(b, c));
+  if (t.jobs.empty() || b->direct) return MAG_OK;
+  HIPCHK(hipStreamBeginCapture(t.s, hipStreamCaptureModeThreadLocal));
+  enqueue_class_replay(b, c, t.s);
+  const hipError_t el = hipGetLastError();
+  const hipError_t e = hipStreamEndCapture(t.s, &t.graph);
+  if (el != hipSuccess) return fail(MAG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(el));
+  if (e != hipSuccess) return fail(MAG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  HIPCHK(hipGraphInstantiate(&t.exec, t.graph, nullptr, nullptr, 0));
+  return MAG_OK;
+}
+
+// launch n replays of idle class c (nothing if it has no members)
+int launch_class(mag_batch* b, int c, int n) {
+  ClassRun& t = b->cls[c];
+  CHK(refresh_class(b, c));
+  if (t.jobs.empty() || n == 0) return MAG_OK;
+  for (int i = 0; i < n; i++) {
+    if (b->direct) enqueue_class_replay(b, c, t.s);
+    else HIPCHK(hipGraphLaunch(t.exec, t.s));
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(t.ev, t.s));
+  t.inflight = true;
+  for (mag_job* j : t.cap_jobs)
+    if (mag::job_mixed32(j) && !j->spin)
+      j->since_rebuild = (j->since_rebuild + n * b->replay) % j->prm.reunit_every;
+  return MAG_OK;
+}
+
+void erase_job(std::vector<mag_job*>& v, mag_job* j) {
+  auto it = std::find(v.begin(), v.end(), j);
+  if (it != v.end()) v.erase(it);
+}
+
+// class c's launched replays are complete: take every member's control block
+// from the gather, move mixed jobs between phases (and classes), drop
+// finished jobs
+int complete_class(mag_batch* b, int c) {
+  ClassRun& t = b->cls[c];
+  t.inflight = false;
+  for (size_t i = 0; i < t.owner.size(); i++) t.owner[i].first->ctl_h[t.owner[i].second] = t.ctl_out[i];
+  for (mag_job* j : t.cap_jobs) {
+    if (std::find(t.jobs.begin(), t.jobs.end(), j) == t.jobs.end()) continue;  // removed meanwhile
+    bool moved = false;
+    if (mag::job_done(j) && mag::job_mixed32(j)) {
+      CHK(mag::job_finalize_mixed(j));  // FP32 phase over: FP64 rebuild, FP64 class
+      moved = true;
+    }
+    bool switched = false;
+    CHK(mag::job_maybe_start_fp32(j, &switched));  // FP64 warm-up over: FP32 class
+    moved = moved || switched;
+    if (moved) HIPCHK(hipStreamSynchronize(j->st));  // before another stream uses the job
+    if (finished(j)) {
+      erase_job(t.jobs, j);
+      erase_job(b->active, j);
+      t.dirty = true;
+    } else if (class_of(j) != c) {
+      erase_job(t.jobs, j);
+      t.dirty = true;
+      ClassRun& u = b->cls[class_of(j)];
+      u.jobs.push_back(j);
+      u.dirty = true;
+    }
   }
   return MAG_OK;
 }
 
-// the launches of one replay (captured into the graph, or issued directly);
-// returns the first error of the stream-ordering calls
-hipError_t enqueue_replay(mag_batch* b) {
-  std::vector<int> live;
-  live_classes(b, live);
-  if (!b->fork || live.size() < 2) {
-    for (int i = 0; i < b->replay; i++) {
-      for (int p = 0; p < 2; p++)
-        for (int c : live) launch_class_half(b, c, p, b->st);
-      mag::launch_reduce_batch(b->d_all, (int)b->all.size(), mag::RED_SWEEP, b->st);
-      for (int c : live) launch_class_rebuild(b, c, i, b->st);
-    }
-    return hipSuccess;
-  }
-  hipError_t e = hipEventRecord(b->ev_fork, b->st);
-  for (size_t k = 0; k < live.size() && e == hipSuccess; k++) {
-    const int c = live[k];
-    hipStream_t s = b->fs[k];
-    e = hipStreamWaitEvent(s, b->ev_fork, 0);
-    if (e != hipSuccess) break;
-    const ClassTable& t = b->cls[c];
-    for (int i = 0; i < b->replay; i++) {
-      launch_class_half(b, c, 0, s);
-      launch_class_half(b, c, 1, s);
-      mag::launch_reduce_batch(t.d_red, (int)t.red.size(), mag::RED_SWEEP, s);
-      launch_class_rebuild(b, c, i, s);
-    }
-    e = hipEventRecord(b->ev_join[k], s);
-    if (e == hipSuccess) e = hipStreamWaitEvent(b->st, b->ev_join[k], 0);
-  }
-  return e;
+// wait for class c's replays and examine them
+int wait_class(mag_batch* b, int c) {
+  ClassRun& t = b->cls[c];
+  if (!t.inflight) return MAG_OK;
+  HIPCHK(hipEventSynchronize(t.ev));
+  return complete_class(b, c);
 }
 
-bool finished(const mag_job* j) { return mag::job_done(j) && !mag::job_mixed32(j); }
+int wait_all(mag_batch* b) {
+  for (int c = 0; c < kClasses; c++) CHK(wait_class(b, c));
+  return MAG_OK;
+}
+
+void counts(const mag_batch* b, int* n_running, int* n_finished) {
+  if (n_running) *n_running = (int)b->active.size();
+  if (n_finished) *n_finished = (int)(b->members.size() - b->active.size());
+}
 
 }  // namespace
 
@@ -291,10 +342,7 @@ int mag_batch_create(int device, mag_batch** out) {
   b->fork = !(sv && sv[0] == '1');
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
   if (e != hipSuccess) {
-    if (b->st) (void)hipStreamDestroy(b->st);
-    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     delete b;
     return fail(MAG_ERR_HIP, "batch setup failed: %s", hipGetErrorString(e));
   }
@@ -306,17 +354,21 @@ int mag_batch_destroy(mag_batch* b) {
   if (!b) return MAG_OK;
   // best-effort teardown: errors here have nowhere to go
   (void)hipSetDevice(b->device);
+  for (int c = 0; c < kClasses; c++) {
+    ClassRun& t = b->cls[c];
+    if (t.inflight) (void)hipEventSynchronize(t.ev);
+  }
   (void)hipStreamSynchronize(b->st);
-  drop_graph(b);
   for (mag_job* j : b->members) j->batch = nullptr;
   for (int c = 0; c < kClasses; c++) {
-    (void)hipFree(b->cls[c].d);
-    (void)hipFree(b->cls[c].d_red);
-    if (b->fs[c]) (void)hipStreamDestroy(b->fs[c]);
-    if (b->ev_join[c]) (void)hipEventDestroy(b->ev_join[c]);
+    ClassRun& t = b->cls[c];
+    drop_graph(t);
+    (void)hipFree(t.d);
+    (void)hipFree(t.d_red);
+    if (t.ctl_out) (void)hipHostFree(t.ctl_out);
+    if (t.ev) (void)hipEventDestroy(t.ev);
+    if (t.s && t.s != b->st) (void)hipStreamDestroy(t.s);
   }
-  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
-  (void)hipFree(b->d_all);
   (void)hipStreamDestroy(b->st);
   delete b;
   return MAG_OK;
@@ -345,8 +397,12 @@ int mag_batch_add(mag_batch* b, mag_job* j) {
   CHK(mag::job_read_ctl(j));
   j->batch = b;
   b->members.push_back(j);
-  if (!finished(j)) b->active.push_back(j);
-  b->dirty = true;
+  if (!finished(j)) {
+    b->active.push_back(j);
+    ClassRun& t = b->cls[class_of(j)];
+    t.jobs.push_back(j);
+    t.dirty = true;
+  }
   return MAG_OK;
 }
 
@@ -355,13 +411,21 @@ int mag_batch_remove(mag_batch* b, mag_job* j) {
   auto it = std::find(b->members.begin(), b->members.end(), j);
   if (it == b->members.end()) return fail(MAG_ERR_ARG, "job is not in this batch");
   HIPCHK(hipSetDevice(b->device));
-  HIPCHK(hipStreamSynchronize(b->st));
-  b->members.erase(it);
-  auto a = std::find(b->active.begin(), b->active.end(), j);
-  if (a != b->active.end()) {
-    b->active.erase(a);
-    b->dirty = true;
+  // a class whose launched replays reference the job must finish them first
+  for (int c = 0; c < kClasses; c++) {
+    ClassRun& t = b->cls[c];
+    if (t.inflight && std::find(t.cap_jobs.begin(), t.cap_jobs.end(), j) != t.cap_jobs.end())
+      CHK(wait_class(b, c));
   }
+  for (int c = 0; c < kClasses; c++) {
+    ClassRun& t = b->cls[c];
+    if (std::find(t.jobs.begin(), t.jobs.end(), j) != t.jobs.end()) {
+      erase_job(t.jobs, j);
+      t.dirty = true;
+    }
+  }
+  b->members.erase(it);
+  erase_job(b->active, j);
   j->batch = nullptr;
   return MAG_OK;
 }
@@ -387,43 +451,53 @@ int mag_batch_stream(mag_batch* b, void** stream) {
 int mag_batch_replays(mag_batch* b, int n) {
   if (!b || n < 0) return fail(MAG_ERR_ARG, "bad argument");
   HIPCHK(hipSetDevice(b->device));
-  CHK(refresh(b));
-  if (b->direct) {
-    if (b->active.empty()) return MAG_OK;
-    for (int i = 0; i < n; i++) HIPCHK(enqueue_replay(b));
-    HIPCHK(hipGetLastError());
-  } else {
-    if (!b->exec) return MAG_OK;
-    for (int i = 0; i < n; i++) HIPCHK(hipGraphLaunch(b->exec, b->st));
+  for (int c = 0; c < kClasses; c++) {
+    ClassRun& t = b->cls[c];
+    if (t.inflight) CHK(wait_class(b, c));
   }
-  for (mag_job* j : b->active)
-    if (mag::job_mixed32(j)) j->since_rebuild = (j->since_rebuild + n * b->replay) % j->prm.reunit_every;
+  for (int c = 0; c < kClasses; c++) {
+    ClassRun& t = b->cls[c];
+    CHK(launch_class(b, c, n));
+    if (t.inflight && t.s != b->st) HIPCHK(hipStreamWaitEvent(b->st, t.ev, 0));
+  }
   return MAG_OK;
 }
 
 int mag_batch_poll(mag_batch* b, int* n_running, int* n_finished) {
   if (!b) return fail(MAG_ERR_ARG, "NULL batch");
   HIPCHK(hipSetDevice(b->device));
+  CHK(wait_all(b));
   HIPCHK(hipStreamSynchronize(b->st));
-  for (mag_job* j : b->active)
-    HIPCHK(hipMemcpyAsync(j->ctl_h, j->ctl, j->ncopies * sizeof(mag::JobCtl),
-                          hipMemcpyDeviceToHost, b->st));
-  HIPCHK(hipStreamSynchronize(b->st));
-  std::vector<mag_job*> still;
-  for (mag_job* j : b->active) {
-    if (mag::job_done(j) && mag::job_mixed32(j)) {
-      CHK(mag::job_finalize_mixed(j));  // FP32 phase over: FP64 rebuild, FP64 class
-      b->dirty = true;
+  counts(b, n_running, n_finished);
+  return MAG_OK;
+}
+
+int mag_batch_step(mag_batch* b, int* n_running, int* n_finished) {
+  if (!b) return fail(MAG_ERR_ARG, "NULL batch");
+  HIPCHK(hipSetDevice(b->device));
+  for (int c = 0; c < kClasses; c++)
+    if (!b->cls[c].inflight) CHK(launch_class(b, c, 1));
+  bool any = false;
+  for (int c = 0; c < kClasses; c++) any = any || b->cls[c].inflight;
+  if (any) {
+    // wait until at least one class has finished its replay
+    for (bool got = false; !got;) {
+      for (int c = 0; c < kClasses; c++) {
+        ClassRun& t = b->cls[c];
+        if (!t.inflight) continue;
+        const hipError_t e = hipEventQuery(t.ev);
+        if (e == hipErrorNotReady) continue;
+        if (e != hipSuccess) return fail(MAG_ERR_HIP, "batch: %s", hipGetErrorString(e));
+        CHK(complete_class(b, c));
+        got = true;
+      }
+      if (!got) std::this_thread::yield();
     }
-    bool switched = false;
-    CHK(mag::job_maybe_start_fp32(j, &switched));  // FP64 warm-up over: FP32 class
-    if (switched) b->dirty = true;
-    if (finished(j)) b->dirty = true;
-    else still.push_back(j);
+    // keep the device busy while the caller looks at the results
+    for (int c = 0; c < kClasses; c++)
+      if (!b->cls[c].inflight) CHK(launch_class(b, c, 1));
   }
-  b->active.swap(still);
-  if (n_running) *n_running = (int)b->active.size();
-  if (n_finished) *n_finished = (int)(b->members.size() - b->active.size());
+  counts(b, n_running, n_finished);
   return MAG_OK;
 }
 
@@ -431,29 +505,33 @@ int mag_batch_run(mag_batch* b) {
   if (!b) return fail(MAG_ERR_ARG, "NULL batch");
   int running = 0;
   CHK(mag_batch_poll(b, &running, nullptr));
-  while (running > 0) {
-    CHK(mag_batch_replays(b, 1));
-    CHK(mag_batch_poll(b, &running, nullptr));
-  }
-  return MAG_OK;
+  while (running > 0) CHK(mag_batch_step(b, &running, nullptr));
+  return mag_batch_poll(b, nullptr, nullptr);
 }
 
 int mag_batch_replays_timed(mag_batch* b, int n, double* ms) {
   if (!b || n < 1 || !ms) return fail(MAG_ERR_ARG, "bad argument");
   HIPCHK(hipSetDevice(b->device));
-  CHK(refresh(b));
+  CHK(wait_all(b));
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, b->st));
-  CHK(mag_batch_replays(b, n));
+  for (int c = 0; c < kClasses; c++) {
+    ClassRun& t = b->cls[c];
+    CHK(refresh_class(b, c));
+    if (t.jobs.empty()) continue;
+    if (t.s != b->st) HIPCHK(hipStreamWaitEvent(t.s, e0, 0));
+    CHK(launch_class(b, c, n));
+    if (t.s != b->st) HIPCHK(hipStreamWaitEvent(b->st, t.ev, 0));
+  }
   HIPCHK(hipEventRecord(e1, b->st));
   HIPCHK(hipEventSynchronize(e1));
   float t = 0.f;
   HIPCHK(hipEventElapsedTime(&t, e0, e1));
   *ms = t;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return MAG_OK;
 }
 

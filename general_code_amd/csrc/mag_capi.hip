@@ -518,6 +518,10 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   j->nreal = (size_t)8 * j->g.Vh * j->nr;
   j->nb = mag::sweep_blocks(j->g, p->group);
   j->nw = mag::sweep_waves(j->g, p->group);
+  if (p->form == MAG_FORM_SPIN && p->group == MAG_SU3) {  // site form: lane pairs
+    j->nb = (int)((mag::site3_lanes(j->g) + mag::kBlock - 1) / mag::kBlock);
+    j->nw = (int)((mag::site3_lanes(j->g) + 63) / 64);
+  }
   j->nf = mag::nblocks(j->g);
   j->invF = 1.0 / (double)(j->V * (p->group == MAG_SU2 ? 4 : 12));
   j->invV = 1.0 / (double)j->V;

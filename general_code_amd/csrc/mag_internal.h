@@ -62,16 +62,26 @@ enum { RED_INIT = 0, RED_SWEEP = 1, RED_PLAIN = 2, RED_RESET = 3, RED_REBUILD = 
 // Partial buffers hold 3 doubles per workgroup: (F sum, residual sum,
 // residual max).  A sweep uses 2*nblocks(g) partial triples (red, black).
 inline int nblocks(const Geom& g) { return (int)((g.Vh + kBlock - 1) / kBlock); }
-// workgroups of a half-sweep at the default shape: SU(2) one lane per
-// site, SU(3) two
+// lanes per site of the link-form SU(3) half-sweep: 2 (lane q holds the
+// links of mu = 2q, 2q+1) or 4 (lane q holds mu = q)
+#ifndef MAG_SU3_LANES
+#define MAG_SU3_LANES 2
+#endif
+constexpr int kSu3Lanes = MAG_SU3_LANES;
+static_assert(kSu3Lanes == 2 || kSu3Lanes == 4, "SU(3) lanes per site: 2 or 4");
+// lanes of a link-form half-sweep: SU(2) one per site, SU(3) kSu3Lanes;
+// the SU(3) site form runs on lane pairs (site_lanes)
+inline size_t sweep_lanes(const Geom& g, int group) {
+  return (size_t)g.Vh * (group == 2 ? 1 : kSu3Lanes);
+}
+inline size_t site3_lanes(const Geom& g) { return (size_t)g.Vh * 2; }
+// workgroups of a half-sweep at the default shape
 inline int sweep_blocks(const Geom& g, int group) {
-  const size_t lanes = (size_t)g.Vh * (group == 2 ? 1 : 2);
-  return (int)((lanes + kBlock - 1) / kBlock);
+  return (int)((sweep_lanes(g, group) + kBlock - 1) / kBlock);
 }
 // partial triples of a half-sweep: one per wave of 64 lanes
 inline int sweep_waves(const Geom& g, int group) {
-  const size_t lanes = (size_t)g.Vh * (group == 2 ? 1 : 2);
-  return (int)((lanes + 63) / 64);
+  return (int)((sweep_lanes(g, group) + 63) / 64);
 }
 // Launch shape of a half-sweep kernel: nt threads per workgroup (64, 128,
 // 256 or 512) and spt tiles of nt lanes per thread (1 or 2).  Partials are
@@ -167,6 +177,8 @@ void launch_su3_site_copies_batch(bool fp32, const BatchJob* jobs, int njobs, in
 // batched reduction, one workgroup per job: mode RED_SWEEP (end of sweep)
 // or RED_REBUILD (after a batched rebuild)
 void launch_reduce_batch(const BatchJob* jobs, int njobs, int mode, hipStream_t st);
+// every entry's control block into out (host-visible memory, n entries)
+void launch_gather_ctl(const BatchJob* jobs, int n, JobCtl* out, hipStream_t st);
 // batched FP64 rebuild of the FP32 working links + gauge normalisation of a
 // class table (mixed FP32 phase); jobs that are done are skipped
 void launch_rebuild_batch(int group, const BatchJob* jobs, int njobs, int nblocks_total,

@@ -4,16 +4,17 @@
 //
 // One scheduler thread per server owns a mag_batch and every admitted job.
 // Its loop:
-//   1. enqueue one replay of the batch graph (mag_batch_replay_sweeps sweeps
-//      of every active job) -- asynchronous on the batch stream;
-//   2. while that runs: admit queued requests in arrival order as long as
+//   1. one batch step (mag_batch_step): every idle class of the batch
+//      launches its next replay (mag_batch_replay_sweeps sweeps of its
+//      jobs), the step returns when some class's replay is done and its
+//      jobs have been examined (mixed jobs moved between phases), with
+//      the classes relaunched;
+//   2. while they run: admit queued requests in arrival order as long as
 //      the HBM footprint of the admitted jobs stays within the budget (and
 //      the active count within max_active): create the job, upload or
 //      generate its links on the job's own stream, add it to the batch (it
-//      joins at the next replay);
-//   3. poll the batch (waits for the replay, reads every job's control
-//      block, moves mixed jobs from the FP32 to the FP64 phase);
-//   4. finished jobs leave the batch, their result is published and their
+//      joins at its class's next launch);
+//   3. finished jobs leave the batch, their result is published and their
 //      HBM freed at once.
 // Requests that arrive while others are on the GPU are therefore admitted
 // as memory frees up and their sweeps run in the same launches as the jobs
@@ -204,13 +205,14 @@ void scheduler(mag_server* s) {
     }
     int rc = MAG_OK;
     if (!s->active.empty()) {
-      rc = mag_batch_replays(s->batch, 1);  // asynchronous
+      // returns once some class's replay is done, with the others (and
+      // that class's next replay) still running
+      rc = mag_batch_step(s->batch, nullptr, nullptr);
       std::lock_guard<std::mutex> g(s->mu);
       s->stats.replays++;
     }
     while (admit_one(s)) {
     }
-    if (rc == MAG_OK && !s->active.empty()) rc = mag_batch_poll(s->batch, nullptr, nullptr);
     if (rc != MAG_OK) {
       fail_all_active(s, rc);
       continue;

@@ -129,9 +129,30 @@ void launch_reduce_batch(const BatchJob* jobs, int njobs, int mode, hipStream_t 
   hipLaunchKernelGGL(k_reduce_batch, dim3(njobs), dim3(kRedBlock), 0, st, jobs, mode);
 }
 
+// control block of every entry of a table into out[i] (pinned host memory):
+// the host reads a finished replay's state without a copy call per job
+__global__ __launch_bounds__(64) void k_gather_ctl(const BatchJob* __restrict__ jobs, int n,
+                                                   JobCtl* __restrict__ out) {
+  constexpr int W = sizeof(JobCtl) / 8;
+  const int i = blockIdx.x * 64 + (int)threadIdx.x;
+  if (i >= n * W) return;
+  const int e = i / W, w = i % W;
+  reinterpret_cast<uint64_t*>(out + e)[w] = reinterpret_cast<const uint64_t*>(jobs[e].ctl)[w];
+}
+void launch_gather_ctl(const BatchJob* jobs, int n, JobCtl* out, hipStream_t st) {
+  constexpr int W = sizeof(JobCtl) / 8;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_gather_ctl, dim3((n * W + 63) / 64), dim3(64), 0, st, jobs, n, out);
+}
+
 // ================================================================== helpers
-// value of the partner lane of the pair (lane ^ 1)
+// value of the partner lane of the pair (lane ^ 1), and of lane ^ 2
 __device__ __forceinline__ double pair_xchg(double v) { return dpp_f64<DPP_QUAD_XOR1>(v); }
+__device__ __forceinline__ double quad_xchg2(double v) { return dpp_f64<DPP_QUAD_XOR2>(v); }
+__device__ __forceinline__ float quad_xchg2(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), DPP_QUAD_XOR2, 0xF, 0xF, true));
+}
 __device__ __forceinline__ float pair_xchg(float v) {
   return __builtin_bit_cast(
       float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), DPP_QUAD_XOR1, 0xF, 0xF, true));
@@ -314,33 +335,52 @@ __device__ __forceinline__ void renorm_rows(M2<T>& m) {
   }
 }
 
-// Cabibbo-Marinari subgroup (I,J) update of the 4 links a lane holds
-// (forward F[k], backward B[k], mu = 2q + k), rows 0,1 in registers, row 2
-// rebuilt where the subgroup touches it.  Returns 1 - cos(theta).
-template <typename T, int I, int J, bool GA>
+// Cabibbo-Marinari subgroup (I,J) update of the 2*NL links a lane holds
+// (forward F[k], backward B[k], mu = NL*q + k; NL = 2 for lane pairs, 1 for
+// lane quads), rows 0,1 in registers, row 2 rebuilt where the subgroup
+// touches it.  Returns 1 - cos(theta).  The field is summed
+// as t_mu = f_mu + b_mu, then (t_0 + t_1) + (t_2 + t_3), the same
+// operations for either split (a + b == b + a exactly), in the order of the
+// oracle.
+template <typename T, int I, int J, bool GA, int NL>
 __device__ __forceinline__ T cm_subgroup(M2<T>* F, M2<T>* B, M2<T>& Gx, T omega) {
   T h0 = T(0), h1 = T(0), h2 = T(0);
-  C<T> Fr2[2][3], Br2[2][3];
+  C<T> Fr2[NL][3], Br2[NL][3];
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < NL; k++) {
     if (J == 2) {
       row2_of(F[k], Fr2[k]);
       row2_of(B[k], Br2[k]);
     }
     auto fe = [&](int a, int b) -> C<T> { return a < 2 ? F[k].r[a][b] : Fr2[k][b]; };
     auto be = [&](int a, int b) -> C<T> { return a < 2 ? B[k].r[a][b] : Br2[k][b]; };
-    blk_fwd(fe(I, I), fe(I, J), fe(J, I), fe(J, J), h0, h1, h2);
-    blk_bwd(be(I, I), be(I, J), be(J, I), be(J, J), h0, h1, h2);
+    T t0 = T(0), t1 = T(0), t2 = T(0);
+    blk_fwd(fe(I, I), fe(I, J), fe(J, I), fe(J, J), t0, t1, t2);
+    blk_bwd(be(I, I), be(I, J), be(J, I), be(J, J), t0, t1, t2);
+    if (k == 0) {
+      h0 = t0;
+      h1 = t1;
+      h2 = t2;
+    } else {
+      h0 += t0;
+      h1 += t1;
+      h2 += t2;
+    }
   }
   h0 += pair_xchg(h0);
   h1 += pair_xchg(h1);
   h2 += pair_xchg(h2);
+  if (NL == 1) {
+    h0 += quad_xchg2(h0);
+    h1 += quad_xchg2(h1);
+    h2 += quad_xchg2(h2);
+  }
   T res;
   const Q<T> gq = omega_gauge(h0, h1, h2, omega, res);
   // r = [[g0 + i g3, g2 + i g1], [-g2 + i g1, g0 - i g3]], g3 = 0
   const C<T> r00{gq.a0, T(0)}, r01{gq.a2, gq.a1}, r10{-gq.a2, gq.a1}, r11{gq.a0, T(0)};
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < NL; k++) {
     // forward: rows I,J <- r (rows I,J); only stored rows are written
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -371,57 +411,60 @@ __device__ __forceinline__ T cm_subgroup(M2<T>* F, M2<T>* B, M2<T>& Gx, T omega)
   return res;
 }
 
-// One workgroup lb of an SU(3) half-sweep, two lanes per site: lane q of
-// the pair holds the forward and backward links of mu = 2q, 2q+1.
+// One workgroup lb of an SU(3) half-sweep, LS = kSu3Lanes lanes per site:
+// lane q of the group holds the forward and backward links of the NL = 4/LS
+// directions mu = NL*q .. NL*q + NL - 1, the group exchanges the partial
+// fields with DPP and every lane derives the same g.
 template <typename T, bool GA, int NT, int SPT>
 __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict__ G,
                                                const Geom& g, int p, T omega,
                                                double* __restrict__ part, uint32_t lb) {
+  constexpr int LS = kSu3Lanes, NL = 4 / kSu3Lanes;
 #pragma unroll
   for (int kk = 0; kk < SPT; kk++) {
-  const uint32_t base = (lb * SPT + kk) * NT;
-  const uint32_t tid = base + threadIdx.x;
-  const uint32_t h = tid >> 1;
-  const int q = (int)(tid & 1u);
-  double fs = 0.0, rs = 0.0, rm = 0.0;
-  if (h < g.Vh) {
-    const SiteIdx s = decode(g, h, p);
-    uint32_t hb[4];
-    back_all(g, h, s, hb);
-    LRef<T> pf[2];
-    LRef<T> pb[2];
-    M2<T> F[2], B[2];
+    const uint32_t base = (lb * SPT + kk) * NT;
+    const uint32_t tid = base + threadIdx.x;
+    const uint32_t h = tid / LS;
+    const int q = (int)(tid % LS);
+    double fs = 0.0, rs = 0.0, rm = 0.0;
+    if (h < g.Vh) {
+      const SiteIdx s = decode(g, h, p);
+      uint32_t hb[4];
+      back_all(g, h, s, hb);  // all four, then a select: no divergent switch
+      LRef<T> pf[NL];
+      LRef<T> pb[NL];
+      M2<T> F[NL], B[NL];
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
-      const int mu = 2 * q + k;
-      const uint32_t hbm = q ? hb[2 + k] : hb[k];
-      pf[k] = link_ptr<12>(U, g, p, mu, h);
-      pb[k] = link_ptr<12>(U, g, 1 - p, mu, hbm);
-      F[k] = ldm(pf[k]);
-      B[k] = ldm(pb[k]);
-    }
-    M2<T> Gx;
-    const LRef<T> gp = site_ptr<12>(G, g, p, h);
-    if (GA) Gx = ldm(gp);
-    T res = cm_subgroup<T, 0, 1, GA>(F, B, Gx, omega);
-    res += cm_subgroup<T, 1, 2, GA>(F, B, Gx, omega);
-    res += cm_subgroup<T, 0, 2, GA>(F, B, Gx, omega);
+      for (int k = 0; k < NL; k++) {
+        const int mu = NL * q + k;
+        const uint32_t hbm = mu == 0 ? hb[0] : mu == 1 ? hb[1] : mu == 2 ? hb[2] : hb[3];
+        pf[k] = link_ptr<12>(U, g, p, mu, h);
+        pb[k] = link_ptr<12>(U, g, 1 - p, mu, hbm);
+        F[k] = ldm(pf[k]);
+        B[k] = ldm(pb[k]);
+      }
+      M2<T> Gx;
+      const LRef<T> gp = site_ptr<12>(G, g, p, h);
+      if (GA) Gx = ldm(gp);
+      T res = cm_subgroup<T, 0, 1, GA, NL>(F, B, Gx, omega);
+      res += cm_subgroup<T, 1, 2, GA, NL>(F, B, Gx, omega);
+      res += cm_subgroup<T, 0, 2, GA, NL>(F, B, Gx, omega);
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
-      stm(pf[k], F[k]);
-      stm(pb[k], B[k]);
-    }
-    if (GA && q == 0) stm(gp, Gx);
-    if (p == 1) {
+      for (int k = 0; k < NL; k++) {
+        stm(pf[k], F[k]);
+        stm(pb[k], B[k]);
+      }
+      if (GA && q == 0) stm(gp, Gx);
+      if (p == 1) {
 #pragma unroll
-      for (int k = 0; k < 2; k++) fs += su3_link_F(F[k]) + su3_link_F(B[k]);
+        for (int k = 0; k < NL; k++) fs += su3_link_F(F[k]) + su3_link_F(B[k]);
+      }
+      if (q == 0) {
+        rs = (double)res;
+        rm = (double)res;
+      }
     }
-    if (q == 0) {
-      rs = (double)res;
-      rm = (double)res;
-    }
-  }
-  wave_partial(part, base + (threadIdx.x & ~63u), 2 * g.Vh, fs, rs, rm);
+    wave_partial(part, base + (threadIdx.x & ~63u), LS * g.Vh, fs, rs, rm);
   }
 }
 
@@ -446,7 +489,7 @@ template <typename T>
 void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega, double* part,
                      const JobCtl* ctl, Shape sh, hipStream_t st) {
   double* pp = part + (size_t)3 * sweep_waves(g, 3) * parity;
-  const uint32_t lanes = 2 * g.Vh;
+  const uint32_t lanes = (uint32_t)sweep_lanes(g, 3);
 #define L_(NT, SPT)                                                                        \
   do {                                                                                     \
     const dim3 grid((lanes + NT * SPT - 1) / (NT * SPT));                                  \
@@ -766,9 +809,9 @@ __device__ __forceinline__ void su3_site_copies_block(const T* __restrict__ U,
           B[k] = rows_mul_dag(rows_mul(Gb, Ub), Gx);
         }
       }
-      T res = cm_subgroup<T, 0, 1, true>(F, B, Gx2, omega);
-      res += cm_subgroup<T, 1, 2, true>(F, B, Gx2, omega);
-      res += cm_subgroup<T, 0, 2, true>(F, B, Gx2, omega);
+      T res = cm_subgroup<T, 0, 1, true, 2>(F, B, Gx2, omega);
+      res += cm_subgroup<T, 1, 2, true, 2>(F, B, Gx2, omega);
+      res += cm_subgroup<T, 0, 2, true, 2>(F, B, Gx2, omega);
       if constexpr (sizeof(T) == 4) renorm_rows(Gx2);
       if (q == 0) stm(gp, Gx2);
       if (p == 1) {
@@ -810,10 +853,10 @@ template <typename T>
 void launch_su3_site_copies(const T* U, T* G, size_t gstride, int ncopies, const Geom& g,
                             int parity, double omega, double* part, size_t pstride,
                             const JobCtl* ctl, hipStream_t st) {
-  const int nb = sweep_blocks(g, 3);
+  const int nb = (int)((site3_lanes(g) + kBlock - 1) / kBlock);
+  const size_t nw = (site3_lanes(g) + 63) / 64;
   hipLaunchKernelGGL((k_su3_site_copies<T>), dim3(nb), dim3(kBlock), 0, st, U, G, gstride,
-                     ncopies, g, parity, (T)omega, part + (size_t)3 * sweep_waves(g, 3) * parity,
-                     pstride, ctl);
+                     ncopies, g, parity, (T)omega, part + 3 * nw * parity, pstride, ctl);
 }
 template void launch_su3_site_copies<float>(const float*, float*, size_t, int, const Geom&, int,
                                             double, double*, size_t, const JobCtl*,

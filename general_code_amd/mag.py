@@ -153,6 +153,7 @@ SIGNATURES = {
     "mag_batch_replay_sweeps": (ctypes.c_int, [_vp, _ip]),
     "mag_batch_replays": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mag_batch_poll": (ctypes.c_int, [_vp, _ip, _ip]),
+    "mag_batch_step": (ctypes.c_int, [_vp, _ip, _ip]),
     "mag_batch_run": (ctypes.c_int, [_vp]),
     "mag_batch_replays_timed": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "mag_batch_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
@@ -610,7 +611,8 @@ def gauge_transform_spins(dims, links, spins) -> np.ndarray:
 # batches -------------------------------------------------------------------
 class MagBatch:
     """Many jobs swept together: batched launches per (group, storage) class,
-    the replay captured in a hipGraph (include/mag_gpu.h, mag_batch_*)."""
+    each class's replay captured in a hipGraph and run as an independent
+    chain on its own stream (include/mag_gpu.h, mag_batch_*)."""
 
     def __init__(self, device: int = 0):
         h = _vp()
@@ -656,6 +658,13 @@ class MagBatch:
         r = ctypes.c_int()
         f = ctypes.c_int()
         _check(lib().mag_batch_poll(self._h, ctypes.byref(r), ctypes.byref(f)))
+        return r.value, f.value
+
+    def step(self):
+        """One scheduling step (mag_batch_step): returns (running, finished)."""
+        r = ctypes.c_int()
+        f = ctypes.c_int()
+        _check(lib().mag_batch_step(self._h, ctypes.byref(r), ctypes.byref(f)))
         return r.value, f.value
 
     def run(self):

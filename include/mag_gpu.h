@@ -218,11 +218,14 @@ int mag_job_halfsweep_timed(mag_job *job, int n, double *avg_ms);
  * one launch per (group, storage type) class whose grid concatenates the
  * member jobs' grids, each sweep ends with one reduction launch (a workgroup
  * per job), and the sequence of `mag_batch_replay_sweeps` sweeps is captured
- * once in a hipGraph and replayed.  Every job runs exactly the code and the
- * partition of its own single launch, so its results are bit-identical alone
- * and in any batch.  Jobs that converge stop consuming work (their
- * workgroups return at once); a mixed-precision job whose FP32 phase ends is
- * rebuilt in FP64 at the next poll and continues in the FP64 class.
+ * once per class in a hipGraph and replayed.  Classes run as independent
+ * chains on streams of their own: mag_batch_step / mag_batch_run relaunch a
+ * class as soon as its previous replay is done, whatever the other classes
+ * are doing.  Every job runs exactly the code and the partition of its own
+ * single launch, so its results are bit-identical alone and in any batch.
+ * Jobs that converge stop consuming work (their workgroups return at once);
+ * a mixed-precision job whose FP32 phase ends is rebuilt in FP64 when its
+ * class's replay has been examined and continues in the FP64 class.
  * Member jobs must not be swept through the mag_job_* calls meanwhile. */
 typedef struct mag_batch mag_batch;
 
@@ -235,12 +238,18 @@ int mag_batch_remove(mag_batch *b, mag_job *job);
 int mag_batch_size(mag_batch *b, int *n);
 /* sweeps per graph replay (default 20; a multiple of reunit_every) */
 int mag_batch_replay_sweeps(mag_batch *b, int *n);
-/* Enqueue n replays (n * replay_sweeps sweeps of every active member). */
+/* Enqueue n replays (n * replay_sweeps sweeps of every active member),
+ * joined on the batch stream. */
 int mag_batch_replays(mag_batch *b, int n);
-/* Blocking: wait for the batch stream, move mixed jobs whose FP32 phase is
+/* Blocking: wait for every class, move mixed jobs whose FP32 phase is
  * done to FP64, report members still running and members finished. */
 int mag_batch_poll(mag_batch *b, int *n_running, int *n_finished);
-/* Blocking: replay until every member is finished. */
+/* Launch a replay of every class that is idle, wait until at least one
+ * class's replay has finished, examine its members (phase changes, finished
+ * jobs) and relaunch the idle classes; returns with replays in flight.
+ * Finished members may be removed (and destroyed) right after it returns. */
+int mag_batch_step(mag_batch *b, int *n_running, int *n_finished);
+/* Blocking: step until every member is finished. */
 int mag_batch_run(mag_batch *b);
 /* Measurement: n replays between HIP events on the batch stream. */
 int mag_batch_replays_timed(mag_batch *b, int n, double *ms);

@@ -1082,13 +1082,15 @@ static void su3_link_halfsweep(const geom_t *g, double *links, int parity,
       double dsite = 0;
       for (int sg = 0; sg < 3; sg++) {
         int i = CM_I[sg], j = CM_J[sg];
-        /* h = [f(0)+b(0)+f(1)+b(1)] + [f(2)+b(2)+f(3)+b(3)] */
-        double hp[2][3] = {{0, 0, 0}, {0, 0, 0}};
+        /* t_mu = f(mu) + b(mu); h = (t_0 + t_1) + (t_2 + t_3) */
+        double t[4][3];
         for (int mu = 0; mu < 4; mu++) {
-          blk_fwd(F[mu].m[i][i], F[mu].m[i][j], F[mu].m[j][i], F[mu].m[j][j], hp[mu >> 1]);
-          blk_bwd(B[mu].m[i][i], B[mu].m[i][j], B[mu].m[j][i], B[mu].m[j][j], hp[mu >> 1]);
+          t[mu][0] = t[mu][1] = t[mu][2] = 0;
+          blk_fwd(F[mu].m[i][i], F[mu].m[i][j], F[mu].m[j][i], F[mu].m[j][j], t[mu]);
+          blk_bwd(B[mu].m[i][i], B[mu].m[i][j], B[mu].m[j][i], B[mu].m[j][j], t[mu]);
         }
-        double h[3] = {hp[0][0] + hp[1][0], hp[0][1] + hp[1][1], hp[0][2] + hp[1][2]};
+        double h[3];
+        for (int a = 0; a < 3; a++) h[a] = (t[0][a] + t[1][a]) + (t[2][a] + t[3][a]);
         dsite += resid_of(h);
         double gq[4];
         or_omega_gauge(h, omega, gq);
