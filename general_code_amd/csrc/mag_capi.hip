@@ -440,6 +440,10 @@ int run_until_done(mag_job* j) {
 
 }  // namespace
 
+void mag::job_copy_links64(mag_job* j, int c, double* soa) {
+  if (su3_site(j)) su3_copy_links(j, c, soa);
+  else mag::launch_spin_links(j->U64, j->S + (size_t)c * j->sstride, soa, j->g, j->st);
+}
 int mag::job_read_ctl(mag_job* j) { return read_ctl(j); }
 int mag::job_finalize_mixed(mag_job* j) { return finalize_mixed(j); }
 bool mag::job_mixed32(const mag_job* j) { return mixed32(j); }

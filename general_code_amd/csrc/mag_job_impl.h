@@ -61,6 +61,9 @@ bool job_done(const mag_job* j);
 // mixed with an FP64 warm-up: enter the FP32 phase once the warm-up sweeps
 // are done (read_ctl first); *switched = whether it did
 int job_maybe_start_fp32(mag_job* j, bool* switched);
+// site form: the FP64 links of copy c (G U G^+ of its site variable) into
+// soa (parity-split SoA, nreal doubles), enqueued on the job's stream
+void job_copy_links64(mag_job* j, int c, double* soa);
 // launch shape of the job's link-form half-sweep (mag_tune.hip): cached per
 // (group, precision, device, lattice), tuned on first use
 int job_shape(const mag_job* j, bool fp32, Shape* out);

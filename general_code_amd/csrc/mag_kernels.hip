@@ -56,15 +56,6 @@ template void launch_su3_functional<float>(const float*, const Geom&, double*, h
 template void launch_su3_functional<double>(const double*, const Geom&, double*, hipStream_t);
 
 // ================================================================== layout
-// thread per lexicographic site s (reference order, indexing.h:111-123)
-__device__ __forceinline__ void lex_site(const Geom& g, uint32_t s, int& p, uint32_t& h) {
-  uint32_t x = s % (uint32_t)g.L0, r = s / (uint32_t)g.L0;
-  uint32_t y = r % (uint32_t)g.L1;
-  r /= (uint32_t)g.L1;
-  uint32_t z = r % (uint32_t)g.L2, t = r / (uint32_t)g.L2;
-  p = (int)((x + y + z + t) & 1u);
-  h = s >> 1;
-}
 
 template <typename T>
 __global__ void k_su2_aos_to_soa(const double* __restrict__ aos, T* __restrict__ U, Geom g) {

@@ -159,6 +159,17 @@ __device__ __forceinline__ void write_part(double* part, double a, double b, dou
   }
 }
 
+// lexicographic site s (reference order, indexing.h:111-123) -> parity p
+// and half index h of the parity-split SoA
+__device__ __forceinline__ void lex_site(const Geom& g, uint32_t s, int& p, uint32_t& h) {
+  uint32_t x = s % (uint32_t)g.L0, r = s / (uint32_t)g.L0;
+  uint32_t y = r % (uint32_t)g.L1;
+  r /= (uint32_t)g.L1;
+  uint32_t z = r % (uint32_t)g.L2, t = r / (uint32_t)g.L2;
+  p = (int)((x + y + z + t) & 1u);
+  h = s >> 1;
+}
+
 // sum_j |U_jj|^2 of an su3 link (mag.cpp:1837-1848), in FP64;
 // U_22 = conj(U00 U11 - U01 U10)
 template <typename T>

@@ -1,0 +1,424 @@
+// mag_monopole.hip -- abelian projection of gauge-fixed links and monopole
+// currents on the device: what the reference's monopole apps
+// (apps/monopoles_su2/monopoles_su2.cpp:110, apps/monopoles_su3/
+// monopoles_su3.cpp:103-120) compute from MAG-fixed configurations, so the
+// links of a MAG job go from fixing to currents without leaving the GPU.
+//
+//   angles     SU(2): phi = atan2(a3, a0) per link (MatrixConverter<su2,
+//              abelian>, matrix.h:1059-1062); SU(3): the phases of the
+//              diagonal less a third of their sum taken into [-pi, pi)
+//              (MatrixConverter<su3, su3_angles>, matrix.h:1082-1100)
+//   plaquettes phi_mu(x) + phi_nu(x+mu) - phi_mu(x+nu) - phi_nu(x) taken
+//              into [-pi, pi] (monopoles.cpp:368-457); SU(3): then the
+//              colour sum brought into (-pi, pi) by moving 2 pi off the
+//              largest / onto the smallest colour (monopoles.cpp:844-884)
+//   currents   J_mu(x) = forward differences at x + mu of the three
+//              plaquette planes orthogonal to mu, over 2 pi
+//              (get_current, monopoles.cpp:44-122)
+// Every kernel runs one thread per lexicographic site.  The plaquette and
+// current kernels do the reference's additions, subtractions and divisions
+// in the reference's order, so from the same angles they give the same
+// bits; the angles go through the device atan2.
+#include "mag_capi_util.h"
+#include "mag_gpu.h"
+#include "mag_internal.h"
+#include "mag_job_impl.h"
+#include "mag_layout.h"
+#include "mag_math.h"
+
+#include <cmath>
+
+namespace mag {
+namespace {
+
+constexpr double kPi = 3.141592653589793;  // M_PI
+constexpr int kMonoBlock = 256;
+// input angles accepted by the host entry points: every plaquette sum is
+// then within 4000 of 0 and its reduction takes at most 640 steps of 2 pi
+constexpr double kMaxAngle = 1000.0;
+constexpr int kMaxWrap = 1024;
+
+// lexicographic lattice of any extents (indexing.h:111-123)
+struct Lat {
+  int L[4];
+  uint32_t st[4];  // site strides
+  uint32_t V;
+};
+
+Lat make_lat(const int* d) {
+  Lat l;
+  uint32_t s = 1;
+  for (int mu = 0; mu < 4; mu++) {
+    l.L[mu] = d[mu];
+    l.st[mu] = s;
+    s *= (uint32_t)d[mu];
+  }
+  l.V = s;
+  return l;
+}
+
+__device__ __forceinline__ void coords_of(const Lat& l, uint32_t s, int* c) {
+  c[0] = (int)(s % (uint32_t)l.L[0]);
+  s /= (uint32_t)l.L[0];
+  c[1] = (int)(s % (uint32_t)l.L[1]);
+  s /= (uint32_t)l.L[1];
+  c[2] = (int)(s % (uint32_t)l.L[2]);
+  c[3] = (int)(s / (uint32_t)l.L[2]);
+}
+// s + mu, periodic (DataPatternLexicographical::move_forward, indexing.h:97-99)
+__device__ __forceinline__ uint32_t fwd(const Lat& l, uint32_t s, const int* c, int mu) {
+  return c[mu] + 1 == l.L[mu] ? s - (uint32_t)(l.L[mu] - 1) * l.st[mu] : s + l.st[mu];
+}
+// s + mu + nu
+__device__ __forceinline__ uint32_t fwd2(const Lat& l, uint32_t s, const int* c, int mu, int nu) {
+  int d[4] = {c[0], c[1], c[2], c[3]};
+  const uint32_t a = fwd(l, s, d, mu);
+  d[mu] = d[mu] + 1 == l.L[mu] ? 0 : d[mu] + 1;
+  return fwd(l, a, d, nu);
+}
+
+// into [-pi, pi] as the reference's two while loops (monopoles.cpp:443-448)
+__device__ __forceinline__ double wrap_pi(double p) {
+  for (int i = 0; i < kMaxWrap && p > kPi; i++) p -= 2 * kPi;
+  for (int i = 0; i < kMaxWrap && p < -kPi; i++) p += 2 * kPi;
+  return p;
+}
+
+// ------------------------------------------------------------------ angles
+template <typename T>
+__global__ __launch_bounds__(kMonoBlock) void k_su2_angles(const T* __restrict__ U, Geom g,
+                                                          double* __restrict__ th) {
+  const uint32_t s = blockIdx.x * kMonoBlock + threadIdx.x;
+  if (s >= 2 * g.Vh) return;
+  int p;
+  uint32_t h;
+  lex_site(g, s, p, h);
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) {
+    const Q<T> q = ldq(link_ptr<4>(U, g, p, mu, h));
+    th[4 * (size_t)s + mu] = atan2((double)q.a3, (double)q.a0);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kMonoBlock) void k_su3_angles(const T* __restrict__ U, Geom g,
+                                                          double* __restrict__ th3) {
+  const uint32_t s = blockIdx.x * kMonoBlock + threadIdx.x;
+  if (s >= 2 * g.Vh) return;
+  int p;
+  uint32_t h;
+  lex_site(g, s, p, h);
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) {
+    const M2<T> o = ldm(link_ptr<12>(U, g, p, mu, h));
+    M2<double> m;
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+      m.r[k / 3][k % 3] = {(double)o.r[k / 3][k % 3].re, (double)o.r[k / 3][k % 3].im};
+    C<double> r2[3];
+    row2_of(m, r2);
+    double a[3];
+    a[0] = atan2(m.r[0][0].im, m.r[0][0].re);
+    a[1] = atan2(m.r[1][1].im, m.r[1][1].re);
+    a[2] = atan2(r2[2].im, r2[2].re);
+    double sum = a[0] + a[1];
+    sum = sum + a[2];
+    for (int i = 0; i < kMaxWrap && sum >= kPi; i++) sum -= 2 * kPi;
+    for (int i = 0; i < kMaxWrap && sum < -kPi; i++) sum += 2 * kPi;
+    double* o3 = th3 + 3 * (4 * (size_t)s + mu);
+#pragma unroll
+    for (int c = 0; c < 3; c++) o3[c] = a[c] - sum / 3;
+  }
+}
+
+// ------------------------------------------------------------------ plaquettes
+// plane index of (mu, nu), mu < nu, in the order of the reference's loops
+__device__ __forceinline__ int plane_of(int mu, int nu) {
+  return mu == 0 ? nu - 1 : (mu == 1 ? nu + 1 : 5);
+}
+
+// angle of the plaquette (mu, nu) at s from th[stride * (4 site + mu) + off]
+__device__ __forceinline__ double plaq_angle(const double* __restrict__ th, int stride, int off,
+                                             const Lat& l, uint32_t s, const int* c, int mu,
+                                             int nu) {
+  const uint32_t smu = fwd(l, s, c, mu), snu = fwd(l, s, c, nu);
+  double p = th[stride * (4 * (size_t)s + mu) + off];
+  p = p + th[stride * (4 * (size_t)smu + nu) + off];
+  p = p - th[stride * (4 * (size_t)snu + mu) + off];
+  p = p - th[stride * (4 * (size_t)s + nu) + off];
+  return wrap_pi(p);
+}
+
+__global__ __launch_bounds__(kMonoBlock) void k_u1_plaquettes(const double* __restrict__ th,
+                                                             Lat l, double* __restrict__ P) {
+  const uint32_t s = blockIdx.x * kMonoBlock + threadIdx.x;
+  if (s >= l.V) return;
+  int c[4];
+  coords_of(l, s, c);
+#pragma unroll
+  for (int mu = 0; mu < 3; mu++)
+#pragma unroll
+    for (int nu = mu + 1; nu < 4; nu++)
+      P[(size_t)plane_of(mu, nu) * l.V + s] = plaq_angle(th, 1, 0, l, s, c, mu, nu);
+}
+
+// P3[colour][plane][site]
+__global__ __launch_bounds__(kMonoBlock) void k_su3_plaquettes(const double* __restrict__ th3,
+                                                              Lat l, double* __restrict__ P3) {
+  const uint32_t s = blockIdx.x * kMonoBlock + threadIdx.x;
+  if (s >= l.V) return;
+  int c[4];
+  coords_of(l, s, c);
+  const size_t n = (size_t)6 * l.V;
+#pragma unroll
+  for (int mu = 0; mu < 3; mu++)
+#pragma unroll
+    for (int nu = mu + 1; nu < 4; nu++) {
+      double q[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) q[k] = plaq_angle(th3, 3, k, l, s, c, mu, nu);
+      double sum = q[0] + q[1];
+      sum = sum + q[2];
+      if (sum >= kPi) {
+        int at = 0;
+        double ext = q[0];
+        if (q[1] >= ext) {
+          ext = q[1];
+          at = 1;
+        }
+        if (q[2] >= ext) at = 2;
+        q[at] -= 2 * kPi;
+      }
+      if (sum <= -kPi) {
+        int at = 0;
+        double ext = q[0];
+        if (q[1] <= ext) {
+          ext = q[1];
+          at = 1;
+        }
+        if (q[2] <= ext) at = 2;
+        q[at] += 2 * kPi;
+      }
+      const size_t o = (size_t)plane_of(mu, nu) * l.V + s;
+#pragma unroll
+      for (int k = 0; k < 3; k++) P3[k * n + o] = q[k];
+    }
+}
+
+// ------------------------------------------------------------------ currents
+__global__ __launch_bounds__(kMonoBlock) void k_monopole_currents(const double* __restrict__ P,
+                                                                 Lat l, double* __restrict__ J) {
+  const uint32_t s = blockIdx.x * kMonoBlock + threadIdx.x;
+  if (s >= l.V) return;
+  int c[4];
+  coords_of(l, s, c);
+  const size_t V = l.V;
+  auto pl = [&](int plane, uint32_t site) { return P[plane * V + site]; };
+  // t: planes yz (3), xz (1), xy (0), at x + t
+  uint32_t b = fwd(l, s, c, 3);
+  const double j3 = pl(3, fwd2(l, s, c, 3, 0)) - pl(3, b) - (pl(1, fwd2(l, s, c, 3, 1)) - pl(1, b)) +
+                    pl(0, fwd2(l, s, c, 3, 2)) - pl(0, b);
+  // x: planes yz (3), zt (5), yt (4), at x + x
+  b = fwd(l, s, c, 0);
+  const double j0 = -(pl(3, fwd2(l, s, c, 0, 3)) - pl(3, b)) - (pl(5, fwd2(l, s, c, 0, 1)) - pl(5, b)) +
+                    pl(4, fwd2(l, s, c, 0, 2)) - pl(4, b);
+  // y: planes xz (1), zt (5), xt (2), at x + y
+  b = fwd(l, s, c, 1);
+  const double j1 = pl(1, fwd2(l, s, c, 1, 3)) - pl(1, b) + pl(5, fwd2(l, s, c, 1, 0)) - pl(5, b) -
+                    (pl(2, fwd2(l, s, c, 1, 2)) - pl(2, b));
+  // z: planes xy (0), yt (4), xt (2), at x + z
+  b = fwd(l, s, c, 2);
+  const double j2 = -(pl(0, fwd2(l, s, c, 2, 3)) - pl(0, b)) - (pl(4, fwd2(l, s, c, 2, 0)) - pl(4, b)) +
+                    pl(2, fwd2(l, s, c, 2, 1)) - pl(2, b);
+  double* o = J + 4 * (size_t)s;
+  o[0] = j0 / 2 / kPi;
+  o[1] = j1 / 2 / kPi;
+  o[2] = j2 / 2 / kPi;
+  o[3] = j3 / 2 / kPi;
+}
+
+int grid_of(uint32_t n) { return (int)((n + kMonoBlock - 1) / kMonoBlock); }
+
+// currents from angles already on the device: th (SU(2): [4V]) or th3
+// (SU(3): [4V][3]) -> J ([4V] or [3][4V]); P: 6V (SU(2)) or 18V scratch
+void launch_currents(int group, const double* th, const Lat& l, double* P, double* J,
+                     hipStream_t st) {
+  const int nb = grid_of(l.V);
+  if (group == MAG_SU2) {
+    hipLaunchKernelGGL(k_u1_plaquettes, dim3(nb), dim3(kMonoBlock), 0, st, th, l, P);
+    hipLaunchKernelGGL(k_monopole_currents, dim3(nb), dim3(kMonoBlock), 0, st, P, l, J);
+  } else {
+    hipLaunchKernelGGL(k_su3_plaquettes, dim3(nb), dim3(kMonoBlock), 0, st, th, l, P);
+    for (int k = 0; k < 3; k++)
+      hipLaunchKernelGGL(k_monopole_currents, dim3(nb), dim3(kMonoBlock), 0, st,
+                         P + (size_t)k * 6 * l.V, l, J + (size_t)k * 4 * l.V);
+  }
+}
+
+// device buffers of one call, freed on every exit path
+struct Bufs {
+  double* p[4] = {nullptr, nullptr, nullptr, nullptr};
+  ~Bufs() {
+    for (double* q : p)
+      if (q) (void)hipFree(q);
+  }
+};
+
+int check_dims(const int* dims) {
+  if (!dims) return fail(MAG_ERR_ARG, "NULL dims");
+  uint64_t v = 1;
+  for (int mu = 0; mu < 4; mu++) {
+    if (dims[mu] < 2) return fail(MAG_ERR_DIMS, "lattice extent %d in direction %d (need >= 2)", dims[mu], mu);
+    v *= (uint64_t)dims[mu];
+  }
+  if (v * 18 >= (1ull << 32)) return fail(MAG_ERR_DIMS, "lattice too large");
+  return MAG_OK;
+}
+
+// host angles -> device, after checking that every angle is finite and
+// within kMaxAngle
+int upload_angles(const double* a, size_t n, double** d, hipStream_t st) {
+  for (size_t i = 0; i < n; i++)
+    if (!(std::fabs(a[i]) <= kMaxAngle))
+      return fail(MAG_ERR_ARG, "angle %zu is %g: angles must be finite with |angle| <= %g", i, a[i],
+                  kMaxAngle);
+  HIPCHK(hipMalloc((void**)d, n * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(*d, a, n * sizeof(double), hipMemcpyHostToDevice, st));
+  return MAG_OK;
+}
+
+int host_currents(int group, const int* dims, const double* angles, double* J) {
+  CHK(check_dims(dims));
+  if (!angles || !J) return fail(MAG_ERR_ARG, "NULL argument");
+  const Lat l = make_lat(dims);
+  const int ncol = group == MAG_SU2 ? 1 : 3;
+  Bufs b;
+  hipStream_t st = nullptr;  // the null stream: one synchronous call
+  CHK(upload_angles(angles, (size_t)4 * l.V * ncol, &b.p[0], st));
+  HIPCHK(hipMalloc((void**)&b.p[1], (size_t)6 * l.V * ncol * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&b.p[2], (size_t)4 * l.V * ncol * sizeof(double)));
+  launch_currents(group, b.p[0], l, b.p[1], b.p[2], st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(J, b.p[2], (size_t)4 * l.V * ncol * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return MAG_OK;
+}
+
+// the job's angles into th (device), FP64 links of copy c taken from the
+// job (link form) or formed in tmp (site form, j->nreal doubles)
+int job_angles(mag_job* j, int c, double* th, double* tmp) {
+  const bool su2 = j->prm.group == MAG_SU2;
+  const int nb = grid_of((uint32_t)j->V);
+  if (j->spin) {
+    job_copy_links64(j, c, tmp);
+    if (su2) hipLaunchKernelGGL((k_su2_angles<double>), dim3(nb), dim3(kMonoBlock), 0, j->st, tmp, j->g, th);
+    else hipLaunchKernelGGL((k_su3_angles<double>), dim3(nb), dim3(kMonoBlock), 0, j->st, tmp, j->g, th);
+  } else if (job_mixed32(j)) {
+    if (su2) hipLaunchKernelGGL((k_su2_angles<float>), dim3(nb), dim3(kMonoBlock), 0, j->st, j->W32, j->g, th);
+    else hipLaunchKernelGGL((k_su3_angles<float>), dim3(nb), dim3(kMonoBlock), 0, j->st, j->W32, j->g, th);
+  } else {
+    if (su2) hipLaunchKernelGGL((k_su2_angles<double>), dim3(nb), dim3(kMonoBlock), 0, j->st, j->U64, j->g, th);
+    else hipLaunchKernelGGL((k_su3_angles<double>), dim3(nb), dim3(kMonoBlock), 0, j->st, j->U64, j->g, th);
+  }
+  HIPCHK(hipGetLastError());
+  return MAG_OK;
+}
+
+int check_job(mag_job* j, int c) {
+  if (!j) return fail(MAG_ERR_ARG, "NULL job");
+  if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  if (c < 0 || c >= j->ncopies) return fail(MAG_ERR_ARG, "copy %d out of range 0..%d", c, j->ncopies - 1);
+  HIPCHK(hipSetDevice(j->prm.device));
+  return MAG_OK;
+}
+
+int lat_dims(const mag_job* j, int* d) {
+  d[0] = j->g.L0;
+  d[1] = j->g.L1;
+  d[2] = j->g.L2;
+  d[3] = j->g.L3;
+  return MAG_OK;
+}
+
+}  // namespace
+}  // namespace mag
+
+using namespace mag;
+
+extern "C" {
+
+int mag_job_abelian_angles(mag_job* j, int copy, double* angles) {
+  CHK(check_job(j, copy));
+  if (!angles) return fail(MAG_ERR_ARG, "NULL angles");
+  const size_t n = (size_t)4 * j->V * (j->prm.group == MAG_SU2 ? 1 : 3);
+  Bufs b;
+  HIPCHK(hipMalloc((void**)&b.p[0], n * sizeof(double)));
+  if (j->spin) HIPCHK(hipMalloc((void**)&b.p[1], j->nreal * sizeof(double)));
+  CHK(job_angles(j, copy, b.p[0], b.p[1]));
+  HIPCHK(hipMemcpyAsync(angles, b.p[0], n * sizeof(double), hipMemcpyDeviceToHost, j->st));
+  HIPCHK(hipStreamSynchronize(j->st));
+  return MAG_OK;
+}
+
+int mag_job_monopole_currents(mag_job* j, int copy, double* J) {
+  CHK(check_job(j, copy));
+  if (!J) return fail(MAG_ERR_ARG, "NULL J");
+  int d[4];
+  lat_dims(j, d);
+  const Lat l = make_lat(d);
+  const int ncol = j->prm.group == MAG_SU2 ? 1 : 3;
+  Bufs b;
+  HIPCHK(hipMalloc((void**)&b.p[0], (size_t)4 * l.V * ncol * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&b.p[1], (size_t)6 * l.V * ncol * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&b.p[2], (size_t)4 * l.V * ncol * sizeof(double)));
+  if (j->spin) HIPCHK(hipMalloc((void**)&b.p[3], j->nreal * sizeof(double)));
+  CHK(job_angles(j, copy, b.p[0], b.p[3]));
+  launch_currents(j->prm.group, b.p[0], l, b.p[1], b.p[2], j->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(J, b.p[2], (size_t)4 * l.V * ncol * sizeof(double), hipMemcpyDeviceToHost, j->st));
+  HIPCHK(hipStreamSynchronize(j->st));
+  return MAG_OK;
+}
+
+int mag_job_monopole_timed(mag_job* j, int reps, double* ms) {
+  CHK(check_job(j, 0));
+  if (reps < 1 || !ms) return fail(MAG_ERR_ARG, "bad argument");
+  int d[4];
+  lat_dims(j, d);
+  const Lat l = make_lat(d);
+  const int ncol = j->prm.group == MAG_SU2 ? 1 : 3;
+  Bufs b;
+  HIPCHK(hipMalloc((void**)&b.p[0], (size_t)4 * l.V * ncol * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&b.p[1], (size_t)6 * l.V * ncol * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&b.p[2], (size_t)4 * l.V * ncol * sizeof(double)));
+  if (j->spin) HIPCHK(hipMalloc((void**)&b.p[3], j->nreal * sizeof(double)));
+  CHK(job_angles(j, 0, b.p[0], b.p[3]));  // warm-up
+  launch_currents(j->prm.group, b.p[0], l, b.p[1], b.p[2], j->st);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  (void)hipEventRecord(e0, j->st);
+  for (int i = 0; i < reps; i++) {
+    (void)job_angles(j, 0, b.p[0], b.p[3]);
+    launch_currents(j->prm.group, b.p[0], l, b.p[1], b.p[2], j->st);
+  }
+  (void)hipEventRecord(e1, j->st);
+  hipError_t e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(MAG_ERR_HIP, "timing failed: %s", hipGetErrorString(e));
+  *ms = (double)t / reps;
+  return MAG_OK;
+}
+
+int mag_u1_monopole_currents(const int* dims, const double* angles, double* J) {
+  return host_currents(MAG_SU2, dims, angles, J);
+}
+
+int mag_su3_monopole_currents(const int* dims, const double* angles3, double* J3) {
+  return host_currents(MAG_SU3, dims, angles3, J3);
+}
+
+}  // extern "C"

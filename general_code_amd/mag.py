@@ -153,6 +153,11 @@ SIGNATURES = {
     "mag_batch_replay_sweeps": (ctypes.c_int, [_vp, _ip]),
     "mag_batch_replays": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mag_batch_poll": (ctypes.c_int, [_vp, _ip, _ip]),
+    "mag_job_abelian_angles": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "mag_job_monopole_currents": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "mag_job_monopole_timed": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "mag_u1_monopole_currents": (ctypes.c_int, [_ip, _dp, _dp]),
+    "mag_su3_monopole_currents": (ctypes.c_int, [_ip, _dp, _dp]),
     "mag_batch_step": (ctypes.c_int, [_vp, _ip, _ip]),
     "mag_batch_run": (ctypes.c_int, [_vp]),
     "mag_batch_replays_timed": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
@@ -456,6 +461,28 @@ class MagJob:
         _check(lib().mag_job_functional(self._h, ctypes.byref(F)))
         return F.value
 
+    def abelian_angles(self, copy: int = 0) -> np.ndarray:
+        """Abelian projection of the links (mag_job_abelian_angles): SU(2)
+        [4V] angles, SU(3) [4V, 3] su3_angles, reference link order."""
+        V = self.params.volume
+        out = np.empty(4 * V if self.params.group == SU2 else (4 * V, 3), dtype=np.float64)
+        _check(lib().mag_job_abelian_angles(self._h, int(copy), _dptr(out)))
+        return out
+
+    def monopole_currents(self, copy: int = 0) -> np.ndarray:
+        """Monopole currents of the links (mag_job_monopole_currents): SU(2)
+        [4V], SU(3) [3, 4V]."""
+        V = self.params.volume
+        out = np.empty(4 * V if self.params.group == SU2 else (3, 4 * V), dtype=np.float64)
+        _check(lib().mag_job_monopole_currents(self._h, int(copy), _dptr(out)))
+        return out
+
+    def monopole_timed(self, reps: int) -> float:
+        """Mean device ms of angles + plaquettes + currents (copy 0)."""
+        ms = ctypes.c_double()
+        _check(lib().mag_job_monopole_timed(self._h, int(reps), ctypes.byref(ms)))
+        return ms.value
+
     def plaquette(self) -> float:
         P = ctypes.c_double()
         _check(lib().mag_job_plaquette(self._h, ctypes.byref(P)))
@@ -605,6 +632,25 @@ def gauge_transform_spins(dims, links, spins) -> np.ndarray:
     out = np.array(links, dtype=np.float64, copy=True)
     _check(lib().mag_su2_gauge_transform_spins(_dims(dims), _dptr(out),
                                                _dptr(np.ascontiguousarray(spins, np.float64))))
+    return out
+
+
+# monopoles -----------------------------------------------------------------
+def u1_monopole_currents(dims, angles) -> np.ndarray:
+    """calculate_current of an abelian configuration (monopoles.cpp:669-687):
+    angles [4V] in reference link order -> J [4V]."""
+    a = np.ascontiguousarray(angles, dtype=np.float64).reshape(-1)
+    out = np.empty_like(a)
+    _check(lib().mag_u1_monopole_currents(_dims(dims), _dptr(a), _dptr(out)))
+    return out
+
+
+def su3_monopole_currents(dims, angles3) -> np.ndarray:
+    """calculate_current of an su3_angles configuration (monopoles.cpp:646-667):
+    angles [4V, 3] -> J [3, 4V]."""
+    a = np.ascontiguousarray(angles3, dtype=np.float64)
+    out = np.empty((3, a.size // 3), dtype=np.float64)
+    _check(lib().mag_su3_monopole_currents(_dims(dims), _dptr(a), _dptr(out)))
     return out
 
 

@@ -25,6 +25,12 @@
  *   make_simulated_annealing ....... mag.h:119-122    -> mag_su2_simulated_annealing
  *   generate_spins_uniform ......... mag.h:179-199    -> mag_su2_generate_spins_uniform
  *   plaket ......................... plaket.h:84-101  -> mag_plaquette
+ *   calculate_current (abelian) .... monopoles.cpp:669-687 -> mag_u1_monopole_currents,
+ *                                    mag_job_monopole_currents
+ *   calculate_current (su3_angles) . monopoles.cpp:646-667 -> mag_su3_monopole_currents,
+ *                                    mag_job_monopole_currents
+ *   MatrixConverter<su2, abelian>,
+ *   MatrixConverter<su3, su3_angles> matrix.h:1059-1062, 1082-1100 -> mag_job_abelian_angles
  *   (none: the reference has no link-form checkerboard MAG, no SU(3) MAG
  *    fixing, no mixed precision, batching or serving)  -> mag_job_* / mag_batch_*
  */
@@ -254,6 +260,35 @@ int mag_batch_run(mag_batch *b);
 /* Measurement: n replays between HIP events on the batch stream. */
 int mag_batch_replays_timed(mag_batch *b, int n, double *ms);
 int mag_batch_stream(mag_batch *b, void **stream);
+
+/* ------------------------------------------------- abelian projection, monopoles
+ * What the reference's monopole apps compute from MAG-fixed links
+ * (apps/monopoles_su2/monopoles_su2.cpp:110, apps/monopoles_su3/
+ * monopoles_su3.cpp:103-120), on the device.  Layouts are the reference's:
+ *   SU(2) angles  [4V]     phi of link 4*site+mu
+ *   SU(3) angles  [4V][3]  the su3_angles of link 4*site+mu
+ *   SU(2) J       [4V]     J of dual link 4*site+mu
+ *   SU(3) J       [3][4V]  per colour
+ * The currents are integers up to rounding (J = n / 1 with |J - n| ~ 1e-15).
+ *
+ * Angles of the job's links (copy `copy` of a site-form job; 0 for the link
+ * form): SU(2) atan2(a3, a0) (MatrixConverter<su2, abelian>,
+ * matrix.h:1059-1062); SU(3) the diagonal phases less a third of their sum
+ * taken into [-pi, pi) (MatrixConverter<su3, su3_angles>, matrix.h:1082-1100). */
+int mag_job_abelian_angles(mag_job *job, int copy, double *angles);
+/* Monopole currents of the job's links: calculate_current of the abelian
+ * (monopoles.cpp:669-687) / su3_angles (monopoles.cpp:646-667, with
+ * make_monopole_plakets 844-884) projection, links kept on the device. */
+int mag_job_monopole_currents(mag_job *job, int copy, double *J);
+/* Measurement: mean device time of angles + plaquettes + currents over reps
+ * runs (copy 0), no host transfer. */
+int mag_job_monopole_timed(mag_job *job, int reps, double *ms);
+/* Currents from host angles (the monopole apps read abelian configurations
+ * from files): calculate_current(LatticeData<abelian>) (monopoles.cpp:669-687)
+ * and calculate_current(LatticeData<su3_angles>) (monopoles.cpp:646-667).
+ * Extents >= 2 (odd allowed); every angle finite with |angle| <= 1000. */
+int mag_u1_monopole_currents(const int *dims, const double *angles, double *J);
+int mag_su3_monopole_currents(const int *dims, const double *angles3, double *J3);
 
 /* ------------------------------------------------------------ serving
  * No counterpart in the reference (one configuration per process,

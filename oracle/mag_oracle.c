@@ -30,6 +30,14 @@
  *   heat bath .............. statistical: distribution of s.h_hat against
  *                            exp(|h| x / T) on [-1,1] (parity unpinned
  *                            stream-wise: the reference RNG is time-seeded)
+ *   abelian projection ..... closed forms (exp(i phi s3) -> phi; SU(3)
+ *                            diagonal phases summing to 0 mod 2 pi)
+ *   monopole currents ...... integer-valued, conserved on the dual lattice,
+ *                            invariant under U(1) (SU(3): U(1)^2) gauge
+ *                            transformations of the angles, equal to the
+ *                            Levi-Civita form -eps_{mu nu rho sigma}
+ *                            d_nu P_{rho sigma}(x + mu) / 4 pi (numpy),
+ *                            colour sum zero for SU(3)
  */
 #include "mag_oracle.h"
 
@@ -1251,4 +1259,127 @@ double or_su3_wilson_loop(const double *links18, const int *dims, int R,
     }
   }
   return tot / (double)(g.V * 3);
+}
+
+/* ------------------------------------------------------------------ */
+/* abelian projection and monopole currents                            */
+/* ------------------------------------------------------------------ */
+/* plane index of (mu, nu), mu < nu: (0,1)=0 (0,2)=1 (0,3)=2 (1,2)=3 (1,3)=4
+ * (2,3)=5, the order of the mu/nu loops of monopoles.cpp:415-457 */
+static int plane_of(int mu, int nu) {
+  static const int P[4][4] = {{-1, 0, 1, 2}, {0, -1, 3, 4}, {1, 3, -1, 5}, {2, 4, 5, -1}};
+  return P[mu][nu];
+}
+
+/* MatrixConverter<su2, abelian>: phi = atan2(a3, a0) (matrix.h:1059-1062) */
+void or_su2_abelian_angles(const double *links, const int *dims, double *angles) {
+  geom_t g = geom(dims);
+  for (int64_t l = 0; l < 4 * g.V; l++) angles[l] = atan2(links[4 * l + 3], links[4 * l]);
+}
+
+/* MatrixConverter<su3, su3_angles> (matrix.h:1082-1100): the phases of the
+ * diagonal, less a third of their sum taken into [-pi, pi).  Output [4V][3]
+ * (the su3_angles element order of LatticeData) */
+void or_su3_abelian_angles(const double *links18, const int *dims, double *angles3) {
+  geom_t g = geom(dims);
+  for (int64_t l = 0; l < 4 * g.V; l++) {
+    const double *m = links18 + 18 * l;
+    double a[3], sum = 0;
+    for (int c = 0; c < 3; c++) {
+      a[c] = atan2(m[2 * (4 * c) + 1], m[2 * (4 * c)]);
+      sum += a[c];
+    }
+    while (sum >= M_PI) sum -= 2 * M_PI;
+    while (sum < -M_PI) sum += 2 * M_PI;
+    for (int c = 0; c < 3; c++) angles3[3 * l + c] = a[c] - sum / 3;
+  }
+}
+
+/* plaquette angle of plane (mu,nu) at every site, taken into [-pi, pi]:
+ * phi_mu(x) + phi_nu(x+mu) - phi_mu(x+nu) - phi_nu(x), in that order
+ * (monopoles.cpp:415-457, the abelian operator* and ^ of matrix.h:194-199);
+ * angles[stride * (4 * site + mu) + off], out[6][V] */
+static void u1_plaquettes(const geom_t *g, const double *angles, int stride, int off,
+                          double *out) {
+  for (int mu = 0; mu < 4; mu++)
+    for (int nu = mu + 1; nu < 4; nu++) {
+      double *o = out + (int64_t)plane_of(mu, nu) * g->V;
+      for (int64_t s = 0; s < g->V; s++) {
+        int c[4];
+        coords(g, s, c);
+        const int64_t smu = nb(g, s, c, mu, +1), snu = nb(g, s, c, nu, +1);
+        double p = angles[stride * (4 * s + mu) + off];
+        p = p + angles[stride * (4 * smu + nu) + off];
+        p = p - angles[stride * (4 * snu + mu) + off];
+        p = p - angles[stride * (4 * s + nu) + off];
+        while (p > M_PI) p -= 2 * M_PI;
+        while (p < -M_PI) p += 2 * M_PI;
+        o[s] = p;
+      }
+    }
+}
+
+void or_u1_monopole_plaquettes(const double *angles, const int *dims, double *out) {
+  geom_t g = geom(dims);
+  u1_plaquettes(&g, angles, 1, 0, out);
+}
+
+/* SU(3): per colour as above (monopoles.cpp:368-413), then the sum of the
+ * three colours brought into (-pi, pi) by moving 2 pi off the largest
+ * (sum >= pi) or onto the smallest (sum <= -pi) colour, later colours
+ * winning ties (monopoles.cpp:844-884).  out[3][6][V] */
+void or_su3_monopole_plaquettes(const double *angles3, const int *dims, double *out) {
+  geom_t g = geom(dims);
+  const int64_t n = 6 * g.V;
+  for (int c = 0; c < 3; c++) u1_plaquettes(&g, angles3, 3, c, out + c * n);
+  for (int64_t k = 0; k < n; k++) {
+    double *p0 = out + k, *p1 = out + n + k, *p2 = out + 2 * n + k;
+    const double sum = *p0 + *p1 + *p2;
+    if (sum >= M_PI) {
+      double *ext = p0;
+      if (*p1 >= *ext) ext = p1;
+      if (*p2 >= *ext) ext = p2;
+      *ext -= 2 * M_PI;
+    }
+    if (sum <= -M_PI) {
+      double *ext = p0;
+      if (*p1 <= *ext) ext = p1;
+      if (*p2 <= *ext) ext = p2;
+      *ext += 2 * M_PI;
+    }
+  }
+}
+
+/* monopole current on the link (x, mu) of the dual lattice from the six
+ * plaquette fields P[6][V] (get_current, monopoles.cpp:44-122): forward
+ * differences of the three planes orthogonal to mu, taken at x + mu, over
+ * 2 pi; J[4V] */
+void or_monopole_currents(const double *P, const int *dims, double *J) {
+  geom_t g = geom(dims);
+  const int64_t V = g.V;
+  for (int64_t s = 0; s < V; s++) {
+    int64_t b[4], bx[4][4];
+    for (int mu = 0; mu < 4; mu++) {
+      b[mu] = shift_n(&g, s, mu, 1);
+      for (int nu = 0; nu < 4; nu++) bx[mu][nu] = shift_n(&g, b[mu], nu, 1);
+    }
+#define PL(pl, site) P[(int64_t)(pl) * V + (site)]
+    /* t: planes yz (3), xz (1), xy (0) */
+    double j3 = PL(3, bx[3][0]) - PL(3, b[3]) - (PL(1, bx[3][1]) - PL(1, b[3])) +
+                PL(0, bx[3][2]) - PL(0, b[3]);
+    /* x: planes yz (3), zt (5), yt (4) */
+    double j0 = -(PL(3, bx[0][3]) - PL(3, b[0])) - (PL(5, bx[0][1]) - PL(5, b[0])) +
+                PL(4, bx[0][2]) - PL(4, b[0]);
+    /* y: planes xz (1), zt (5), xt (2) */
+    double j1 = PL(1, bx[1][3]) - PL(1, b[1]) + PL(5, bx[1][0]) - PL(5, b[1]) -
+                (PL(2, bx[1][2]) - PL(2, b[1]));
+    /* z: planes xy (0), yt (4), xt (2) */
+    double j2 = -(PL(0, bx[2][3]) - PL(0, b[2])) - (PL(4, bx[2][0]) - PL(4, b[2])) +
+                PL(2, bx[2][1]) - PL(2, b[2]);
+#undef PL
+    J[4 * s + 0] = j0 / 2 / M_PI;
+    J[4 * s + 1] = j1 / 2 / M_PI;
+    J[4 * s + 2] = j2 / 2 / M_PI;
+    J[4 * s + 3] = j3 / 2 / M_PI;
+  }
 }

@@ -109,6 +109,13 @@ double or_su3_plaquette(const double *links18, const int *dims);
 double or_su2_wilson_loop(const double *links, const int *dims, int R, int T); /* wilson_loops.h:666-754,1049-1073 */
 double or_su3_wilson_loop(const double *links18, const int *dims, int R, int T);
 
+/* ---------------- abelian projection, monopoles (monopoles.cpp) ---------- */
+void or_su2_abelian_angles(const double *links, const int *dims, double *angles);  /* matrix.h:1059-1062 */
+void or_su3_abelian_angles(const double *links18, const int *dims, double *angles3); /* matrix.h:1082-1100, [4V][3] */
+void or_u1_monopole_plaquettes(const double *angles, const int *dims, double *out);  /* monopoles.cpp:415-457, [6][V] */
+void or_su3_monopole_plaquettes(const double *angles3, const int *dims, double *out); /* monopoles.cpp:368-413,844-884, [3][6][V] */
+void or_monopole_currents(const double *P, const int *dims, double *J);              /* monopoles.cpp:44-122, [4V] */
+
 /* ---------------- synthetic configurations (build-defined) ------------- */
 void or_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 void or_gen_su2_config(double *links, const int *dims, double eps, uint64_t seed);

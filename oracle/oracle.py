@@ -80,6 +80,11 @@ def lib():
             "or_su3_plaquette": (ctypes.c_double, [_dp, _ip]),
             "or_su2_wilson_loop": (ctypes.c_double, [_dp, _ip, ctypes.c_int, ctypes.c_int]),
             "or_su3_wilson_loop": (ctypes.c_double, [_dp, _ip, ctypes.c_int, ctypes.c_int]),
+            "or_su2_abelian_angles": (None, [_dp, _ip, _dp]),
+            "or_su3_abelian_angles": (None, [_dp, _ip, _dp]),
+            "or_u1_monopole_plaquettes": (None, [_dp, _ip, _dp]),
+            "or_su3_monopole_plaquettes": (None, [_dp, _ip, _dp]),
+            "or_monopole_currents": (None, [_dp, _ip, _dp]),
             "or_philox4x32_10": (None, [_u32p, _u32p, _u32p]),
             "or_gen_su2_config": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64]),
             "or_gen_su3_config": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64]),
@@ -319,6 +324,42 @@ def su3_wilson_loop(links, dims, R, T):
 
 
 # ---------------------------------------------------------------- generators
+def su2_abelian_angles(links, dims):
+    """phi = atan2(a3, a0) per link, [4V] (matrix.h:1059-1062)."""
+    out = np.empty(4 * volume(dims))
+    lib().or_su2_abelian_angles(_d(links), _dims(dims), _d(out))
+    return out
+
+
+def su3_abelian_angles(links18, dims):
+    """su3_angles of every link, [4V, 3] (matrix.h:1082-1100)."""
+    out = np.empty((4 * volume(dims), 3))
+    lib().or_su3_abelian_angles(_d(links18), _dims(dims), _d(out))
+    return out
+
+
+def u1_monopole_plaquettes(angles, dims):
+    """[6, V] plaquette angles in [-pi, pi] (monopoles.cpp:415-457)."""
+    out = np.empty((6, volume(dims)))
+    lib().or_u1_monopole_plaquettes(_d(angles), _dims(dims), _d(out))
+    return out
+
+
+def su3_monopole_plaquettes(angles3, dims):
+    """[3, 6, V] per-colour plaquette angles with the colour-sum rule
+    (monopoles.cpp:368-413, 844-884)."""
+    out = np.empty((3, 6, volume(dims)))
+    lib().or_su3_monopole_plaquettes(_d(angles3), _dims(dims), _d(out))
+    return out
+
+
+def monopole_currents(plaq, dims):
+    """[4V] currents J[4 site + mu] from [6, V] plaquettes (monopoles.cpp:44-122)."""
+    out = np.empty(4 * volume(dims))
+    lib().or_monopole_currents(_d(plaq), _dims(dims), _d(out))
+    return out
+
+
 def philox(ctr, key):
     c = (ctypes.c_uint32 * 4)(*ctr)
     k = (ctypes.c_uint32 * 2)(*key)
