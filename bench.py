@@ -100,6 +100,32 @@ def measure(G, group, dims, precision, eps, omega, steps, warmup, seed=2024):
     }
 
 
+# algorithmic bytes per site of abelian projection + plaquettes + currents:
+# SU(2) angles 4 links x 4 doubles read + 4 angles written, plaquettes 4
+# angles read + 6 written, currents 6 read + 4 written; SU(3) 12-real links
+# and three colours
+MONO_B_SITE = {2: (16 + 4) * 8 + (4 + 6) * 8 + (6 + 4) * 8,
+               3: (48 + 12) * 8 + (12 + 18) * 8 + (18 + 12) * 8}
+
+
+def monopole_measure(G, group, dims, eps, reps=20):
+    """Device time of the abelian projection and monopole currents of a
+    job's links (mag_job_monopole_timed: angles + plaquettes + currents,
+    no host transfer) and its HBM rate against MONO_B_SITE."""
+    V = 1
+    for d in dims:
+        V *= d
+    with G.MagJob(group=group, dims=dims) as job:
+        job.generate(eps, 5)
+        ms = job.monopole_timed(reps)
+        nmono = int(abs(job.monopole_currents().round()).sum())
+    gbs = V * MONO_B_SITE[group] / (ms * 1e-3) / 1e9
+    return {"ms": ms, "sites_per_s": V / (ms * 1e-3), "b_site": MONO_B_SITE[group],
+            "achieved_GBs": round(gbs, 1), "roofline_frac": round(gbs / PEAK_HBM_GBS, 4),
+            "current_links": nmono,
+            "workload": f"generated links eps {eps} (unfixed), 3 kernels per call"}
+
+
 def _median_rate(fn, V, seconds, min_runs=5):
     """site-updates/s of fn() (one sweep): median over runs after one
     warm-up, for about `seconds` (at least min_runs runs)."""
@@ -200,25 +226,31 @@ def run_plain(G, specs, max_sweeps):
 
 
 # FP64 warm-up sweeps before the mixed phase: the spin form's FP32-rounded
-# links are a fixed perturbation of the whole problem and need a longer
-# FP64 start on large lattices to keep the FP64 run's maximum (48^4: 100
-# sweeps were not enough for one seed, 300 were)
-SU2_WARMUP, SU3_WARMUP = 300, 100
+# links are a fixed perturbation of the whole problem, and a trajectory
+# that passes near a saddle during the FP32 phase can end on a neighbouring
+# maximum (another Gribov copy).  Measured on the batched and served
+# workloads: 100 sweeps moved one 48^4 request, 300 moved one 24^4 copy of
+# the batch, 600 moved none; FP64 spin sweeps throughout never move any
+# and cost 7% more time on the batch.
+SU2_WARMUP, SU3_WARMUP = 600, 100
 
 
 def run_batched(G, specs, max_sweeps, su3_precision, su3_warmup, su3_site=False,
-                su2_warmup=SU2_WARMUP):
+                su2_warmup=SU2_WARMUP, su2_precision=None):
     """All configurations in one graph-captured batch: each SU(2)
     configuration is one spin-form copy set (links read once for all its
     copies; mixed: FP32-rounded links after the FP64 warm-up, spins and
     arithmetic in FP64), each SU(3) copy a link-form job (mixed: FP32 sweeps
-    after the warm-up); `su3_precision` applies to both."""
+    after the warm-up); `su3_precision` applies to both unless
+    `su2_precision` is given."""
+    su2_precision = su3_precision if su2_precision is None else su2_precision
     jobs = []
     for sp in specs:
         if sp["group"] == 2:
             j = G.MagJob(group=2, dims=sp["dims"], omega=sp["omega"], tol=sp["tol"],
                          max_sweeps=max_sweeps, form=G.FORM_SPIN, copies=sp["copies"],
-                         precision=su3_precision, tol_mixed=sp["tol"], fp64_warmup=su2_warmup)
+                         precision=su2_precision, tol_mixed=sp["tol"],
+                         fp64_warmup=su2_warmup if su2_precision == G.MIXED else 0)
             j.generate(sp["eps"], sp["seed"])
             jobs.append((j, list(range(sp["copies"]))))
         elif su3_site:
@@ -413,6 +445,11 @@ def main():
                 "b_site": r["roofline"]["b_site"],
                 "launch_shape": r["shape"],
             }
+
+        for name, group, d, e in [("monopoles_su2_32^4", 2, dims, 1.5),
+                                  ("monopoles_su3_24^3x48", 3, (24, 24, 24, 48), 1.2)]:
+            print(f"[bench] {name}", file=sys.stderr, flush=True)
+            extra[name] = monopole_measure(G, group, d, e)
 
     batched = None
     if not args.no_batch:

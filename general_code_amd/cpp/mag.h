@@ -59,6 +59,14 @@ struct su3 {
   static constexpr int data_size = 18;
   double v[18] = {1, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 1, 0};
 };
+// abelian link r e^{i phi} (matrix.h:119-130) and the three U(1) angles of
+// an su3_angles link (matrix.h:579-585), the layouts calculate_current reads
+struct abelian {
+  double r = 1, phi = 0;
+};
+struct su3_angles {
+  std::array<double, 3> matrix{0, 0, 0};
+};
 
 // link index 4*site + mu (indexing.h:71-126)
 class DataPatternLexicographical {
@@ -279,9 +287,12 @@ double plaket(const Lattice& conf) {
 // The north-star path: checkerboard omega-overrelaxation on the links
 // (SU(3): Cabibbo-Marinari), FP64 or mixed precision, in place.  Returns
 // the number of sweeps; the functional history is in *history if given.
+// With `currents`, also the monopole currents of the fixed links'
+// abelian projection, computed before the links leave the device
+// (calculate_current's layout: J[4*site+mu], SU(3) J[colour][4*site+mu]).
 template <class Lattice>
 int fix_mag_links(Lattice& conf, double omega, double tol, int max_sweeps, int precision,
-                  std::vector<double>* history = nullptr) {
+                  std::vector<double>* history = nullptr, std::vector<double>* currents = nullptr) {
   mag_params p;
   mag_params_default(&p);
   constexpr bool is3 = sizeof(typename Lattice::matrix_type) == 18 * sizeof(double);
@@ -320,8 +331,41 @@ int fix_mag_links(Lattice& conf, double omega, double tol, int max_sweeps, int p
     int n = 0;
     check(mag_job_history(job, history->data(), nullptr, nullptr, sweeps, &n));
   }
+  if (currents) {
+    currents->assign(conf.array.size() * (is3 ? 3 : 1), 0.0);
+    check(mag_job_monopole_currents(job, 0, currents->data()));
+  }
   check(rc);
   return sweeps;
+}
+
+// ---------------------------------------------------------------- monopoles
+// calculate_current (monopoles.cpp:646-687) on the GPU.  The lattice holds
+// abelian links {r, phi} (16 bytes; the reference's
+// LatticeData<DataPatternLexicographical, abelian>) or su3_angles links (24
+// bytes); returns J[4*site + mu] (abelian) or J[colour][4*site + mu]
+// (su3_angles), as the reference does.  Angles must be finite with
+// |angle| <= 1000 (the reference's unbounded wrap loops are not kept).
+template <class Lattice>
+auto calculate_current(const Lattice& conf) {
+  using M = typename Lattice::matrix_type;
+  int d[4];
+  detail::dims_of(conf, d);
+  const size_t n = conf.array.size();
+  const double* in = reinterpret_cast<const double*>(conf.array.data());
+  if constexpr (sizeof(M) == 2 * sizeof(double)) {
+    std::vector<double> phi(n), J(n);
+    for (size_t l = 0; l < n; l++) phi[l] = in[2 * l + 1];
+    check(mag_u1_monopole_currents(d, phi.data(), J.data()));
+    return J;
+  } else {
+    static_assert(sizeof(M) == 3 * sizeof(double), "abelian (r, phi) or su3_angles links");
+    std::vector<double> J(3 * n);
+    check(mag_su3_monopole_currents(d, in, J.data()));
+    std::vector<std::vector<double>> out(3);
+    for (int c = 0; c < 3; c++) out[c].assign(J.begin() + c * n, J.begin() + (c + 1) * n);
+    return out;
+  }
 }
 
 // ---------------------------------------------------------------- file formats

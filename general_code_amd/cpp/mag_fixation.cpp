@@ -12,7 +12,12 @@
 //                       (--omega, --tol, --max_sweeps) instead of the spin
 //                       drivers
 //   --max_iter N        iteration cap of the maximisation loops
+//   --path_monopoles_output P
+//                       monopole currents of the fixed links' abelian
+//                       projection (calculate_current, monopoles.cpp:669-687)
+//                       as 4V doubles J[4*site + mu]
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <iostream>
 #include <sstream>
@@ -25,6 +30,7 @@ using namespace mag_amd;
 int main(int argc, char** argv) {
   std::string path_conf, conf_format = "lexicographical", file_precision = "double";
   std::string path_conf_output, path_spins_output, path_previous, path_functional_output;
+  std::string path_monopoles_output;
   int bytes_skip = 0;
   double T_step = 0.01, T_init = 2.5, T_final = 0.01;
   int OR_steps = 6, thermalization_steps = 50;
@@ -72,6 +78,7 @@ int main(int argc, char** argv) {
     else if (a == "--seed") seed = std::stoull(next());
     else if (a == "--synthetic_eps") synthetic_eps = std::stod(next());
     else if (a == "--link_form") std::istringstream(next()) >> link_form;
+    else if (a == "--path_monopoles_output") path_monopoles_output = next();
     else if (a == "--omega") omega = std::stod(next());
     else if (a == "--tol") tol = std::stod(next());
     else if (a == "--max_sweeps") max_sweeps = std::stoi(next());
@@ -103,9 +110,11 @@ int main(int argc, char** argv) {
     std::cout.precision(17);
     std::cout << "plaquette " << plaket(conf) << std::endl;
     const auto t0 = std::chrono::steady_clock::now();
+    std::vector<double> J;
     if (link_form) {
       std::vector<double> hist;
-      const int sweeps = fix_mag_links(conf, omega, tol, max_sweeps, MAG_PREC_F64, &hist);
+      const int sweeps = fix_mag_links(conf, omega, tol, max_sweeps, MAG_PREC_F64, &hist,
+                                       path_monopoles_output.empty() ? nullptr : &J);
       std::cout << "link-form sweeps " << sweeps << std::endl;
     } else {
       DataPatternLexicographical pattern(dims);
@@ -139,6 +148,22 @@ int main(int argc, char** argv) {
     std::cout << "final functional is " << MAG_functional_su2(conf) << std::endl;
     std::cout << "plaquette " << plaket(conf) << std::endl;
     if (!path_conf_output.empty()) write_links_lexicographical(conf, path_conf_output);
+    if (!path_monopoles_output.empty()) {
+      if (J.empty()) {
+        // spin path: the fixed links are on the host; project them as the
+        // monopole app reads them (MatrixConverter<su2, abelian>)
+        Data::LatticeData<DataPatternLexicographical, abelian> ab(dims);
+        for (size_t l = 0; l < conf.array.size(); l++)
+          ab.array[l] = abelian{1.0, std::atan2(conf.array[l].a3, conf.array[l].a0)};
+        J = calculate_current(ab);
+      }
+      std::ofstream f(path_monopoles_output, std::ios::binary);
+      f.write(reinterpret_cast<const char*>(J.data()), (std::streamsize)(J.size() * sizeof(double)));
+      if (!f) throw error(MAG_ERR_ARG, "cannot write " + path_monopoles_output);
+      long n = 0;
+      for (double v : J) n += (long)std::lround(std::fabs(v));
+      std::cout << "monopole current links " << n << std::endl;
+    }
   } catch (const error& e) {
     std::cerr << "mag_fixation: " << e.what() << " (status " << e.code << ")" << std::endl;
     return 1;

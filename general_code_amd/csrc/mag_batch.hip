@@ -65,6 +65,7 @@ struct ClassRun {
   size_t cap_out = 0;
   int nblocks = 0;   // half-sweep grid
   int nrblocks = 0;  // rebuild grid (FP32 link-form classes)
+  mag::Shape shape;  // link-form classes: launch shape of the half-sweep
   hipStream_t s = nullptr;
   hipEvent_t ev = nullptr;
   hipGraph_t graph = nullptr;
@@ -153,7 +154,7 @@ void drop_graph(ClassRun& t) {
 void launch_class_half(const mag_batch* b, int c, int p, hipStream_t st) {
   const ClassRun& t = b->cls[c];
   if (c < kSpinClass) {
-    mag::launch_half_batch(t.group, t.fp32, t.d, (int)t.h.size(), t.nblocks, p, st);
+    mag::launch_half_batch(t.group, t.fp32, t.shape, t.d, (int)t.h.size(), t.nblocks, p, st);
   } else if (c >= kSiteClass3) {
     mag::launch_su3_site_copies_batch(c == kSiteClass3 + 1, t.d, (int)t.h.size(), t.nblocks, p,
                                       st);
@@ -204,8 +205,15 @@ int refresh_class(mag_batch* b, int c) {
   t.cap_jobs = t.jobs;
   t.nblocks = 0;
   t.nrblocks = 0;
+  t.shape = mag::Shape{};
+  if (c < kSpinClass && !t.jobs.empty())
+    CHK(mag::batch_shape(t.group, t.fp32, t.jobs, &t.shape));  // autotuned; same results any shape
   for (mag_job* j : t.jobs) {
     BatchJob d = describe(j);
+    if (c < kSpinClass) {
+      const size_t tile = (size_t)t.shape.nt * t.shape.spt;
+      d.nb = (int)((mag::sweep_lanes(j->g, j->prm.group) + tile - 1) / tile);
+    }
     d.block0 = t.nblocks;
     d.rb0 = t.nrblocks;
     t.nblocks += d.nb;

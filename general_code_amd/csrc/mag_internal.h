@@ -164,8 +164,9 @@ struct BatchJob {
   int ncopies;
 };
 
-// batched half-sweep of one class (group, FP32 phase or FP64) table
-void launch_half_batch(int group, bool fp32, const BatchJob* jobs, int njobs,
+// batched half-sweep of one class (group, FP32 phase or FP64) table with
+// launch shape sh (block0 / nblocks_total in workgroups of that shape)
+void launch_half_batch(int group, bool fp32, Shape sh, const BatchJob* jobs, int njobs,
                        int nblocks_total, int parity, hipStream_t st);
 // batched spin-form (gauge-copy) half-sweep of a table of copy sets that
 // all hold ncopies copies

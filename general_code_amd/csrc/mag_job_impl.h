@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "mag_gpu.h"
 #include "mag_internal.h"
 
@@ -67,4 +69,8 @@ void job_copy_links64(mag_job* j, int c, double* soa);
 // launch shape of the job's link-form half-sweep (mag_tune.hip): cached per
 // (group, precision, device, lattice), tuned on first use
 int job_shape(const mag_job* j, bool fp32, Shape* out);
+// launch shape of a batched link-form half-sweep over `jobs` (mag_tune.hip):
+// tuned on scratch copies of the jobs through the batched kernel, cached
+// per (group, precision, device, log2 of the total lanes)
+int batch_shape(int group, bool fp32, const std::vector<mag_job*>& jobs, Shape* out);
 }  // namespace mag

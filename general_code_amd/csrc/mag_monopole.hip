@@ -15,10 +15,13 @@
 //   currents   J_mu(x) = forward differences at x + mu of the three
 //              plaquette planes orthogonal to mu, over 2 pi
 //              (get_current, monopoles.cpp:44-122)
-// Every kernel runs one thread per lexicographic site.  The plaquette and
-// current kernels do the reference's additions, subtractions and divisions
-// in the reference's order, so from the same angles they give the same
-// bits; the angles go through the device atan2.
+// Every kernel runs one thread per lexicographic site.  Between the kernels
+// the angles are planar, th[(colour * 4 + mu) * V + site], so that the
+// neighbour loads of a wave are contiguous; the reference's link-major
+// layouts ([4V], [4V][3]) are produced or read only at the boundary.  The
+// plaquette and current kernels do the reference's additions, subtractions
+// and divisions in the reference's order, so from the same angles they give
+// the same bits; the angles go through the device atan2.
 #include "mag_capi_util.h"
 #include "mag_gpu.h"
 #include "mag_internal.h"
@@ -93,10 +96,11 @@ __global__ __launch_bounds__(kMonoBlock) void k_su2_angles(const T* __restrict__
   int p;
   uint32_t h;
   lex_site(g, s, p, h);
+  const size_t V = 2 * (size_t)g.Vh;
 #pragma unroll
   for (int mu = 0; mu < 4; mu++) {
     const Q<T> q = ldq(link_ptr<4>(U, g, p, mu, h));
-    th[4 * (size_t)s + mu] = atan2((double)q.a3, (double)q.a0);
+    th[mu * V + s] = atan2((double)q.a3, (double)q.a0);
   }
 }
 
@@ -108,6 +112,7 @@ __global__ __launch_bounds__(kMonoBlock) void k_su3_angles(const T* __restrict__
   int p;
   uint32_t h;
   lex_site(g, s, p, h);
+  const size_t V = 2 * (size_t)g.Vh;
 #pragma unroll
   for (int mu = 0; mu < 4; mu++) {
     const M2<T> o = ldm(link_ptr<12>(U, g, p, mu, h));
@@ -125,10 +130,27 @@ __global__ __launch_bounds__(kMonoBlock) void k_su3_angles(const T* __restrict__
     sum = sum + a[2];
     for (int i = 0; i < kMaxWrap && sum >= kPi; i++) sum -= 2 * kPi;
     for (int i = 0; i < kMaxWrap && sum < -kPi; i++) sum += 2 * kPi;
-    double* o3 = th3 + 3 * (4 * (size_t)s + mu);
 #pragma unroll
-    for (int c = 0; c < 3; c++) o3[c] = a[c] - sum / 3;
+    for (int c = 0; c < 3; c++) th3[(c * 4 + mu) * V + s] = a[c] - sum / 3;
   }
+}
+
+// reference link-major angles [4V][ncol] <-> planar [(c * 4 + mu) * V + site]
+__global__ __launch_bounds__(kMonoBlock) void k_angles_to_planar(const double* __restrict__ ref,
+                                                                int ncol, uint32_t V,
+                                                                double* __restrict__ th) {
+  const uint32_t s = blockIdx.x * kMonoBlock + threadIdx.x;
+  if (s >= V) return;
+  for (int mu = 0; mu < 4; mu++)
+    for (int c = 0; c < ncol; c++) th[(size_t)(c * 4 + mu) * V + s] = ref[(4 * (size_t)s + mu) * ncol + c];
+}
+__global__ __launch_bounds__(kMonoBlock) void k_angles_to_ref(const double* __restrict__ th,
+                                                             int ncol, uint32_t V,
+                                                             double* __restrict__ ref) {
+  const uint32_t s = blockIdx.x * kMonoBlock + threadIdx.x;
+  if (s >= V) return;
+  for (int mu = 0; mu < 4; mu++)
+    for (int c = 0; c < ncol; c++) ref[(4 * (size_t)s + mu) * ncol + c] = th[(size_t)(c * 4 + mu) * V + s];
 }
 
 // ------------------------------------------------------------------ plaquettes
@@ -137,15 +159,15 @@ __device__ __forceinline__ int plane_of(int mu, int nu) {
   return mu == 0 ? nu - 1 : (mu == 1 ? nu + 1 : 5);
 }
 
-// angle of the plaquette (mu, nu) at s from th[stride * (4 site + mu) + off]
-__device__ __forceinline__ double plaq_angle(const double* __restrict__ th, int stride, int off,
-                                             const Lat& l, uint32_t s, const int* c, int mu,
-                                             int nu) {
+// angle of the plaquette (mu, nu) at s from the planar angles th[mu * V + site]
+__device__ __forceinline__ double plaq_angle(const double* __restrict__ th, const Lat& l,
+                                             uint32_t s, const int* c, int mu, int nu) {
   const uint32_t smu = fwd(l, s, c, mu), snu = fwd(l, s, c, nu);
-  double p = th[stride * (4 * (size_t)s + mu) + off];
-  p = p + th[stride * (4 * (size_t)smu + nu) + off];
-  p = p - th[stride * (4 * (size_t)snu + mu) + off];
-  p = p - th[stride * (4 * (size_t)s + nu) + off];
+  const size_t V = l.V;
+  double p = th[mu * V + s];
+  p = p + th[nu * V + smu];
+  p = p - th[mu * V + snu];
+  p = p - th[nu * V + s];
   return wrap_pi(p);
 }
 
@@ -159,7 +181,7 @@ __global__ __launch_bounds__(kMonoBlock) void k_u1_plaquettes(const double* __re
   for (int mu = 0; mu < 3; mu++)
 #pragma unroll
     for (int nu = mu + 1; nu < 4; nu++)
-      P[(size_t)plane_of(mu, nu) * l.V + s] = plaq_angle(th, 1, 0, l, s, c, mu, nu);
+      P[(size_t)plane_of(mu, nu) * l.V + s] = plaq_angle(th, l, s, c, mu, nu);
 }
 
 // P3[colour][plane][site]
@@ -176,7 +198,7 @@ __global__ __launch_bounds__(kMonoBlock) void k_su3_plaquettes(const double* __r
     for (int nu = mu + 1; nu < 4; nu++) {
       double q[3];
 #pragma unroll
-      for (int k = 0; k < 3; k++) q[k] = plaq_angle(th3, 3, k, l, s, c, mu, nu);
+      for (int k = 0; k < 3; k++) q[k] = plaq_angle(th3 + (size_t)k * 4 * l.V, l, s, c, mu, nu);
       double sum = q[0] + q[1];
       sum = sum + q[2];
       if (sum >= kPi) {
@@ -239,8 +261,8 @@ __global__ __launch_bounds__(kMonoBlock) void k_monopole_currents(const double* 
 
 int grid_of(uint32_t n) { return (int)((n + kMonoBlock - 1) / kMonoBlock); }
 
-// currents from angles already on the device: th (SU(2): [4V]) or th3
-// (SU(3): [4V][3]) -> J ([4V] or [3][4V]); P: 6V (SU(2)) or 18V scratch
+// currents from planar angles on the device (SU(2) 4V, SU(3) 12V) -> J
+// ([4V] or [3][4V]); P: 6V (SU(2)) or 18V scratch
 void launch_currents(int group, const double* th, const Lat& l, double* P, double* J,
                      hipStream_t st) {
   const int nb = grid_of(l.V);
@@ -297,7 +319,10 @@ int host_currents(int group, const int* dims, const double* angles, double* J) {
   CHK(upload_angles(angles, (size_t)4 * l.V * ncol, &b.p[0], st));
   HIPCHK(hipMalloc((void**)&b.p[1], (size_t)6 * l.V * ncol * sizeof(double)));
   HIPCHK(hipMalloc((void**)&b.p[2], (size_t)4 * l.V * ncol * sizeof(double)));
-  launch_currents(group, b.p[0], l, b.p[1], b.p[2], st);
+  HIPCHK(hipMalloc((void**)&b.p[3], (size_t)4 * l.V * ncol * sizeof(double)));
+  hipLaunchKernelGGL(k_angles_to_planar, dim3(grid_of(l.V)), dim3(kMonoBlock), 0, st, b.p[0], ncol,
+                     l.V, b.p[3]);
+  launch_currents(group, b.p[3], l, b.p[1], b.p[2], st);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(J, b.p[2], (size_t)4 * l.V * ncol * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -353,9 +378,13 @@ int mag_job_abelian_angles(mag_job* j, int copy, double* angles) {
   const size_t n = (size_t)4 * j->V * (j->prm.group == MAG_SU2 ? 1 : 3);
   Bufs b;
   HIPCHK(hipMalloc((void**)&b.p[0], n * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&b.p[2], n * sizeof(double)));
   if (j->spin) HIPCHK(hipMalloc((void**)&b.p[1], j->nreal * sizeof(double)));
   CHK(job_angles(j, copy, b.p[0], b.p[1]));
-  HIPCHK(hipMemcpyAsync(angles, b.p[0], n * sizeof(double), hipMemcpyDeviceToHost, j->st));
+  hipLaunchKernelGGL(k_angles_to_ref, dim3(grid_of((uint32_t)j->V)), dim3(kMonoBlock), 0, j->st,
+                     b.p[0], j->prm.group == MAG_SU2 ? 1 : 3, (uint32_t)j->V, b.p[2]);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(angles, b.p[2], n * sizeof(double), hipMemcpyDeviceToHost, j->st));
   HIPCHK(hipStreamSynchronize(j->st));
   return MAG_OK;
 }

@@ -273,13 +273,16 @@ __global__ __launch_bounds__(NT) void k_su2_half(T* __restrict__ U, T* __restric
   su2_half_block<T, GA, NT, SPT>(U, G, g, p, omega, part, blockIdx.x);
 }
 
-template <typename T, bool GA>
-__global__ __launch_bounds__(kBlock) void k_su2_half_batch(const BatchJob* __restrict__ jobs,
-                                                           int njobs, int p) {
+// batched: workgroup b runs block b - block0 of its job with the class's
+// launch shape (NT, SPT); the job's partials are per wave, so the shape
+// leaves its results unchanged
+template <typename T, bool GA, int NT, int SPT>
+__global__ __launch_bounds__(NT) void k_su2_half_batch(const BatchJob* __restrict__ jobs,
+                                                       int njobs, int p) {
   const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
   if (j.ctl->done) return;
-  su2_half_block<T, GA, kBlock, 1>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega,
-                                   j.part + (size_t)3 * j.nw * p, blockIdx.x - j.block0);
+  su2_half_block<T, GA, NT, SPT>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega,
+                                 j.part + (size_t)3 * j.nw * p, blockIdx.x - j.block0);
 }
 
 // launch of a shape (NT, SPT) of the supported set
@@ -476,13 +479,13 @@ __global__ __launch_bounds__(NT) void k_su3_half(T* __restrict__ U, T* __restric
   su3_half_block<T, GA, NT, SPT>(U, G, g, p, omega, part, blockIdx.x);
 }
 
-template <typename T, bool GA>
-__global__ __launch_bounds__(kBlock) void k_su3_half_batch(const BatchJob* __restrict__ jobs,
-                                                           int njobs, int p) {
+template <typename T, bool GA, int NT, int SPT>
+__global__ __launch_bounds__(NT) void k_su3_half_batch(const BatchJob* __restrict__ jobs,
+                                                       int njobs, int p) {
   const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
   if (j.ctl->done) return;
-  su3_half_block<T, GA, kBlock, 1>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega,
-                                   j.part + (size_t)3 * j.nw * p, blockIdx.x - j.block0);
+  su3_half_block<T, GA, NT, SPT>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega,
+                                 j.part + (size_t)3 * j.nw * p, blockIdx.x - j.block0);
 }
 
 template <typename T>
@@ -509,18 +512,32 @@ template void launch_su3_half<double>(double*, double*, const Geom&, int, double
                                       const JobCtl*, Shape, hipStream_t);
 
 // batched half-sweep over a class table (group, precision): every job of
-// the table has the same group and storage type; GA = mixed FP32 phase
-void launch_half_batch(int group, bool fp32, const BatchJob* jobs, int njobs, int nblocks_total,
-                       int parity, hipStream_t st) {
+// the table has the same group and storage type; GA = mixed FP32 phase;
+// block0 / nblocks_total counted in workgroups of the shape sh
+void launch_half_batch(int group, bool fp32, Shape sh, const BatchJob* jobs, int njobs,
+                       int nblocks_total, int parity, hipStream_t st) {
   if (njobs == 0 || nblocks_total == 0) return;
-  const dim3 grid(nblocks_total), block(kBlock);
-  if (group == 2) {
-    if (fp32) hipLaunchKernelGGL((k_su2_half_batch<float, true>), grid, block, 0, st, jobs, njobs, parity);
-    else hipLaunchKernelGGL((k_su2_half_batch<double, false>), grid, block, 0, st, jobs, njobs, parity);
-  } else {
-    if (fp32) hipLaunchKernelGGL((k_su3_half_batch<float, true>), grid, block, 0, st, jobs, njobs, parity);
-    else hipLaunchKernelGGL((k_su3_half_batch<double, false>), grid, block, 0, st, jobs, njobs, parity);
-  }
+  const dim3 grid(nblocks_total);
+#define L_(NT, SPT)                                                                          \
+  do {                                                                                       \
+    if (group == 2) {                                                                        \
+      if (fp32)                                                                              \
+        hipLaunchKernelGGL((k_su2_half_batch<float, true, NT, SPT>), grid, dim3(NT), 0, st,  \
+                           jobs, njobs, parity);                                             \
+      else                                                                                   \
+        hipLaunchKernelGGL((k_su2_half_batch<double, false, NT, SPT>), grid, dim3(NT), 0,    \
+                           st, jobs, njobs, parity);                                         \
+    } else {                                                                                 \
+      if (fp32)                                                                              \
+        hipLaunchKernelGGL((k_su3_half_batch<float, true, NT, SPT>), grid, dim3(NT), 0, st,  \
+                           jobs, njobs, parity);                                             \
+      else                                                                                   \
+        hipLaunchKernelGGL((k_su3_half_batch<double, false, NT, SPT>), grid, dim3(NT), 0,    \
+                           st, jobs, njobs, parity);                                         \
+    }                                                                                        \
+  } while (0)
+  MAG_SHAPE_SWITCH(sh, L_)
+#undef L_
 }
 
 // ================================================================== gauge copies

@@ -11,8 +11,9 @@
 // before times every shape on a scratch copy of the job's own links (8
 // back-to-back half-sweeps per shape between HIP events, after 2 warm-up
 // launches), keeps the fastest, and caches it for the process.  The job's
-// data are not touched.  mag_tune_set switches to a fixed shape or to the
-// default shape (256, 1) without tuning.
+// data are not touched.  Batched launches are tuned on their own (see
+// tune_batch).  mag_tune_set switches to a fixed shape or to the default
+// shape (256, 1) without tuning.
 #include "mag_capi_util.h"
 #include "mag_gpu.h"
 #include "mag_internal.h"
@@ -21,6 +22,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <vector>
 
 namespace mag {
 namespace {
@@ -109,7 +111,131 @@ int tune(const mag_job* j, bool fp32, Shape* out) {
   return rc;
 }
 
+// Batched link-form half-sweeps (mag_batch.hip) concatenate many jobs'
+// grids, so the best shape of one job alone is not that of the batch: the
+// batch is tuned on its own launch.  Scratch copies of the class's jobs (in
+// order, up to kBatchScratch bytes, at least one job) are swept with each
+// shape through the batched kernel (2 warm-up + 6 timed launches); the
+// fastest is cached per (group, precision, device, log2 of the class's
+// total lanes).
+constexpr size_t kBatchScratch = (size_t)1 << 30;
+
+int tune_batch(int group, bool fp32, const std::vector<mag_job*>& jobs, Shape* out) {
+  const hipStream_t st = jobs[0]->st;
+  const size_t esz = fp32 ? sizeof(float) : sizeof(double);
+  std::vector<void*> bufs;
+  std::vector<BatchJob> tab;
+  BatchJob* d_tab = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(st);
+    for (void* b : bufs) (void)hipFree(b);
+    if (d_tab) (void)hipFree(d_tab);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  auto alloc = [&](size_t n) -> void* {
+    void* p = nullptr;
+    if (hipMalloc(&p, n) != hipSuccess) return nullptr;
+    bufs.push_back(p);
+    return p;
+  };
+  size_t used = 0;
+  hipError_t e = hipSuccess;
+  for (const mag_job* j : jobs) {
+    const size_t ub = j->nreal * esz, gb = fp32 ? (size_t)2 * j->g.Vh * j->nr * sizeof(float) : 0;
+    if (!tab.empty() && used + ub + gb > kBatchScratch) break;
+    used += ub + gb;
+    BatchJob b{};
+    b.U = alloc(ub);
+    b.G = fp32 ? alloc(gb) : nullptr;
+    b.part = (double*)alloc((size_t)2 * j->nw * 3 * sizeof(double));
+    b.ctl = (JobCtl*)alloc(sizeof(JobCtl));
+    if (!b.U || (fp32 && !b.G) || !b.part || !b.ctl) {
+      cleanup();
+      return fail(MAG_ERR_NOMEM, "batch autotune: scratch allocation failed");
+    }
+    b.g = j->g;
+    b.omega = j->prm.omega;
+    b.nw = j->nw;
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(b.U, fp32 ? (const void*)j->W32 : (const void*)j->U64, ub,
+                         hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess && fp32) e = hipMemcpyAsync(b.G, j->G32, gb, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(b.ctl, 0, sizeof(JobCtl), st);  // done = 0
+    tab.push_back(b);
+  }
+  if (e == hipSuccess) e = hipMalloc((void**)&d_tab, tab.size() * sizeof(BatchJob));
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(MAG_ERR_HIP, "batch autotune: %s", hipGetErrorString(e));
+  }
+  float best = 0.f;
+  Shape bs;
+  for (const Shape& sh : kShapes) {
+    const size_t tile = (size_t)sh.nt * sh.spt;
+    int nblocks = 0;
+    for (BatchJob& b : tab) {
+      b.block0 = nblocks;
+      nblocks += (int)((sweep_lanes(b.g, group) + tile - 1) / tile);
+    }
+    e = hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st);
+    for (int i = 0; i < 2 && e == hipSuccess; i++)
+      launch_half_batch(group, fp32, sh, d_tab, (int)tab.size(), nblocks, i & 1, st);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int i = 0; i < 6 && e == hipSuccess; i++)
+      launch_half_batch(group, fp32, sh, d_tab, (int)tab.size(), nblocks, i & 1, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) break;
+    if (best == 0.f || ms < best) {
+      best = ms;
+      bs = sh;
+    }
+  }
+  cleanup();
+  if (e != hipSuccess) return fail(MAG_ERR_HIP, "batch autotune: %s", hipGetErrorString(e));
+  *out = bs;
+  return MAG_OK;
+}
+
+std::map<std::tuple<int, int, int, int>, Shape> g_batch_cache;
+
 }  // namespace
+
+int batch_shape(int group, bool fp32, const std::vector<mag_job*>& jobs, Shape* out) {
+  size_t lanes = 0;
+  for (const mag_job* j : jobs) lanes += sweep_lanes(j->g, group);
+  int bucket = 0;
+  while (((size_t)2 << bucket) <= lanes) bucket++;
+  std::tuple<int, int, int, int> key{group, fp32 ? 1 : 0, jobs[0]->prm.device, bucket};
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_mode == MAG_TUNE_FIXED) {
+      *out = g_fixed;
+      return MAG_OK;
+    }
+    if (g_mode == MAG_TUNE_DEFAULT) {
+      *out = Shape{};
+      return MAG_OK;
+    }
+    auto it = g_batch_cache.find(key);
+    if (it != g_batch_cache.end()) {
+      *out = it->second;
+      return MAG_OK;
+    }
+  }
+  Shape sh;
+  CHK(tune_batch(group, fp32, jobs, &sh));
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_batch_cache[key] = sh;
+  *out = sh;
+  return MAG_OK;
+}
 
 int job_shape(const mag_job* j, bool fp32, Shape* out) {
   std::tuple<int, int, int, int, int, int, int> key{j->prm.group, fp32 ? 1 : 0, j->prm.device,
@@ -156,6 +282,7 @@ int mag_tune_set(int mode, int nt, int spt) {
 int mag_tune_clear(void) {
   std::lock_guard<std::mutex> lk(mag::g_mu);
   mag::g_cache.clear();
+  mag::g_batch_cache.clear();
   return MAG_OK;
 }
 

@@ -61,3 +61,29 @@ def test_link_form_synthetic(oracle, tmp_path):
     assert abs(_num(out, "link-form sweeps")[0] - (n + 1)) <= 1
     fixed = np.fromfile(outp).reshape(-1, 4)
     assert abs(oracle.su2_functional(fixed, dims) - F) < 1e-12
+
+
+@pytest.mark.parametrize("link_form", [1, 0])
+def test_monopole_output(oracle, tmp_path, link_form):
+    """--path_monopoles_output: link form from the device-resident fixed links
+    (mag_job_monopole_currents), spin path from the host links through
+    calculate_current; both against the oracle on the written links."""
+    outp, mono = tmp_path / "fixed.bin", tmp_path / "J.bin"
+    args = ["--synthetic_eps", "1.5", "--seed", "7", "--x_size", "8", "--y_size", "8",
+            "--z_size", "8", "--t_size", "8", "--path_conf_output", str(outp),
+            "--path_monopoles_output", str(mono)]
+    if link_form:
+        args += ["--link_form", "1", "--omega", "1.7", "--tol", "1e-10", "--max_sweeps", "20000"]
+    else:
+        args += ["--T_init", "2.0", "--T_final", "0.5", "--T_step", "0.5", "--OR_steps", "2",
+                 "--thermalization_steps", "2", "--tolerance_maximal", "1e-6",
+                 "--tolerance_average", "1e-9", "--max_iter", "200"]
+    out = _run(args)
+    dims = (8, 8, 8, 8)
+    fixed = np.fromfile(outp).reshape(-1, 4)
+    J = np.fromfile(mono)
+    ang = oracle.su2_abelian_angles(np.ascontiguousarray(fixed), dims)
+    Jr = oracle.monopole_currents(oracle.u1_monopole_plaquettes(ang, dims), dims)
+    assert np.abs(J - Jr).max() < 1e-12
+    n = int(np.abs(np.round(J)).sum())
+    assert n == _num(out, "monopole current links")[0]
