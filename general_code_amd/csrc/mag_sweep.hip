@@ -338,6 +338,18 @@ __device__ __forceinline__ void renorm_rows(M2<T>& m) {
   }
 }
 
+#ifndef MAG_SU3_ROW2_REMAT
+#define MAG_SU3_ROW2_REMAT 1
+#endif
+// a value the compiler must treat as changed (no instruction is emitted)
+template <typename T>
+__device__ __forceinline__ void opaque(M2<T>& m) {
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) asm volatile("" : "+v"(m.r[i][c].re), "+v"(m.r[i][c].im));
+}
+
 // Cabibbo-Marinari subgroup (I,J) update of the 2*NL links a lane holds
 // (forward F[k], backward B[k], mu = NL*q + k; NL = 2 for lane pairs, 1 for
 // lane quads), rows 0,1 in registers, row 2 rebuilt where the subgroup
@@ -382,6 +394,17 @@ __device__ __forceinline__ T cm_subgroup(M2<T>* F, M2<T>* B, M2<T>& Gx, T omega)
   const Q<T> gq = omega_gauge(h0, h1, h2, omega, res);
   // r = [[g0 + i g3, g2 + i g1], [-g2 + i g1, g0 - i g3]], g3 = 0
   const C<T> r00{gq.a0, T(0)}, r01{gq.a2, gq.a1}, r10{-gq.a2, gq.a1}, r11{gq.a0, T(0)};
+#if MAG_SU3_ROW2_REMAT
+  // row 2 of the forward links again rather than kept live across the
+  // gauge computation (the empty asm hides that F is unchanged)
+  if (J == 2) {
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+      opaque(F[k]);
+      row2_of(F[k], Fr2[k]);
+    }
+  }
+#endif
 #pragma unroll
   for (int k = 0; k < NL; k++) {
     // forward: rows I,J <- r (rows I,J); only stored rows are written
