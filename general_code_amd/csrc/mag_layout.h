@@ -159,6 +159,17 @@ __device__ __forceinline__ void write_part(double* part, double a, double b, dou
   }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8),
+// each with its own L2.  For kernels that read neighbour data written or
+// read by other workgroups, workgroup b processes the logical block that
+// gives XCD b % 8 one contiguous range of blocks: neighbouring sites then
+// mostly run on the same XCD and share its L2.  A bijection of [0, n); it
+// only reorders which workgroup computes which block.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+  const uint32_t q = n / 8, r = n % 8, x = b % 8, i = b / 8;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // lexicographic site s (reference order, indexing.h:111-123) -> parity p
 // and half index h of the parity-split SoA
 __device__ __forceinline__ void lex_site(const Geom& g, uint32_t s, int& p, uint32_t& h) {

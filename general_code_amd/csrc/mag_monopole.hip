@@ -15,7 +15,9 @@
 //   currents   J_mu(x) = forward differences at x + mu of the three
 //              plaquette planes orthogonal to mu, over 2 pi
 //              (get_current, monopoles.cpp:44-122)
-// Every kernel runs one thread per lexicographic site.  Between the kernels
+// Every kernel runs one thread per lexicographic site (the plaquette and
+// current kernels, which read neighbours, in XCD-aware block order).
+// Between the kernels
 // the angles are planar, th[(colour * 4 + mu) * V + site], so that the
 // neighbour loads of a wave are contiguous; the reference's link-major
 // layouts ([4V], [4V][3]) are produced or read only at the boundary.  The
@@ -173,7 +175,7 @@ __device__ __forceinline__ double plaq_angle(const double* __restrict__ th, cons
 
 __global__ __launch_bounds__(kMonoBlock) void k_u1_plaquettes(const double* __restrict__ th,
                                                              Lat l, double* __restrict__ P) {
-  const uint32_t s = blockIdx.x * kMonoBlock + threadIdx.x;
+  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kMonoBlock + threadIdx.x;
   if (s >= l.V) return;
   int c[4];
   coords_of(l, s, c);
@@ -187,7 +189,7 @@ __global__ __launch_bounds__(kMonoBlock) void k_u1_plaquettes(const double* __re
 // P3[colour][plane][site]
 __global__ __launch_bounds__(kMonoBlock) void k_su3_plaquettes(const double* __restrict__ th3,
                                                               Lat l, double* __restrict__ P3) {
-  const uint32_t s = blockIdx.x * kMonoBlock + threadIdx.x;
+  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kMonoBlock + threadIdx.x;
   if (s >= l.V) return;
   int c[4];
   coords_of(l, s, c);
@@ -230,7 +232,7 @@ __global__ __launch_bounds__(kMonoBlock) void k_su3_plaquettes(const double* __r
 // ------------------------------------------------------------------ currents
 __global__ __launch_bounds__(kMonoBlock) void k_monopole_currents(const double* __restrict__ P,
                                                                  Lat l, double* __restrict__ J) {
-  const uint32_t s = blockIdx.x * kMonoBlock + threadIdx.x;
+  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kMonoBlock + threadIdx.x;
   if (s >= l.V) return;
   int c[4];
   coords_of(l, s, c);

@@ -176,17 +176,8 @@ __device__ __forceinline__ int batch_job(const BatchJob* __restrict__ jobs, int 
   return lo;
 }
 
-// ================================================================== XCD-aware order
-// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8), each
-// with its own L2.  The site-form kernels reuse neighbour spins / gauges
-// (read by up to 8 sites), so workgroup b processes the logical block that
-// gives XCD b % 8 one contiguous range of blocks: neighbouring sites then
-// mostly run on the same XCD and share its L2.  A bijection of [0, n); it
-// only reorders which workgroup computes which block.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
-  const uint32_t q = n / 8, r = n % 8, x = b % 8, i = b / 8;
-  return x * q + (x < r ? x : r) + i;
-}
+// XCD-aware workgroup order (xcd_block, mag_layout.h): the site-form
+// kernels reuse neighbour spins / gauges (read by up to 8 sites).
 
 // ================================================================== launch shapes
 // The partial triples of a half-sweep are per wave: wave w of a job covers
