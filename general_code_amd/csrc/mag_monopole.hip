@@ -302,10 +302,11 @@ int check_dims(const int* dims) {
 // host angles -> device, after checking that every angle is finite and
 // within kMaxAngle
 int upload_angles(const double* a, size_t n, double** d, hipStream_t st) {
-  for (size_t i = 0; i < n; i++)
-    if (!(std::fabs(a[i]) <= kMaxAngle))
-      return fail(MAG_ERR_ARG, "angle %zu is %g: angles must be finite with |angle| <= %g", i, a[i],
-                  kMaxAngle);
+  for (size_t i = 0; i < n; i++) {
+    if (!std::isfinite(a[i])) return fail(MAG_ERR_NONFINITE, "angle %zu is %g", i, a[i]);
+    if (std::fabs(a[i]) > kMaxAngle)
+      return fail(MAG_ERR_ARG, "angle %zu is %g: |angle| must be <= %g", i, a[i], kMaxAngle);
+  }
   HIPCHK(hipMalloc((void**)d, n * sizeof(double)));
   HIPCHK(hipMemcpyAsync(*d, a, n * sizeof(double), hipMemcpyHostToDevice, st));
   return MAG_OK;

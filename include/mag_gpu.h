@@ -286,7 +286,8 @@ int mag_job_monopole_timed(mag_job *job, int reps, double *ms);
 /* Currents from host angles (the monopole apps read abelian configurations
  * from files): calculate_current(LatticeData<abelian>) (monopoles.cpp:669-687)
  * and calculate_current(LatticeData<su3_angles>) (monopoles.cpp:646-667).
- * Extents >= 2 (odd allowed); every angle finite with |angle| <= 1000. */
+ * Extents >= 2 (odd allowed); every angle finite (else MAG_ERR_NONFINITE)
+ * with |angle| <= 1000 (else MAG_ERR_ARG). */
 int mag_u1_monopole_currents(const int *dims, const double *angles, double *J);
 int mag_su3_monopole_currents(const int *dims, const double *angles3, double *J3);
 

@@ -93,11 +93,17 @@ def test_monopole_argument_errors(mag):
     dims = (4, 4, 4, 4)
     th = np.zeros(4 * 256)
     th[17] = np.inf
-    with pytest.raises(mag.MagError):
+    with pytest.raises(mag.MagError) as e:
         mag.u1_monopole_currents(dims, th)
+    assert e.value.code == -4  # MAG_ERR_NONFINITE
+    th[17] = np.nan
+    with pytest.raises(mag.MagError) as e:
+        mag.su3_monopole_currents(dims, np.zeros((4 * 256, 3)) + th[:, None])
+    assert e.value.code == -4  # MAG_ERR_NONFINITE
     th[17] = 2e3
-    with pytest.raises(mag.MagError):
+    with pytest.raises(mag.MagError) as e:
         mag.u1_monopole_currents(dims, th)
+    assert e.value.code == -1  # MAG_ERR_ARG
     with pytest.raises(mag.MagError):
         mag.u1_monopole_currents((4, 4, 1, 4), np.zeros(4 * 64))
     with mag.MagJob(group=mag.SU2, dims=dims) as job:
