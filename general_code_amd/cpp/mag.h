@@ -369,11 +369,70 @@ auto calculate_current(const Lattice& conf) {
 }
 
 // ---------------------------------------------------------------- file formats
-// Readers of the reference's on-disk link formats (data.h:114-142,
-// indexing.h:190-323); precision "double" or "float".
+namespace detail {
+// LIME records (the container of ILDG files): a 144-byte header -- magic
+// 0x456789ab (big-endian 32 bit), version (16 bit), a flag byte with
+// message-begin 0x80 / message-end 0x40, the data length (big-endian 64
+// bit) at byte 8 and a NUL-padded 128-byte type string at byte 16 --, then
+// the data padded to a multiple of 8 bytes.
+constexpr uint32_t kLimeMagic = 0x456789abu;
+inline uint64_t be_load(const unsigned char* p, int n) {
+  uint64_t v = 0;
+  for (int i = 0; i < n; i++) v = (v << 8) | p[i];
+  return v;
+}
+inline void be_store(unsigned char* p, uint64_t v, int n) {
+  for (int i = n - 1; i >= 0; i--, v >>= 8) p[i] = (unsigned char)(v & 0xff);
+}
+inline double be_double(const unsigned char* p) {
+  const uint64_t u = be_load(p, 8);
+  double d;
+  std::memcpy(&d, &u, 8);
+  return d;
+}
+// the data of the first record of type `type`, as read
+inline std::vector<unsigned char> lime_record(std::ifstream& in, const std::string& path,
+                                              const std::string& type) {
+  for (;;) {
+    unsigned char h[144];
+    if (!in.read(reinterpret_cast<char*>(h), sizeof(h)))
+      throw error(MAG_ERR_ARG, "no " + type + " record in " + path);
+    if (be_load(h, 4) != kLimeMagic) throw error(MAG_ERR_ARG, "not a LIME file: " + path);
+    const uint64_t len = be_load(h + 8, 8);
+    const std::string t(reinterpret_cast<const char*>(h + 16), strnlen(reinterpret_cast<const char*>(h + 16), 128));
+    const uint64_t padded = (len + 7) / 8 * 8;
+    if (t == type) {
+      std::vector<unsigned char> d(len);
+      if (!in.read(reinterpret_cast<char*>(d.data()), (std::streamsize)len))
+        throw error(MAG_ERR_ARG, "short LIME record in " + path);
+      return d;
+    }
+    in.seekg((std::streamoff)padded, std::ios::cur);
+  }
+}
+inline void lime_write(std::ofstream& out, const std::string& type, const void* data, uint64_t len,
+                       bool mb, bool me) {
+  unsigned char h[144] = {0};
+  be_store(h, kLimeMagic, 4);
+  be_store(h + 4, 1, 2);
+  h[6] = (unsigned char)((mb ? 0x80 : 0) | (me ? 0x40 : 0));
+  be_store(h + 8, len, 8);
+  std::memcpy(h + 16, type.data(), type.size() < 127 ? type.size() : 127);
+  out.write(reinterpret_cast<const char*>(h), sizeof(h));
+  out.write(reinterpret_cast<const char*>(data), (std::streamsize)len);
+  const char pad[8] = {0};
+  out.write(pad, (std::streamsize)((8 - len % 8) % 8));
+}
+}  // namespace detail
+
+// Readers of the reference's on-disk link formats (data.h:114-176,
+// indexing.h:190-351); precision "double" or "float" (ildg: always double).
 //   lexicographical: element e of link (site, mu) at (4*site + mu)*N + e
 //   qcdstag:         at (site + mu1*V)*N + e', mu1 = (mu + 1) % 4 (t first),
 //                    and for su2 the element order reversed: e' = 4 - e, e > 0
+//   ildg:            the "ildg-binary-data" record of a LIME file, big-endian
+//                    doubles in the lexicographical order (data.h:144-176,
+//                    FilePatternILDG indexing.h:325-351); bytes_skip unused
 template <class Lattice>
 void read_links(Lattice& conf, const std::string& path, const std::string& format,
                 long bytes_skip, const std::string& precision) {
@@ -383,12 +442,19 @@ void read_links(Lattice& conf, const std::string& path, const std::string& forma
   const size_t n = V * 4 * N;
   std::ifstream in(path, std::ios::binary);
   if (!in) throw error(MAG_ERR_ARG, "cannot open " + path);
-  in.ignore(bytes_skip);
   std::vector<double> raw(n);
-  if (precision == "double") {
+  if (format == "ildg") {
+    const std::vector<unsigned char> d = detail::lime_record(in, path, "ildg-binary-data");
+    if (d.size() != n * sizeof(double))
+      throw error(MAG_ERR_ARG, "ildg-binary-data holds " + std::to_string(d.size()) +
+                                   " bytes, the lattice needs " + std::to_string(n * sizeof(double)));
+    for (size_t i = 0; i < n; i++) raw[i] = detail::be_double(d.data() + 8 * i);
+  } else if (precision == "double") {
+    in.ignore(bytes_skip);
     if (!in.read(reinterpret_cast<char*>(raw.data()), n * sizeof(double)))
       throw error(MAG_ERR_ARG, "short read: " + path);
   } else if (precision == "float") {
+    in.ignore(bytes_skip);
     std::vector<float> f(n);
     if (!in.read(reinterpret_cast<char*>(f.data()), n * sizeof(float)))
       throw error(MAG_ERR_ARG, "short read: " + path);
@@ -401,7 +467,7 @@ void read_links(Lattice& conf, const std::string& path, const std::string& forma
     for (int mu = 0; mu < 4; mu++)
       for (int e = 0; e < N; e++) {
         size_t idx;
-        if (format == "lexicographical") {
+        if (format == "lexicographical" || format == "ildg") {
           idx = (s * 4 + mu) * N + e;
         } else if (format == "qcdstag") {
           const int mu1 = (mu == 3) ? 0 : mu + 1;
@@ -432,6 +498,40 @@ void write_links_lexicographical(const Lattice& conf, const std::string& path) {
     const std::vector<double> rm = detail::su3_links(conf);
     out.write(reinterpret_cast<const char*>(rm.data()), rm.size() * sizeof(double));
   }
+  if (!out) throw error(MAG_ERR_ARG, "write failed: " + path);
+}
+
+// ILDG file (a LIME message): an "ildg-format" XML record and the
+// "ildg-binary-data" record of big-endian doubles in the lexicographical
+// order that read_links(..., "ildg", ...) and the reference read
+// (data.h:144-176)
+template <class Lattice>
+void write_links_ildg(const Lattice& conf, const std::string& path) {
+  using M = typename Lattice::matrix_type;
+  constexpr int N = sizeof(M) / sizeof(double);
+  std::vector<double> rm;
+  if constexpr (N == 4) {
+    const double* p = reinterpret_cast<const double*>(conf.array.data());
+    rm.assign(p, p + conf.array.size() * 4);
+  } else {
+    rm = detail::su3_links(conf);
+  }
+  std::vector<unsigned char> be(rm.size() * 8);
+  for (size_t i = 0; i < rm.size(); i++) {
+    uint64_t u;
+    std::memcpy(&u, &rm[i], 8);
+    detail::be_store(be.data() + 8 * i, u, 8);
+  }
+  const std::string xml =
+      std::string("<?xml version=\"1.0\" encoding=\"UTF-8\"?>\n<ildgFormat>\n<version>1.0</version>\n") +
+      "<field>" + (N == 4 ? "su2gauge" : "su3gauge") + "</field>\n<precision>64</precision>\n" +
+      "<lx>" + std::to_string(conf.lat_dim[0]) + "</lx><ly>" + std::to_string(conf.lat_dim[1]) +
+      "</ly><lz>" + std::to_string(conf.lat_dim[2]) + "</lz><lt>" + std::to_string(conf.lat_dim[3]) +
+      "</lt>\n</ildgFormat>";
+  std::ofstream out(path, std::ios::binary);
+  if (!out) throw error(MAG_ERR_ARG, "cannot open " + path);
+  detail::lime_write(out, "ildg-format", xml.data(), xml.size(), true, false);
+  detail::lime_write(out, "ildg-binary-data", be.data(), be.size(), false, true);
   if (!out) throw error(MAG_ERR_ARG, "write failed: " + path);
 }
 

@@ -33,6 +33,96 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+_REF_LIME = os.path.join(_HERE, "_ref", "liblime_ref.so")
+_lime = None
+
+
+def build_ref():
+    """Compile the reference's LIME library from its sources under
+    /root/reference (oracle/Makefile target `ref`, output in oracle/_ref/);
+    returns its path, or None where the reference is absent."""
+    if not os.path.isdir("/root/reference/lib/cpu/src/c-lime"):
+        return _REF_LIME if os.path.exists(_REF_LIME) else None
+    subprocess.run(["make", "-C", _HERE, "ref"], check=True, stdout=subprocess.DEVNULL)
+    return _REF_LIME
+
+
+def lime_ref():
+    """ctypes handle of oracle/_ref/liblime_ref.so (None if not built)."""
+    global _lime
+    if _lime is None:
+        if not os.path.exists(_REF_LIME):
+            return None
+        L = ctypes.CDLL(_REF_LIME)
+        vp, u64p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)
+        for name, res, args in [
+            ("limeCreateWriter", vp, [vp]), ("limeDestroyWriter", ctypes.c_int, [vp]),
+            ("limeCreateHeader", vp, [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint64]),
+            ("limeDestroyHeader", None, [vp]),
+            ("limeWriteRecordHeader", ctypes.c_int, [vp, vp]),
+            ("limeWriteRecordData", ctypes.c_int, [vp, u64p, vp]),
+            ("limeWriterCloseRecord", ctypes.c_int, [vp]),
+            ("limeCreateReader", vp, [vp]), ("limeDestroyReader", None, [vp]),
+            ("limeReaderNextRecord", ctypes.c_int, [vp]),
+            ("limeReaderMBFlag", ctypes.c_int, [vp]), ("limeReaderMEFlag", ctypes.c_int, [vp]),
+            ("limeReaderType", ctypes.c_char_p, [vp]), ("limeReaderBytes", ctypes.c_uint64, [vp]),
+            ("limeReaderReadData", ctypes.c_int, [vp, u64p, vp]),
+        ]:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lime = L
+    return _lime
+
+
+def _libc():
+    c = ctypes.CDLL(None)
+    c.fopen.restype = ctypes.c_void_p
+    c.fopen.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    c.fclose.argtypes = [ctypes.c_void_p]
+    return c
+
+
+def lime_write(path, records):
+    """Write [(type, bytes), ...] as one LIME message with the reference's
+    writer (limeCreateHeader / limeWriteRecordHeader / limeWriteRecordData)."""
+    L, c = lime_ref(), _libc()
+    fp = c.fopen(str(path).encode(), b"wb")
+    w = L.limeCreateWriter(fp)
+    for i, (t, data) in enumerate(records):
+        h = L.limeCreateHeader(int(i == 0), int(i == len(records) - 1), t.encode(), len(data))
+        assert L.limeWriteRecordHeader(h, w) == 0
+        n = ctypes.c_uint64(len(data))
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        assert L.limeWriteRecordData(buf, ctypes.byref(n), w) == 0
+        L.limeWriterCloseRecord(w)
+        L.limeDestroyHeader(h)
+    L.limeDestroyWriter(w)
+    c.fclose(fp)
+
+
+def lime_read(path):
+    """[(type, MB, ME, bytes), ...] of a LIME file, with the reference's reader."""
+    L, c = lime_ref(), _libc()
+    fp = c.fopen(str(path).encode(), b"rb")
+    r = L.limeCreateReader(fp)
+    out = []
+    while True:
+        st = L.limeReaderNextRecord(r)
+        if st == -7:  # LIME_EOF
+            break
+        assert st == 0, st
+        n = L.limeReaderBytes(r)
+        buf = ctypes.create_string_buffer(int(n))
+        nn = ctypes.c_uint64(n)
+        assert L.limeReaderReadData(buf, ctypes.byref(nn), r) == 0
+        out.append((L.limeReaderType(r).decode(), L.limeReaderMBFlag(r), L.limeReaderMEFlag(r),
+                    buf.raw[:n]))
+    L.limeDestroyReader(r)
+    c.fclose(fp)
+    return out
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -66,3 +66,59 @@ def test_bad_file_reports_error(reader, tmp_path):
     r = subprocess.run([reader, "2", "2", "2", "2", "2", str(p), "lexicographical", "double",
                         str(tmp_path / "o.bin")], capture_output=True, text=True)
     assert r.returncode == 1 and "short read" in r.stderr
+
+
+def _be(links):
+    return np.ascontiguousarray(links, dtype=">f8").tobytes()
+
+
+@pytest.mark.parametrize("group", [2, 3])
+def test_ildg_read_write(oracle, reader, tmp_path, group):
+    """ILDG (LIME) files: the "ildg-binary-data" record holds big-endian
+    doubles in the lexicographical order (data.h:144-176, FilePatternILDG
+    indexing.h:325-351).  Files written by the reference's own LIME writer
+    (c-lime, compiled from the reference sources into oracle/_ref) are read by
+    read_links(..., "ildg"), and files from write_links_ildg are read back by
+    the reference's LIME reader; without the reference checkout only the
+    self round trip runs."""
+    dims = (4, 2, 2, 4)
+    links = (oracle.gen_su2_config(dims, 0.5, 8) if group == 2
+             else oracle.gen_su3_config(dims, 0.4, 8))
+    N = links.shape[1]
+    have_ref = oracle.build_ref() is not None and oracle.lime_ref() is not None
+    lex = tmp_path / "lex.bin"
+    links.tofile(lex)
+    mine = tmp_path / "mine.lime"
+    subprocess.run([reader, str(group), *map(str, dims), str(lex), "lexicographical", "double",
+                    str(mine), "ildg"], check=True)
+    back = tmp_path / "back.bin"
+    subprocess.run([reader, str(group), *map(str, dims), str(mine), "ildg", "double", str(back)],
+                   check=True)
+    assert np.array_equal(np.fromfile(back).reshape(-1, N), links)
+    if not have_ref:
+        pytest.skip("reference LIME library not built (no reference checkout)")
+    recs = oracle.lime_read(mine)
+    assert [r[0] for r in recs] == ["ildg-format", "ildg-binary-data"]
+    assert (recs[0][1], recs[-1][2]) == (1, 1)  # message begin / end flags
+    assert recs[1][3] == _be(links)
+    theirs = tmp_path / "theirs.lime"
+    xml = b"<?xml version=\"1.0\"?><ildgFormat><field>gauge</field></ildgFormat>"
+    oracle.lime_write(theirs, [("ildg-format", xml), ("ildg-data-lfn", b"lfn://test"),
+                               ("ildg-binary-data", _be(links))])
+    out = tmp_path / "out.bin"
+    subprocess.run([reader, str(group), *map(str, dims), str(theirs), "ildg", "double", str(out)],
+                   check=True)
+    assert np.array_equal(np.fromfile(out).reshape(-1, N), links)
+
+
+def test_ildg_wrong_size_is_refused(oracle, reader, tmp_path):
+    dims = (4, 2, 2, 4)
+    links = oracle.gen_su2_config(dims, 0.5, 8)
+    lex = tmp_path / "lex.bin"
+    links.tofile(lex)
+    f = tmp_path / "f.lime"
+    subprocess.run([reader, "2", *map(str, dims), str(lex), "lexicographical", "double", str(f),
+                    "ildg"], check=True)
+    r = subprocess.run([reader, "2", "4", "2", "2", "2", str(f), "ildg", "double",
+                        str(tmp_path / "o.bin")], capture_output=True, text=True)
+    assert r.returncode == 1 and "ildg-binary-data holds" in r.stderr
