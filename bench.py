@@ -126,6 +126,68 @@ def monopole_measure(G, group, dims, eps, reps=20):
             "workload": f"generated links eps {eps} (unfixed), 3 kernels per call"}
 
 
+# the annealing schedule of the reference app's defaults
+# (apps/mag_su2/mag_fixation.cpp: T_init 2.5, T_final 0.01, T_step 0.01,
+# OR_steps 6, thermalization_steps 50)
+ANNEAL = dict(T_init=2.5, T_final=0.01, T_step=0.01, OR_steps=6, thermalization_steps=50)
+
+
+def anneal_sweeps(T_init, T_final, T_step, OR_steps, thermalization_steps):
+    """Sweeps of make_simulated_annealing (mag.cpp:1585-1603): the
+    thermalisation heat-bath sweeps, then per temperature one heat-bath and
+    OR_steps overrelaxation sweeps, steps of T_step / 4 in [1.2, 1.6]."""
+    n, T = thermalization_steps, T_init
+    while T > T_final / 2:
+        n += 1 + OR_steps
+        T -= T_step / 4 if (1.2 <= T <= 1.6 + T_step) else T_step
+    return n
+
+
+def annealing_measure(G, dims, seed=11):
+    """Simulated annealing of the spin form on the GPU (the reference app's
+    first and longest phase) with the app's default schedule."""
+    V = 1
+    for d in dims:
+        V *= d
+    with G.MagJob(group=2, dims=dims) as job:
+        job.generate(0.5, seed)
+        links = job.download()
+    with G.SpinSession(dims) as s:
+        s.upload_links(links)
+        s.generate_spins(seed)
+        s.simulated_annealing(1.0, 0.9, 0.1, 1, 0, seed)  # warm-up
+        t0 = time.perf_counter()
+        s.simulated_annealing(seed=seed, **ANNEAL)
+        t = time.perf_counter() - t0
+    n = anneal_sweeps(**ANNEAL)
+    return {"s": round(t, 4), "sweeps": n, "site_updates_per_s": V * n / t, "schedule": ANNEAL}
+
+
+def cpu_annealing(dims, seconds):
+    """The oracle's checkerboard heat-bath + 6 OR sweeps per temperature on
+    the host's threads (the algorithm the GPU runs), for about `seconds`."""
+    from oracle import oracle as O
+    O.build()
+    V = 1
+    for d in dims:
+        V *= d
+    links = O.gen_su2_config(dims, 0.5, 11)
+    spins = O.gen_spins_uniform(dims, 11)
+    steps, k, t0 = 0, 0, time.perf_counter()
+    while steps < 2 or time.perf_counter() - t0 < seconds:
+        for p in (0, 1):
+            O.su2_heatbath_halfsweep(links, spins, dims, p, 1.0, 11, k)
+        k += 1
+        for _ in range(ANNEAL["OR_steps"]):
+            for p in (0, 1):
+                O.su2_spin_halfsweep(links, spins, dims, p, 0)
+        steps += 1
+    t = time.perf_counter() - t0
+    return {"site_updates_per_s": V * (1 + ANNEAL["OR_steps"]) * steps / t,
+            "threads": O.get_threads(), "temperature_steps": steps,
+            "sample": "checkerboard heat bath + 6 OR sweeps per step (oracle), same lattice"}
+
+
 def _median_rate(fn, V, seconds, min_runs=5):
     """site-updates/s of fn() (one sweep): median over runs after one
     warm-up, for about `seconds` (at least min_runs runs)."""
@@ -450,6 +512,10 @@ def main():
                                   ("monopoles_su3_24^3x48", 3, (24, 24, 24, 48), 1.2)]:
             print(f"[bench] {name}", file=sys.stderr, flush=True)
             extra[name] = monopole_measure(G, group, d, e)
+        print("[bench] su2_32^4_annealing", file=sys.stderr, flush=True)
+        extra["su2_32^4_annealing"] = annealing_measure(G, dims)
+        if not args.no_cpu:
+            extra["su2_32^4_annealing"]["cpu"] = cpu_annealing(dims, 8.0)
 
     batched = None
     if not args.no_batch:
