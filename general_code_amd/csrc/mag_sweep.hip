@@ -329,6 +329,9 @@ __device__ __forceinline__ void renorm_rows(M2<T>& m) {
   }
 }
 
+#ifndef MAG_SU3_PTR_REMAT
+#define MAG_SU3_PTR_REMAT 1
+#endif
 #ifndef MAG_SU3_ROW2_REMAT
 #define MAG_SU3_ROW2_REMAT 1
 #endif
@@ -466,6 +469,18 @@ __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict_
       T res = cm_subgroup<T, 0, 1, GA, NL>(F, B, Gx, omega);
       res += cm_subgroup<T, 1, 2, GA, NL>(F, B, Gx, omega);
       res += cm_subgroup<T, 0, 2, GA, NL>(F, B, Gx, omega);
+#if MAG_SU3_PTR_REMAT
+      // the store addresses again from the 32-bit indices rather than 64-bit
+      // pointers kept live through the three subgroups
+#pragma unroll
+      for (int k = 0; k < NL; k++) {
+        const int mu = NL * q + k;
+        uint32_t hh = h, hbm = mu == 0 ? hb[0] : mu == 1 ? hb[1] : mu == 2 ? hb[2] : hb[3];
+        asm volatile("" : "+v"(hh), "+v"(hbm));
+        pf[k] = link_ptr<12>(U, g, p, mu, hh);
+        pb[k] = link_ptr<12>(U, g, 1 - p, mu, hbm);
+      }
+#endif
 #pragma unroll
       for (int k = 0; k < NL; k++) {
         stm(pf[k], F[k]);
