@@ -498,6 +498,7 @@ int mag_job_footprint(const mag_params* p, uint64_t* bytes) {
   }
   const uint64_t lanes = (uint64_t)(V / 2) * (p->group == MAG_SU2 ? 1 : 2);
   const uint64_t nw = (lanes + 63) / 64;  // partial triples per half-sweep
+  b += (uint64_t)4 << 20;  // pitch padding of the chunk arrays (4 KB per array)
   b += nc * (nw * 2 * 3 * 8 + (uint64_t)p->max_sweeps * 3 * 8 + sizeof(JobCtl)) + 3 * 8;
   *bytes = b;
   return MAG_OK;
@@ -519,7 +520,7 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   j->g = mag::make_geom(p->dims);
   j->nr = p->group == MAG_SU2 ? 4 : 12;
   j->nr_in = p->group == MAG_SU2 ? 4 : 18;
-  j->nreal = (size_t)8 * j->g.Vh * j->nr;
+  j->nreal = (size_t)8 * j->g.hp * j->nr;  // chunk arrays at the padded pitch
   j->nb = mag::sweep_blocks(j->g, p->group);
   j->nw = mag::sweep_waves(j->g, p->group);
   if (p->form == MAG_FORM_SPIN && p->group == MAG_SU3) {  // site form: lane pairs
@@ -531,7 +532,8 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   j->invV = 1.0 / (double)j->V;
   j->spin = p->form == MAG_FORM_SPIN;
   j->ncopies = j->spin ? p->copies : 1;
-  j->sstride = (size_t)2 * j->g.Vh * (p->group == MAG_SU2 ? 3 : 12);
+  // SU(2) spins: plain SoA at pitch Vh; SU(3) site gauges: chunk arrays
+  j->sstride = p->group == MAG_SU2 ? (size_t)2 * j->g.Vh * 3 : (size_t)2 * j->g.hp * 12;
   j->pstride = (size_t)2 * j->nw * 3;
 #define ALLOC(ptr, bytes)                                                          \
   do {                                                                             \
@@ -550,7 +552,7 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   ALLOC(j->U64, j->nreal * sizeof(double));
   if (p->precision == MAG_PREC_MIXED) {
     ALLOC(j->W32, j->nreal * sizeof(float));
-    if (!j->spin) ALLOC(j->G32, (size_t)2 * j->g.Vh * j->nr * sizeof(float));
+    if (!j->spin) ALLOC(j->G32, (size_t)2 * j->g.hp * j->nr * sizeof(float));
     if (j->spin && p->group == MAG_SU3) ALLOC(j->S32, j->sstride * j->ncopies * sizeof(float));
   }
   if (j->spin) ALLOC(j->S, j->sstride * j->ncopies * sizeof(double));

@@ -37,20 +37,20 @@ template <int NR, typename P>
 __device__ __forceinline__ LRef<P> link_ptr(P* U, const Geom& g, int p, int mu, uint32_t h) {
   using T = std::remove_const_t<P>;
   constexpr int CW = chunk_reals<T>(), NC = NR / CW;
-  return LRef<P>{U + ((size_t)(p * 4 + mu) * NC * g.Vh + h) * CW, (size_t)g.Vh * CW};
+  return LRef<P>{U + ((size_t)(p * 4 + mu) * NC * g.hp + h) * CW, (size_t)g.hp * CW};
 }
 // pm = p*4 + mu in [0, 8)
 template <int NR, typename P>
 __device__ __forceinline__ LRef<P> link_ptr_pm(P* U, const Geom& g, uint32_t pm, uint32_t h) {
   using T = std::remove_const_t<P>;
   constexpr int CW = chunk_reals<T>(), NC = NR / CW;
-  return LRef<P>{U + ((size_t)pm * NC * g.Vh + h) * CW, (size_t)g.Vh * CW};
+  return LRef<P>{U + ((size_t)pm * NC * g.hp + h) * CW, (size_t)g.hp * CW};
 }
 template <int NR, typename P>
 __device__ __forceinline__ LRef<P> site_ptr(P* G, const Geom& g, int p, uint32_t h) {
   using T = std::remove_const_t<P>;
   constexpr int CW = chunk_reals<T>(), NC = NR / CW;
-  return LRef<P>{G + ((size_t)p * NC * g.Vh + h) * CW, (size_t)g.Vh * CW};
+  return LRef<P>{G + ((size_t)p * NC * g.hp + h) * CW, (size_t)g.hp * CW};
 }
 
 // ------------------------------------------------------------------ su2 links

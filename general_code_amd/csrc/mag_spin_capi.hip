@@ -121,7 +121,7 @@ int mag_spin_session_create(const int* dims, int device, mag_spin_session** out)
   s->nb = mag::nblocks(s->g);
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMalloc((void**)&s->U, (size_t)V * 16 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&s->U, (size_t)8 * s->g.hp * 4 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&s->S, (size_t)V * 3 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&s->part, (size_t)2 * s->nb * 3 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&s->out3, 3 * sizeof(double));

@@ -57,7 +57,7 @@ void launch_scratch(const mag_job* j, bool fp32, void* U, void* G, double* part,
 int tune(const mag_job* j, bool fp32, Shape* out) {
   const size_t esz = fp32 ? sizeof(float) : sizeof(double);
   const size_t ubytes = j->nreal * esz;
-  const size_t gbytes = (size_t)2 * j->g.Vh * j->nr * sizeof(float);
+  const size_t gbytes = (size_t)2 * j->g.hp * j->nr * sizeof(float);
   void *U = nullptr, *G = nullptr;
   double* part = nullptr;
   JobCtl* ctl = nullptr;
@@ -143,7 +143,7 @@ int tune_batch(int group, bool fp32, const std::vector<mag_job*>& jobs, Shape* o
   size_t used = 0;
   hipError_t e = hipSuccess;
   for (const mag_job* j : jobs) {
-    const size_t ub = j->nreal * esz, gb = fp32 ? (size_t)2 * j->g.Vh * j->nr * sizeof(float) : 0;
+    const size_t ub = j->nreal * esz, gb = fp32 ? (size_t)2 * j->g.hp * j->nr * sizeof(float) : 0;
     if (!tab.empty() && used + ub + gb > kBatchScratch) break;
     used += ub + gb;
     BatchJob b{};
