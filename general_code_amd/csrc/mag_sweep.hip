@@ -202,8 +202,9 @@ __device__ __forceinline__ void wave_partial(double* __restrict__ part, uint32_t
 
 // ================================================================== SU(2)
 // Tile k of workgroup lb of a half-sweep over the sites of parity p, one
-// site per lane; the 8 links of an su2 site fit comfortably in registers (6
-// waves/SIMD in FP64), and measured faster than a lane pair.  The black
+// site per lane; the 8 links of an su2 site fit in registers (3 waves/SIMD
+// in FP64, enough to stream at the copy rate), and one site per lane
+// measured faster than a lane pair.  The black
 // (p = 1) half also sums the functional of the links just written.
 template <typename T, bool GA, int NT, int SPT>
 __device__ __forceinline__ void su2_half_block(T* __restrict__ U, T* __restrict__ G,
