@@ -71,3 +71,27 @@ def test_mixed_against_fp64(mag, oracle, group, dims, eps, omega, tol):
     assert dlink < 1e-3 and dlink_rms < 1e-5
     assert dW < 1e-12
     assert b["fp64_start"] > 100
+
+
+def test_su3_32_mixed_functional_within_1e10(mag):
+    """SU(3) 32^4 (eps 0.4, omega 1.6, tol 1e-11, cap 8000), the lattice of
+    the mixed-precision roofline target: the mixed run's functional within
+    1e-10 of the FP64 run's, plaquette of either run equal to the input's
+    to 1e-12 (GPU FP64 plaquette)."""
+    dims = (32, 32, 32, 32)
+    out = {}
+    for name, prec in (("fp64", mag.F64), ("mixed", mag.MIXED)):
+        with mag.MagJob(group=mag.SU3, dims=dims, precision=prec, omega=1.6, tol=1e-11,
+                        max_sweeps=8000, reunit_every=10, tol_mixed=1e-10, fp64_warmup=100) as j:
+            j.generate(0.4, 7)
+            P0 = j.plaquette()
+            j.run()  # False at the cap
+            s, conv, F = j.status()
+            out[name] = dict(sweeps=s, converged=conv, F=F, P=j.plaquette(), P0=P0)
+    a, b = out["fp64"], out["mixed"]
+    dF = abs(b["F"] - a["F"]) / a["F"]
+    print(json.dumps({"dims": dims, "sweeps_fp64": a["sweeps"], "sweeps_mixed": b["sweeps"],
+                      "F_fp64": a["F"], "rel_dF": dF, "dP_fp64": abs(a["P"] - a["P0"]),
+                      "dP_mixed": abs(b["P"] - b["P0"])}))
+    assert dF < 1e-10
+    assert abs(a["P"] - a["P0"]) < 1e-12 and abs(b["P"] - b["P0"]) < 1e-12
