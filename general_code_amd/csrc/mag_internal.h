@@ -13,12 +13,20 @@ constexpr int kWaves = kBlock / 64;
 // Lattice geometry in the parity-split layout.  Site (x,y,z,t) with
 // lexicographic index i = x + Lx(y + Ly(z + Lz t)) (indexing.h:111-123) has
 // parity p = (x+y+z+t)&1 and half index h = i>>1; every dimension is even.
-// Pitch padding of the 16-byte chunk arrays, in half-sites: consecutive
-// chunk arrays of a link or site field start 16*Vh + 4096 bytes apart.  With
-// a power-of-two pitch the many streams of one half-sweep (SU(3) FP64: 48
-// loads and 48 stores) fall on the same DRAM channels; measured on a
-// 48-stream copy, 4 KB of padding raised 4.3 to 5.3 TB/s.
+// Pitch padding of the chunk fields, in half-sites: consecutive fields of a
+// link or site array start NC*(16*Vh + 4096) bytes apart (Vh rounded up to
+// whole tiles).  With a power-of-two pitch the many streams of one
+// half-sweep fall on the same DRAM channels; measured on a 48-stream copy,
+// 4 KB of padding raised 4.3 to 5.3 TB/s.
 constexpr uint32_t kChunkPad = 256;
+
+// MAG_TILE_LAYOUT=1 (default): the chunks of 64 consecutive half-sites are
+// stored together (tile-interleaved); 0: one array per chunk index.  The
+// pitch is rounded up to whole tiles of 64 half-sites.
+#ifndef MAG_TILE_LAYOUT
+#define MAG_TILE_LAYOUT 1
+#endif
+constexpr uint32_t kTile = 64;
 
 struct Geom {
   int L0, L1, L2, L3;
@@ -27,7 +35,7 @@ struct Geom {
   uint32_t s1;  // half-index stride of y  (= hx)
   uint32_t s2;  // half-index stride of z  (= hx*L1)
   uint32_t s3;  // half-index stride of t  (= hx*L1*L2)
-  uint32_t hp;  // pitch of a chunk array in half-sites (Vh + kChunkPad)
+  uint32_t hp;  // pitch of a chunk field in half-sites (Vh in whole tiles + kChunkPad)
 };
 
 inline Geom make_geom(const int* d) {
@@ -38,7 +46,7 @@ inline Geom make_geom(const int* d) {
   g.s2 = g.hx * (uint32_t)d[1];
   g.s3 = g.s2 * (uint32_t)d[2];
   g.Vh = g.s3 * (uint32_t)d[3];
-  g.hp = g.Vh + kChunkPad;
+  g.hp = (MAG_TILE_LAYOUT ? (g.Vh + kTile - 1) / kTile * kTile : g.Vh) + kChunkPad;
   return g;
 }
 
