@@ -108,7 +108,7 @@ MONO_B_SITE = {2: (16 + 4) * 8 + (4 + 6) * 8 + (6 + 4) * 8,
                3: (48 + 12) * 8 + (12 + 18) * 8 + (18 + 12) * 8}
 
 
-def monopole_measure(G, group, dims, eps, reps=20):
+def monopole_measure(G, group, dims, eps, reps=20, cpu=False):
     """Device time of the abelian projection and monopole currents of a
     job's links (mag_job_monopole_timed: angles + plaquettes + currents,
     no host transfer) and its HBM rate against MONO_B_SITE."""
@@ -118,12 +118,36 @@ def monopole_measure(G, group, dims, eps, reps=20):
     with G.MagJob(group=group, dims=dims) as job:
         job.generate(eps, 5)
         ms = job.monopole_timed(reps)
-        nmono = int(abs(job.monopole_currents().round()).sum())
+        J = job.monopole_currents()
+        nmono = int(abs(J.round()).sum())
+        cl_ms = job.clusters_timed(reps)
+        recs = job.monopole_clusters()
+    recs = [recs] if group == 2 else recs
     gbs = V * MONO_B_SITE[group] / (ms * 1e-3) / 1e9
-    return {"ms": ms, "sites_per_s": V / (ms * 1e-3), "b_site": MONO_B_SITE[group],
-            "achieved_GBs": round(gbs, 1), "roofline_frac": round(gbs / PEAK_HBM_GBS, 4),
-            "current_links": nmono,
-            "workload": f"generated links eps {eps} (unfixed), 3 kernels per call"}
+    out = {"ms": ms, "sites_per_s": V / (ms * 1e-3), "b_site": MONO_B_SITE[group],
+           "achieved_GBs": round(gbs, 1), "roofline_frac": round(gbs / PEAK_HBM_GBS, 4),
+           "current_links": nmono,
+           "workload": f"generated links eps {eps} (unfixed), 3 kernels per call",
+           "clusters": {"ms": cl_ms, "count": [len(r) for r in recs],
+                        "largest": [int(r[:, 1].max()) if len(r) else 0 for r in recs],
+                        "kernels": "5 per colour (init, hook, flatten, sums, collect)"}}
+    if cpu:
+        out["clusters"]["cpu"] = cpu_clusters(J.reshape(-1, 4 * V), dims)
+    return out
+
+
+def cpu_clusters(J, dims):
+    """The oracle's restatement of the reference's cluster walk
+    (calculate_clusters, one host thread as in the reference) on the same
+    currents, every colour."""
+    import numpy as np
+    from oracle import oracle as O
+    O.build()
+    t0 = time.perf_counter()
+    for k in range(J.shape[0]):
+        O.monopole_clusters(np.ascontiguousarray(J[k]), dims)
+    return {"ms": (time.perf_counter() - t0) * 1e3, "threads": 1,
+            "sample": "the same currents, all colours, one run"}
 
 
 # the annealing schedule of the reference app's defaults
@@ -511,7 +535,7 @@ def main():
         for name, group, d, e in [("monopoles_su2_32^4", 2, dims, 1.5),
                                   ("monopoles_su3_24^3x48", 3, (24, 24, 24, 48), 1.2)]:
             print(f"[bench] {name}", file=sys.stderr, flush=True)
-            extra[name] = monopole_measure(G, group, d, e)
+            extra[name] = monopole_measure(G, group, d, e, cpu=not args.no_cpu)
         print("[bench] su2_32^4_annealing", file=sys.stderr, flush=True)
         extra["su2_32^4_annealing"] = annealing_measure(G, dims)
         if not args.no_cpu:

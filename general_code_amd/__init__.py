@@ -8,5 +8,5 @@ from .mag import (  # noqa: F401
     SPIN_HEATBATH, SPIN_OMEGA, SPIN_OR, SPIN_RELAX, SpinSession, device_count,
     SHAPES, TUNE_AUTO, TUNE_DEFAULT, TUNE_FIXED, functional_spin, gauge_transform_spins,
     tune_clear, tune_set, generate_spins_uniform, hbm_copy_bandwidth, lib, plaquette, su2_functional, su3_functional,
-    su3_monopole_currents, u1_monopole_currents,
+    su3_monopole_currents, u1_monopole_currents, current_clusters, CLUSTER_INTS,
 )

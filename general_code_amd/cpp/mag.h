@@ -22,8 +22,11 @@
 //   * the maximisation loops have an iteration cap (max_iter).
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cstdint>
+#include <map>
+#include <tuple>
 #include <cstring>
 #include <fstream>
 #include <stdexcept>
@@ -366,6 +369,168 @@ auto calculate_current(const Lattice& conf) {
     for (int c = 0; c < 3; c++) out[c].assign(J.begin() + c * n, J.begin() + (c + 1) * n);
     return out;
   }
+}
+
+// ---------------------------------------------------------------- clusters
+// A monopole cluster as the apps use it.  The reference's cluster is the
+// root of a tree of loop_new nodes (loop.h:21-32) that the apps only walk
+// through cluster_length, length_mu and currents_directions
+// (monopoles.cpp:1343-1449); the mirror's loop_new carries those sums,
+// computed on the GPU, with the coordinate of the cluster's first site.
+struct loop_new {
+  std::array<int, 4> coordinate{0, 0, 0, 0};
+  int length = 0;                          // sum of |charge|
+  std::array<int, 4> lengths_mu{0, 0, 0, 0};  // sum of charges per direction
+  int space = 0, time = 0;                 // |charge| over mu < 3, mu = 3
+  int sites = 0;                           // distinct sites (site_number)
+};
+
+namespace detail {
+inline std::vector<loop_new*> clusters_of(const int32_t* r, int64_t n, const int* d) {
+  std::vector<loop_new*> LL;
+  LL.reserve((size_t)n);
+  for (int64_t i = 0; i < n; i++, r += MAG_CLUSTER_INTS) {
+    auto* c = new loop_new;
+    int s = r[0];
+    for (int mu = 0; mu < 4; mu++) {
+      c->coordinate[mu] = s % d[mu];
+      s /= d[mu];
+    }
+    c->length = r[1];
+    for (int mu = 0; mu < 4; mu++) c->lengths_mu[mu] = r[2 + mu];
+    c->space = r[6];
+    c->time = r[7];
+    c->sites = r[8];
+    LL.push_back(c);
+  }
+  return LL;
+}
+// one cluster entry point, called with the exact capacity after a first
+// guess that was short
+template <class Call>
+std::vector<std::vector<loop_new*>> run_clusters(Call call, int ncol, const int* d) {
+  int64_t cap = 1 << 12;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    std::vector<int32_t> rec((size_t)ncol * cap * MAG_CLUSTER_INTS);
+    std::vector<int64_t> n(ncol, 0);
+    const int rc = call(rec.data(), cap, n.data());
+    int64_t most = 0;
+    for (int64_t v : n) most = v > most ? v : most;
+    if (rc == MAG_ERR_ARG && most > cap) {
+      cap = most;
+      continue;
+    }
+    check(rc);
+    std::vector<std::vector<loop_new*>> out(ncol);
+    for (int k = 0; k < ncol; k++)
+      out[k] = clusters_of(rec.data() + (size_t)k * cap * MAG_CLUSTER_INTS, n[k], d);
+    return out;
+  }
+  throw error(MAG_ERR_ARG, "cluster count changed between calls");
+}
+}  // namespace detail
+
+// calculate_clusters(J, data_pattern) (monopoles.cpp:1277-1303) on the GPU:
+// the clusters of the currents J[4*site + mu] in the reference's order.
+// J is left as it is (the reference zeroes it while walking).  The nodes
+// belong to the caller, as in the reference; free_clusters deletes them.
+template <class T>
+std::vector<loop_new*> calculate_clusters(const std::vector<T>& J, const DataPatternLexicographical& dp) {
+  const int d[4] = {dp.lat_dim[0], dp.lat_dim[1], dp.lat_dim[2], dp.lat_dim[3]};
+  if (J.size() != (size_t)dp.get_data_size())
+    throw error(MAG_ERR_ARG, "J has " + std::to_string(J.size()) + " values, the lattice 4V = " +
+                                 std::to_string(dp.get_data_size()));
+  const std::vector<double> Jd(J.begin(), J.end());
+  return detail::run_clusters(
+      [&](int32_t* rec, int64_t cap, int64_t* n) { return mag_current_clusters(d, 1, Jd.data(), rec, cap, n); },
+      1, d)[0];
+}
+
+inline void free_clusters(std::vector<loop_new*>& LL) {
+  for (loop_new* c : LL) delete c;
+  LL.clear();
+}
+// the observables of monopoles.cpp:1343-1449 and 1510-1531
+inline int cluster_length(const loop_new* c) { return c->length; }
+inline std::vector<int> length_mu(const loop_new* c) {
+  return {c->lengths_mu[0], c->lengths_mu[1], c->lengths_mu[2], c->lengths_mu[3]};
+}
+inline std::tuple<int, int> currents_directions(const loop_new* c) { return {c->space, c->time}; }
+inline int site_number(const loop_new* c) { return c->sites; }
+
+// group_percolating (monopoles.cpp:1533-1590): the first group of wrapped
+// clusters whose winding vectors sum to zero, searched over groups of 2, 3,
+// ... in the reference's enumeration.  That enumeration keeps cluster 0 in
+// every group; it runs the last position to the end, then moves up by one
+// the innermost position i >= 1 whose successor is more than 2 ahead and
+// sets the following positions right after it; when there is no such
+// position the groups of this size are done.  (So some groups, e.g.
+// {0, m-2, m-1} of three out of m, are never tried -- kept, since the app's
+// percolating labels depend on it.)
+inline std::vector<int> group_percolating(const std::vector<std::vector<int>>& w) {
+  const int m = (int)w.size();
+  for (int order = 2; order <= m; order++) {
+    std::vector<int> pos(order);
+    for (int i = 0; i < order; i++) pos[i] = i;
+    for (;;) {
+      bool zero = true;
+      for (int mu = 0; mu < 4 && zero; mu++) {
+        int sum = 0;
+        for (int i : pos) sum += w[i][mu];
+        zero = sum == 0;
+      }
+      if (zero) return pos;
+      if (pos[order - 1] < m - 1) {
+        pos[order - 1]++;
+        continue;
+      }
+      int i = order - 2;
+      while (i > 0 && pos[i + 1] - pos[i] <= 2) i--;
+      if (i <= 0) break;
+      pos[i]++;
+      for (int k = i + 1; k < order; k++) pos[k] = pos[i] + k - i;
+    }
+  }
+  return {};
+}
+
+// The tables apps/monopoles_su2/monopoles_su2.cpp:110-197 (and per colour
+// apps/monopoles_su3/monopoles_su3.cpp:117-213) write for one colour's
+// clusters: unwrapped clusters counted by length; wrapped clusters (some
+// length_mu != 0) with their windings length_mu / L (C++ division), longest
+// first -- equal lengths keep the cluster order (the reference's std::sort
+// leaves their order unspecified) -- and which of them form the
+// percolating group; the asymmetry (space/3 - time) / (space/3 + time).
+struct cluster_tables {
+  std::map<int, int> unwrapped;
+  std::vector<int> wrapped_lengths;
+  std::vector<std::vector<int>> wrappings;
+  std::vector<int> percolating;  // positions in wrapped_lengths
+  double asymmetry = 0;
+};
+inline cluster_tables make_cluster_tables(const std::vector<loop_new*>& LL, const std::array<int, 4>& dims) {
+  cluster_tables t;
+  int space = 0, time = 0;
+  std::vector<std::pair<int, std::vector<int>>> wrapped;
+  for (const loop_new* c : LL) {
+    space += c->space;
+    time += c->time;
+    const std::vector<int> lmu = length_mu(c);
+    if (lmu[0] == 0 && lmu[1] == 0 && lmu[2] == 0 && lmu[3] == 0) {
+      t.unwrapped[c->length]++;
+    } else {
+      wrapped.push_back({c->length, {lmu[0] / dims[0], lmu[1] / dims[1], lmu[2] / dims[2], lmu[3] / dims[3]}});
+    }
+  }
+  std::stable_sort(wrapped.begin(), wrapped.end(),
+                   [](const auto& a, const auto& b) { return a.first > b.first; });
+  for (auto& w : wrapped) {
+    t.wrapped_lengths.push_back(w.first);
+    t.wrappings.push_back(w.second);
+  }
+  if (t.wrappings.size() > 1) t.percolating = group_percolating(t.wrappings);
+  t.asymmetry = (space / 3. - time) / (space / 3. + time);
+  return t;
 }
 
 // ---------------------------------------------------------------- file formats

@@ -16,6 +16,11 @@
 //                       monopole currents of the fixed links' abelian
 //                       projection (calculate_current, monopoles.cpp:669-687)
 //                       as 4V doubles J[4*site + mu]
+//   --path_output_clusters_unwrapped, --path_output_clusters_wrapped,
+//   --path_output_windings, --path_output_monopoles P
+//                       the monopole cluster tables of the fixed links, in
+//                       the CSV formats of apps/monopoles_su2/
+//                       monopoles_su2.cpp:97-197 (clusters on the GPU)
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -31,6 +36,7 @@ int main(int argc, char** argv) {
   std::string path_conf, conf_format = "lexicographical", file_precision = "double";
   std::string path_conf_output, path_spins_output, path_previous, path_functional_output;
   std::string path_monopoles_output;
+  std::string path_clusters_unwrapped, path_clusters_wrapped, path_windings, path_monopoles;
   int bytes_skip = 0;
   double T_step = 0.01, T_init = 2.5, T_final = 0.01;
   int OR_steps = 6, thermalization_steps = 50;
@@ -79,6 +85,10 @@ int main(int argc, char** argv) {
     else if (a == "--synthetic_eps") synthetic_eps = std::stod(next());
     else if (a == "--link_form") std::istringstream(next()) >> link_form;
     else if (a == "--path_monopoles_output") path_monopoles_output = next();
+    else if (a == "--path_output_clusters_unwrapped") path_clusters_unwrapped = next();
+    else if (a == "--path_output_clusters_wrapped") path_clusters_wrapped = next();
+    else if (a == "--path_output_windings") path_windings = next();
+    else if (a == "--path_output_monopoles") path_monopoles = next();
     else if (a == "--omega") omega = std::stod(next());
     else if (a == "--tol") tol = std::stod(next());
     else if (a == "--max_sweeps") max_sweeps = std::stoi(next());
@@ -111,10 +121,13 @@ int main(int argc, char** argv) {
     std::cout << "plaquette " << plaket(conf) << std::endl;
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<double> J;
+    const bool want_clusters = !path_clusters_unwrapped.empty() || !path_clusters_wrapped.empty() ||
+                               !path_windings.empty() || !path_monopoles.empty();
+    const bool want_J = !path_monopoles_output.empty() || want_clusters;
     if (link_form) {
       std::vector<double> hist;
       const int sweeps = fix_mag_links(conf, omega, tol, max_sweeps, MAG_PREC_F64, &hist,
-                                       path_monopoles_output.empty() ? nullptr : &J);
+                                       want_J ? &J : nullptr);
       std::cout << "link-form sweeps " << sweeps << std::endl;
     } else {
       DataPatternLexicographical pattern(dims);
@@ -148,7 +161,7 @@ int main(int argc, char** argv) {
     std::cout << "final functional is " << MAG_functional_su2(conf) << std::endl;
     std::cout << "plaquette " << plaket(conf) << std::endl;
     if (!path_conf_output.empty()) write_links_lexicographical(conf, path_conf_output);
-    if (!path_monopoles_output.empty()) {
+    if (want_J) {
       if (J.empty()) {
         // spin path: the fixed links are on the host; project them as the
         // monopole app reads them (MatrixConverter<su2, abelian>)
@@ -157,12 +170,51 @@ int main(int argc, char** argv) {
           ab.array[l] = abelian{1.0, std::atan2(conf.array[l].a3, conf.array[l].a0)};
         J = calculate_current(ab);
       }
-      std::ofstream f(path_monopoles_output, std::ios::binary);
-      f.write(reinterpret_cast<const char*>(J.data()), (std::streamsize)(J.size() * sizeof(double)));
-      if (!f) throw error(MAG_ERR_ARG, "cannot write " + path_monopoles_output);
+      if (!path_monopoles_output.empty()) {
+        std::ofstream f(path_monopoles_output, std::ios::binary);
+        f.write(reinterpret_cast<const char*>(J.data()), (std::streamsize)(J.size() * sizeof(double)));
+        if (!f) throw error(MAG_ERR_ARG, "cannot write " + path_monopoles_output);
+      }
       long n = 0;
       for (double v : J) n += (long)std::lround(std::fabs(v));
       std::cout << "monopole current links " << n << std::endl;
+    }
+    if (want_clusters) {
+      DataPatternLexicographical pattern(dims);
+      std::vector<loop_new*> LL = calculate_clusters(J, pattern);
+      const cluster_tables t = make_cluster_tables(LL, dims);
+      std::cout << "monopole clusters " << LL.size() << std::endl;
+      free_clusters(LL);
+      auto open = [](const std::string& path) {
+        std::ofstream f(path);
+        if (!f) throw error(MAG_ERR_ARG, "cannot write " + path);
+        return f;  // default stream precision, as the reference app's files
+      };
+      if (!path_clusters_unwrapped.empty()) {
+        auto f = open(path_clusters_unwrapped);
+        f << "length,number" << std::endl;
+        for (const auto& [len, num] : t.unwrapped) f << len << "," << num << std::endl;
+      }
+      if (!path_clusters_wrapped.empty()) {
+        auto f = open(path_clusters_wrapped);
+        f << "length,x0_wrap,x1_wrap,x2_wrap,x3_wrap,percolating_group" << std::endl;
+        for (size_t i = 0; i < t.wrapped_lengths.size(); i++) {
+          const bool perc = std::find(t.percolating.begin(), t.percolating.end(), (int)i) != t.percolating.end();
+          f << t.wrapped_lengths[i];
+          for (int w : t.wrappings[i]) f << "," << w;
+          f << (perc ? ",percolating" : ",non-percolating") << std::endl;
+        }
+      }
+      if (!path_windings.empty()) {
+        // the reference app writes the header only: its winding maps are
+        // never filled (monopoles_su2.cpp:122-123, 186-191)
+        auto f = open(path_windings);
+        f << "winding_number,cluster_number,direction" << std::endl;
+      }
+      if (!path_monopoles.empty()) {
+        auto f = open(path_monopoles);
+        f << "asymmetry" << std::endl << t.asymmetry << std::endl;
+      }
     }
   } catch (const error& e) {
     std::cerr << "mag_fixation: " << e.what() << " (status " << e.code << ")" << std::endl;

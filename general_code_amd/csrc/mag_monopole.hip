@@ -15,6 +15,10 @@
 //   currents   J_mu(x) = forward differences at x + mu of the three
 //              plaquette planes orthogonal to mu, over 2 pi
 //              (get_current, monopoles.cpp:44-122)
+//   clusters   connected components of the current links (calculate_clusters,
+//              monopoles.cpp:1277-1303) with the sums the apps take over
+//              each cluster (cluster_length, length_mu,
+//              currents_directions, monopoles.cpp:1343-1449)
 // Every kernel runs one thread per lexicographic site (the plaquette and
 // current kernels, which read neighbours, in XCD-aware block order).
 // Between the kernels
@@ -31,7 +35,9 @@
 #include "mag_layout.h"
 #include "mag_math.h"
 
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace mag {
 namespace {
@@ -279,6 +285,287 @@ void launch_currents(int group, const double* th, const Lat& l, double* P, doubl
   }
 }
 
+// ------------------------------------------------------------------ clusters
+// The reference finds a cluster by walking from its first site along the
+// links that carry current out of each site reached (|J| > 0.3, charge
+// lround(J); find_current / find_paths / find_cluster, monopoles.cpp:
+// 1120-1250).  For conserved integer charges (in-flow = out-flow at every
+// site, which the currents of k_monopole_currents satisfy) every link of a
+// connected set of current links lies on a directed cycle, so that walk
+// reaches exactly the connected component: the clusters are the connected
+// components of the graph of current links, found here by a lock-free
+// union-find over all links at once (roots always the smallest site, so a
+// cluster's root is its first site in the reference's scan order).  The
+// hook kernel checks the premise and flags any site where it fails.
+//
+// acc[k V + r] of root r, k = 0..6: length, length_mu[4], time length,
+// sites (one array per counter, so that the counters of one root lie on
+// different cache lines)
+constexpr int kAcc = 7;
+constexpr int kSumBlock = 1024;  // threads per workgroup of the sums kernel
+
+__device__ __forceinline__ bool is_current(double j) { return j > 0.3 || j < -0.3; }
+// lround: nearest integer, halves away from zero
+__device__ __forceinline__ int charge_of(double j) { return (int)round(j); }
+
+__device__ __forceinline__ uint32_t bwd(const Lat& l, uint32_t s, const int* c, int mu) {
+  return c[mu] == 0 ? s + (uint32_t)(l.L[mu] - 1) * l.st[mu] : s - l.st[mu];
+}
+__device__ __forceinline__ uint32_t ld_par(uint32_t* par, uint32_t i) {
+  return __hip_atomic_load(par + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// root of i.  parent[i] <= i always and roots point to themselves, so the
+// walk ends within V steps (the bound raises the error flag otherwise).
+__device__ uint32_t cc_find(uint32_t* par, uint32_t i, uint32_t bound, int* err) {
+  for (uint32_t it = 0; it <= bound; it++) {
+    const uint32_t p = ld_par(par, i);
+    if (p == i) return i;
+    const uint32_t gp = ld_par(par, p);
+    // path halving: any ancestor is a valid parent, so a plain store does
+    // (a store that loses a race leaves a longer but valid path)
+    if (gp != p) __hip_atomic_store(par + i, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    i = gp;
+  }
+  atomicOr(err, 2);
+  return i;
+}
+// join the components of a and b: the larger root is hooked under the
+// smaller one; a failed hook means another thread hooked that root first
+// (at most V - 1 hooks succeed in all, so the loop ends)
+__device__ void cc_unite(uint32_t* par, uint32_t a, uint32_t b, uint32_t bound, int* err) {
+  for (uint32_t it = 0; it <= bound; it++) {
+    a = cc_find(par, a, bound, err);
+    b = cc_find(par, b, bound, err);
+    if (a == b) return;
+    const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+    if (atomicCAS(par + hi, hi, lo) == hi) return;
+  }
+  atomicOr(err, 2);
+}
+
+// parent of s: the smallest of s and its neighbours across current links
+// (already one union per site, without atomics)
+__global__ __launch_bounds__(kMonoBlock) void k_cc_init(const double* __restrict__ J, Lat l,
+                                                        uint32_t* __restrict__ par,
+                                                        int* __restrict__ acc, int* __restrict__ meta) {
+  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kMonoBlock + threadIdx.x;
+  if (s == 0) {
+    meta[0] = 0;  // error flags
+    meta[1] = 0;  // clusters
+  }
+  if (s >= l.V) return;
+  int c[4];
+  coords_of(l, s, c);
+  uint32_t m = s;
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) {
+    const uint32_t f = fwd(l, s, c, mu), b = bwd(l, s, c, mu);
+    if (is_current(J[4 * (size_t)s + mu]) && f < m) m = f;
+    if (is_current(J[4 * (size_t)b + mu]) && b < m) m = b;
+  }
+  par[s] = m;
+#pragma unroll
+  for (int k = 0; k < kAcc; k++) acc[(size_t)k * l.V + s] = 0;
+}
+
+// err bit 0: a site where the charges do not balance, or a link with
+// 0.3 < |J| < 0.5 (a current of charge 0)
+__global__ __launch_bounds__(kMonoBlock) void k_cc_hook(const double* __restrict__ J, Lat l,
+                                                        uint32_t* __restrict__ par, int* __restrict__ meta) {
+  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kMonoBlock + threadIdx.x;
+  if (s >= l.V) return;
+  int c[4];
+  coords_of(l, s, c);
+  int div = 0;
+  bool bad = false;
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) {
+    const double jf = J[4 * (size_t)s + mu], jb = J[4 * (size_t)bwd(l, s, c, mu) + mu];
+    if (is_current(jf)) {
+      bad |= charge_of(jf) == 0;
+      div += charge_of(jf);
+    }
+    if (is_current(jb)) div -= charge_of(jb);
+  }
+  if (bad || div != 0) atomicOr(meta, 1);
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++)
+    if (is_current(J[4 * (size_t)s + mu])) cc_unite(par, s, fwd(l, s, c, mu), l.V, meta);
+}
+
+__global__ __launch_bounds__(kMonoBlock) void k_cc_flatten(uint32_t V, uint32_t* __restrict__ par,
+                                                           int* __restrict__ meta) {
+  const uint32_t s = blockIdx.x * kMonoBlock + threadIdx.x;
+  if (s >= V) return;
+  par[s] = cc_find(par, s, V, meta);
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// per-root sums over the cluster's links (each link counted at its base
+// site) and over its sites (sites with a current on one of their 8 links).
+// Most current links usually belong to one large cluster, whose counters
+// every thread would hit: the lanes of a wave that share the root of its
+// first contributing lane add up in the wave, the waves of a workgroup
+// that share a root add up in LDS, and that sum takes one atomic per
+// counter; other lanes add their own.  Every lane takes part in the wave
+// sums (lanes past V contribute nothing).
+__global__ __launch_bounds__(kSumBlock) void k_cc_sums(const double* __restrict__ J, Lat l,
+                                                       const uint32_t* __restrict__ par,
+                                                       int* __restrict__ acc) {
+  constexpr int NW = kSumBlock / 64;
+  __shared__ uint32_t wr[NW];
+  __shared__ int wt[NW][kAcc];
+  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kSumBlock + threadIdx.x;
+  int v[kAcc] = {0, 0, 0, 0, 0, 0, 0};
+  uint32_t r = 0;
+  if (s < l.V) {
+    int c[4];
+    coords_of(l, s, c);
+    r = par[s];
+    bool site = false;
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) {
+      const double jf = J[4 * (size_t)s + mu];
+      site |= is_current(J[4 * (size_t)bwd(l, s, c, mu) + mu]);
+      if (!is_current(jf)) continue;
+      site = true;
+      const int q = charge_of(jf), aq = q < 0 ? -q : q;
+      v[0] += aq;
+      v[1 + mu] += q;
+      if (mu == 3) v[5] += aq;
+    }
+    v[6] = site ? 1 : 0;
+  }
+  const bool any = v[6] != 0;  // a site without a current link adds nothing
+  const uint64_t mask = __ballot(any);
+  const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
+  const int lead = mask ? __builtin_ctzll(mask) : 0;
+  const uint32_t r0 = __shfl(r, lead);
+  const bool mine = any && r == r0;
+#pragma unroll
+  for (int k = 0; k < kAcc; k++) {
+    const int t = wave_sum(mine ? v[k] : 0);
+    if (lane == 0) wt[w][k] = t;
+    if (any && !mine && v[k] != 0) atomicAdd(acc + (size_t)k * l.V + r, v[k]);
+  }
+  if (lane == 0) wr[w] = mask ? r0 : 0xffffffffu;
+  __syncthreads();
+  if (threadIdx.x < NW) {
+    // wave w' adds the wave sums of the waves of its root, if it is the
+    // first wave with that root
+    const uint32_t rw = wr[threadIdx.x];
+    bool first = rw != 0xffffffffu;
+    for (int u = 0; u < (int)threadIdx.x && first; u++) first = wr[u] != rw;
+    if (first) {
+      int t[kAcc] = {0, 0, 0, 0, 0, 0, 0};
+      for (int u = (int)threadIdx.x; u < NW; u++)
+        if (wr[u] == rw)
+#pragma unroll
+          for (int k = 0; k < kAcc; k++) t[k] += wt[u][k];
+#pragma unroll
+      for (int k = 0; k < kAcc; k++)
+        if (t[k] != 0) atomicAdd(acc + (size_t)k * l.V + rw, t[k]);
+    }
+  }
+}
+
+// one record per root: {first site, length, length_mu[4], space, time, sites}
+__global__ __launch_bounds__(kMonoBlock) void k_cc_collect(uint32_t V, const uint32_t* __restrict__ par,
+                                                           const int* __restrict__ acc,
+                                                           int32_t* __restrict__ rec, int* __restrict__ meta) {
+  const uint32_t s = blockIdx.x * kMonoBlock + threadIdx.x;
+  if (s >= V || par[s] != s) return;
+  int a[kAcc];
+#pragma unroll
+  for (int k = 0; k < kAcc; k++) a[k] = acc[(size_t)k * V + s];
+  if (a[6] == 0) return;
+  const int i = atomicAdd(meta + 1, 1);  // distinct roots: i < V
+  int32_t* o = rec + (size_t)MAG_CLUSTER_INTS * i;
+  o[0] = (int32_t)s;
+  o[1] = a[0];
+  o[2] = a[1];
+  o[3] = a[2];
+  o[4] = a[3];
+  o[5] = a[4];
+  o[6] = a[0] - a[5];
+  o[7] = a[5];
+  o[8] = a[6];
+}
+
+// device memory of one cluster call, freed on every exit path
+struct CcBufs {
+  uint32_t* par = nullptr;
+  int* acc = nullptr;
+  int32_t* rec = nullptr;
+  int* meta = nullptr;
+  ~CcBufs() {
+    if (par) (void)hipFree(par);
+    if (acc) (void)hipFree(acc);
+    if (rec) (void)hipFree(rec);
+    if (meta) (void)hipFree(meta);
+  }
+  int alloc(uint32_t V) {
+    HIPCHK(hipMalloc((void**)&par, (size_t)V * sizeof(uint32_t)));
+    HIPCHK(hipMalloc((void**)&acc, (size_t)kAcc * V * sizeof(int)));
+    HIPCHK(hipMalloc((void**)&rec, (size_t)MAG_CLUSTER_INTS * V * sizeof(int32_t)));
+    HIPCHK(hipMalloc((void**)&meta, 2 * sizeof(int)));
+    return MAG_OK;
+  }
+};
+
+// the five cluster kernels for the currents J ([4V]) of one colour
+void launch_clusters(const double* J, const Lat& l, CcBufs& b, hipStream_t st) {
+  const int nb = grid_of(l.V);
+  hipLaunchKernelGGL(k_cc_init, dim3(nb), dim3(kMonoBlock), 0, st, J, l, b.par, b.acc, b.meta);
+  hipLaunchKernelGGL(k_cc_hook, dim3(nb), dim3(kMonoBlock), 0, st, J, l, b.par, b.meta);
+  hipLaunchKernelGGL(k_cc_flatten, dim3(nb), dim3(kMonoBlock), 0, st, l.V, b.par, b.meta);
+  hipLaunchKernelGGL(k_cc_sums, dim3((l.V + kSumBlock - 1) / kSumBlock), dim3(kSumBlock), 0, st, J, l,
+                     b.par, b.acc);
+  hipLaunchKernelGGL(k_cc_collect, dim3(nb), dim3(kMonoBlock), 0, st, l.V, b.par, b.acc, b.rec, b.meta);
+}
+
+// clusters of ncol colours of device currents dJ ([ncol][4V]) -> caller's
+// records (colour k at records + k * cap * MAG_CLUSTER_INTS), sorted by
+// first site: the reference's order
+int device_clusters(const double* dJ, const Lat& l, int ncol, int32_t* records, int64_t cap,
+                    int64_t* n, hipStream_t st) {
+  CcBufs b;
+  CHK(b.alloc(l.V));
+  bool short_cap = false;
+  for (int k = 0; k < ncol; k++) {
+    launch_clusters(dJ + (size_t)k * 4 * l.V, l, b, st);
+    HIPCHK(hipGetLastError());
+    int meta[2];
+    HIPCHK(hipMemcpyAsync(meta, b.meta, sizeof(meta), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (meta[0] & 1)
+      return fail(MAG_ERR_ARG, "colour %d: currents not conserved (charges lround(J) of the links "
+                  "with |J| > 0.3 must balance at every site, none of them 0)", k);
+    if (meta[0] & 2) return fail(MAG_ERR_HIP, "colour %d: cluster union-find did not finish", k);
+    const int m = meta[1];
+    std::vector<int32_t> h((size_t)m * MAG_CLUSTER_INTS);
+    if (m > 0)
+      HIPCHK(hipMemcpyAsync(h.data(), b.rec, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<int> order(m);
+    for (int i = 0; i < m; i++) order[i] = i;
+    std::sort(order.begin(), order.end(),
+              [&](int x, int y) { return h[(size_t)MAG_CLUSTER_INTS * x] < h[(size_t)MAG_CLUSTER_INTS * y]; });
+    int32_t* out = records + (size_t)k * cap * MAG_CLUSTER_INTS;
+    for (int i = 0; i < m && i < cap; i++)
+      std::copy_n(&h[(size_t)MAG_CLUSTER_INTS * order[i]], MAG_CLUSTER_INTS, out + (size_t)MAG_CLUSTER_INTS * i);
+    n[k] = m;
+    short_cap |= m > cap;
+  }
+  if (short_cap) return fail(MAG_ERR_ARG, "more clusters than the capacity %lld (counts returned in n)", (long long)cap);
+  return MAG_OK;
+}
+
 // device buffers of one call, freed on every exit path
 struct Bufs {
   double* p[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -443,6 +730,77 @@ int mag_job_monopole_timed(mag_job* j, int reps, double* ms) {
   if (e != hipSuccess) return fail(MAG_ERR_HIP, "timing failed: %s", hipGetErrorString(e));
   *ms = (double)t / reps;
   return MAG_OK;
+}
+
+int mag_job_monopole_clusters(mag_job* j, int copy, int32_t* records, int64_t cap, int64_t* n) {
+  CHK(check_job(j, copy));
+  if ((!records && cap > 0) || cap < 0 || !n) return fail(MAG_ERR_ARG, "bad records / capacity / n");
+  int d[4];
+  lat_dims(j, d);
+  const Lat l = make_lat(d);
+  const int ncol = j->prm.group == MAG_SU2 ? 1 : 3;
+  Bufs b;
+  HIPCHK(hipMalloc((void**)&b.p[0], (size_t)4 * l.V * ncol * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&b.p[1], (size_t)6 * l.V * ncol * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&b.p[2], (size_t)4 * l.V * ncol * sizeof(double)));
+  if (j->spin) HIPCHK(hipMalloc((void**)&b.p[3], j->nreal * sizeof(double)));
+  CHK(job_angles(j, copy, b.p[0], b.p[3]));
+  launch_currents(j->prm.group, b.p[0], l, b.p[1], b.p[2], j->st);
+  HIPCHK(hipGetLastError());
+  return device_clusters(b.p[2], l, ncol, records, cap, n, j->st);
+}
+
+int mag_job_clusters_timed(mag_job* j, int reps, double* ms) {
+  CHK(check_job(j, 0));
+  if (reps < 1 || !ms) return fail(MAG_ERR_ARG, "bad argument");
+  int d[4];
+  lat_dims(j, d);
+  const Lat l = make_lat(d);
+  const int ncol = j->prm.group == MAG_SU2 ? 1 : 3;
+  Bufs b;
+  HIPCHK(hipMalloc((void**)&b.p[0], (size_t)4 * l.V * ncol * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&b.p[1], (size_t)6 * l.V * ncol * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&b.p[2], (size_t)4 * l.V * ncol * sizeof(double)));
+  if (j->spin) HIPCHK(hipMalloc((void**)&b.p[3], j->nreal * sizeof(double)));
+  CHK(job_angles(j, 0, b.p[0], b.p[3]));
+  launch_currents(j->prm.group, b.p[0], l, b.p[1], b.p[2], j->st);
+  CcBufs cb;
+  CHK(cb.alloc(l.V));
+  for (int k = 0; k < ncol; k++) launch_clusters(b.p[2] + (size_t)k * 4 * l.V, l, cb, j->st);  // warm-up
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  (void)hipEventRecord(e0, j->st);
+  for (int i = 0; i < reps; i++)
+    for (int k = 0; k < ncol; k++) launch_clusters(b.p[2] + (size_t)k * 4 * l.V, l, cb, j->st);
+  (void)hipEventRecord(e1, j->st);
+  hipError_t e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(MAG_ERR_HIP, "timing failed: %s", hipGetErrorString(e));
+  int meta[2];
+  HIPCHK(hipMemcpy(meta, cb.meta, sizeof(meta), hipMemcpyDeviceToHost));
+  if (meta[0]) return fail(MAG_ERR_HIP, "cluster kernels flagged %d", meta[0]);
+  *ms = (double)t / reps;
+  return MAG_OK;
+}
+
+int mag_current_clusters(const int* dims, int ncol, const double* J, int32_t* records, int64_t cap,
+                         int64_t* n) {
+  CHK(check_dims(dims));
+  if (ncol != 1 && ncol != 3) return fail(MAG_ERR_ARG, "ncol %d (1 or 3)", ncol);
+  if (!J || (!records && cap > 0) || cap < 0 || !n) return fail(MAG_ERR_ARG, "bad J / records / capacity / n");
+  const Lat l = make_lat(dims);
+  const size_t nj = (size_t)4 * l.V * ncol;
+  for (size_t i = 0; i < nj; i++)
+    if (!std::isfinite(J[i])) return fail(MAG_ERR_NONFINITE, "J[%zu] is %g", i, J[i]);
+  Bufs b;
+  hipStream_t st = nullptr;  // the null stream: one synchronous call
+  HIPCHK(hipMalloc((void**)&b.p[0], nj * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(b.p[0], J, nj * sizeof(double), hipMemcpyHostToDevice, st));
+  return device_clusters(b.p[0], l, ncol, records, cap, n, st);
 }
 
 int mag_u1_monopole_currents(const int* dims, const double* angles, double* J) {

@@ -158,6 +158,9 @@ SIGNATURES = {
     "mag_job_monopole_timed": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "mag_u1_monopole_currents": (ctypes.c_int, [_ip, _dp, _dp]),
     "mag_su3_monopole_currents": (ctypes.c_int, [_ip, _dp, _dp]),
+    "mag_job_monopole_clusters": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64, _vp]),
+    "mag_current_clusters": (ctypes.c_int, [_ip, ctypes.c_int, _dp, _vp, ctypes.c_int64, _vp]),
+    "mag_job_clusters_timed": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "mag_batch_step": (ctypes.c_int, [_vp, _ip, _ip]),
     "mag_batch_run": (ctypes.c_int, [_vp]),
     "mag_batch_replays_timed": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
@@ -477,6 +480,22 @@ class MagJob:
         _check(lib().mag_job_monopole_currents(self._h, int(copy), _dptr(out)))
         return out
 
+    def monopole_clusters(self, copy: int = 0):
+        """Clusters of the copy's monopole currents, computed on the device
+        (mag_job_monopole_clusters): int32 [n, 9] records {first site,
+        length, length_mu[4], space, time, sites} in the reference's order;
+        SU(3): a list of the three colours' records."""
+        ncol = 1 if self.params.group == SU2 else 3
+        recs = _clusters(lambda buf, cap, n: lib().mag_job_monopole_clusters(self._h, int(copy), buf, cap, n),
+                         ncol, self.params.volume)
+        return recs[0] if ncol == 1 else recs
+
+    def clusters_timed(self, reps: int) -> float:
+        """Mean device ms of the cluster kernels (copy 0, all colours)."""
+        ms = ctypes.c_double()
+        _check(lib().mag_job_clusters_timed(self._h, int(reps), ctypes.byref(ms)))
+        return ms.value
+
     def monopole_timed(self, reps: int) -> float:
         """Mean device ms of angles + plaquettes + currents (copy 0)."""
         ms = ctypes.c_double()
@@ -652,6 +671,39 @@ def su3_monopole_currents(dims, angles3) -> np.ndarray:
     out = np.empty((3, a.size // 3), dtype=np.float64)
     _check(lib().mag_su3_monopole_currents(_dims(dims), _dptr(a), _dptr(out)))
     return out
+
+
+CLUSTER_INTS = 9
+
+
+def _clusters(call, ncol, V):
+    """Run a cluster entry point with a first guess of the capacity and once
+    more with the exact count if that was short."""
+    cap = min(V, 1 << 16)
+    for _ in range(2):
+        buf = np.zeros((ncol, cap, CLUSTER_INTS), dtype=np.int32)
+        n = np.zeros(ncol, dtype=np.int64)
+        rc = call(buf.ctypes.data, cap, n.ctypes.data)
+        if rc == MAG_OK:
+            return [buf[k, :n[k]].copy() for k in range(ncol)]
+        if rc == MAG_ERR_ARG and n.max() > cap:
+            cap = int(n.max())
+            continue
+        _check(rc)
+    raise MagError(MAG_ERR_ARG, "cluster count changed between calls")
+
+
+def current_clusters(dims, J):
+    """calculate_clusters of host currents (mag_current_clusters): J [4V]
+    (one colour) or [3, 4V]; records as MagJob.monopole_clusters."""
+    J = np.ascontiguousarray(J, dtype=np.float64)
+    V = int(np.prod(dims))
+    ncol = 1 if J.ndim == 1 else J.shape[0]
+    if J.size != ncol * 4 * V:
+        raise ValueError(f"J has {J.size} values, expected {ncol} x 4V = {ncol * 4 * V}")
+    recs = _clusters(lambda buf, cap, n: lib().mag_current_clusters(_dims(dims), ncol, _dptr(J), buf, cap, n),
+                     ncol, V)
+    return recs[0] if J.ndim == 1 else recs
 
 
 # batches -------------------------------------------------------------------

@@ -29,6 +29,9 @@
  *                                    mag_job_monopole_currents
  *   calculate_current (su3_angles) . monopoles.cpp:646-667 -> mag_su3_monopole_currents,
  *                                    mag_job_monopole_currents
+ *   calculate_clusters + cluster_length / length_mu / currents_directions
+ *                                    monopoles.cpp:1277-1303, 1343-1449
+ *                                    -> mag_current_clusters, mag_job_monopole_clusters
  *   MatrixConverter<su2, abelian>,
  *   MatrixConverter<su3, su3_angles> matrix.h:1059-1062, 1082-1100 -> mag_job_abelian_angles
  *   (none: the reference has no link-form checkerboard MAG, no SU(3) MAG
@@ -290,6 +293,37 @@ int mag_job_monopole_timed(mag_job *job, int reps, double *ms);
  * with |angle| <= 1000 (else MAG_ERR_ARG). */
 int mag_u1_monopole_currents(const int *dims, const double *angles, double *J);
 int mag_su3_monopole_currents(const int *dims, const double *angles3, double *J3);
+
+/* Monopole clusters: the connected clusters of current links
+ * (calculate_clusters(J, data_pattern), monopoles.cpp:1277-1303) and, per
+ * cluster, what the monopole apps read from its tree (cluster_length,
+ * length_mu, currents_directions, monopoles.cpp:1343-1449; site_number,
+ * 1510-1531).  A link carries a current when |J| > 0.3, with charge
+ * lround(J) (monopoles.cpp:1105-1228).  One record of MAG_CLUSTER_INTS
+ * int32 per cluster:
+ *   {first site, length = sum |charge|, length_mu[4] = sum of charges per
+ *    direction, space = sum |charge| over mu < 3, time = over mu = 3,
+ *    number of sites}
+ * in the reference's order (by first site in lexicographic order; the
+ * first site is the cluster's smallest site index).  Records of colour k
+ * at records + k * cap * MAG_CLUSTER_INTS, n[k] = number of clusters of
+ * colour k; more than cap clusters: n is still set, cap records are
+ * written, MAG_ERR_ARG.  The charges must balance at every site (true of
+ * every current from the functions above) with no current of charge 0;
+ * otherwise MAG_ERR_ARG.
+ * mag_job_monopole_clusters: angles -> currents -> clusters of a job's
+ *   copy on the device (SU(2) one colour, SU(3) three), only the records
+ *   leave the GPU.
+ * mag_current_clusters: from host currents J ([ncol][4V], ncol 1 or 3),
+ *   as the apps call calculate_clusters (extents >= 2; J finite, else
+ *   MAG_ERR_NONFINITE).
+ * mag_job_clusters_timed: mean device time of the cluster kernels over
+ *   reps runs (copy 0's currents, all colours), no host transfer. */
+#define MAG_CLUSTER_INTS 9
+int mag_job_monopole_clusters(mag_job *job, int copy, int32_t *records, int64_t cap, int64_t *n);
+int mag_current_clusters(const int *dims, int ncol, const double *J, int32_t *records, int64_t cap,
+                         int64_t *n);
+int mag_job_clusters_timed(mag_job *job, int reps, double *ms);
 
 /* ------------------------------------------------------------ serving
  * No counterpart in the reference (one configuration per process,

@@ -38,6 +38,11 @@
  *                            Levi-Civita form -eps_{mu nu rho sigma}
  *                            d_nu P_{rho sigma}(x + mu) / 4 pi (numpy),
  *                            colour sum zero for SU(3)
+ *   monopole clusters ...... brute force: connected components of the
+ *                            current links by an independent union-find
+ *                            (scipy), totals equal to the sums over all
+ *                            links, hand-built loops, windings and charges
+ *                            with known answers
  */
 #include "mag_oracle.h"
 
@@ -1382,4 +1387,88 @@ void or_monopole_currents(const double *P, const int *dims, double *J) {
     J[4 * s + 2] = j2 / 2 / M_PI;
     J[4 * s + 3] = j3 / 2 / M_PI;
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* monopole clusters (monopoles.cpp:1120-1303, 1343-1449)               */
+/* ------------------------------------------------------------------ */
+/* The reference's walk, step for step: sites in lexicographic order; a
+ * site starts a cluster when one of its 8 links carries |J| > 0.3
+ * (find_current, 1120-1138); from the cluster's node list it follows, for
+ * every node in turn, the forward links (x, mu) with J > 0.3, then the
+ * backward links (x - mu, mu) with J < -0.3, mu = 0..3, zeroing each link
+ * it takes and adding the far site as a new node with charge lround(J)
+ * (find_paths, 1185-1228), until a pass adds no node (find_cluster,
+ * 1241-1250).  Per cluster it keeps what the observables sum over the
+ * tree's edges: cluster_length = sum |charge| (1343-1354), length_mu[mu] =
+ * sum charge over edges in direction mu (1380-1398), currents_directions =
+ * (sum |charge| over mu < 3, over mu = 3) (1423-1447), and the number of
+ * distinct sites of the nodes (site_number, 1510-1531).
+ * rec[9 n + .] = {first site, length, length_mu[4], space, time, sites};
+ * returns the number of clusters (records beyond cap are not written). */
+int64_t or_monopole_clusters(const double *J_in, const int *dims, int32_t *rec, int64_t cap) {
+  geom_t g = geom(dims);
+  const int64_t V = g.V;
+  double *J = (double *)malloc(sizeof(double) * 4 * V);
+  int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * 8 * V + 8);
+  int64_t *nxt = (int64_t *)malloc(sizeof(int64_t) * 8 * V + 8);
+  int64_t *mark = (int64_t *)malloc(sizeof(int64_t) * V);
+  memcpy(J, J_in, sizeof(double) * 4 * V);
+  for (int64_t s = 0; s < V; s++) mark[s] = -1;
+  int64_t n = 0;
+  for (int64_t s = 0; s < V; s++) {
+    int start = 0;
+    for (int mu = 0; mu < 4 && !start; mu++) {
+      if (fabs(J[4 * s + mu]) > 0.3) start = 1;
+      if (fabs(J[4 * shift_n(&g, s, mu, -1) + mu]) > 0.3) start = 1;
+    }
+    if (!start) continue;
+    int32_t length = 0, lmu[4] = {0, 0, 0, 0}, space = 0, time = 0, sites = 1;
+    mark[s] = n;
+    int64_t ncur = 1, nnext;
+    cur[0] = s;
+    do {
+      nnext = 0;
+      for (int64_t i = 0; i < ncur; i++) {
+        const int64_t x = cur[i];
+        for (int dir = 0; dir < 8; dir++) {
+          const int mu = dir & 3;
+          const int64_t l = dir < 4 ? 4 * x + mu : 4 * shift_n(&g, x, mu, -1) + mu;
+          const double j = J[l];
+          if (dir < 4 ? !(j > 0.3) : !(j < -0.3)) continue;
+          const int32_t q = (int32_t)lround(j);
+          J[l] = 0.;
+          const int64_t y = shift_n(&g, x, mu, dir < 4 ? 1 : -1);
+          nxt[nnext++] = y;
+          length += abs(q);
+          lmu[mu] += q;
+          if (mu == 3) time += abs(q);
+          else space += abs(q);
+          if (mark[y] != n) {
+            mark[y] = n;
+            sites++;
+          }
+        }
+      }
+      int64_t *t = cur;
+      cur = nxt;
+      nxt = t;
+      ncur = nnext;
+    } while (ncur > 0);
+    if (n < cap) {
+      int32_t *r = rec + 9 * n;
+      r[0] = (int32_t)s;
+      r[1] = length;
+      for (int mu = 0; mu < 4; mu++) r[2 + mu] = lmu[mu];
+      r[6] = space;
+      r[7] = time;
+      r[8] = sites;
+    }
+    n++;
+  }
+  free(J);
+  free(cur);
+  free(nxt);
+  free(mark);
+  return n;
 }

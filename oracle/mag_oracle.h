@@ -115,6 +115,9 @@ void or_su3_abelian_angles(const double *links18, const int *dims, double *angle
 void or_u1_monopole_plaquettes(const double *angles, const int *dims, double *out);  /* monopoles.cpp:415-457, [6][V] */
 void or_su3_monopole_plaquettes(const double *angles3, const int *dims, double *out); /* monopoles.cpp:368-413,844-884, [3][6][V] */
 void or_monopole_currents(const double *P, const int *dims, double *J);              /* monopoles.cpp:44-122, [4V] */
+/* clusters of the currents J[4V]: rec[9n] = {first site, length, length_mu[4], space, time, sites},
+ * in the reference's order; returns the count (monopoles.cpp:1120-1303, 1343-1449, 1510-1531) */
+int64_t or_monopole_clusters(const double *J, const int *dims, int32_t *rec, int64_t cap);
 
 /* ---------------- synthetic configurations (build-defined) ------------- */
 void or_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);

@@ -175,6 +175,8 @@ def lib():
             "or_u1_monopole_plaquettes": (None, [_dp, _ip, _dp]),
             "or_su3_monopole_plaquettes": (None, [_dp, _ip, _dp]),
             "or_monopole_currents": (None, [_dp, _ip, _dp]),
+            "or_monopole_clusters": (ctypes.c_int64, [_dp, _ip, ctypes.POINTER(ctypes.c_int32),
+                                                      ctypes.c_int64]),
             "or_philox4x32_10": (None, [_u32p, _u32p, _u32p]),
             "or_gen_su2_config": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64]),
             "or_gen_su3_config": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64]),
@@ -448,6 +450,78 @@ def monopole_currents(plaq, dims):
     out = np.empty(4 * volume(dims))
     lib().or_monopole_currents(_d(plaq), _dims(dims), _d(out))
     return out
+
+
+CLUSTER_FIELDS = ("site", "length", "l0", "l1", "l2", "l3", "space", "time", "sites")
+
+
+def monopole_clusters(J, dims):
+    """Clusters of the currents J[4V] in the reference's order
+    (calculate_clusters, monopoles.cpp:1277-1303): int32 [n, 9] rows
+    {first site, length, length_mu[4], space, time, sites}."""
+    J = np.ascontiguousarray(J, dtype=np.float64)
+    V = volume(dims)
+    assert J.shape == (4 * V,)
+    rec = np.zeros((V, 9), dtype=np.int32)  # roots are distinct sites: n <= V
+    n = lib().or_monopole_clusters(_d(J), _dims(dims),
+                                   rec.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), V)
+    return rec[:n].copy()
+
+
+def group_percolating(wrappings):
+    """The reference's search for a group of wrapped clusters whose winding
+    vectors sum to zero (monopoles.cpp:1533-1590), restated with its
+    enumeration: groups of 2, 3, ... clusters, the first cluster always in
+    the group, the other positions advanced from the last one, an inner
+    position advanced only while its successor is more than 2 ahead."""
+    w = [list(x) for x in wrappings]
+    if len(w) < 2:
+        return []
+    last = len(w) - 1
+    for order in range(2, len(w) + 1):
+        pos = list(range(order))
+        while True:
+            if all(sum(w[i][mu] for i in pos) == 0 for mu in range(4)):
+                return pos
+            if pos[-1] < last:
+                pos[-1] += 1
+                continue
+            moved = False
+            for i in range(order - 2, 0, -1):
+                if pos[i + 1] - pos[i] > 2:
+                    pos[i] += 1
+                    for j in range(i + 1, order):
+                        pos[j] = pos[i] + j - i
+                    moved = True
+                    break
+            if not moved:
+                break
+    return []
+
+
+def cluster_summary(rec, dims):
+    """What apps/monopoles_su2/monopoles_su2.cpp:110-197 writes for one set of
+    clusters: unwrapped {length: count}; wrapped clusters as (length,
+    windings[4], percolating) sorted by length, longest first (ties in
+    cluster order); asymmetry (space/3 - time)/(space/3 + time)."""
+    unwrapped = {}
+    wrapped = []
+    space = int(rec[:, 6].sum()) if len(rec) else 0
+    time = int(rec[:, 7].sum()) if len(rec) else 0
+    for r in rec:
+        lmu = [int(v) for v in r[2:6]]
+        if all(v == 0 for v in lmu):
+            unwrapped[int(r[1])] = unwrapped.get(int(r[1]), 0) + 1
+        else:
+            # C++ integer division truncates toward zero
+            wrapped.append((int(r[1]), [int(np.fix(lmu[mu] / dims[mu])) for mu in range(4)]))
+    wrapped.sort(key=lambda t: -t[0])
+    perc = group_percolating([w for _, w in wrapped]) if len(wrapped) > 1 else []
+    den = space / 3.0 + time
+    asym = (space / 3.0 - time) / den if den != 0 else float("nan")
+    return {"unwrapped": dict(sorted(unwrapped.items())),
+            "wrapped": [(L, w, i in perc) for i, (L, w) in enumerate(wrapped)],
+            "asymmetry": asym}
 
 
 def philox(ctr, key):

@@ -69,9 +69,14 @@ def test_monopole_output(oracle, tmp_path, link_form):
     (mag_job_monopole_currents), spin path from the host links through
     calculate_current; both against the oracle on the written links."""
     outp, mono = tmp_path / "fixed.bin", tmp_path / "J.bin"
+    tabs = {k: tmp_path / f"{k}.csv" for k in ("unwrapped", "wrapped", "windings", "monopoles")}
     args = ["--synthetic_eps", "1.5", "--seed", "7", "--x_size", "8", "--y_size", "8",
             "--z_size", "8", "--t_size", "8", "--path_conf_output", str(outp),
-            "--path_monopoles_output", str(mono)]
+            "--path_monopoles_output", str(mono),
+            "--path_output_clusters_unwrapped", str(tabs["unwrapped"]),
+            "--path_output_clusters_wrapped", str(tabs["wrapped"]),
+            "--path_output_windings", str(tabs["windings"]),
+            "--path_output_monopoles", str(tabs["monopoles"])]
     if link_form:
         args += ["--link_form", "1", "--omega", "1.7", "--tol", "1e-10", "--max_sweeps", "20000"]
     else:
@@ -87,3 +92,18 @@ def test_monopole_output(oracle, tmp_path, link_form):
     assert np.abs(J - Jr).max() < 1e-12
     n = int(np.abs(np.round(J)).sum())
     assert n == _num(out, "monopole current links")[0]
+    # the cluster tables, in the CSV formats of apps/monopoles_su2/monopoles_su2.cpp:97-197
+    rec = oracle.monopole_clusters(J, dims)
+    assert len(rec) == _num(out, "monopole clusters")[0]
+    summ = oracle.cluster_summary(rec, dims)
+    lines = tabs["unwrapped"].read_text().splitlines()
+    assert lines[0] == "length,number"
+    assert {int(a): int(b) for a, b in (l.split(",") for l in lines[1:])} == summ["unwrapped"]
+    lines = tabs["wrapped"].read_text().splitlines()
+    assert lines[0] == "length,x0_wrap,x1_wrap,x2_wrap,x3_wrap,percolating_group"
+    got = [(int(f[0]), [int(v) for v in f[1:5]], f[5] == "percolating") for f in (l.split(",") for l in lines[1:])]
+    assert got == summ["wrapped"]
+    assert tabs["windings"].read_text().splitlines() == ["winding_number,cluster_number,direction"]
+    lines = tabs["monopoles"].read_text().splitlines()
+    assert lines[0] == "asymmetry"
+    assert float(lines[1]) == pytest.approx(summ["asymmetry"], rel=1e-5)  # 6 digits, as the app
