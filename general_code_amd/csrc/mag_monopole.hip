@@ -389,8 +389,15 @@ __global__ __launch_bounds__(kMonoBlock) void k_cc_hook(const double* __restrict
   }
   if (bad || div != 0) atomicOr(meta, 1);
 #pragma unroll
-  for (int mu = 0; mu < 4; mu++)
-    if (is_current(J[4 * (size_t)s + mu])) cc_unite(par, s, fwd(l, s, c, mu), l.V, meta);
+  for (int mu = 0; mu < 4; mu++) {
+    if (!is_current(J[4 * (size_t)s + mu])) continue;
+    const uint32_t f = fwd(l, s, c, mu);
+    // already joined when f hangs under s (usually from k_cc_init: s is the
+    // smallest current neighbour of f).  A plain load may return an older
+    // parent, but every parent f ever had is an ancestor of f.
+    if (par[f] == s) continue;
+    cc_unite(par, s, f, l.V, meta);
+  }
 }
 
 __global__ __launch_bounds__(kMonoBlock) void k_cc_flatten(uint32_t V, uint32_t* __restrict__ par,
