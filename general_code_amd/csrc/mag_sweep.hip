@@ -484,8 +484,13 @@ __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict_
 #endif
 #pragma unroll
       for (int k = 0; k < NL; k++) {
-        stm(pf[k], F[k]);
-        stm(pb[k], B[k]);
+        if constexpr (sizeof(T) == 8) {
+          stm_stream(pf[k], F[k]);
+          stm_stream(pb[k], B[k]);
+        } else {
+          stm(pf[k], F[k]);
+          stm(pb[k], B[k]);
+        }
       }
       if (GA && q == 0) stm(gp, Gx);
       if (p == 1) {
