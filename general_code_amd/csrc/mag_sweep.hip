@@ -39,15 +39,16 @@ __device__ void reduce_job(const double* __restrict__ part, int nparts, int mode
                            double* out3) {
   double a = 0.0, b = 0.0, c = 0.0;
   int i = threadIdx.x;
-  // four triples in flight per thread, combined in index order
-  for (; i + 3 * kRedBlock < nparts; i += 4 * kRedBlock) {
-    double x[4][3];
+  // kRedUnroll triples in flight per thread, combined in index order
+  constexpr int kRedUnroll = 8;
+  for (; i + (kRedUnroll - 1) * kRedBlock < nparts; i += kRedUnroll * kRedBlock) {
+    double x[kRedUnroll][3];
 #pragma unroll
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < kRedUnroll; k++)
 #pragma unroll
       for (int j = 0; j < 3; j++) x[k][j] = part[3 * (i + k * kRedBlock) + j];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < kRedUnroll; k++) {
       a += x[k][0];
       b += x[k][1];
       c = fmax(c, x[k][2]);
