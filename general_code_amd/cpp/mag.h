@@ -287,7 +287,7 @@ double plaket(const Lattice& conf) {
 }
 
 // ---------------------------------------------------------------- link form
-// The north-star path: checkerboard omega-overrelaxation on the links
+// The link-form path: checkerboard omega-overrelaxation on the links
 // (SU(3): Cabibbo-Marinari), FP64 or mixed precision, in place.  Returns
 // the number of sweeps; the functional history is in *history if given.
 // With `currents`, also the monopole currents of the fixed links'
