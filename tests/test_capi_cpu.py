@@ -52,3 +52,32 @@ def test_bad_params_rejected_without_gpu(mag):
         assert e.code == -2
     else:
         raise AssertionError("odd extent accepted")
+
+
+def test_monopole_and_cluster_arguments_rejected_without_gpu(mag):
+    """The host checks of the monopole and cluster entry points run before
+    any HIP call, so they hold on a machine without a GPU."""
+    import numpy as np
+    L = mag.lib()
+    dims = (4, 4, 4, 4)
+    d = (ctypes.c_int * 4)(*dims)
+    V = 256
+    J = np.zeros(4 * V)
+    rec = np.zeros((1, 9), dtype=np.int32)
+    n = np.zeros(3, dtype=np.int64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    Jp = J.ctypes.data_as(dp)
+    assert L.mag_current_clusters(None, 1, Jp, rec.ctypes.data, 1, n.ctypes.data) == -1
+    assert L.mag_current_clusters((ctypes.c_int * 4)(4, 1, 4, 4), 1, Jp, rec.ctypes.data, 1, n.ctypes.data) == -2
+    assert L.mag_current_clusters(d, 2, Jp, rec.ctypes.data, 1, n.ctypes.data) == -1
+    assert L.mag_current_clusters(d, 1, Jp, rec.ctypes.data, -1, n.ctypes.data) == -1
+    assert L.mag_current_clusters(d, 1, Jp, rec.ctypes.data, 1, None) == -1
+    bad = J.copy()
+    bad[17] = np.inf
+    assert L.mag_current_clusters(d, 1, bad.ctypes.data_as(dp), rec.ctypes.data, 1, n.ctypes.data) == -4
+    ang = np.zeros(4 * V)
+    ang[3] = np.nan
+    assert L.mag_u1_monopole_currents(d, ang.ctypes.data_as(dp), Jp) == -4
+    ang[3] = 1001.0
+    assert L.mag_u1_monopole_currents(d, ang.ctypes.data_as(dp), Jp) == -1
+    assert "1001" in L.mag_last_error().decode()
