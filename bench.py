@@ -517,6 +517,7 @@ def main():
             ("su3_32^4_mixed", 3, dims, G.MIXED, 0.4, 1.6),
             ("su3_32^4_fp64", 3, dims, G.F64, 0.4, 1.6),
             ("su3_24^3x48_mixed", 3, (24, 24, 24, 48), G.MIXED, 0.4, 1.6),
+            ("su3_24^3x48_fp64", 3, (24, 24, 24, 48), G.F64, 0.4, 1.6),
         ]:
             print(f"[bench] {name}", file=sys.stderr, flush=True)
             r = measure(G, group, d, prec, e, om, args.steps, args.warmup)
