@@ -754,66 +754,146 @@ void launch_copies_half_batch(int ncopies, bool fp32, const BatchJob* jobs, int 
 
 // ================================================================== SU(3) gauge copies
 // Site form of SU(3) (the SU(3) counterpart of the spin form): the links U
-// stay read-only and each gauge copy c holds a site gauge field G_c.  A
-// site's effective links
-//   forward  G_c(x) U_x,mu G_c(x+mu)^+,   backward  G_c(x-mu) U_x-mu,mu G_c(x)^+
-// are formed in registers and updated by the Cabibbo-Marinari code of the
-// link form (cm_subgroup, accumulating r into G_c(x)); only G_c(x) is
-// written.  Copy c therefore follows the link-form trajectory of the links
-// G_c U G_c^+, and all copies of a configuration share the link reads
-// (copy 0 reads U from HBM, the others mostly from cache).  Two lanes per
-// site as in the link form.
+// stay read-only and each gauge copy c holds a site gauge field G_c.  The
+// site's effective links are
+//   forward  W_f = G X,  X = U_x,mu G_c(x+mu)^+      backward  W_b = Y G^+,  Y = G_c(x-mu) U_x-mu,mu
+// with G = G_c(x).  Their diagonals are bilinear in the rows g_i of G:
+// (W_f)_ii = g_i . x_i (x_i = column i of X) and (W_b)_ii = y_i . conj(g_i)
+// (y_i = row i of Y), so every quantity the Cabibbo-Marinari update of G
+// needs is a Hermitian form of three 3x3 matrices that do not depend on G:
+//   H^(i) = sum_fwd x_i x_i^+ + sum_bwd conj(y_i) y_i^T.
+// For the subgroup (I,J), with D = H^(I) - H^(J), the field of the link
+// form's block sums (blk_fwd / blk_bwd) is
+//   h0 - i h1 = g_I D g_J^+,   h2 = (g_I D g_I^+ - g_J D g_J^+) / 2,
+// and the update rotates rows I, J of G.  A site builds H once from its 8
+// links (X and Y each from two products of rows and the unitary row-2 rule)
+// and then updates only G; only G is written.  Copy c follows the
+// link-form trajectory of the links G_c U G_c^+; all copies of a
+// configuration share the link reads.  Two lanes per site as in the link
+// form: each builds H from two directions, the pair adds them.
 template <typename T>
-__device__ __forceinline__ M3<T> full3t(const M2<T>& m) {
-  M3<T> M;
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int k = 0; k < 3; k++) M.m[i][k] = m.r[i][k];
-  row2_of(m, M.m[2]);
-  return M;
+struct Herm3 {
+  T d0, d1, d2;       // diagonal
+  C<T> o01, o02, o12;  // upper off-diagonal (H_ba = conj(H_ab))
+};
+// H += v v^+  (H_ab += v_a conj(v_b))
+template <typename T>
+__device__ __forceinline__ void herm_add_col(Herm3<T>& H, C<T> v0, C<T> v1, C<T> v2) {
+  H.d0 += cnorm2(v0);
+  H.d1 += cnorm2(v1);
+  H.d2 += cnorm2(v2);
+  H.o01 = cadd(H.o01, cmulc(v0, v1));
+  H.o02 = cadd(H.o02, cmulc(v0, v2));
+  H.o12 = cadd(H.o12, cmulc(v1, v2));
 }
-// A B^+
+// H += conj(y) y^T  (H_ab += conj(y_a) y_b)
 template <typename T>
-__device__ __forceinline__ M3<T> mul3_dag(const M3<T>& A, const M3<T>& B) {
-  M3<T> R;
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      C<T> acc = cmulc(A.m[i][0], B.m[j][0]);
-      acc = cadd(acc, cmulc(A.m[i][1], B.m[j][1]));
-      R.m[i][j] = cadd(acc, cmulc(A.m[i][2], B.m[j][2]));
-    }
-  return R;
+__device__ __forceinline__ void herm_add_row(Herm3<T>& H, C<T> y0, C<T> y1, C<T> y2) {
+  H.d0 += cnorm2(y0);
+  H.d1 += cnorm2(y1);
+  H.d2 += cnorm2(y2);
+  H.o01 = cadd(H.o01, cmulc(y1, y0));
+  H.o02 = cadd(H.o02, cmulc(y2, y0));
+  H.o12 = cadd(H.o12, cmulc(y2, y1));
 }
-// rows 0, 1 of A B
+// r = D w^+ (a column: r_a = sum_b D_ab conj(w_b))
 template <typename T>
-__device__ __forceinline__ M2<T> rows_mul(const M3<T>& A, const M3<T>& B) {
-  M2<T> R;
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      C<T> acc = cmul(A.m[i][0], B.m[0][j]);
-      acc = cadd(acc, cmul(A.m[i][1], B.m[1][j]));
-      R.r[i][j] = cadd(acc, cmul(A.m[i][2], B.m[2][j]));
-    }
-  return R;
+__device__ __forceinline__ void herm_mul_dag(const Herm3<T>& D, const C<T>* w, C<T>* r) {
+  const C<T> c0 = cconj(w[0]), c1 = cconj(w[1]), c2 = cconj(w[2]);
+  r[0] = cadd(cadd(C<T>{D.d0 * c0.re, D.d0 * c0.im}, cmul(D.o01, c1)), cmul(D.o02, c2));
+  r[1] = cadd(cadd(cmul(cconj(D.o01), c0), C<T>{D.d1 * c1.re, D.d1 * c1.im}), cmul(D.o12, c2));
+  r[2] = cadd(cadd(cmul(cconj(D.o02), c0), cmul(cconj(D.o12), c1)), C<T>{D.d2 * c2.re, D.d2 * c2.im});
 }
-// (2x3 R) B^+
 template <typename T>
-__device__ __forceinline__ M2<T> rows_mul_dag(const M2<T>& A, const M3<T>& B) {
-  M2<T> R;
+__device__ __forceinline__ C<T> dot3(const C<T>* u, const C<T>* r) {
+  return cadd(cadd(cmul(u[0], r[0]), cmul(u[1], r[1])), cmul(u[2], r[2]));
+}
+template <typename T>
+__device__ __forceinline__ void herm_pair_sum(Herm3<T>& H) {
+  H.d0 += pair_xchg(H.d0);
+  H.d1 += pair_xchg(H.d1);
+  H.d2 += pair_xchg(H.d2);
+  H.o01.re += pair_xchg(H.o01.re);
+  H.o01.im += pair_xchg(H.o01.im);
+  H.o02.re += pair_xchg(H.o02.re);
+  H.o02.im += pair_xchg(H.o02.im);
+  H.o12.re += pair_xchg(H.o12.re);
+  H.o12.im += pair_xchg(H.o12.im);
+}
+
+// subgroup (I,J) update of the site gauge G (3 rows); returns 1 - cos(theta)
+template <typename T, int I, int J>
+__device__ __forceinline__ T site_subgroup(const Herm3<T>* H, C<T> (&G)[3][3], T omega) {
+  Herm3<T> D;
+  D.d0 = H[I].d0 - H[J].d0;
+  D.d1 = H[I].d1 - H[J].d1;
+  D.d2 = H[I].d2 - H[J].d2;
+  D.o01 = csub(H[I].o01, H[J].o01);
+  D.o02 = csub(H[I].o02, H[J].o02);
+  D.o12 = csub(H[I].o12, H[J].o12);
+  C<T> rJ[3], rI[3];
+  herm_mul_dag(D, G[J], rJ);
+  herm_mul_dag(D, G[I], rI);
+  const C<T> z = dot3(G[I], rJ);
+  const T h0 = z.re, h1 = -z.im;
+  const T h2 = T(0.5) * (dot3(G[I], rI).re - dot3(G[J], rJ).re);
+  T res;
+  const Q<T> gq = omega_gauge(h0, h1, h2, omega, res);
+  // r = [[g0 + i g3, g2 + i g1], [-g2 + i g1, g0 - i g3]], g3 = 0 (as in cm_subgroup)
+  const C<T> r00{gq.a0, T(0)}, r01{gq.a2, gq.a1}, r10{-gq.a2, gq.a1}, r11{gq.a0, T(0)};
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int c = 0; c < 3; c++) {
+    const C<T> a = G[I][c], b = G[J][c];
+    G[I][c] = cadd(cmul(r00, a), cmul(r01, b));
+    G[J][c] = cadd(cmul(r10, a), cmul(r11, b));
+  }
+  return res;
+}
+
+// the contributions of one direction to H^(0..2): the forward link U_f
+// with the site gauge G_f at x + mu, the backward link U_b with G_b at x - mu
+template <typename T>
+__device__ __forceinline__ void site_fwd_field(Herm3<T>* H, const M2<T>& Uf, const M2<T>& Gf) {
+  {
+    // X = U_f G_f^+: rows 0, 1 from the stored rows of U_f and all rows of
+    // G_f, row 2 = conj(row0 x row1)
+    C<T> gf2[3];
+    row2_of(Gf, gf2);
+    M2<T> X;
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-      C<T> acc = cmulc(A.r[i][0], B.m[j][0]);
-      acc = cadd(acc, cmulc(A.r[i][1], B.m[j][1]));
-      R.r[i][j] = cadd(acc, cmulc(A.r[i][2], B.m[j][2]));
-    }
-  return R;
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        const C<T>* gb = b < 2 ? Gf.r[b] : gf2;
+        X.r[a][b] = cadd(cadd(cmulc(Uf.r[a][0], gb[0]), cmulc(Uf.r[a][1], gb[1])),
+                         cmulc(Uf.r[a][2], gb[2]));
+      }
+    C<T> x2[3];
+    row2_of(X, x2);
+#pragma unroll
+    for (int i = 0; i < 3; i++) herm_add_col(H[i], X.r[0][i], X.r[1][i], x2[i]);
+  }
+}
+template <typename T>
+__device__ __forceinline__ void site_bwd_field(Herm3<T>* H, const M2<T>& Ub, const M2<T>& Gb) {
+  {
+    // Y = G_b U_b: rows 0, 1 from the stored rows of G_b and all rows of
+    // U_b, row 2 = conj(row0 x row1)
+    C<T> ub2[3];
+    row2_of(Ub, ub2);
+    M2<T> Y;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++)
+        Y.r[a][b] = cadd(cadd(cmul(Gb.r[a][0], Ub.r[0][b]), cmul(Gb.r[a][1], Ub.r[1][b])),
+                         cmul(Gb.r[a][2], ub2[b]));
+    C<T> y2[3];
+    row2_of(Y, y2);
+    herm_add_row(H[0], Y.r[0][0], Y.r[0][1], Y.r[0][2]);
+    herm_add_row(H[1], Y.r[1][0], Y.r[1][1], Y.r[1][2]);
+    herm_add_row(H[2], y2[0], y2[1], y2[2]);
+  }
 }
 
 template <typename T>
@@ -842,38 +922,79 @@ __device__ __forceinline__ void su3_site_copies_block(const T* __restrict__ U,
     T* Gc = G + (size_t)c * gstride;
     double fs = 0.0, rs = 0.0, rm = 0.0;
     if (in) {
+      Herm3<T> H[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) H[i] = Herm3<T>{T(0), T(0), T(0), {T(0), T(0)}, {T(0), T(0)}, {T(0), T(0)}};
+      // FP32: the eight loads of the two directions first, then the
+      // arithmetic (16^3x32, one copy: 24.3 -> 11.3 us per half-sweep);
+      // FP64 loads each direction's links where it uses them (its
+      // registers are full: 250 VGPRs)
+      if constexpr (sizeof(T) == 4) {
+        M2<T> Lf[2], Lgf[2], Lb[2], Lgb[2];
+        auto load_dir = [&](int k) {
+          const int mu = 2 * q + k;
+          const uint32_t hfm = fwd_h(g, h, s, mu);
+          const uint32_t hbm = q ? hb[2 + k] : hb[k];
+          Lf[k] = ldm(link_ptr<12>(U, g, p, mu, h));
+          Lgf[k] = ldm(site_ptr<12>(Gc, g, 1 - p, hfm));
+          Lb[k] = ldm(link_ptr<12>(U, g, 1 - p, mu, hbm));
+          Lgb[k] = ldm(site_ptr<12>(Gc, g, 1 - p, hbm));
+        };
+        load_dir(0);
+        load_dir(1);
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+          site_fwd_field(H, Lf[k], Lgf[k]);
+          site_bwd_field(H, Lb[k], Lgb[k]);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+          const int mu = 2 * q + k;
+          const uint32_t hbm = q ? hb[2 + k] : hb[k];
+          site_fwd_field(H, ldm(link_ptr<12>(U, g, p, mu, h)),
+                         ldm(site_ptr<12>(Gc, g, 1 - p, fwd_h(g, h, s, mu))));
+          site_bwd_field(H, ldm(link_ptr<12>(U, g, 1 - p, mu, hbm)), ldm(site_ptr<12>(Gc, g, 1 - p, hbm)));
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 3; i++) herm_pair_sum(H[i]);
       const LRef<T> gp = site_ptr<12>(Gc, g, p, h);
       M2<T> Gx2 = ldm(gp);
-      const M3<T> Gx = full3t(Gx2);
-      M2<T> F[2], B[2];
+      C<T> Gx[3][3];
 #pragma unroll
-      for (int k = 0; k < 2; k++) {
-        const int mu = 2 * q + k;
-        const uint32_t hfm = fwd_h(g, h, s, mu);
-        const uint32_t hbm = q ? hb[2 + k] : hb[k];
-        {
-          const M3<T> Uf = full3t(ldm(link_ptr<12>(U, g, p, mu, h)));
-          const M3<T> Gf = full3t(ldm(site_ptr<12>(Gc, g, 1 - p, hfm)));
-          F[k] = rows_mul(Gx, mul3_dag(Uf, Gf));
-        }
-        {
-          const M3<T> Ub = full3t(ldm(link_ptr<12>(U, g, 1 - p, mu, hbm)));
-          const M3<T> Gb = full3t(ldm(site_ptr<12>(Gc, g, 1 - p, hbm)));
-          B[k] = rows_mul_dag(rows_mul(Gb, Ub), Gx);
-        }
-      }
-      T res = cm_subgroup<T, 0, 1, true, 2>(F, B, Gx2, omega);
-      res += cm_subgroup<T, 1, 2, true, 2>(F, B, Gx2, omega);
-      res += cm_subgroup<T, 0, 2, true, 2>(F, B, Gx2, omega);
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) Gx[a][b] = Gx2.r[a][b];
+      row2_of(Gx2, Gx[2]);
+      T res = site_subgroup<T, 0, 1>(H, Gx, omega);
+      res += site_subgroup<T, 1, 2>(H, Gx, omega);
+      res += site_subgroup<T, 0, 2>(H, Gx, omega);
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) Gx2.r[a][b] = Gx[a][b];
       if constexpr (sizeof(T) == 4) renorm_rows(Gx2);
-      if (q == 0) stm(gp, Gx2);
-      if (p == 1) {
-#pragma unroll
-        for (int k = 0; k < 2; k++) fs += su3_link_F(F[k]) + su3_link_F(B[k]);
-      }
       if (q == 0) {
+        stm(gp, Gx2);
         rs = (double)res;
         rm = (double)res;
+        if (p == 1) {
+          // sum_i g_i H^(i) g_i^+ = sum over the site's 8 links of
+          // sum_i |W_ii|^2, in double
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            Herm3<double> Hd{(double)H[i].d0, (double)H[i].d1, (double)H[i].d2,
+                             {(double)H[i].o01.re, (double)H[i].o01.im},
+                             {(double)H[i].o02.re, (double)H[i].o02.im},
+                             {(double)H[i].o12.re, (double)H[i].o12.im}};
+            C<double> gi[3], r[3];
+#pragma unroll
+            for (int b = 0; b < 3; b++) gi[b] = C<double>{(double)Gx[i][b].re, (double)Gx[i][b].im};
+            herm_mul_dag(Hd, gi, r);
+            fs += dot3(gi, r).re;
+          }
+        }
       }
     }
     wave_partial(part + (size_t)c * pstride, lb * kBlock + (threadIdx.x & ~63u), 2 * g.Vh, fs,
