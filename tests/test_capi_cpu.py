@@ -81,3 +81,19 @@ def test_monopole_and_cluster_arguments_rejected_without_gpu(mag):
     ang[3] = 1001.0
     assert L.mag_u1_monopole_currents(d, ang.ctypes.data_as(dp), Jp) == -1
     assert "1001" in L.mag_last_error().decode()
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """Without libmag_gpu.so the front end raises: there is no CPU fallback."""
+    import subprocess
+    import sys
+    code = ("import general_code_amd.mag as M\n"
+            f"M.LIB_PATH = {str(tmp_path / 'absent.so')!r}\n"
+            "try:\n"
+            "    M.MagJob(group=M.SU2, dims=(4, 4, 4, 4))\n"
+            "except ImportError as e:\n"
+            "    print('ImportError', e)\n"
+            "else:\n"
+            "    print('no error')\n")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert out.stdout.startswith("ImportError") and "build it" in out.stdout, out.stdout + out.stderr
