@@ -14,6 +14,9 @@ REQS = [
          eps=0.4, seed=3),
     dict(group=3, dims=(4, 4, 4, 4), omega=1.6, tol=1e-11, eps=0.4, seed=4),
     dict(group=2, dims=(8, 8, 8, 8), precision=1, omega=1.7, tol=1e-12, eps=0.5, seed=5),
+    # SU(3) in the site form (one site gauge over read-only links), as bench.py serves it
+    dict(group=3, dims=(4, 4, 4, 8), form=1, copies=1, precision=1, fp64_warmup=20, omega=1.6,
+         tol=1e-11, eps=0.4, seed=6),
 ]
 
 
@@ -106,3 +109,22 @@ def test_served_from_host_links(mag, oracle):
         r = srv.wait(srv.submit(p, links=links))
     assert r["F"] == F and r["sweeps"] == s and r["status"] == 0
     assert np.isfinite(r["F"])
+
+
+def test_su3_site_form_request_reaches_fp64_maximum(mag):
+    """A mixed-precision SU(3) request in the site form ends at the maximum
+    the plain FP64 link-form job reaches.  Both stop on |dF/F| < 1e-11 per
+    sweep, which leaves F short of the maximum by what the remaining sweeps
+    would add (about 3e-10 here, the site-form run being the further
+    along), so the check is the serving benchmark's 1e-9."""
+    dims = (4, 4, 4, 8)
+    with mag.MagJob(group=mag.SU3, dims=dims, omega=1.6, tol=1e-11, max_sweeps=4000) as lj:
+        lj.generate(0.4, 6)
+        lj.run()
+        _, conv_l, F_l = lj.status()
+    with mag.MagServer() as srv:
+        p = mag.MagParams(group=mag.SU3, dims=dims, form=mag.FORM_SPIN, copies=1, precision=mag.MIXED,
+                          fp64_warmup=20, omega=1.6, tol=1e-11, tol_mixed=1e-11, max_sweeps=4000)
+        r = srv.wait(srv.submit(p, eps=0.4, seed=6))
+    assert conv_l and r["converged"]
+    assert abs(r["F"] - F_l) / F_l < 1e-9, (r["F"], F_l)
