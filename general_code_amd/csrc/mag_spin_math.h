@@ -134,12 +134,16 @@ __device__ __forceinline__ Q<double> gauge_of_spin(const V3& s) {
 }
 
 // The adjoint 3x3 matrix M(A) of a link, entries as in contribution1 with
-// the off-diagonal factor 2 left out: computed once per link and applied to
+// the off-diagonal factor 2 folded in: computed once per link and applied to
 // the neighbour spin of every gauge copy.  add_M / add_MT evaluate the
-// expressions of contribution1 / contribution1_conj term for term.
+// terms of contribution1 / contribution1_conj as a chain of explicit fused
+// multiply-adds (fma(), not contraction: the same operations in every
+// kernel instance, so a copy's result does not depend on the copy count);
+// the off-diagonal entries are stored doubled (exact), which gives the
+// reference's 2 b_j M_ij terms.
 struct Adj {
   double d1, d2, d3;             // A5 + 2 q_i
-  double p12, m12, p13, m13, p23, m23;  // q12 +- q03, q13 -+ q02, q23 +- q01
+  double p12, m12, p13, m13, p23, m23;  // 2 (q12 +- q03), 2 (q13 -+ q02), 2 (q23 +- q01)
 };
 __device__ __forceinline__ Adj adj_of(const Q<double>& A) {
 #pragma clang fp contract(off)
@@ -147,22 +151,21 @@ __device__ __forceinline__ Adj adj_of(const Q<double>& A) {
   const double q12 = A.a1 * A.a2, q13 = A.a1 * A.a3, q23 = A.a2 * A.a3;
   const double q01 = A.a0 * A.a1, q02 = A.a0 * A.a2, q03 = A.a0 * A.a3;
   const double A5 = A.a0 * A.a0 - q1 - q2 - q3;
-  return Adj{A5 + 2 * q1, A5 + 2 * q2, A5 + 2 * q3, q12 + q03, q12 - q03,
-             q13 - q02,   q13 + q02,   q23 + q01,   q23 - q01};
+  return Adj{A5 + 2 * q1,       A5 + 2 * q2,       A5 + 2 * q3,
+             2 * (q12 + q03),   2 * (q12 - q03),   2 * (q13 - q02),
+             2 * (q13 + q02),   2 * (q23 + q01),   2 * (q23 - q01)};
 }
 // h += M(A) b
 __device__ __forceinline__ void add_M(V3& h, const Adj& M, const V3& b) {
-#pragma clang fp contract(off)
-  h.a1 += b.a1 * M.d1 + 2 * b.a2 * M.p12 + 2 * b.a3 * M.p13;
-  h.a2 += 2 * b.a1 * M.m12 + b.a2 * M.d2 + 2 * b.a3 * M.p23;
-  h.a3 += 2 * b.a1 * M.m13 + 2 * b.a2 * M.m23 + b.a3 * M.d3;
+  h.a1 = fma(b.a3, M.p13, fma(b.a2, M.p12, fma(b.a1, M.d1, h.a1)));
+  h.a2 = fma(b.a3, M.p23, fma(b.a2, M.d2, fma(b.a1, M.m12, h.a2)));
+  h.a3 = fma(b.a3, M.d3, fma(b.a2, M.m23, fma(b.a1, M.m13, h.a3)));
 }
 // h += M(A)^T b
 __device__ __forceinline__ void add_MT(V3& h, const Adj& M, const V3& b) {
-#pragma clang fp contract(off)
-  h.a1 += b.a1 * M.d1 + 2 * b.a2 * M.m12 + 2 * b.a3 * M.m13;
-  h.a2 += 2 * b.a1 * M.p12 + b.a2 * M.d2 + 2 * b.a3 * M.m23;
-  h.a3 += 2 * b.a1 * M.p13 + 2 * b.a2 * M.p23 + b.a3 * M.d3;
+  h.a1 = fma(b.a3, M.m13, fma(b.a2, M.m12, fma(b.a1, M.d1, h.a1)));
+  h.a2 = fma(b.a3, M.m23, fma(b.a2, M.d2, fma(b.a1, M.p12, h.a2)));
+  h.a3 = fma(b.a3, M.d3, fma(b.a2, M.p23, fma(b.a1, M.p13, h.a3)));
 }
 
 }  // namespace mag
