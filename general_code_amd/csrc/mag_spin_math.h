@@ -85,7 +85,8 @@ __device__ __forceinline__ void spin_omega(V3& s, const V3& h, double omega) {
 #pragma clang fp contract(off)
   const double hn = sqrt(h.a1 * h.a1 + h.a2 * h.a2 + h.a3 * h.a3);
   if (hn == 0.0) return;
-  const V3 hh{h.a1 / hn, h.a2 / hn, h.a3 / hn};
+  const double ih = 1.0 / hn;  // one division, three products
+  const V3 hh{h.a1 * ih, h.a2 * ih, h.a3 * ih};
   const double c = s.a1 * hh.a1 + s.a2 * hh.a2 + s.a3 * hh.a3;
   V3 t{hh.a1 - c * s.a1, hh.a2 - c * s.a2, hh.a3 - c * s.a3};
   double tn = sqrt(t.a1 * t.a1 + t.a2 * t.a2 + t.a3 * t.a3);
@@ -100,7 +101,8 @@ __device__ __forceinline__ void spin_omega(V3& s, const V3& h, double omega) {
   const double theta = atan2(tn, c);
   double sa, ca;
   sincos(omega * theta, &sa, &ca);
-  s = V3{ca * s.a1 + sa * t.a1 / tn, ca * s.a2 + sa * t.a2 / tn, ca * s.a3 + sa * t.a3 / tn};
+  const double st = sa / tn;
+  s = V3{ca * s.a1 + st * t.a1, ca * s.a2 + st * t.a2, ca * s.a3 + st * t.a3};
   // back onto the sphere: for omega > 1 this step multiplies a spin's
   // deviation from unit length by about 2 omega - 1 per update, so rounding
   // would grow exponentially over the sweeps without it
