@@ -20,7 +20,7 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmag_gpu.so")
 SOURCES = ["mag_sweep.hip", "mag_kernels.hip", "mag_generate.hip", "mag_spin.hip",
            "mag_capi.hip", "mag_spin_capi.hip", "mag_batch.hip", "mag_server.hip",
-           "mag_tune.hip", "mag_monopole.hip"]
+           "mag_tune.hip", "mag_monopole.hip", "mag_noalloc.hip"]
 ARCH = os.environ.get("MAG_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

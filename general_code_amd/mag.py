@@ -143,6 +143,7 @@ SIGNATURES = {
     "mag_su2_functional": (ctypes.c_int, [_ip, _dp, _dp]),
     "mag_su3_functional": (ctypes.c_int, [_ip, _dp, _dp]),
     "mag_plaquette": (ctypes.c_int, [ctypes.c_int, _ip, _dp, _dp]),
+    "mag_functional_noalloc": (ctypes.c_int, [ctypes.c_int, _ip, _vp, _vp, _vp]),
     "mag_hbm_copy_bandwidth": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, _dp]),
     # batches
     "mag_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),

@@ -447,6 +447,20 @@ int mag_su2_functional(const int *dims, const double *links, double *F);
 int mag_su3_functional(const int *dims, const double *links18, double *F);
 int mag_plaquette(int group, const int *dims, const double *links, double *P);
 
+/* The functional of mag_su2_functional / mag_su3_functional (group
+ * MAG_SU2 / MAG_SU3) from links already in device memory (reference
+ * layout, 4*V*4 or 4*V*18 doubles), written to dev_F (one double in device
+ * memory), enqueued on `stream` (a hipStream_t, NULL: the default stream)
+ * of the current device; returns without waiting.  It allocates nothing:
+ * its partial sums live in a fixed 64 KiB device array of the library,
+ * one per device, so calls that overlap on one device race.  At most
+ * 16 777 216 sites (MAG_ERR_DIMS beyond); extents even and >= 2; the links
+ * are not checked for NaN / Inf (a non-finite link gives a non-finite F).
+ * No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, serving loops that forbid hipMalloc). */
+int mag_functional_noalloc(int group, const int *dims, const double *dev_links, double *dev_F,
+                           void *stream);
+
 #ifdef __cplusplus
 }
 #endif
