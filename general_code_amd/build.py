@@ -18,7 +18,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmag_gpu.so")
-SOURCES = ["mag_sweep.hip", "mag_kernels.hip", "mag_generate.hip", "mag_spin.hip",
+SOURCES = ["mag_sweep.hip", "mag_sweep64.hip", "mag_kernels.hip", "mag_generate.hip", "mag_spin.hip",
            "mag_capi.hip", "mag_spin_capi.hip", "mag_batch.hip", "mag_server.hip",
            "mag_tune.hip", "mag_monopole.hip", "mag_noalloc.hip"]
 ARCH = os.environ.get("MAG_OFFLOAD_ARCH", "gfx950")
@@ -34,9 +34,17 @@ COMMON = [
 ]
 
 
+# Sources built without floating-point contraction: the FP64 link-form
+# half-sweeps and the layout / rebuild kernels, whose FP64 results then
+# follow the source's IEEE operations exactly (as the oracle's C code, also
+# built with -ffp-contract=off, does), bit for bit.
+NO_CONTRACT = {"mag_sweep64.hip", "mag_kernels.hip"}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *COMMON, "-c", os.path.join(CSRC, src), "-o", obj]
+    extra = ["-ffp-contract=off"] if src in NO_CONTRACT else []
+    cmd = [HIPCC, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")

@@ -76,8 +76,11 @@ struct JobCtl {
 enum { RED_INIT = 0, RED_SWEEP = 1, RED_PLAIN = 2, RED_RESET = 3, RED_REBUILD = 4 };
 
 // ---------------------------------------------------------------- launchers
-// Partial buffers hold 3 doubles per workgroup: (F sum, residual sum,
-// residual max).  A sweep uses 2*nblocks(g) partial triples (red, black).
+// Partial buffers hold triples of doubles (F sum, residual sum, residual
+// max).  The half-sweep kernels write one triple per wave of 64 lanes
+// (sweep_waves below): a sweep uses 2*nw triples, parity p at 3*nw*p.
+// nblocks(g) is the workgroup count of a kBlock-thread launch over Vh
+// sites (the functional, spin-form and copy launches).
 inline int nblocks(const Geom& g) { return (int)((g.Vh + kBlock - 1) / kBlock); }
 // lanes per site of the link-form SU(3) half-sweep: 2 (lane q holds the
 // links of mu = 2q, 2q+1) or 4 (lane q holds mu = q)
@@ -122,8 +125,8 @@ void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega,
 // Gauge-copy (spin form, SU(2), FP64) half-sweep over all copies of one
 // configuration: the links U (read-only) are loaded once per site and
 // every copy not yet done updates its spins, copy c at S + c*sstride; its
-// partial triples go to part + c*pstride + 3*nb*parity (nb = nblocks(g))
-// and it is skipped once ctl[c].done.
+// partial triples (one per wave) go to part + c*pstride + 3*nw*parity
+// (nw = sweep_waves(g, 2)) and it is skipped once ctl[c].done.
 // TL = float: the FP32-rounded links of the mixed form (converted to FP64
 // on load; spins and arithmetic stay FP64).
 template <typename TL>
@@ -132,7 +135,7 @@ void launch_su2_copies_half(const TL* U, double* S, size_t sstride, int ncopies,
                             size_t pstride, const JobCtl* ctl, hipStream_t st);
 // SU(3) site form: links U read-only, site gauge of copy c at G + c*gstride
 // (2 rows, chunked site layout, the links' type); partials and control as
-// for the spin form, nb = sweep_blocks(g, 3)
+// for the spin form, one triple per wave of the site3_lanes(g) lanes
 template <typename T>
 void launch_su3_site_copies(const T* U, T* G, size_t gstride, int ncopies, const Geom& g,
                             int parity, double omega, double* part, size_t pstride,
@@ -280,6 +283,8 @@ struct SpinCtl {
   int max_iter;
   int converged;
   int digits;      // make_maximization_approximate tolerance digits
+  float scale;     // powf(10, digits), evaluated on the host by the C library
+                   // as the reference's criterion does (mag.cpp:1745-1746)
   double tol_max;  // make_maximization_final criteria
   double tol_avg;
   double dmax;     // last relaxation residuals

@@ -8,15 +8,22 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mag_trig.h"
+
 namespace mag {
 
 // ----------------------------------------------------------------- scalars
 __device__ __forceinline__ float m_sqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float m_atan2(float y, float x) { return atan2f(y, x); }
-__device__ __forceinline__ double m_atan2(double y, double x) { return atan2(y, x); }
-__device__ __forceinline__ void m_sincos(float a, float* s, float* c) { sincosf(a, s, c); }
-__device__ __forceinline__ void m_sincos(double a, double* s, double* c) { sincos(a, s, c); }
+// angle functions of the omega rule (y > 0; angles in [0, pi]): FP32 the
+// device library's, FP64 the fixed operation sequence of mag_trig.h that
+// the host restatement repeats bit for bit
+__device__ __forceinline__ float omega_atan2(float y, float x) { return atan2f(y, x); }
+__device__ __forceinline__ double omega_atan2(double y, double x) { return trig::atan2_pos(y, x); }
+__device__ __forceinline__ void omega_sincos(float a, float* s, float* c) { sincosf(a, s, c); }
+__device__ __forceinline__ void omega_sincos(double a, double* s, double* c) {
+  trig::sincos_0_2pi(a, s, c);
+}
 template <typename T> __device__ __forceinline__ T m_pi();
 template <> __device__ __forceinline__ float m_pi<float>() { return 3.14159265358979323846f; }
 template <> __device__ __forceinline__ double m_pi<double>() { return 3.14159265358979323846; }
@@ -106,15 +113,15 @@ __device__ __forceinline__ Q<T> omega_gauge(T h0, T h1, T h2, T omega, T& resid)
       g.a0 = T(1); g.a1 = T(0); g.a2 = T(0);
     } else {
       T s, c;
-      m_sincos(T(0.5) * omega * m_pi<T>(), &s, &c);
+      omega_sincos(T(0.5) * omega * m_pi<T>(), &s, &c);
       g.a0 = c; g.a1 = s; g.a2 = T(0);
     }
     return g;
   }
   resid = (h2 >= T(0)) ? hp2 / (hn * (hn + h2)) : T(1) - h2 / hn;
-  T theta = m_atan2(hp, h2);
+  T theta = omega_atan2(hp, h2);
   T s, c;
-  m_sincos(T(0.5) * omega * theta, &s, &c);
+  omega_sincos(T(0.5) * omega * theta, &s, &c);
   T inv = T(1) / hp;
   g.a0 = c;
   g.a1 = -s * h1 * inv;

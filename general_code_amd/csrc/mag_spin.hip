@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_spin_reduce(const double* __restrict
   } else if (mode == SRED_FUNC_APPROX) {
     // make_maximization_approximate (mag.cpp:1733-1750), powf kept
     const double F = a * invF, Fo = ctl->F_prev;
-    const float scale = powf(10.f, (float)ctl->digits);
+    const float scale = ctl->scale;  // the host C library's powf(10, digits)
     const bool eq = (trunc(scale * F) == trunc(scale * Fo)) && (signbit(F) == signbit(Fo));
     ctl->F_prev = F;
     ctl->iter += 1;

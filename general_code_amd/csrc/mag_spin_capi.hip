@@ -88,6 +88,7 @@ int reset_ctl(mag_spin_session* s, int max_iter, double tol_max, double tol_avg,
   s->ctl_h->tol_max = tol_max;
   s->ctl_h->tol_avg = tol_avg;
   s->ctl_h->digits = digits;
+  s->ctl_h->scale = powf(10.f, (float)digits);
   HIPCHK(hipMemcpyAsync(s->ctl, s->ctl_h, sizeof(SpinCtl), hipMemcpyHostToDevice, s->st));
   return MAG_OK;
 }

@@ -1,30 +1,13 @@
-// mag_sweep.hip -- checkerboard omega-overrelaxation half-sweep kernels of
-// the MAG path (gfx950) and the per-sweep reduction.
-//
-// The site update (build-defined; the reference has only the spin-form
-// omega=1 and omega=2 updates, lib/cpu/src/mag.cpp:1375-1512):
-//   h      = sum_mu [ M(U_x,mu) e3 + M(U_x-mu,mu)^T e3 ]     (mag.cpp:569-626
-//            with every spin e3: the links carry the gauge transformation)
-//   g      = geodesic omega step of e3 towards h_hat          (mag_math.h)
-//   U_x,mu <- g U_x,mu ;  U_x-mu,mu <- U_x-mu,mu g^+
-// SU(3): the same update for the Cabibbo-Marinari subgroups (0,1), (1,2),
-// (0,2) in turn, with the block field terms of mag_math.h.
-//
-// Work split: SU(2) runs one site per lane.  SU(3) runs a site on a pair of
-// adjacent lanes: lane q of the pair holds the 4 links of directions
-// mu = 2q, 2q+1 (forward and backward), sums their field terms, and the
-// pair exchanges the two partial fields with one DPP quad_perm; both lanes
-// then derive the same g and update their own links.  Control flow is
-// identical in both lanes and the register footprint per lane is halved
-// (4 links instead of 8): FP64 SU(3) goes from 1 to 3 waves per SIMD.
-//
-// Field summation order (matched by the oracle):
-//   h = [f(0) + b(0) + f(1) + b(1)] + [f(2) + b(2) + f(3) + b(3)]
+// mag_sweep.hip -- per-sweep reductions, the FP32 instantiations of the
+// link-form half-sweeps (mag_sweep_link.h; the FP64 ones are in
+// mag_sweep64.hip), and the site-form gauge-copy half-sweeps (SU(2) spins,
+// SU(3) site gauges) of the MAG path (gfx950).
 #include "mag_internal.h"
 #include "mag_layout.h"
 #include "mag_math.h"
 #include "mag_reduce.h"
 #include "mag_spin_math.h"
+#include "mag_sweep_link.h"
 
 namespace mag {
 
@@ -146,434 +129,20 @@ void launch_gather_ctl(const BatchJob* jobs, int n, JobCtl* out, hipStream_t st)
   hipLaunchKernelGGL(k_gather_ctl, dim3((n * W + 63) / 64), dim3(64), 0, st, jobs, n, out);
 }
 
-// ================================================================== helpers
-// value of the partner lane of the pair (lane ^ 1), and of lane ^ 2
-__device__ __forceinline__ double pair_xchg(double v) { return dpp_f64<DPP_QUAD_XOR1>(v); }
-__device__ __forceinline__ double quad_xchg2(double v) { return dpp_f64<DPP_QUAD_XOR2>(v); }
-__device__ __forceinline__ float quad_xchg2(float v) {
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), DPP_QUAD_XOR2, 0xF, 0xF, true));
-}
-__device__ __forceinline__ float pair_xchg(float v) {
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), DPP_QUAD_XOR1, 0xF, 0xF, true));
-}
-
-// half indices of x - mu for all mu
-__device__ __forceinline__ void back_all(const Geom& g, uint32_t h, const SiteIdx& s,
-                                         uint32_t* hb) {
-#pragma unroll
-  for (int mu = 0; mu < 4; mu++) hb[mu] = back_h(g, h, s, mu);
-}
-
-// job of workgroup b in a batch table (block0 ascending)
-__device__ __forceinline__ int batch_job(const BatchJob* __restrict__ jobs, int n, int b) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].block0 <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-// XCD-aware workgroup order (xcd_block, mag_layout.h): the site-form
-// kernels reuse neighbour spins / gauges (read by up to 8 sites).
-
-// ================================================================== launch shapes
-// The partial triples of a half-sweep are per wave: wave w of a job covers
-// its lanes 64w .. 64w+63 (64 SU(2) sites, 32 SU(3) sites) and its triple
-// goes to part[3w].  A launch shape -- NT threads per workgroup, SPT
-// 64-lane-aligned tiles of NT lanes per thread (workgroup lb, tile k covers
-// lanes (lb*SPT + k)*NT ...) -- only decides which thread computes which
-// wave, never the arithmetic or the order of a partial, so every shape
-// gives bit-identical results and the shape is free for the autotuner.
-__device__ __forceinline__ void wave_partial(double* __restrict__ part, uint32_t lane0,
-                                             uint32_t nlanes, double f, double r, double m) {
-  f = wave_sum(f);
-  r = wave_sum(r);
-  m = wave_max(m);
-  if ((threadIdx.x & 63u) == 0 && lane0 < nlanes) {
-    const uint32_t w = lane0 >> 6;
-    part[3 * w] = f;
-    part[3 * w + 1] = r;
-    part[3 * w + 2] = m;
-  }
-}
-
-// ================================================================== SU(2)
-// Tile k of workgroup lb of a half-sweep over the sites of parity p, one
-// site per lane; the 8 links of an su2 site fit in registers (3 waves/SIMD
-// in FP64, enough to stream at the copy rate), and one site per lane
-// measured faster than a lane pair.  The black
-// (p = 1) half also sums the functional of the links just written.
-template <typename T, bool GA, int NT, int SPT>
-__device__ __forceinline__ void su2_half_block(T* __restrict__ U, T* __restrict__ G,
-                                               const Geom& g, int p, T omega,
-                                               double* __restrict__ part, uint32_t lb) {
-#pragma unroll
-  for (int k = 0; k < SPT; k++) {
-  const uint32_t base = (lb * SPT + k) * NT;
-  const uint32_t h = base + threadIdx.x;
-  double fs = 0.0, rs = 0.0, rm = 0.0;
-  if (h < g.Vh) {
-    const SiteIdx s = decode(g, h, p);
-    uint32_t hb[4];
-    back_all(g, h, s, hb);
-    Q<T> F[4], B[4];
-#pragma unroll
-    for (int mu = 0; mu < 4; mu++) {
-      F[mu] = ldq(link_ptr<4>(U, g, p, mu, h));
-      B[mu] = ldq(link_ptr<4>(U, g, 1 - p, mu, hb[mu]));
-    }
-    T a0 = T(0), a1 = T(0), a2 = T(0), b0 = T(0), b1 = T(0), b2 = T(0);
-#pragma unroll
-    for (int mu = 0; mu < 2; mu++) {
-      field_fwd(F[mu], a0, a1, a2);
-      field_bwd(B[mu], a0, a1, a2);
-      field_fwd(F[mu + 2], b0, b1, b2);
-      field_bwd(B[mu + 2], b0, b1, b2);
-    }
-    T res;
-    const Q<T> gq = omega_gauge(a0 + b0, a1 + b1, a2 + b2, omega, res);
-#pragma unroll
-    for (int mu = 0; mu < 4; mu++) {
-      F[mu] = gmul(gq, F[mu]);
-      B[mu] = mul_gdag(B[mu], gq);
-      stq(link_ptr<4>(U, g, p, mu, h), F[mu]);
-      stq(link_ptr<4>(U, g, 1 - p, mu, hb[mu]), B[mu]);
-    }
-    if (GA) {
-      const LRef<T> gp = site_ptr<4>(G, g, p, h);
-      stq(gp, gmul(gq, ldq(gp)));
-    }
-    if (p == 1) {
-#pragma unroll
-      for (int mu = 0; mu < 4; mu++) fs += link_F(F[mu]) + link_F(B[mu]);
-    }
-    rs = (double)res;
-    rm = (double)res;
-  }
-  wave_partial(part, base + (threadIdx.x & ~63u), g.Vh, fs, rs, rm);
-  }
-}
-
-template <typename T, bool GA, int NT, int SPT>
-__global__ __launch_bounds__(NT) void k_su2_half(T* __restrict__ U, T* __restrict__ G, Geom g,
-                                                 int p, T omega, double* __restrict__ part,
-                                                 const JobCtl* __restrict__ ctl) {
-  if (ctl->done) return;
-  su2_half_block<T, GA, NT, SPT>(U, G, g, p, omega, part, blockIdx.x);
-}
-
-// batched: workgroup b runs block b - block0 of its job with the class's
-// launch shape (NT, SPT); the job's partials are per wave, so the shape
-// leaves its results unchanged
-template <typename T, bool GA, int NT, int SPT>
-__global__ __launch_bounds__(NT) void k_su2_half_batch(const BatchJob* __restrict__ jobs,
-                                                       int njobs, int p) {
-  const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
-  if (j.ctl->done) return;
-  su2_half_block<T, GA, NT, SPT>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega,
-                                 j.part + (size_t)3 * j.nw * p, blockIdx.x - j.block0);
-}
-
-// launch of a shape (NT, SPT) of the supported set
-#define MAG_SHAPE_SWITCH(sh, L_)                  \
-  switch (sh.nt * 4 + sh.spt) {                   \
-    case 64 * 4 + 1: L_(64, 1); break;            \
-    case 64 * 4 + 2: L_(64, 2); break;            \
-    case 128 * 4 + 1: L_(128, 1); break;          \
-    case 128 * 4 + 2: L_(128, 2); break;          \
-    case 256 * 4 + 2: L_(256, 2); break;          \
-    case 512 * 4 + 1: L_(512, 1); break;          \
-    case 512 * 4 + 2: L_(512, 2); break;          \
-    default: L_(256, 1); break;                   \
-  }
-
-template <typename T>
-void launch_su2_half(T* U, T* G, const Geom& g, int parity, double omega, double* part,
-                     const JobCtl* ctl, Shape sh, hipStream_t st) {
-  double* pp = part + (size_t)3 * sweep_waves(g, 2) * parity;
-  const uint32_t lanes = g.Vh;
-#define L_(NT, SPT)                                                                        \
-  do {                                                                                     \
-    const dim3 grid((lanes + NT * SPT - 1) / (NT * SPT));                                  \
-    if (G)                                                                                 \
-      hipLaunchKernelGGL((k_su2_half<T, true, NT, SPT>), grid, dim3(NT), 0, st, U, G, g,   \
-                         parity, (T)omega, pp, ctl);                                       \
-    else                                                                                   \
-      hipLaunchKernelGGL((k_su2_half<T, false, NT, SPT>), grid, dim3(NT), 0, st, U, G, g,  \
-                         parity, (T)omega, pp, ctl);                                       \
-  } while (0)
-  MAG_SHAPE_SWITCH(sh, L_)
-#undef L_
-}
+// ================================================================== link form, FP32
 template void launch_su2_half<float>(float*, float*, const Geom&, int, double, double*,
                                      const JobCtl*, Shape, hipStream_t);
-template void launch_su2_half<double>(double*, double*, const Geom&, int, double, double*,
-                                      const JobCtl*, Shape, hipStream_t);
-
-// ================================================================== SU(3)
-// rows 0, 1 scaled by (3 - |row|^2) / 2, the Newton step of 1/|row| at 1
-// (the FP32 site gauges of the site form, which are never rebuilt in FP64
-// during their FP32 phase, lose about 1e-7 of row norm per sweep without it)
-template <typename T>
-__device__ __forceinline__ void renorm_rows(M2<T>& m) {
-#pragma unroll
-  for (int i = 0; i < 2; i++) {
-    T n = T(0);
-#pragma unroll
-    for (int c = 0; c < 3; c++) n += m.r[i][c].re * m.r[i][c].re + m.r[i][c].im * m.r[i][c].im;
-    const T f = T(1.5) - T(0.5) * n;
-#pragma unroll
-    for (int c = 0; c < 3; c++) m.r[i][c] = C<T>{m.r[i][c].re * f, m.r[i][c].im * f};
-  }
-}
-
-#ifndef MAG_SU3_PTR_REMAT
-#define MAG_SU3_PTR_REMAT 1
-#endif
-#ifndef MAG_SU3_ROW2_REMAT
-#define MAG_SU3_ROW2_REMAT 1
-#endif
-// a value the compiler must treat as changed (no instruction is emitted)
-template <typename T>
-__device__ __forceinline__ void opaque(M2<T>& m) {
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int c = 0; c < 3; c++) asm volatile("" : "+v"(m.r[i][c].re), "+v"(m.r[i][c].im));
-}
-
-// Cabibbo-Marinari subgroup (I,J) update of the 2*NL links a lane holds
-// (forward F[k], backward B[k], mu = NL*q + k; NL = 2 for lane pairs, 1 for
-// lane quads), rows 0,1 in registers, row 2 rebuilt where the subgroup
-// touches it.  Returns 1 - cos(theta).  The field is summed
-// as t_mu = f_mu + b_mu, then (t_0 + t_1) + (t_2 + t_3), the same
-// operations for either split (a + b == b + a exactly), in the order of the
-// oracle.
-template <typename T, int I, int J, bool GA, int NL>
-__device__ __forceinline__ T cm_subgroup(M2<T>* F, M2<T>* B, M2<T>& Gx, T omega) {
-  T h0 = T(0), h1 = T(0), h2 = T(0);
-  C<T> Fr2[NL][3], Br2[NL][3];
-#pragma unroll
-  for (int k = 0; k < NL; k++) {
-    if (J == 2) {
-      row2_of(F[k], Fr2[k]);
-      row2_of(B[k], Br2[k]);
-    }
-    auto fe = [&](int a, int b) -> C<T> { return a < 2 ? F[k].r[a][b] : Fr2[k][b]; };
-    auto be = [&](int a, int b) -> C<T> { return a < 2 ? B[k].r[a][b] : Br2[k][b]; };
-    T t0 = T(0), t1 = T(0), t2 = T(0);
-    blk_fwd(fe(I, I), fe(I, J), fe(J, I), fe(J, J), t0, t1, t2);
-    blk_bwd(be(I, I), be(I, J), be(J, I), be(J, J), t0, t1, t2);
-    if (k == 0) {
-      h0 = t0;
-      h1 = t1;
-      h2 = t2;
-    } else {
-      h0 += t0;
-      h1 += t1;
-      h2 += t2;
-    }
-  }
-  h0 += pair_xchg(h0);
-  h1 += pair_xchg(h1);
-  h2 += pair_xchg(h2);
-  if (NL == 1) {
-    h0 += quad_xchg2(h0);
-    h1 += quad_xchg2(h1);
-    h2 += quad_xchg2(h2);
-  }
-  T res;
-  const Q<T> gq = omega_gauge(h0, h1, h2, omega, res);
-  // r = [[g0 + i g3, g2 + i g1], [-g2 + i g1, g0 - i g3]], g3 = 0
-  const C<T> r00{gq.a0, T(0)}, r01{gq.a2, gq.a1}, r10{-gq.a2, gq.a1}, r11{gq.a0, T(0)};
-#if MAG_SU3_ROW2_REMAT
-  // row 2 of the forward links again rather than kept live across the
-  // gauge computation (the empty asm hides that F is unchanged)
-  if (J == 2) {
-#pragma unroll
-    for (int k = 0; k < NL; k++) {
-      opaque(F[k]);
-      row2_of(F[k], Fr2[k]);
-    }
-  }
-#endif
-#pragma unroll
-  for (int k = 0; k < NL; k++) {
-    // forward: rows I,J <- r (rows I,J); only stored rows are written
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const C<T> a = F[k].r[I][c];
-      const C<T> b = (J < 2) ? F[k].r[J][c] : Fr2[k][c];
-      F[k].r[I][c] = cadd(cmul(r00, a), cmul(r01, b));
-      if (J < 2) F[k].r[J][c] = cadd(cmul(r10, a), cmul(r11, b));
-    }
-    // backward: cols I,J of rows 0,1 <- (cols I,J) r^+
-#pragma unroll
-    for (int c = 0; c < 2; c++) {
-      const C<T> a = B[k].r[c][I], b = B[k].r[c][J];
-      B[k].r[c][I] = cadd(cmulc(a, r00), cmulc(b, r01));
-      B[k].r[c][J] = cadd(cmulc(a, r10), cmulc(b, r11));
-    }
-  }
-  if (GA) {
-    C<T> g2[3];
-    if (J == 2) row2_of(Gx, g2);
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const C<T> a = Gx.r[I][c];
-      const C<T> b = (J < 2) ? Gx.r[J][c] : g2[c];
-      Gx.r[I][c] = cadd(cmul(r00, a), cmul(r01, b));
-      if (J < 2) Gx.r[J][c] = cadd(cmul(r10, a), cmul(r11, b));
-    }
-  }
-  return res;
-}
-
-// One workgroup lb of an SU(3) half-sweep, LS = kSu3Lanes lanes per site:
-// lane q of the group holds the forward and backward links of the NL = 4/LS
-// directions mu = NL*q .. NL*q + NL - 1, the group exchanges the partial
-// fields with DPP and every lane derives the same g.
-template <typename T, bool GA, int NT, int SPT>
-__device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict__ G,
-                                               const Geom& g, int p, T omega,
-                                               double* __restrict__ part, uint32_t lb) {
-  constexpr int LS = kSu3Lanes, NL = 4 / kSu3Lanes;
-#pragma unroll
-  for (int kk = 0; kk < SPT; kk++) {
-    const uint32_t base = (lb * SPT + kk) * NT;
-    const uint32_t tid = base + threadIdx.x;
-    const uint32_t h = tid / LS;
-    const int q = (int)(tid % LS);
-    double fs = 0.0, rs = 0.0, rm = 0.0;
-    if (h < g.Vh) {
-      const SiteIdx s = decode(g, h, p);
-      uint32_t hb[4];
-      back_all(g, h, s, hb);  // all four, then a select: no divergent switch
-      LRef<T> pf[NL];
-      LRef<T> pb[NL];
-      M2<T> F[NL], B[NL];
-#pragma unroll
-      for (int k = 0; k < NL; k++) {
-        const int mu = NL * q + k;
-        const uint32_t hbm = mu == 0 ? hb[0] : mu == 1 ? hb[1] : mu == 2 ? hb[2] : hb[3];
-        pf[k] = link_ptr<12>(U, g, p, mu, h);
-        pb[k] = link_ptr<12>(U, g, 1 - p, mu, hbm);
-        F[k] = ldm(pf[k]);
-        B[k] = ldm(pb[k]);
-      }
-      M2<T> Gx;
-      const LRef<T> gp = site_ptr<12>(G, g, p, h);
-      if (GA) Gx = ldm(gp);
-      T res = cm_subgroup<T, 0, 1, GA, NL>(F, B, Gx, omega);
-      res += cm_subgroup<T, 1, 2, GA, NL>(F, B, Gx, omega);
-      res += cm_subgroup<T, 0, 2, GA, NL>(F, B, Gx, omega);
-#if MAG_SU3_PTR_REMAT
-      // the store addresses again from the 32-bit indices rather than 64-bit
-      // pointers kept live through the three subgroups
-#pragma unroll
-      for (int k = 0; k < NL; k++) {
-        const int mu = NL * q + k;
-        uint32_t hh = h, hbm = mu == 0 ? hb[0] : mu == 1 ? hb[1] : mu == 2 ? hb[2] : hb[3];
-        asm volatile("" : "+v"(hh), "+v"(hbm));
-        pf[k] = link_ptr<12>(U, g, p, mu, hh);
-        pb[k] = link_ptr<12>(U, g, 1 - p, mu, hbm);
-      }
-#endif
-#pragma unroll
-      for (int k = 0; k < NL; k++) {
-        if constexpr (sizeof(T) == 8) {
-          stm_stream(pf[k], F[k]);
-          stm_stream(pb[k], B[k]);
-        } else {
-          stm(pf[k], F[k]);
-          stm(pb[k], B[k]);
-        }
-      }
-      if (GA && q == 0) stm(gp, Gx);
-      if (p == 1) {
-#pragma unroll
-        for (int k = 0; k < NL; k++) fs += su3_link_F(F[k]) + su3_link_F(B[k]);
-      }
-      if (q == 0) {
-        rs = (double)res;
-        rm = (double)res;
-      }
-    }
-    wave_partial(part, base + (threadIdx.x & ~63u), LS * g.Vh, fs, rs, rm);
-  }
-}
-
-template <typename T, bool GA, int NT, int SPT>
-__global__ __launch_bounds__(NT) void k_su3_half(T* __restrict__ U, T* __restrict__ G, Geom g,
-                                                 int p, T omega, double* __restrict__ part,
-                                                 const JobCtl* __restrict__ ctl) {
-  if (ctl->done) return;
-  su3_half_block<T, GA, NT, SPT>(U, G, g, p, omega, part, blockIdx.x);
-}
-
-template <typename T, bool GA, int NT, int SPT>
-__global__ __launch_bounds__(NT) void k_su3_half_batch(const BatchJob* __restrict__ jobs,
-                                                       int njobs, int p) {
-  const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
-  if (j.ctl->done) return;
-  su3_half_block<T, GA, NT, SPT>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega,
-                                 j.part + (size_t)3 * j.nw * p, blockIdx.x - j.block0);
-}
-
-template <typename T>
-void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega, double* part,
-                     const JobCtl* ctl, Shape sh, hipStream_t st) {
-  double* pp = part + (size_t)3 * sweep_waves(g, 3) * parity;
-  const uint32_t lanes = (uint32_t)sweep_lanes(g, 3);
-#define L_(NT, SPT)                                                                        \
-  do {                                                                                     \
-    const dim3 grid((lanes + NT * SPT - 1) / (NT * SPT));                                  \
-    if (G)                                                                                 \
-      hipLaunchKernelGGL((k_su3_half<T, true, NT, SPT>), grid, dim3(NT), 0, st, U, G, g,   \
-                         parity, (T)omega, pp, ctl);                                       \
-    else                                                                                   \
-      hipLaunchKernelGGL((k_su3_half<T, false, NT, SPT>), grid, dim3(NT), 0, st, U, G, g,  \
-                         parity, (T)omega, pp, ctl);                                       \
-  } while (0)
-  MAG_SHAPE_SWITCH(sh, L_)
-#undef L_
-}
 template void launch_su3_half<float>(float*, float*, const Geom&, int, double, double*,
                                      const JobCtl*, Shape, hipStream_t);
-template void launch_su3_half<double>(double*, double*, const Geom&, int, double, double*,
-                                      const JobCtl*, Shape, hipStream_t);
 
-// batched half-sweep over a class table (group, precision): every job of
-// the table has the same group and storage type; GA = mixed FP32 phase;
-// block0 / nblocks_total counted in workgroups of the shape sh
+// FP64 tables: mag_sweep64.hip
+void launch_half_batch_f64(int group, Shape sh, const BatchJob* jobs, int njobs,
+                           int nblocks_total, int parity, hipStream_t st);
+
 void launch_half_batch(int group, bool fp32, Shape sh, const BatchJob* jobs, int njobs,
                        int nblocks_total, int parity, hipStream_t st) {
-  if (njobs == 0 || nblocks_total == 0) return;
-  const dim3 grid(nblocks_total);
-#define L_(NT, SPT)                                                                          \
-  do {                                                                                       \
-    if (group == 2) {                                                                        \
-      if (fp32)                                                                              \
-        hipLaunchKernelGGL((k_su2_half_batch<float, true, NT, SPT>), grid, dim3(NT), 0, st,  \
-                           jobs, njobs, parity);                                             \
-      else                                                                                   \
-        hipLaunchKernelGGL((k_su2_half_batch<double, false, NT, SPT>), grid, dim3(NT), 0,    \
-                           st, jobs, njobs, parity);                                         \
-    } else {                                                                                 \
-      if (fp32)                                                                              \
-        hipLaunchKernelGGL((k_su3_half_batch<float, true, NT, SPT>), grid, dim3(NT), 0, st,  \
-                           jobs, njobs, parity);                                             \
-      else                                                                                   \
-        hipLaunchKernelGGL((k_su3_half_batch<double, false, NT, SPT>), grid, dim3(NT), 0,    \
-                           st, jobs, njobs, parity);                                         \
-    }                                                                                        \
-  } while (0)
-  MAG_SHAPE_SWITCH(sh, L_)
-#undef L_
+  if (fp32) launch_half_batch_t<float>(group, sh, jobs, njobs, nblocks_total, parity, st);
+  else launch_half_batch_f64(group, sh, jobs, njobs, nblocks_total, parity, st);
 }
 
 // ================================================================== gauge copies

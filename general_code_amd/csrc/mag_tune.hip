@@ -141,10 +141,13 @@ int tune_batch(int group, bool fp32, const std::vector<mag_job*>& jobs, Shape* o
     return p;
   };
   size_t used = 0;
+  // scratch stays within a quarter of the free device memory as well
+  size_t cap = kBatchScratch, free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b / 4 < cap) cap = free_b / 4;
   hipError_t e = hipSuccess;
   for (const mag_job* j : jobs) {
     const size_t ub = j->nreal * esz, gb = fp32 ? (size_t)2 * j->g.hp * j->nr * sizeof(float) : 0;
-    if (!tab.empty() && used + ub + gb > kBatchScratch) break;
+    if (!tab.empty() && used + ub + gb > cap) break;
     used += ub + gb;
     BatchJob b{};
     b.U = alloc(ub);
@@ -229,8 +232,14 @@ int batch_shape(int group, bool fp32, const std::vector<mag_job*>& jobs, Shape* 
       return MAG_OK;
     }
   }
+  // a failed tuning run (e.g. no memory for its scratch copies) falls back
+  // to the default shape, which gives the same results as any other; its
+  // error is cleared so that it is not reported by a later HIP call
   Shape sh;
-  CHK(tune_batch(group, fp32, jobs, &sh));
+  if (tune_batch(group, fp32, jobs, &sh) != MAG_OK) {
+    (void)hipGetLastError();
+    sh = Shape{};
+  }
   std::lock_guard<std::mutex> lk(g_mu);
   g_batch_cache[key] = sh;
   *out = sh;
@@ -257,7 +266,10 @@ int job_shape(const mag_job* j, bool fp32, Shape* out) {
     }
   }
   Shape sh;
-  CHK(tune(j, fp32, &sh));
+  if (tune(j, fp32, &sh) != MAG_OK) {  // as for batches: the default shape
+    (void)hipGetLastError();
+    sh = Shape{};
+  }
   std::lock_guard<std::mutex> lk(g_mu);
   g_cache[key] = sh;
   *out = sh;

@@ -244,6 +244,23 @@ def _dims(d):
     return (ctypes.c_int * 4)(*[int(x) for x in d])
 
 
+def _volume(dims) -> int:
+    d = [int(x) for x in dims]
+    if len(d) != 4:
+        raise ValueError(f"dims must hold 4 extents, got {len(d)}")
+    return int(np.prod(d))
+
+
+def _sized(a, dims, per_site: int, name: str) -> np.ndarray:
+    """a as contiguous float64, checked to hold per_site reals per lattice
+    site of dims (the C side reads and writes that many)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    V = _volume(dims)
+    if a.size != per_site * V:
+        raise ValueError(f"{name} holds {a.size} reals, expected {per_site} x V = {per_site * V}")
+    return a
+
+
 def hbm_copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20, device: int = 0) -> float:
     """GB/s (read + write) of a plain 16-byte streaming copy kernel."""
     g = ctypes.c_double()
@@ -512,7 +529,7 @@ class MagJob:
 # one-shot helpers ---------------------------------------------------------
 def su2_functional(dims, links) -> float:
     """MAG_functional_su2 (lib/cpu/src/mag.cpp:993-1001) on the GPU."""
-    a = np.ascontiguousarray(links, dtype=np.float64)
+    a = _sized(links, dims, 16, "links")
     F = ctypes.c_double()
     _check(lib().mag_su2_functional(_dims(dims), _dptr(a), ctypes.byref(F)))
     return F.value
@@ -520,7 +537,7 @@ def su2_functional(dims, links) -> float:
 
 def su3_functional(dims, links) -> float:
     """mag_functional_su3 (lib/cpu/src/mag.cpp:1837-1862) on the GPU."""
-    a = np.ascontiguousarray(links, dtype=np.float64)
+    a = _sized(links, dims, 72, "links")
     F = ctypes.c_double()
     _check(lib().mag_su3_functional(_dims(dims), _dptr(a), ctypes.byref(F)))
     return F.value
@@ -528,7 +545,7 @@ def su3_functional(dims, links) -> float:
 
 def plaquette(group, dims, links) -> float:
     """plaket (lib/cpu/include/plaket.h:84-101) on the GPU."""
-    a = np.ascontiguousarray(links, dtype=np.float64)
+    a = _sized(links, dims, 16 if int(group) == SU2 else 72, "links")
     P = ctypes.c_double()
     _check(lib().mag_plaquette(int(group), _dims(dims), _dptr(a), ctypes.byref(P)))
     return P.value
@@ -639,19 +656,18 @@ def generate_spins_uniform(dims, seed: int) -> np.ndarray:
 def functional_spin(dims, links, spins) -> float:
     """MAG_functional_su2_spin (lib/cpu/src/mag.cpp:925-953) on the GPU."""
     F = ctypes.c_double()
-    _check(lib().mag_su2_functional_spin(_dims(dims),
-                                         _dptr(np.ascontiguousarray(links, np.float64)),
-                                         _dptr(np.ascontiguousarray(spins, np.float64)),
-                                         ctypes.byref(F)))
+    a = _sized(links, dims, 16, "links")
+    s = _sized(spins, dims, 3, "spins")
+    _check(lib().mag_su2_functional_spin(_dims(dims), _dptr(a), _dptr(s), ctypes.byref(F)))
     return F.value
 
 
 def gauge_transform_spins(dims, links, spins) -> np.ndarray:
     """gauge_tranformation_spins (lib/cpu/src/mag.cpp:1045-1068); returns the
     transformed links."""
-    out = np.array(links, dtype=np.float64, copy=True)
-    _check(lib().mag_su2_gauge_transform_spins(_dims(dims), _dptr(out),
-                                               _dptr(np.ascontiguousarray(spins, np.float64))))
+    out = _sized(links, dims, 16, "links").reshape(4 * _volume(dims), 4).copy()
+    s = _sized(spins, dims, 3, "spins")
+    _check(lib().mag_su2_gauge_transform_spins(_dims(dims), _dptr(out), _dptr(s)))
     return out
 
 
@@ -659,8 +675,8 @@ def gauge_transform_spins(dims, links, spins) -> np.ndarray:
 def u1_monopole_currents(dims, angles) -> np.ndarray:
     """calculate_current of an abelian configuration (monopoles.cpp:669-687):
     angles [4V] in reference link order -> J [4V]."""
-    a = np.ascontiguousarray(angles, dtype=np.float64).reshape(-1)
-    out = np.empty_like(a)
+    a = _sized(angles, dims, 4, "angles").reshape(-1)
+    out = np.empty(4 * _volume(dims), dtype=np.float64)
     _check(lib().mag_u1_monopole_currents(_dims(dims), _dptr(a), _dptr(out)))
     return out
 
@@ -668,8 +684,8 @@ def u1_monopole_currents(dims, angles) -> np.ndarray:
 def su3_monopole_currents(dims, angles3) -> np.ndarray:
     """calculate_current of an su3_angles configuration (monopoles.cpp:646-667):
     angles [4V, 3] -> J [3, 4V]."""
-    a = np.ascontiguousarray(angles3, dtype=np.float64)
-    out = np.empty((3, a.size // 3), dtype=np.float64)
+    a = _sized(angles3, dims, 12, "angles3")
+    out = np.empty((3, 4 * _volume(dims)), dtype=np.float64)
     _check(lib().mag_su3_monopole_currents(_dims(dims), _dptr(a), _dptr(out)))
     return out
 

@@ -170,8 +170,10 @@ int mag_job_plaquette(mag_job *job, double *P);
  * on a new (group, precision, lattice) times every shape on a scratch copy
  * of the job's links and keeps the fastest for the process (MAG_TUNE_AUTO);
  * MAG_TUNE_FIXED uses (nt, spt) everywhere, MAG_TUNE_DEFAULT uses (256, 1)
- * without tuning.  Batches always use (256, 1).  No counterpart in the
- * reference. */
+ * without tuning.  A batch class's half-sweep is tuned on its own batched
+ * launch (every shape over scratch copies of the class's jobs, cached per
+ * group, precision, device and log2 of the class's lanes) under the same
+ * modes.  No counterpart in the reference. */
 #define MAG_TUNE_AUTO 0
 #define MAG_TUNE_FIXED 1
 #define MAG_TUNE_DEFAULT 2

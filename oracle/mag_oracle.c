@@ -378,6 +378,16 @@ int or_su2_maximize_final(const double *links, double *spins, const int *dims,
   return converged ? it : -1;
 }
 
+/* the stopping test of make_maximization_approximate (mag.cpp:1744-1746):
+ * the leading `digits` decimals of F, scaled by the single-precision
+ * powf(10, digits) (inexact from 11 digits on) and truncated towards zero,
+ * are equal, and so is the sign bit */
+int or_approx_same(double functional, double functional_old, int tolerance_digits) {
+  return (trunc(powf(10.f, (float)tolerance_digits) * functional) ==
+          trunc(powf(10.f, (float)tolerance_digits) * functional_old)) &&
+         (signbit(functional) == signbit(functional_old));
+}
+
 /* make_maximization_approximate (mag.cpp:1733-1750), powf quirk kept */
 int or_su2_maximize_approximate(const double *links, double *spins,
                                 const int *dims, int or_steps,
@@ -391,13 +401,87 @@ int or_su2_maximize_approximate(const double *links, double *spins,
     or_su2_relax_sweep_lexi(links, spins, dims, &dmax, &davg);
     for (int i = 0; i < or_steps; i++) or_su2_or_sweep_lexi(links, spins, dims);
     functional = or_su2_functional_spin(links, spins, dims);
-    is_equal = (trunc(powf(10.f, (float)tolerance_digits) * functional) ==
-                trunc(powf(10.f, (float)tolerance_digits) * functional_old)) &&
-               (signbit(functional) == signbit(functional_old));
+    is_equal = or_approx_same(functional, functional_old, tolerance_digits);
     functional_old = functional;
     it++;
   } while (!is_equal && it < max_iter);
   return is_equal ? it : -1;
+}
+
+/* ------------------------------------------------------------------ */
+/* angle functions of the omega rule                                    */
+/* ------------------------------------------------------------------ */
+/* A fixed sequence of IEEE operations (the library is compiled with
+ * -ffp-contract=off, no fma), the same sequence as the device's:
+ *   atan2 for y > 0: fold onto t in [0, 1], t = c + ..., c = k/4 nearest,
+ *     atan t = atan(k/4) + atan((t - c)/(1 + t c)), Taylor to u^19;
+ *   sin / cos on [0, 2 pi]: Cody-Waite reduction by pi/2 (two-part
+ *     constant), Taylor to r^17 / r^18, quadrant.
+ * Both are checked against the C library's atan2 / sin / cos. */
+static double atan_small(double u) {
+  const double z = u * u;
+  double p = -1.0 / 19.0;
+  p = p * z + 1.0 / 17.0;
+  p = p * z + -1.0 / 15.0;
+  p = p * z + 1.0 / 13.0;
+  p = p * z + -1.0 / 11.0;
+  p = p * z + 1.0 / 9.0;
+  p = p * z + -1.0 / 7.0;
+  p = p * z + 1.0 / 5.0;
+  p = p * z + -1.0 / 3.0;
+  return u + (u * z) * p;
+}
+
+static double atan01(double t) {
+  static const double atan_quarter[5] = {0.0, 0.24497866312686414, 0.4636476090008061,
+                                         0.6435011087932844, 0.7853981633974483};
+  const int k = (int)(t * 4.0 + 0.5);
+  const double c = 0.25 * (double)k;
+  const double u = (t - c) / (1.0 + t * c);
+  return atan_quarter[k] + atan_small(u);
+}
+
+double or_atan2_pos(double y, double x) {
+  const double ax = fabs(x);
+  const double pio2 = 1.5707963267948966, pi = 3.141592653589793;
+  if (y <= ax) {
+    const double a = atan01(y / ax);
+    return x >= 0.0 ? a : pi - a;
+  }
+  const double a = atan01(ax / y);
+  return x >= 0.0 ? pio2 - a : pio2 + a;
+}
+
+void or_sincos(double a, double *s, double *c) {
+  const int q = (int)(a * 0.6366197723675814 + 0.5);
+  const double qd = (double)q;
+  const double r = (a - qd * 1.5707963267948966) - qd * 6.123233995736766e-17;
+  const double z = r * r;
+  double ps = 1.0 / 355687428096000.0;
+  ps = ps * z + -1.0 / 1307674368000.0;
+  ps = ps * z + 1.0 / 6227020800.0;
+  ps = ps * z + -1.0 / 39916800.0;
+  ps = ps * z + 1.0 / 362880.0;
+  ps = ps * z + -1.0 / 5040.0;
+  ps = ps * z + 1.0 / 120.0;
+  ps = ps * z + -1.0 / 6.0;
+  const double sr = r + (r * z) * ps;
+  double pc = -1.0 / 6402373705728000.0;
+  pc = pc * z + 1.0 / 20922789888000.0;
+  pc = pc * z + -1.0 / 87178291200.0;
+  pc = pc * z + 1.0 / 479001600.0;
+  pc = pc * z + -1.0 / 3628800.0;
+  pc = pc * z + 1.0 / 40320.0;
+  pc = pc * z + -1.0 / 720.0;
+  pc = pc * z + 1.0 / 24.0;
+  pc = pc * z + -0.5;
+  const double cr = 1.0 + z * pc;
+  switch (q & 3) {
+    case 0: *s = sr; *c = cr; break;
+    case 1: *s = cr; *c = -sr; break;
+    case 2: *s = -sr; *c = -cr; break;
+    default: *s = -cr; *c = sr; break;
+  }
 }
 
 /* ------------------------------------------------------------------ */
@@ -417,12 +501,14 @@ void or_omega_gauge(const double *h, double omega, double *g) {
     }
     /* anti-aligned: rotate about e1 */
     double phi = 0.5 * omega * M_PI;
-    g[0] = cos(phi); g[1] = sin(phi); g[2] = 0; g[3] = 0;
+    or_sincos(phi, &g[1], &g[0]);
+    g[2] = 0; g[3] = 0;
     return;
   }
-  double theta = atan2(hp, h[2]);
+  double theta = or_atan2_pos(hp, h[2]);
   double phi = 0.5 * omega * theta;
-  double sp = sin(phi), cp = cos(phi);
+  double sp, cp;
+  or_sincos(phi, &sp, &cp);
   double inv = 1.0 / hp;
   g[0] = cp;
   g[1] = -sp * h[1] * inv;
@@ -1075,6 +1161,15 @@ void or_block_field_bwd(const double *B, double *h) {
 static const int CM_I[3] = {0, 1, 0};
 static const int CM_J[3] = {1, 2, 2};
 
+/* 2-row storage (the build's SU(3) link form): row 2 = conj(row0 x row1),
+ * recomputed from rows 0 and 1 wherever it is read */
+static void m3_row2(m3 *M) {
+  const cx *a = M->m[0], *b = M->m[1];
+  M->m[2][0] = cconj(csub(cmul(a[1], b[2]), cmul(a[2], b[1])));
+  M->m[2][1] = cconj(csub(cmul(a[2], b[0]), cmul(a[0], b[2])));
+  M->m[2][2] = cconj(csub(cmul(a[0], b[1]), cmul(a[1], b[0])));
+}
+
 static void su3_link_halfsweep(const geom_t *g, double *links, int parity,
                                double omega, double *rsum, double *rmax) {
   int64_t rows = g->V / g->L[0];
@@ -1095,6 +1190,13 @@ static void su3_link_halfsweep(const geom_t *g, double *links, int parity,
       double dsite = 0;
       for (int sg = 0; sg < 3; sg++) {
         int i = CM_I[sg], j = CM_J[sg];
+        /* rows 0 and 1 are the state; row 2 is rebuilt before a subgroup
+         * that reads it (the updates below leave it stale) */
+        if (j == 2)
+          for (int mu = 0; mu < 4; mu++) {
+            m3_row2(&F[mu]);
+            m3_row2(&B[mu]);
+          }
         /* t_mu = f(mu) + b(mu); h = (t_0 + t_1) + (t_2 + t_3) */
         double t[4][3];
         for (int mu = 0; mu < 4; mu++) {
@@ -1128,6 +1230,8 @@ static void su3_link_halfsweep(const geom_t *g, double *links, int parity,
       rsum[r] += dsite;
       if (dsite > rmax[r]) rmax[r] = dsite;
       for (int mu = 0; mu < 4; mu++) {
+        m3_row2(&F[mu]);
+        m3_row2(&B[mu]);
         m3store(links + 18 * (4 * s + mu), &F[mu]);
         m3store(links + 18 * (4 * bn[mu] + mu), &B[mu]);
       }

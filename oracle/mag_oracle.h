@@ -62,6 +62,11 @@ void or_su2_relax_sweep_lexi(const double *links, double *spins, const int *dims
 int or_su2_maximize_final(const double *links, double *spins, const int *dims,
                           int or_steps, double tol_max, double tol_avg,
                           int max_iter);                                 /* mag.cpp:1800-1816 */
+/* the omega rule's angle functions (the device's operation sequence):
+ * atan2 for y > 0, sin and cos on [0, 2 pi] */
+double or_atan2_pos(double y, double x);
+void or_sincos(double a, double *s, double *c);
+int or_approx_same(double functional, double functional_old, int tolerance_digits);
 int or_su2_maximize_approximate(const double *links, double *spins,
                                 const int *dims, int or_steps,
                                 int tolerance_digits, int max_iter);     /* mag.cpp:1733-1750 */

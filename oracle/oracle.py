@@ -149,6 +149,9 @@ def lib():
             "or_su2_maximize_final": (ctypes.c_int, [_dp, _dp, _ip, ctypes.c_int,
                                                      ctypes.c_double, ctypes.c_double,
                                                      ctypes.c_int]),
+            "or_atan2_pos": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+            "or_sincos": (None, [ctypes.c_double, _dp, _dp]),
+            "or_approx_same": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_int]),
             "or_su2_maximize_approximate": (ctypes.c_int, [_dp, _dp, _ip, ctypes.c_int,
                                                            ctypes.c_int, ctypes.c_int]),
             "or_su2_spin_halfsweep": (None, [_dp, _dp, _ip, ctypes.c_int, ctypes.c_int,
@@ -334,6 +337,23 @@ def su2_maximize_final(links, spins, dims, or_steps, tol_max, tol_avg, max_iter=
 def su2_maximize_approximate(links, spins, dims, or_steps, tol_digits, max_iter=100000):
     return lib().or_su2_maximize_approximate(_d(links), _d(spins), _dims(dims),
                                              int(or_steps), int(tol_digits), int(max_iter))
+
+
+def atan2_pos(y, x) -> float:
+    """The omega rule's atan2 (y > 0): a fixed sequence of IEEE operations."""
+    return lib().or_atan2_pos(float(y), float(x))
+
+
+def sincos(a):
+    """The omega rule's (sin a, cos a) for a in [0, 2 pi]."""
+    s, c = ctypes.c_double(), ctypes.c_double()
+    lib().or_sincos(float(a), ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value
+
+
+def approx_same(F, F_old, digits) -> bool:
+    """The stopping test of make_maximization_approximate (mag.cpp:1744-1746)."""
+    return bool(lib().or_approx_same(float(F), float(F_old), int(digits)))
 
 
 def su2_spin_halfsweep(links, spins, dims, parity, mode, omega=1.0):

@@ -97,3 +97,32 @@ def test_missing_library_fails_loudly(tmp_path):
             "    print('no error')\n")
     out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
     assert out.stdout.startswith("ImportError") and "build it" in out.stdout, out.stdout + out.stderr
+
+
+def test_one_shot_wrappers_check_array_sizes_against_dims(mag):
+    """The one-shot helpers read (and write) as many reals as dims imply; a
+    short or long array is a ValueError before the C side is called."""
+    import numpy as np
+    import pytest
+
+    d = (4, 4, 4, 4)
+    V = 256
+    short16 = np.zeros(16 * V - 4)
+    with pytest.raises(ValueError):
+        mag.su2_functional(d, short16)
+    with pytest.raises(ValueError):
+        mag.su3_functional(d, np.zeros(16 * V))
+    with pytest.raises(ValueError):
+        mag.plaquette(mag.SU3, d, np.zeros(16 * V))
+    with pytest.raises(ValueError):
+        mag.functional_spin(d, np.zeros(16 * V), np.zeros(3 * V - 3))
+    with pytest.raises(ValueError):
+        mag.gauge_transform_spins(d, short16, np.zeros(3 * V))
+    with pytest.raises(ValueError):
+        mag.gauge_transform_spins(d, np.zeros(16 * V), np.zeros(2 * V))
+    with pytest.raises(ValueError):
+        mag.u1_monopole_currents(d, np.zeros(4 * V + 1))
+    with pytest.raises(ValueError):
+        mag.su3_monopole_currents(d, np.zeros((4 * V, 2)))
+    with pytest.raises(ValueError):
+        mag.su2_functional((4, 4, 4), np.zeros(16 * 64))

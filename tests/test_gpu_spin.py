@@ -135,17 +135,46 @@ def test_maximize_final_with_reflections(mag, oracle, cfg):
         assert mx < 1e-8
 
 
-def test_maximize_approximate(mag, oracle, cfg):
+def _oracle_approximate(oracle, links, spins, or_steps, digits, max_iter):
+    """make_maximization_approximate (mag.cpp:1733-1750) composed from the
+    oracle's checkerboard half-sweeps and its criterion (or_approx_same):
+    the GPU driver's order."""
+    F_old = oracle.su2_functional_spin(links, spins, DIMS)
+    it = 0
+    while True:
+        oracle.su2_spin_halfsweep(links, spins, DIMS, 0, 1)
+        oracle.su2_spin_halfsweep(links, spins, DIMS, 1, 1)
+        for _ in range(or_steps):
+            oracle.su2_spin_halfsweep(links, spins, DIMS, 0, 0)
+            oracle.su2_spin_halfsweep(links, spins, DIMS, 1, 0)
+        F = oracle.su2_functional_spin(links, spins, DIMS)
+        it += 1
+        same = oracle.approx_same(F, F_old, digits)
+        F_old = F
+        if same or it >= max_iter:
+            return it, same, F
+
+
+@pytest.mark.parametrize("or_steps,digits", [(4, 4), (1, 6), (0, 9), (0, 11)])
+def test_maximize_approximate_matches_oracle(mag, oracle, cfg, or_steps, digits):
+    """The GPU approximate driver stops at the oracle's iteration with the
+    oracle's spins; 11 digits exercise the inexact single-precision
+    powf(10, 11) of the reference's criterion.  Reflection sweeps do not
+    contract rounding differences, so runs with them stop at few digits;
+    relaxation-only runs go to 9 and 11."""
     links, spins = cfg
+    ref = spins.copy()
+    it_ref, same, F_ref = _oracle_approximate(oracle, links, ref, or_steps, digits, 20000)
+    assert same and it_ref > 1
     with mag.SpinSession(DIMS) as s:
         s.upload_links(links)
         s.set_spins(spins)
-        it = s.maximize_approximate(4, 7, 20000)
-        assert it > 1
+        it = s.maximize_approximate(or_steps, digits, 20000)
+        got = s.get_spins()
         F = s.functional_spin()
-    # one more iteration does not change the 7 leading digits much
-    ref = spins.copy()
-    assert F > oracle.su2_functional_spin(links, ref, DIMS)
+    assert it == it_ref
+    assert np.max(np.abs(got - ref)) < 1e-10
+    assert abs(F - F_ref) < 1e-12 * abs(F_ref)
 
 
 def test_annealing_then_fixing_end_to_end(mag, oracle, cfg):

@@ -183,6 +183,85 @@ def test_relaxation_is_monotone_and_maximisation_converges(oracle):
     assert oracle.su2_maximize_approximate(links, spins2, dims, 4, 7, 10000) > 0
 
 
+def test_approximate_criterion_hand_cases(oracle):
+    """The stopping test of make_maximization_approximate (mag.cpp:1744-1746):
+    trunc(powf(10, d) * F) == trunc(powf(10, d) * F_old) and equal sign
+    bits.  powf is single precision: 10^d is exact in float up to d = 10,
+    and powf(10, 11) = 99999997952 (the nearest float to 1e11).  The cases
+    below were chosen by hand so that pow in place of powf, floor in place
+    of trunc, or a dropped sign test each flips one of them."""
+    # d = 11, F_old = 0.3, F = 0.3 + 4.5e-12:
+    #   with 1e11: 30000000000 vs 30000000000.45 -> same
+    #   with powf: 29999999385.6 vs 29999999386.05 -> 29999999385 != 29999999386
+    assert oracle.approx_same(0.3000000000045, 0.3, 11) is False
+    # F = 0.3 - 3e-12: 1e11 gives 29999999999 != 30000000000, powf gives
+    # 29999999385.3 and 29999999385.6 -> same
+    assert oracle.approx_same(0.3 - 3e-12, 0.3, 11) is True
+    # d = 10 (powf exact): the decimal digits decide
+    assert oracle.approx_same(0.3000000000045, 0.3, 10) is True
+    assert oracle.approx_same(0.3 - 3e-12, 0.3, 10) is False
+    # trunc, not floor: -1.0 vs -0.5 -> -1 != -0 (floor: -1 == -1)
+    assert oracle.approx_same(-0.001, -0.0005, 3) is False
+    # both truncate to zero but the signs differ
+    assert oracle.approx_same(-1e-13, 1e-13, 3) is False
+    assert oracle.approx_same(1e-13, 2e-13, 3) is True
+    # ordinary cases at 7 digits
+    assert oracle.approx_same(0.61234567, 0.61234568, 7) is True
+    assert oracle.approx_same(0.61234567, 0.61234557, 7) is False
+
+
+def test_maximize_approximate_stops_at_the_first_equal_digits(oracle):
+    """or_su2_maximize_approximate is the loop {relax, OR x n, F_s} until
+    the criterion holds: replayed step by step with the oracle's own
+    sweeps, it stops at the first iteration whose F_s matches the previous
+    one and returns that iteration count."""
+    dims = (4, 4, 4, 4)
+    links = oracle.gen_su2_config(dims, 0.5, 23)
+    spins = oracle.gen_spins_uniform(dims, 23)
+    ref = spins.copy()
+    it = oracle.su2_maximize_approximate(links, spins, dims, 3, 6, 10000)
+    F_old = oracle.su2_functional_spin(links, ref, dims)
+    k = 0
+    while True:
+        oracle.su2_relax_sweep_lexi(links, ref, dims)
+        for _ in range(3):
+            oracle.su2_or_sweep_lexi(links, ref, dims)
+        F = oracle.su2_functional_spin(links, ref, dims)
+        k += 1
+        # criterion by hand in decimal (powf(10, 6) is exact)
+        same = int(F * 1e6) == int(F_old * 1e6) and (F < 0) == (F_old < 0)
+        F_old = F
+        if same:
+            break
+    assert it == k > 1
+    assert np.array_equal(spins, ref)
+
+
+def test_omega_rule_angle_functions_against_libm(oracle):
+    """The omega rule's atan2 (y > 0) and sin / cos on [0, 2 pi] are a fixed
+    sequence of IEEE operations shared with the device; they agree with the
+    C library's atan2 / sin / cos to 2 ulp, at random points and at the ends
+    and switch points of their argument reductions."""
+    rng = np.random.default_rng(77)
+    ys = np.concatenate([rng.random(3000), 10.0 ** rng.uniform(-12, 3, 2000)])
+    xs = np.concatenate([rng.normal(size=3000), rng.choice([-1, 1], 2000) * 10.0 ** rng.uniform(-12, 3, 2000)])
+    # t = y/|x| on the quarter switch points and the octant boundary
+    edge_t = np.array([0.125, 0.375, 0.625, 0.875, 1.0, 1e-300, 0.5])
+    ys = np.concatenate([ys, edge_t, edge_t, np.ones(7)])
+    xs = np.concatenate([xs, np.ones(7), -np.ones(7), 1 / edge_t[:7]])
+    for y, x in zip(ys, xs):
+        ref = np.arctan2(y, x)
+        got = oracle.atan2_pos(y, x)
+        assert abs(got - ref) <= 2 * np.spacing(ref), (y, x, got, ref)
+    pio4 = np.pi / 4
+    angles = np.concatenate([rng.random(5000) * 2 * np.pi, [0.0, 2 * np.pi, np.pi],
+                             pio4 * np.arange(1, 8), np.nextafter(pio4 * np.arange(1, 8), 0)])
+    for a in angles:
+        s, c = oracle.sincos(a)
+        assert abs(s - np.sin(a)) <= 2 * np.spacing(abs(np.sin(a))) + 1e-300, (a, s)
+        assert abs(c - np.cos(a)) <= 2 * np.spacing(abs(np.cos(a))) + 1e-300, (a, c)
+
+
 def test_omega_gauge_rotates_by_omega_theta(oracle):
     rng = np.random.default_rng(6)
     for _ in range(30):

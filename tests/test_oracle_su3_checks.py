@@ -68,6 +68,31 @@ def test_su3_reunitarize_and_proj(oracle):
         assert np.max(np.abs(R[0] - r0)) < 1e-14
 
 
+def test_su3_proj_is_the_special_unitary_polar_factor(oracle):
+    """su3::proj (matrix.h:279-292): Newton steps A <- 3/2 A - 1/2 A A^+ A
+    converge to the unitary polar factor W of M = W P, and the factor
+    1 - (det A - 1)/3 removes the determinant's phase (to third order per
+    step), so six steps give W det(W)^(-1/3) (principal root) to rounding.
+    W comes from scipy.linalg.polar.  Gram-Schmidt of the same M (another
+    unitary matrix, off by O(|M - U|)) fails the 1e-12 bound."""
+    rng = np.random.default_rng(12)
+    for amp in (1e-3, 1e-2, 3e-2):
+        for _ in range(10):
+            U = scipy.linalg.expm(1j * np.einsum("a,aij->ij", rng.normal(size=8), gell_mann()))
+            M = U + amp * (rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3)))
+            W, _ = scipy.linalg.polar(M)
+            d = np.linalg.det(W)
+            ref = W * np.exp(-1j * np.angle(d) / 3)
+            got = m3(oracle.su3_proj(v18(M)))
+            assert np.max(np.abs(got - ref)) < 1e-12, amp
+            gs = m3(oracle.su3_reunitarize(v18(M)))
+            assert np.max(np.abs(gs - ref)) > 1e3 * np.max(np.abs(got - ref))
+    # an SU(3) matrix is its own projection; a positive multiple projects to it
+    U = scipy.linalg.expm(1j * np.einsum("a,aij->ij", rng.normal(size=8), gell_mann()))
+    assert np.max(np.abs(m3(oracle.su3_proj(v18(U))) - U)) < 1e-14
+    assert np.max(np.abs(m3(oracle.su3_proj(v18(0.98 * U))) - U)) < 1e-13
+
+
 def test_su3_products(oracle):
     rng = np.random.default_rng(3)
     A = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
