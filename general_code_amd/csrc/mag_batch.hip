@@ -6,9 +6,9 @@
 // a class is
 //   for parity 0, 1: one half-sweep launch whose grid concatenates the
 //                    member jobs' grids
-//   one reduction launch (a workgroup per job, per copy of a copy set)
-//   every reunit_every sweeps (link-form FP32 classes): one FP64 rebuild,
-//   one normalisation and one rebuild reduction
+//   one reduction launch (a workgroup per job, per copy of a copy set);
+//   link-form FP32 classes instead reduce once per reunit_every sweeps,
+//   after the black half-sweep that rebuilds the links from the FP64 master
 // and `replay` sweeps (default 20) plus a gather of every member's control
 // block into pinned host memory are captured in one hipGraph per class.
 //
@@ -64,7 +64,6 @@ struct ClassRun {
   JobCtl* ctl_out = nullptr;  // pinned host memory, one entry per red[i]
   size_t cap_out = 0;
   int nblocks = 0;   // half-sweep grid
-  int nrblocks = 0;  // rebuild grid (FP32 link-form classes)
   mag::Shape shape;  // link-form classes: launch shape of the half-sweep
   hipStream_t s = nullptr;
   hipEvent_t ev = nullptr;
@@ -133,9 +132,7 @@ BatchJob describe(mag_job* j) {
   b.nb = j->nb;
   b.nw = j->nw;
   b.nparts = 2 * j->nw;
-  b.nrparts = mag::rebuild_blocks_of(j->g);
   b.block0 = 0;
-  b.rb0 = 0;
   b.S = (j->spin && j->prm.group == MAG_SU3 && fp32) ? (void*)j->S32 : (void*)j->S;
   b.sstride = j->sstride;
   b.pstride = j->pstride;
@@ -168,14 +165,25 @@ void launch_class_half(const mag_batch* b, int c, int p, hipStream_t st) {
 // one replay of class c: `replay` sweeps, then the control-block gather
 void enqueue_class_replay(const mag_batch* b, int c, hipStream_t s) {
   const ClassRun& t = b->cls[c];
+  const int nred = (int)t.red.size();
   for (int i = 0; i < b->replay; i++) {
+    if (c < kSpinClass && t.fp32) {
+      // mixed FP32 phase, as a job alone (mag_capi.hip enqueue_sweep):
+      // periods of reunit sweeps, the replay a whole number of them
+      const int R = b->reunit, k = i % R;
+      mag::launch_half_batch(t.group, true, t.shape, t.d, (int)t.h.size(), t.nblocks, 0, s, k,
+                             k == 0 ? mag::HALF_RENORM : mag::HALF_PLAIN);
+      mag::launch_half_batch(t.group, true, t.shape, t.d, (int)t.h.size(), t.nblocks, 1, s, k,
+                             k == R - 1 ? mag::HALF_REBUILD : mag::HALF_PLAIN);
+      if (k == R - 1) {
+        mag::launch_reduce_slots_batch(t.d_red, nred, R, s);
+        mag::launch_reduce_period_batch(t.d_red, nred, R, 1, s);
+      }
+      continue;
+    }
     launch_class_half(b, c, 0, s);
     launch_class_half(b, c, 1, s);
-    mag::launch_reduce_batch(t.d_red, (int)t.red.size(), mag::RED_SWEEP, s);
-    if (c < kSpinClass && t.fp32 && b->reunit > 0 && (i + 1) % b->reunit == 0) {
-      mag::launch_rebuild_batch(t.group, t.d, (int)t.h.size(), t.nrblocks, s);
-      mag::launch_reduce_batch(t.d_red, (int)t.red.size(), mag::RED_REBUILD, s);
-    }
+    mag::launch_reduce_batch(t.d_red, nred, mag::RED_SWEEP, s);
   }
   mag::launch_gather_ctl(t.d_red, (int)t.red.size(), t.ctl_out, s);
 }
@@ -204,7 +212,6 @@ int refresh_class(mag_batch* b, int c) {
   t.owner.clear();
   t.cap_jobs = t.jobs;
   t.nblocks = 0;
-  t.nrblocks = 0;
   t.shape = mag::Shape{};
   if (c < kSpinClass && !t.jobs.empty())
     CHK(mag::batch_shape(t.group, t.fp32, t.jobs, &t.shape));  // autotuned; same results any shape
@@ -215,9 +222,7 @@ int refresh_class(mag_batch* b, int c) {
       d.nb = (int)((mag::sweep_lanes(j->g, j->prm.group) + tile - 1) / tile);
     }
     d.block0 = t.nblocks;
-    d.rb0 = t.nrblocks;
     t.nblocks += d.nb;
-    t.nrblocks += mag::rebuild_blocks_of(j->g);
     t.h.push_back(d);
     if (!j->spin) {
       t.red.push_back(d);

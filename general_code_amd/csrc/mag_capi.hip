@@ -116,7 +116,7 @@ mag::Shape link_shape(mag_job* j, bool fp32) {
   return j->shape[k];
 }
 
-void launch_half(mag_job* j, int parity) {
+void launch_half(mag_job* j, int parity, const mag::HalfOpt& o = mag::HalfOpt{}) {
   if (j->spin && j->prm.group == MAG_SU3) {
     if (mixed32(j))
       mag::launch_su3_site_copies<float>(j->W32, j->S32, j->sstride, j->ncopies, j->g, parity,
@@ -132,11 +132,18 @@ void launch_half(mag_job* j, int parity) {
       mag::launch_su2_copies_half<double>(j->U64, j->S, j->sstride, j->ncopies, j->g, parity,
                                           j->prm.omega, j->part, j->pstride, j->ctl, j->st);
   } else if (mixed32(j)) {
+    // red sites accumulate the site gauge; the rebuilding black half reads it
     const mag::Shape sh = link_shape(j, true);
+    float* G = (parity == 0 || o.mode == mag::HALF_REBUILD) ? j->G32 : nullptr;
+    mag::HalfOpt oo = o;
+    if (oo.mode == mag::HALF_REBUILD) {
+      oo.U0 = j->U64;
+      oo.part_rb = j->part + (size_t)mag::kSlots * j->pstride;
+    }
     if (j->prm.group == MAG_SU2)
-      mag::launch_su2_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, j->part, j->ctl, sh, j->st);
+      mag::launch_su2_half<float>(j->W32, G, j->g, parity, j->prm.omega, j->part, j->ctl, sh, j->st, oo);
     else
-      mag::launch_su3_half<float>(j->W32, j->G32, j->g, parity, j->prm.omega, j->part, j->ctl, sh, j->st);
+      mag::launch_su3_half<float>(j->W32, G, j->g, parity, j->prm.omega, j->part, j->ctl, sh, j->st, oo);
   } else {
     const mag::Shape sh = link_shape(j, false);
     if (j->prm.group == MAG_SU2)
@@ -146,28 +153,47 @@ void launch_half(mag_job* j, int parity) {
   }
 }
 
-// periodic rebuild (skipped by the kernels once the job is done) or the
-// final FP64 rebuild (forced)
-// (the periodic one also sums F of the rebuilt links and tests convergence
-// on it: RED_REBUILD)
-void launch_rebuild(mag_job* j, bool final64) {
-  const JobCtl* c = final64 ? nullptr : j->ctl;
-  double* part = final64 ? nullptr : j->part;
-  if (j->prm.group == MAG_SU2)
-    mag::launch_su2_rebuild(j->U64, j->G32, final64 ? nullptr : j->W32, final64 ? j->U64 : nullptr,
-                            j->g, c, part, j->st);
-  else
-    mag::launch_su3_rebuild(j->U64, j->G32, final64 ? nullptr : j->W32, final64 ? j->U64 : nullptr,
-                            j->g, c, part, j->st);
-  if (!final64) {
-    mag::launch_gauge_normalize(j->G32, j->g, j->prm.group, j->ctl, j->st);
-    mag::launch_reduce(j->part, mag::rebuild_blocks_of(j->g), mag::RED_REBUILD, j->invF, j->invV,
-                       j->ctl, j->hF, j->hR, j->hA, nullptr, j->st);
-  }
+// mixed FP32 phase: reduce the slots swept since the last reduction into
+// the history and the sweep count; after a rebuilding sweep also test
+// convergence on the rebuilt links' F
+void flush_slots(mag_job* j, bool rebuild) {
+  if (j->pending == 0) return;
+  mag::launch_reduce_slots(j->part, j->pstride, 2 * j->nw, j->pending, j->invF, j->invV, j->ctl,
+                           j->hF, j->hR, j->hA, j->st);
+  mag::launch_reduce_period(j->part + (size_t)mag::kSlots * j->pstride, j->nw, j->pending,
+                            rebuild ? 1 : 0, j->invF, j->ctl, j->hF, j->st);
+  j->pending = 0;
 }
 
-// sweep tail shared by the plain and timed paths
-void enqueue_after_sweep(mag_job* j) {
+// the FP64 links at the end of the FP32 phase (in place over the master)
+void launch_final_rebuild(mag_job* j) {
+  if (j->prm.group == MAG_SU2)
+    mag::launch_su2_rebuild_final(j->U64, j->G32, j->W32, j->U64, j->g, j->st);
+  else
+    mag::launch_su3_rebuild_final(j->U64, j->G32, j->W32, j->U64, j->g, j->st);
+}
+
+// One sweep.  Mixed link form, FP32 phase: periods of reunit_every sweeps;
+// the first red half of a period renormalises the red gauges, the last
+// black half rebuilds the links from the master; the sweeps' partials go to
+// slots reduced at the end of a period or when the slots are full.
+void enqueue_sweep(mag_job* j) {
+  if (mixed32(j) && !j->spin) {
+    const int R = j->prm.reunit_every;
+    mag::HalfOpt o;
+    o.slot = j->pending;
+    o.mode = j->since_rebuild == 0 ? mag::HALF_RENORM : mag::HALF_PLAIN;
+    launch_half(j, 0, o);
+    const bool rebuild = j->since_rebuild == R - 1;
+    o.mode = rebuild ? mag::HALF_REBUILD : mag::HALF_PLAIN;
+    launch_half(j, 1, o);
+    j->pending++;
+    j->since_rebuild = rebuild ? 0 : j->since_rebuild + 1;
+    if (rebuild || j->pending == mag::kSlots) flush_slots(j, rebuild);
+    return;
+  }
+  launch_half(j, 0);
+  launch_half(j, 1);
   if (j->spin) {
     mag::launch_reduce_copies(j->part, j->pstride, 2 * j->nw, j->ncopies, j->invF, j->invV,
                               j->ctl, j->hF, j->hR, j->hA, (size_t)j->prm.max_sweeps, j->st);
@@ -175,16 +201,6 @@ void enqueue_after_sweep(mag_job* j) {
   }
   mag::launch_reduce(j->part, 2 * j->nw, mag::RED_SWEEP, j->invF, j->invV, j->ctl, j->hF,
                      j->hR, j->hA, nullptr, j->st);
-  if (mixed32(j) && ++j->since_rebuild >= j->prm.reunit_every) {
-    launch_rebuild(j, false);
-    j->since_rebuild = 0;
-  }
-}
-
-void enqueue_sweep(mag_job* j) {
-  launch_half(j, 0);
-  launch_half(j, 1);
-  enqueue_after_sweep(j);
 }
 
 // functional partials of the current working links
@@ -280,6 +296,7 @@ int start_fp32_phase(mag_job* j) {
                                  j->g, j->st);
   j->phase = 0;
   j->since_rebuild = 0;
+  j->pending = 0;
   for (int c = 0; c < j->ncopies; c++) {
     j->ctl_h[c].tol = j->prm.tol_mixed;
     j->ctl_h[c].window = j->spin ? 0 : -1;
@@ -317,6 +334,7 @@ int reset_after_upload(mag_job* j) {
   }
   j->fp64_start = mixed ? -1 : 0;
   j->since_rebuild = 0;
+  j->pending = 0;
   // initial functional from the FP64 links (and each copy's site variable)
   double* tmp = nullptr;
   CHK(alloc_tmp(j, &tmp));
@@ -410,8 +428,9 @@ int finalize_mixed_spin(mag_job* j) {
 int finalize_mixed(mag_job* j) {
   if (!mixed32(j)) return MAG_OK;
   if (j->spin) return finalize_mixed_spin(j);
+  flush_slots(j, false);
   CHK(read_ctl(j));
-  launch_rebuild(j, true);
+  launch_final_rebuild(j);
   j->phase = 1;
   j->fp64_start = j->ctl_h->sweep;
   if (j->prm.group == MAG_SU2) mag::launch_su2_functional<double>(j->U64, j->g, j->part, j->st);
@@ -434,6 +453,7 @@ int run_until_done(mag_job* j) {
     bool sw;
     CHK(maybe_start_fp32(j, &sw));
     for (int i = 0; i < kChunk; i++) enqueue_sweep(j);
+    flush_slots(j, false);
     HIPCHK(hipGetLastError());
   }
 }
@@ -499,7 +519,8 @@ int mag_job_footprint(const mag_params* p, uint64_t* bytes) {
   const uint64_t lanes = (uint64_t)(V / 2) * (p->group == MAG_SU2 ? 1 : 2);
   const uint64_t nw = (lanes + 63) / 64;  // partial triples per half-sweep
   b += (uint64_t)4 << 20;  // pitch padding of the chunk arrays (4 KB per array)
-  b += nc * (nw * 2 * 3 * 8 + (uint64_t)p->max_sweeps * 3 * 8 + sizeof(JobCtl)) + 3 * 8;
+  const uint64_t slots = (p->precision == MAG_PREC_MIXED && p->form == MAG_FORM_LINK) ? mag::kSlots + 1 : 1;
+  b += nc * (nw * 2 * 3 * 8 * slots + (uint64_t)p->max_sweeps * 3 * 8 + sizeof(JobCtl)) + 3 * 8;
   *bytes = b;
   return MAG_OK;
 }
@@ -556,7 +577,9 @@ int mag_job_create(const mag_params* p, mag_job** out) {
     if (j->spin && p->group == MAG_SU3) ALLOC(j->S32, j->sstride * j->ncopies * sizeof(float));
   }
   if (j->spin) ALLOC(j->S, j->sstride * j->ncopies * sizeof(double));
-  ALLOC(j->part, j->pstride * j->ncopies * sizeof(double));
+  // mixed link form: kSlots sweep slots and the rebuild partials
+  const size_t nslots = (p->precision == MAG_PREC_MIXED && !j->spin) ? mag::kSlots + 1 : 1;
+  ALLOC(j->part, j->pstride * j->ncopies * nslots * sizeof(double));
   ALLOC(j->ctl, j->ncopies * sizeof(JobCtl));
   ALLOC(j->hF, (size_t)p->max_sweeps * j->ncopies * sizeof(double));
   ALLOC(j->hR, (size_t)p->max_sweeps * j->ncopies * sizeof(double));
@@ -655,6 +678,7 @@ int mag_job_sweeps(mag_job* j, int n) {
   if (j->batch) return fail(MAG_ERR_STATE, "job is in a batch: sweep it through the batch");
   CHK(set_device(j));
   for (int i = 0; i < n; i++) enqueue_sweep(j);
+  flush_slots(j, false);
   HIPCHK(hipGetLastError());
   return MAG_OK;
 }
@@ -795,6 +819,7 @@ int mag_job_sweeps_timed(mag_job* j, int n, double* total_ms) {
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, j->st));
   for (int i = 0; i < n; i++) enqueue_sweep(j);
+  flush_slots(j, false);
   HIPCHK(hipEventRecord(e1, j->st));
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventSynchronize(e1));

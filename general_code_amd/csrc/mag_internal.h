@@ -71,9 +71,7 @@ struct JobCtl {
 // reduction kernel modes
 //   RED_INIT : F0 = F_prev = F      RED_SWEEP: append a sweep to the history
 //   RED_PLAIN: out3 only            RED_RESET: F_prev = F (phase change)
-//   RED_REBUILD: F of freshly rebuilt links (FP64); converged when the mean
-//                change per sweep since the previous rebuild is below tol
-enum { RED_INIT = 0, RED_SWEEP = 1, RED_PLAIN = 2, RED_RESET = 3, RED_REBUILD = 4 };
+enum { RED_INIT = 0, RED_SWEEP = 1, RED_PLAIN = 2, RED_RESET = 3 };
 
 // ---------------------------------------------------------------- launchers
 // Partial buffers hold triples of doubles (F sum, residual sum, residual
@@ -111,16 +109,50 @@ struct Shape {
   int spt = 1;
 };
 
+// Mixed-precision link form, FP32 phase: sweeps run in periods of
+// reunit_every.  Only red (parity 0) sites accumulate a site gauge G; the
+// red half-sweep that opens a period first normalises its G in FP64
+// (HALF_RENORM); the black half-sweep that closes it rebuilds every link
+// from the FP64 master U0 with the red gauges and the black gauges
+// recovered from the swept links (HALF_REBUILD, mag_gauge.h) and writes
+// the functional partials of the rebuilt links to part_rb.  The per-sweep
+// partials of the FP32 phase are kept in kSlots slots (slot s at
+// part + s * 2*nw*3) and reduced together at the end of a period or when
+// the slots are full (launch_reduce_slots / launch_reduce_period); the
+// rebuild partials sit after the slots, at part + kSlots * 2*nw*3.
+enum { HALF_PLAIN = 0, HALF_RENORM = 1, HALF_REBUILD = 2 };
+constexpr int kSlots = 20;
+
+struct HalfOpt {
+  int slot = 0;               // sweep slot of the partials (0 outside the FP32 phase)
+  int mode = HALF_PLAIN;
+  const double* U0 = nullptr; // HALF_REBUILD: FP64 master
+  double* part_rb = nullptr;  // HALF_REBUILD: partials of the rebuilt links' F
+};
+
 // One half-sweep of parity `parity`; partial triples go to
-// part[3*nw*parity ...], nw = sweep_waves(g, group); G (mixed precision) is
-// the accumulated site gauge or nullptr.  The kernels return at once when
-// ctl->done is set.
+// part + slot*2*3*nw + 3*nw*parity, nw = sweep_waves(g, group); G is the
+// red site gauge (mixed FP32 phase: accumulated by red PLAIN / RENORM
+// half-sweeps, read by HALF_REBUILD) or nullptr.  The kernels return at
+// once when ctl->done is set or ctl->sweep + slot reaches ctl->max_sweeps.
 template <typename T>
 void launch_su2_half(T* U, T* G, const Geom& g, int parity, double omega,
-                     double* part, const JobCtl* ctl, Shape sh, hipStream_t st);
+                     double* part, const JobCtl* ctl, Shape sh, hipStream_t st,
+                     const HalfOpt& o = HalfOpt{});
 template <typename T>
 void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega,
-                     double* part, const JobCtl* ctl, Shape sh, hipStream_t st);
+                     double* part, const JobCtl* ctl, Shape sh, hipStream_t st,
+                     const HalfOpt& o = HalfOpt{});
+// FP32 phase: reduce slots 0..n-1 into the history (entries ctl->sweep + k,
+// below max_sweeps), then advance ctl->sweep by n (capped at max_sweeps,
+// done at the cap); with rebuild: the rebuilt links' F from part_rb (nrb
+// triples) decides convergence: the mean change per sweep since the
+// previous rebuild below tol
+void launch_reduce_slots(const double* part, size_t pstride, int nparts, int n, double invF,
+                         double invV, JobCtl* ctl, double* hF, double* hR, double* hA,
+                         hipStream_t st);
+void launch_reduce_period(const double* part_rb, int nrb, int n, int rebuild, double invF,
+                          JobCtl* ctl, double* hF, hipStream_t st);
 
 // Gauge-copy (spin form, SU(2), FP64) half-sweep over all copies of one
 // configuration: the links U (read-only) are loaded once per site and
@@ -173,9 +205,7 @@ struct BatchJob {
   int nb;      // workgroups of one half-sweep of this job (default shape)
   int nw;      // partial triples of one half-sweep (one per wave)
   int nparts;  // partial triples of a sweep (2*nw)
-  int nrparts; // partial triples of a rebuild
   int block0;  // first workgroup of this job in a batched half-sweep grid
-  int rb0;     // first workgroup of this job in a batched rebuild grid
   // spin form (gauge copies): spins of copy c at S + c*sstride, partials
   // at part + c*pstride, control block ctl + c; ncopies = 0 in the link form
   void* S;  // spins (double) / site gauges (the link type)
@@ -185,9 +215,16 @@ struct BatchJob {
 };
 
 // batched half-sweep of one class (group, FP32 phase or FP64) table with
-// launch shape sh (block0 / nblocks_total in workgroups of that shape)
+// launch shape sh (block0 / nblocks_total in workgroups of that shape);
+// slot and mode as in HalfOpt (the FP32 class: G of the red sites, U0 and
+// the rebuild partials of each job from its table entry)
 void launch_half_batch(int group, bool fp32, Shape sh, const BatchJob* jobs, int njobs,
-                       int nblocks_total, int parity, hipStream_t st);
+                       int nblocks_total, int parity, hipStream_t st, int slot = 0,
+                       int mode = HALF_PLAIN);
+// batched launch_reduce_slots / launch_reduce_period over a table
+void launch_reduce_slots_batch(const BatchJob* jobs, int njobs, int n, hipStream_t st);
+void launch_reduce_period_batch(const BatchJob* jobs, int njobs, int n, int rebuild,
+                                hipStream_t st);
 // batched spin-form (gauge-copy) half-sweep of a table of copy sets that
 // all hold ncopies copies
 void launch_copies_half_batch(int ncopies, bool fp32, const BatchJob* jobs, int njobs,
@@ -196,14 +233,9 @@ void launch_copies_half_batch(int ncopies, bool fp32, const BatchJob* jobs, int 
 void launch_su3_site_copies_batch(bool fp32, const BatchJob* jobs, int njobs, int nblocks_total,
                                   int parity, hipStream_t st);
 // batched reduction, one workgroup per job: mode RED_SWEEP (end of sweep)
-// or RED_REBUILD (after a batched rebuild)
 void launch_reduce_batch(const BatchJob* jobs, int njobs, int mode, hipStream_t st);
 // every entry's control block into out (host-visible memory, n entries)
 void launch_gather_ctl(const BatchJob* jobs, int n, JobCtl* out, hipStream_t st);
-// batched FP64 rebuild of the FP32 working links + gauge normalisation of a
-// class table (mixed FP32 phase); jobs that are done are skipped
-void launch_rebuild_batch(int group, const BatchJob* jobs, int njobs, int nblocks_total,
-                          hipStream_t st);
 
 // functional partials of a whole configuration (both parities): 2*nblocks
 template <typename T>
@@ -226,23 +258,14 @@ void launch_links_f64_to_f32(const double* a, float* b, const Geom& g, int group
 // site gauges set to the identity (NR reals per site, both parities)
 void launch_gauge_identity(float* G, const Geom& g, int group, hipStream_t st);
 
-// Rebuild links from the FP64 master and the accumulated site gauge:
-// out = G(x) U0 G(x+mu)^+ with G normalised in FP64 (SU(2): q/|q|, SU(3):
-// Gram-Schmidt).  out32 (FP32 working links) or out64 (may alias U0).
-// Skipped when ctl->done (ctl = nullptr forces it: the final rebuild).
-// part != nullptr: the F partials of the rebuilt links (rebuild_blocks_of(g)
-// triples) for a RED_REBUILD reduction
-void launch_su2_rebuild(const double* U0, const float* G, float* out32,
-                        double* out64, const Geom& g, const JobCtl* ctl, double* part,
-                        hipStream_t st);
-void launch_su3_rebuild(const double* U0, const float* G, float* out32,
-                        double* out64, const Geom& g, const JobCtl* ctl, double* part,
-                        hipStream_t st);
-// G <- float(normalise_fp64(G)); skipped when ctl->done
-void launch_gauge_normalize(float* G, const Geom& g, int group, const JobCtl* ctl,
-                            hipStream_t st);
-// workgroups of one rebuild / normalisation over both parities
-inline int rebuild_blocks_of(const Geom& g) { return 2 * (int)((g.Vh + 255) / 256); }
+// The FP64 links at the end of the FP32 phase: out64 (may alias U0) =
+// G(x) U0 G(x+mu)^+, one thread per black site rebuilding its 8 links,
+// with the red gauges G_r normalised in FP64 and the black gauges recovered
+// from the working links W (mag_gauge.h).
+void launch_su2_rebuild_final(const double* U0, const float* Gr, const float* W, double* out64,
+                              const Geom& g, hipStream_t st);
+void launch_su3_rebuild_final(const double* U0, const float* Gr, const float* W, double* out64,
+                              const Geom& g, hipStream_t st);
 // FP64 reunitarisation of links in place (SU(2) q/|q|, SU(3) Gram-Schmidt)
 void launch_su2_reunit64(double* U, const Geom& g, hipStream_t st);
 void launch_su3_reunit64(double* U, const Geom& g, hipStream_t st);

@@ -37,6 +37,7 @@ struct mag_job {
   int phase = 0;  // mixed: -1 = FP64 warm-up, 0 = FP32 sweeps, 1 = FP64 after rebuild
   int fp64_start = -1;
   int since_rebuild = 0;  // sweeps enqueued since the last FP64 rebuild
+  int pending = 0;        // mixed FP32 phase: swept slots not yet reduced
   double invF = 0, invV = 0;
   mag_batch* batch = nullptr;  // batch the job belongs to, if any
   // site form (gauge copies): U64 stays read-only; copy c has its site

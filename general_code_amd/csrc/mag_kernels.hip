@@ -4,6 +4,7 @@
 // gauge (mixed precision), reunitarisation, plaquette, and the streaming-copy
 // bandwidth reference.  The sweep kernels live in mag_sweep.hip.
 #include "mag_internal.h"
+#include "mag_gauge.h"
 #include "mag_layout.h"
 #include "mag_math.h"
 #include "mag_reduce.h"
@@ -202,257 +203,74 @@ void launch_gauge_identity(float* G, const Geom& g, int group, hipStream_t st) {
 }
 
 // ================================================================== rebuild
-template <typename P>
-__device__ __forceinline__ Q<double> norm_q(LRef<P> r) {
-  const Q<float> f = ldq(r);
-  Q<double> q{(double)f.a0, (double)f.a1, (double)f.a2, (double)f.a3};
-  double n = 1.0 / sqrt(q.a0 * q.a0 + q.a1 * q.a1 + q.a2 * q.a2 + q.a3 * q.a3);
-  q.a0 *= n; q.a1 *= n; q.a2 *= n; q.a3 *= n;
-  return q;
-}
-
-// full 3x3 from rows 0,1 after FP64 Gram-Schmidt
-__device__ __forceinline__ M3<double> gs3(const M2<double>& in) {
-  M2<double> m = in;
-  double n0 = 0.0;
+// End of the FP32 phase: every link in FP64 from the master, one thread per
+// black site x rebuilding its 4 forward and 4 backward links (each link has
+// exactly one black end, so out64 may alias U0).  Red gauges G_r(y) are
+// normalised in FP64; the black gauge is recovered from W(x,0) (mag_gauge.h).
+__global__ __launch_bounds__(256) void k_su2_rebuild_final(const double* U0,
+                                                           const float* __restrict__ Gr,
+                                                           const float* __restrict__ W,
+                                                           double* out64, Geom g) {
+  const uint32_t h = blockIdx.x * 256 + threadIdx.x;
+  if (h >= g.Vh) return;
+  const SiteIdx s = decode(g, h, 1);
+  uint32_t hf[4], hb[4];
 #pragma unroll
-  for (int k = 0; k < 3; k++) n0 += cnorm2(m.r[0][k]);
-  n0 = 1.0 / sqrt(n0);
+  for (int mu = 0; mu < 4; mu++) {
+    hf[mu] = fwd_h(g, h, s, mu);
+    hb[mu] = back_h(g, h, s, mu);
+  }
+  const Q<double> gx = black_gauge(to_d(ldq(link_ptr<4>(W, g, 1, 0, h))),
+                                   norm_q(site_ptr<4>(Gr, g, 0, hf[0])),
+                                   ldq(link_ptr<4>(U0, g, 1, 0, h)));
+  Q<double> wf[4], wb[4];
 #pragma unroll
-  for (int k = 0; k < 3; k++) m.r[0][k] = {m.r[0][k].re * n0, m.r[0][k].im * n0};
-  C<double> d{0.0, 0.0};
+  for (int mu = 0; mu < 4; mu++) {
+    wf[mu] = qmul_dag(qmul(gx, ldq(link_ptr<4>(U0, g, 1, mu, h))),
+                      norm_q(site_ptr<4>(Gr, g, 0, hf[mu])));
+    wb[mu] = qmul_dag(qmul(norm_q(site_ptr<4>(Gr, g, 0, hb[mu])),
+                           ldq(link_ptr<4>(U0, g, 0, mu, hb[mu]))), gx);
+  }
 #pragma unroll
-  for (int k = 0; k < 3; k++) d = cadd(d, cmulc(m.r[1][k], m.r[0][k]));
-#pragma unroll
-  for (int k = 0; k < 3; k++) m.r[1][k] = csub(m.r[1][k], cmul(d, m.r[0][k]));
-  double n1 = 0.0;
-#pragma unroll
-  for (int k = 0; k < 3; k++) n1 += cnorm2(m.r[1][k]);
-  n1 = 1.0 / sqrt(n1);
-#pragma unroll
-  for (int k = 0; k < 3; k++) m.r[1][k] = {m.r[1][k].re * n1, m.r[1][k].im * n1};
-  M3<double> M;
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int k = 0; k < 3; k++) M.m[i][k] = m.r[i][k];
-  row2_of(m, M.m[2]);
-  return M;
-}
-__device__ __forceinline__ M2<double> to_d(const M2<double>& m) { return m; }
-__device__ __forceinline__ M2<double> to_d(const M2<float>& m) {
-  M2<double> r;
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int k = 0; k < 3; k++) r.r[i][k] = {(double)m.r[i][k].re, (double)m.r[i][k].im};
-  return r;
-}
-
-// the functional partial of a rebuild workgroup (F of the links it wrote,
-// in FP64) to part[3*lb]; part = nullptr: none
-__device__ __forceinline__ void rebuild_partial(double* part, double fs, uint32_t lb) {
-  if (!part) return;  // uniform
-  double z1 = 0.0, z2 = 0.0;
-  block_reduce3<4>(fs, z1, z2);
-  if (threadIdx.x == 0) {
-    part[3 * lb] = fs;
-    part[3 * lb + 1] = 0.0;
-    part[3 * lb + 2] = 0.0;
+  for (int mu = 0; mu < 4; mu++) {
+    stq(link_ptr<4>(out64, g, 1, mu, h), wf[mu]);
+    stq(link_ptr<4>(out64, g, 0, mu, hb[mu]), wb[mu]);
   }
 }
 
-// workgroup lb in [0, 2*ceil(Vh/256)) of a rebuild over both parities
-__device__ __forceinline__ void su2_rebuild_block(const double* __restrict__ U0,
-                                                  const float* __restrict__ G,
-                                                  float* __restrict__ out32, double* out64,
-                                                  const Geom& g, uint32_t lb, double* part) {
-  const uint32_t nb = (g.Vh + 255) / 256;
-  const int p = (int)(lb / nb);
-  const uint32_t h = (lb % nb) * 256 + threadIdx.x;
-  double fs = 0.0;
-  if (h < g.Vh) {
-    SiteIdx s = decode(g, h, p);
-    Q<double> gx = norm_q(site_ptr<4>(G, g, p, h));
-#pragma unroll
-    for (int mu = 0; mu < 4; mu++) {
-      uint32_t hf = fwd_h(g, h, s, mu);
-      Q<double> gf = norm_q(site_ptr<4>(G, g, 1 - p, hf));
-      Q<double> u = ldq(link_ptr<4>(U0, g, p, mu, h));
-      Q<double> w = qmul_dag(qmul(gx, u), gf);
-      if (part) fs += link_F(w);
-      if (out32) stq(link_ptr<4>(out32, g, p, mu, h), Q<float>{(float)w.a0, (float)w.a1, (float)w.a2, (float)w.a3});
-      else stq(link_ptr<4>(out64, g, p, mu, h), w);
-    }
-  }
-  rebuild_partial(part, fs, lb);
-}
-
-__device__ __forceinline__ M3<double> full3(const M2<double>& m) {
-  M3<double> M;
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int k = 0; k < 3; k++) M.m[i][k] = m.r[i][k];
-  row2_of(m, M.m[2]);
-  return M;
-}
-
-template <typename TG>
-__device__ __forceinline__ void su3_rebuild_block(const double* __restrict__ U0,
-                                                  const TG* __restrict__ G,
-                                                  float* __restrict__ out32, double* out64,
-                                                  const Geom& g, uint32_t lb, double* part) {
-  const uint32_t nb = (g.Vh + 255) / 256;
-  const int p = (int)(lb / nb);
-  const uint32_t h = (lb % nb) * 256 + threadIdx.x;
-  double fs = 0.0;
-  if (h < g.Vh) {
-    SiteIdx s = decode(g, h, p);
-    M3<double> gx = gs3(to_d(ldm(site_ptr<12>(G, g, p, h))));
+__global__ __launch_bounds__(256) void k_su3_rebuild_final(const double* U0,
+                                                           const float* __restrict__ Gr,
+                                                           const float* __restrict__ W,
+                                                           double* out64, Geom g) {
+  const uint32_t h = blockIdx.x * 256 + threadIdx.x;
+  if (h >= g.Vh) return;
+  const SiteIdx s = decode(g, h, 1);
+  const M3<double> gx = gs3(black_gauge_rows(
+      to_d(ldm(link_ptr<12>(W, g, 1, 0, h))), gs3(to_d(ldm(site_ptr<12>(Gr, g, 0, fwd_h(g, h, s, 0))))),
+      ldm(link_ptr<12>(U0, g, 1, 0, h))));
+  // every read of U0 below precedes the write of the same link (this
+  // thread owns the 8 links), so the rebuild may run in place
 #pragma unroll 1
-    for (int mu = 0; mu < 4; mu++) {
-      uint32_t hf = fwd_h(g, h, s, mu);
-      M3<double> gf = gs3(to_d(ldm(site_ptr<12>(G, g, 1 - p, hf))));
-      M3<double> u = full3(ldm(link_ptr<12>(U0, g, p, mu, h)));
-      // V = u gf^+
-      M3<double> v;
-#pragma unroll
-      for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-          C<double> acc{0.0, 0.0};
-#pragma unroll
-          for (int k = 0; k < 3; k++) acc = cadd(acc, cmulc(u.m[i][k], gf.m[j][k]));
-          v.m[i][j] = acc;
-        }
-      M2<double> w;
-#pragma unroll
-      for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-          C<double> acc{0.0, 0.0};
-#pragma unroll
-          for (int k = 0; k < 3; k++) acc = cadd(acc, cmul(gx.m[i][k], v.m[k][j]));
-          w.r[i][j] = acc;
-        }
-      if (part) fs += su3_link_F(w);
-      if (out32) {
-        M2<float> wf;
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-          for (int j = 0; j < 3; j++) wf.r[i][j] = {(float)w.r[i][j].re, (float)w.r[i][j].im};
-        stm(link_ptr<12>(out32, g, p, mu, h), wf);
-      } else {
-        stm(link_ptr<12>(out64, g, p, mu, h), w);
-      }
-    }
-  }
-  rebuild_partial(part, fs, lb);
-}
-
-// ctl: skip when the job is done (the periodic rebuild of a finished job
-// must not move its links); nullptr forces the rebuild (final rebuild)
-__global__ __launch_bounds__(256) void k_su2_rebuild(const double* __restrict__ U0,
-                                                     const float* __restrict__ G,
-                                                     float* __restrict__ out32, double* out64,
-                                                     Geom g, const JobCtl* __restrict__ ctl,
-                                                     double* part) {
-  if (ctl && ctl->done) return;
-  su2_rebuild_block(U0, G, out32, out64, g, blockIdx.x, part);
-}
-__global__ __launch_bounds__(256) void k_su3_rebuild(const double* __restrict__ U0,
-                                                     const float* __restrict__ G,
-                                                     float* __restrict__ out32, double* out64,
-                                                     Geom g, const JobCtl* __restrict__ ctl,
-                                                     double* part) {
-  if (ctl && ctl->done) return;
-  su3_rebuild_block(U0, G, out32, out64, g, blockIdx.x, part);
-}
-
-static inline int rebuild_blocks(const Geom& g) { return 2 * (int)((g.Vh + 255) / 256); }
-
-void launch_su2_rebuild(const double* U0, const float* G, float* out32, double* out64,
-                        const Geom& g, const JobCtl* ctl, double* part, hipStream_t st) {
-  hipLaunchKernelGGL(k_su2_rebuild, dim3(rebuild_blocks(g)), dim3(256), 0, st, U0, G, out32,
-                     out64, g, ctl, part);
-}
-void launch_su3_rebuild(const double* U0, const float* G, float* out32, double* out64,
-                        const Geom& g, const JobCtl* ctl, double* part, hipStream_t st) {
-  hipLaunchKernelGGL(k_su3_rebuild, dim3(rebuild_blocks(g)), dim3(256), 0, st, U0, G, out32,
-                     out64, g, ctl, part);
-}
-
-__device__ __forceinline__ void gauge_normalize_site(float* __restrict__ G, const Geom& g,
-                                                     int group, uint32_t s) {
-  if (s >= 2 * g.Vh) return;
-  const int par = (int)(s / g.Vh);
-  const uint32_t h = s % g.Vh;
-  if (group == 2) {
-    const LRef<float> p = site_ptr<4>(G, g, par, h);
-    Q<double> q = norm_q(p);
-    stq(p, Q<float>{(float)q.a0, (float)q.a1, (float)q.a2, (float)q.a3});
-  } else {
-    const LRef<float> p = site_ptr<12>(G, g, par, h);
-    M3<double> m = gs3(to_d(ldm(p)));
-    M2<float> o;
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int k = 0; k < 3; k++) o.r[i][k] = {(float)m.m[i][k].re, (float)m.m[i][k].im};
-    stm(p, o);
+  for (int mu = 0; mu < 4; mu++) {
+    const uint32_t hf = fwd_h(g, h, s, mu), hb = back_h(g, h, s, mu);
+    const M2<double> wf = rebuild_link(gx, ldm(link_ptr<12>(U0, g, 1, mu, h)),
+                                       gs3(to_d(ldm(site_ptr<12>(Gr, g, 0, hf)))));
+    const M2<double> wb = rebuild_link(gs3(to_d(ldm(site_ptr<12>(Gr, g, 0, hb)))),
+                                       ldm(link_ptr<12>(U0, g, 0, mu, hb)), gx);
+    stm(link_ptr<12>(out64, g, 1, mu, h), wf);
+    stm(link_ptr<12>(out64, g, 0, mu, hb), wb);
   }
 }
-__global__ void k_gauge_normalize(float* __restrict__ G, Geom g, int group,
-                                  const JobCtl* __restrict__ ctl) {
-  if (ctl && ctl->done) return;
-  gauge_normalize_site(G, g, group, blockIdx.x * blockDim.x + threadIdx.x);
-}
-void launch_gauge_normalize(float* G, const Geom& g, int group, const JobCtl* ctl,
-                            hipStream_t st) {
-  hipLaunchKernelGGL(k_gauge_normalize, dim3(rebuild_blocks(g)), dim3(256), 0, st, G, g, group,
-                     ctl);
-}
 
-// batched rebuild + normalisation: workgroup b -> job via rb0 (ascending)
-__device__ __forceinline__ int batch_rjob(const BatchJob* __restrict__ jobs, int n, int b) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].rb0 <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
+void launch_su2_rebuild_final(const double* U0, const float* Gr, const float* W, double* out64,
+                              const Geom& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_su2_rebuild_final, dim3((g.Vh + 255) / 256), dim3(256), 0, st, U0, Gr, W,
+                     out64, g);
 }
-template <int GROUP>
-__global__ __launch_bounds__(256) void k_rebuild_batch(const BatchJob* __restrict__ jobs,
-                                                       int njobs) {
-  const BatchJob& j = jobs[batch_rjob(jobs, njobs, blockIdx.x)];
-  if (j.ctl->done) return;
-  if (GROUP == 2)
-    su2_rebuild_block(j.U0, (const float*)j.G, (float*)j.U, nullptr, j.g, blockIdx.x - j.rb0,
-                      j.part);
-  else
-    su3_rebuild_block(j.U0, (const float*)j.G, (float*)j.U, nullptr, j.g, blockIdx.x - j.rb0,
-                      j.part);
-}
-template <int GROUP>
-__global__ void k_normalize_batch(const BatchJob* __restrict__ jobs, int njobs) {
-  const BatchJob& j = jobs[batch_rjob(jobs, njobs, blockIdx.x)];
-  if (j.ctl->done) return;
-  gauge_normalize_site((float*)j.G, j.g, GROUP, (blockIdx.x - j.rb0) * blockDim.x + threadIdx.x);
-}
-void launch_rebuild_batch(int group, const BatchJob* jobs, int njobs, int nblocks_total,
-                          hipStream_t st) {
-  if (njobs == 0 || nblocks_total == 0) return;
-  if (group == 2) {
-    hipLaunchKernelGGL(k_rebuild_batch<2>, dim3(nblocks_total), dim3(256), 0, st, jobs, njobs);
-    hipLaunchKernelGGL(k_normalize_batch<2>, dim3(nblocks_total), dim3(256), 0, st, jobs, njobs);
-  } else {
-    hipLaunchKernelGGL(k_rebuild_batch<3>, dim3(nblocks_total), dim3(256), 0, st, jobs, njobs);
-    hipLaunchKernelGGL(k_normalize_batch<3>, dim3(nblocks_total), dim3(256), 0, st, jobs, njobs);
-  }
+void launch_su3_rebuild_final(const double* U0, const float* Gr, const float* W, double* out64,
+                              const Geom& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_su3_rebuild_final, dim3((g.Vh + 255) / 256), dim3(256), 0, st, U0, Gr, W,
+                     out64, g);
 }
 
 __global__ void k_su2_reunit64(double* __restrict__ U, Geom g) {
@@ -584,10 +402,43 @@ void launch_copy(const void* a, void* b, size_t bytes, hipStream_t st) {
 }
 // ================================================================== SU(3) site form
 // out = G(x) U0 G(x+mu)^+ (FP64) from FP64 site gauges: the links of a copy
+// (one thread per site of either parity, its 4 forward links)
 __global__ __launch_bounds__(256) void k_su3_site_links(const double* __restrict__ U0,
                                                         const double* __restrict__ G,
                                                         double* __restrict__ out, Geom g) {
-  su3_rebuild_block<double>(U0, G, nullptr, out, g, blockIdx.x, nullptr);
+  const uint32_t nb = (g.Vh + 255) / 256;
+  const int p = (int)(blockIdx.x / nb);
+  const uint32_t h = (blockIdx.x % nb) * 256 + threadIdx.x;
+  if (h >= g.Vh) return;
+  const SiteIdx s = decode(g, h, p);
+  const M3<double> gx = gs3(ldm(site_ptr<12>(G, g, p, h)));
+#pragma unroll 1
+  for (int mu = 0; mu < 4; mu++) {
+    const M3<double> gf = gs3(ldm(site_ptr<12>(G, g, 1 - p, fwd_h(g, h, s, mu))));
+    const M3<double> u = full3(ldm(link_ptr<12>(U0, g, p, mu, h)));
+    // V = u gf^+, then rows 0,1 of gx V
+    M3<double> v;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        C<double> acc{0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < 3; k++) acc = cadd(acc, cmulc(u.m[i][k], gf.m[j][k]));
+        v.m[i][j] = acc;
+      }
+    M2<double> w;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        C<double> acc{0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < 3; k++) acc = cadd(acc, cmul(gx.m[i][k], v.m[k][j]));
+        w.r[i][j] = acc;
+      }
+    stm(link_ptr<12>(out, g, p, mu, h), w);
+  }
 }
 void launch_su3_site_links(const double* U0, const double* G, double* out, const Geom& g,
                            hipStream_t st) {

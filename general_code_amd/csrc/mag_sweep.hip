@@ -17,10 +17,12 @@ namespace mag {
 // threads.  Thread t combines partials t, t + kRedBlock, ... in order, then
 // the fixed-order workgroup tree, so the result depends only on the job's
 // partials (alone or in a batch).
-__device__ void reduce_job(const double* __restrict__ part, int nparts, int mode, double invF,
-                           double invV, JobCtl* ctl, double* hF, double* hR, double* hA,
-                           double* out3) {
-  double a = 0.0, b = 0.0, c = 0.0;
+// (sum a, sum b, max c) of nparts triples in a fixed order; valid in thread 0
+__device__ void reduce_triples(const double* __restrict__ part, int nparts, double& a, double& b,
+                               double& c) {
+  a = 0.0;
+  b = 0.0;
+  c = 0.0;
   int i = threadIdx.x;
   // kRedUnroll triples in flight per thread, combined in index order
   constexpr int kRedUnroll = 8;
@@ -43,6 +45,26 @@ __device__ void reduce_job(const double* __restrict__ part, int nparts, int mode
     c = fmax(c, part[3 * i + 2]);
   }
   block_reduce3<kRedBlock / 64>(a, b, c);
+}
+
+// F of freshly rebuilt FP64 links (mixed FP32 phase): converged when the
+// mean change per sweep since the previous rebuild is below tol (thread 0)
+__device__ __forceinline__ void apply_rebuild(double F, JobCtl* ctl) {
+  if (ctl->F_ref != 0.0) {
+    const double rel = fabs(F - ctl->F_ref) / (ctl->rb_every * fabs(F));
+    if (rel < ctl->tol) {
+      ctl->converged = 1;
+      ctl->done = 1;
+    }
+  }
+  ctl->F_ref = F;
+}
+
+__device__ void reduce_job(const double* __restrict__ part, int nparts, int mode, double invF,
+                           double invV, JobCtl* ctl, double* hF, double* hR, double* hA,
+                           double* out3) {
+  double a, b, c;
+  reduce_triples(part, nparts, a, b, c);
   if (threadIdx.x == 0) {
     const double F = a * invF;
     if (mode == RED_INIT) {
@@ -66,16 +88,6 @@ __device__ void reduce_job(const double* __restrict__ part, int nparts, int mode
         ctl->done = 1;
       }
       if (sw + 1 >= ctl->max_sweeps) ctl->done = 1;
-    } else if (mode == RED_REBUILD) {
-      // FP64 links rebuilt from the master: no FP32 rounding in this F
-      if (ctl->F_ref != 0.0) {
-        const double rel = fabs(F - ctl->F_ref) / (ctl->rb_every * fabs(F));
-        if (rel < ctl->tol) {
-          ctl->converged = 1;
-          ctl->done = 1;
-        }
-      }
-      ctl->F_ref = F;
     }
     if (out3) {
       out3[0] = F;
@@ -89,7 +101,7 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce(const double* __restrict__
                                                       int nparts, int mode, double invF,
                                                       double invV, JobCtl* ctl, double* hF,
                                                       double* hR, double* hA, double* out3) {
-  if ((mode == RED_SWEEP || mode == RED_REBUILD) && ctl->done) return;
+  if (mode == RED_SWEEP && ctl->done) return;
   reduce_job(part, nparts, mode, invF, invV, ctl, hF, hR, hA, out3);
 }
 
@@ -98,7 +110,7 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_batch(const BatchJob* __re
                                                             int mode) {
   const BatchJob& j = jobs[blockIdx.x];
   if (j.ctl->done) return;
-  reduce_job(j.part, mode == RED_REBUILD ? j.nrparts : j.nparts, mode, j.invF, j.invV, j.ctl,
+  reduce_job(j.part, j.nparts, mode, j.invF, j.invV, j.ctl,
              j.hF, j.hR, j.hA, nullptr);
 }
 
@@ -111,6 +123,91 @@ void launch_reduce(const double* part, int nparts, int mode, double invF, double
 void launch_reduce_batch(const BatchJob* jobs, int njobs, int mode, hipStream_t st) {
   if (njobs == 0) return;
   hipLaunchKernelGGL(k_reduce_batch, dim3(njobs), dim3(kRedBlock), 0, st, jobs, mode);
+}
+
+// ------------------------------------------------------ FP32-phase slots
+// Block k: the history entry of slot k (sweep ctl->sweep + k, below the cap),
+// reduced exactly as RED_SWEEP reduces one sweep.  Nothing in the control
+// block changes here, so every block sees the same ctl.
+__device__ __forceinline__ void reduce_slot(const double* __restrict__ part, size_t pstride,
+                                            int nparts, double invF, double invV,
+                                            const JobCtl* ctl, double* hF, double* hR,
+                                            double* hA, int k) {
+  if (ctl->done) return;
+  const int sw = ctl->sweep + k;
+  if (sw >= ctl->max_sweeps) return;
+  double a, b, c;
+  reduce_triples(part + (size_t)k * pstride, nparts, a, b, c);
+  if (threadIdx.x == 0) {
+    hF[sw] = a * invF;
+    hR[sw] = c;
+    hA[sw] = b * invV;
+  }
+}
+__global__ __launch_bounds__(kRedBlock) void k_reduce_slots(const double* __restrict__ part,
+                                                            size_t pstride, int nparts,
+                                                            double invF, double invV,
+                                                            const JobCtl* ctl, double* hF,
+                                                            double* hR, double* hA) {
+  reduce_slot(part, pstride, nparts, invF, invV, ctl, hF, hR, hA, blockIdx.x);
+}
+// After the slots: the sweep count advances by n (up to the cap), F_prev is
+// the last recorded F, and a period that ended with a rebuild (its
+// rebuilding sweep ran) tests convergence on the rebuilt links' F.
+__device__ __forceinline__ void reduce_period(const double* __restrict__ part_rb, int nrb, int n,
+                                              int rebuild, double invF, JobCtl* ctl,
+                                              const double* hF) {
+  if (ctl->done) return;
+  const int s0 = ctl->sweep, mx = ctl->max_sweeps;
+  const bool rb = rebuild && s0 + n <= mx;
+  double a = 0.0, b = 0.0, c = 0.0;
+  if (rb) reduce_triples(part_rb, nrb, a, b, c);
+  if (threadIdx.x == 0) {
+    const int sw = s0 + n < mx ? s0 + n : mx;
+    if (sw > s0) ctl->F_prev = hF[sw - 1];
+    ctl->sweep = sw;
+    if (sw >= mx) ctl->done = 1;
+    if (rb) apply_rebuild(a * invF, ctl);
+  }
+}
+__global__ __launch_bounds__(kRedBlock) void k_reduce_period(const double* __restrict__ part_rb,
+                                                             int nrb, int n, int rebuild,
+                                                             double invF, JobCtl* ctl,
+                                                             const double* hF) {
+  reduce_period(part_rb, nrb, n, rebuild, invF, ctl, hF);
+}
+void launch_reduce_slots(const double* part, size_t pstride, int nparts, int n, double invF,
+                         double invV, JobCtl* ctl, double* hF, double* hR, double* hA,
+                         hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_reduce_slots, dim3(n), dim3(kRedBlock), 0, st, part, pstride, nparts,
+                     invF, invV, ctl, hF, hR, hA);
+}
+void launch_reduce_period(const double* part_rb, int nrb, int n, int rebuild, double invF,
+                          JobCtl* ctl, double* hF, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_period, dim3(1), dim3(kRedBlock), 0, st, part_rb, nrb, n, rebuild,
+                     invF, ctl, hF);
+}
+// batched: slots of job b / n, slot b % n; the period: one block per job
+__global__ __launch_bounds__(kRedBlock) void k_reduce_slots_batch(const BatchJob* __restrict__ jobs,
+                                                                  int n) {
+  const BatchJob& j = jobs[blockIdx.x / n];
+  reduce_slot(j.part, j.pstride, j.nparts, j.invF, j.invV, j.ctl, j.hF, j.hR, j.hA,
+              (int)(blockIdx.x % n));
+}
+__global__ __launch_bounds__(kRedBlock) void k_reduce_period_batch(
+    const BatchJob* __restrict__ jobs, int n, int rebuild) {
+  const BatchJob& j = jobs[blockIdx.x];
+  reduce_period(j.part + (size_t)kSlots * j.pstride, j.nw, n, rebuild, j.invF, j.ctl, j.hF);
+}
+void launch_reduce_slots_batch(const BatchJob* jobs, int njobs, int n, hipStream_t st) {
+  if (njobs == 0 || n <= 0) return;
+  hipLaunchKernelGGL(k_reduce_slots_batch, dim3(njobs * n), dim3(kRedBlock), 0, st, jobs, n);
+}
+void launch_reduce_period_batch(const BatchJob* jobs, int njobs, int n, int rebuild,
+                                hipStream_t st) {
+  if (njobs == 0) return;
+  hipLaunchKernelGGL(k_reduce_period_batch, dim3(njobs), dim3(kRedBlock), 0, st, jobs, n, rebuild);
 }
 
 // control block of every entry of a table into out[i] (pinned host memory):
@@ -131,17 +228,19 @@ void launch_gather_ctl(const BatchJob* jobs, int n, JobCtl* out, hipStream_t st)
 
 // ================================================================== link form, FP32
 template void launch_su2_half<float>(float*, float*, const Geom&, int, double, double*,
-                                     const JobCtl*, Shape, hipStream_t);
+                                     const JobCtl*, Shape, hipStream_t,
+                                     const HalfOpt&);
 template void launch_su3_half<float>(float*, float*, const Geom&, int, double, double*,
-                                     const JobCtl*, Shape, hipStream_t);
+                                     const JobCtl*, Shape, hipStream_t,
+                                     const HalfOpt&);
 
 // FP64 tables: mag_sweep64.hip
 void launch_half_batch_f64(int group, Shape sh, const BatchJob* jobs, int njobs,
                            int nblocks_total, int parity, hipStream_t st);
 
 void launch_half_batch(int group, bool fp32, Shape sh, const BatchJob* jobs, int njobs,
-                       int nblocks_total, int parity, hipStream_t st) {
-  if (fp32) launch_half_batch_t<float>(group, sh, jobs, njobs, nblocks_total, parity, st);
+                       int nblocks_total, int parity, hipStream_t st, int slot, int mode) {
+  if (fp32) launch_half_batch_t<float>(group, sh, jobs, njobs, nblocks_total, parity, st, slot, mode);
   else launch_half_batch_f64(group, sh, jobs, njobs, nblocks_total, parity, st);
 }
 

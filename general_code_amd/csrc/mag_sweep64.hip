@@ -9,13 +9,15 @@
 namespace mag {
 
 template void launch_su2_half<double>(double*, double*, const Geom&, int, double, double*,
-                                      const JobCtl*, Shape, hipStream_t);
+                                      const JobCtl*, Shape, hipStream_t,
+                                     const HalfOpt&);
 template void launch_su3_half<double>(double*, double*, const Geom&, int, double, double*,
-                                      const JobCtl*, Shape, hipStream_t);
+                                      const JobCtl*, Shape, hipStream_t,
+                                     const HalfOpt&);
 
 void launch_half_batch_f64(int group, Shape sh, const BatchJob* jobs, int njobs,
                            int nblocks_total, int parity, hipStream_t st) {
-  launch_half_batch_t<double>(group, sh, jobs, njobs, nblocks_total, parity, st);
+  launch_half_batch_t<double>(group, sh, jobs, njobs, nblocks_total, parity, st, 0, HALF_PLAIN);
 }
 
 }  // namespace mag

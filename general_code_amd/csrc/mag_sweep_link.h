@@ -26,6 +26,7 @@
 // Field summation order (matched by the oracle):
 //   h = [f(0) + b(0) + f(1) + b(1)] + [f(2) + b(2) + f(3) + b(3)]
 #pragma once
+#include "mag_gauge.h"
 #include "mag_internal.h"
 #include "mag_layout.h"
 #include "mag_math.h"
@@ -92,17 +93,23 @@ __device__ __forceinline__ void wave_partial(double* __restrict__ part, uint32_t
 // Tile k of workgroup lb of a half-sweep over the sites of parity p, one
 // site per lane; the 8 links of an su2 site fit in registers (3 waves/SIMD
 // in FP64, enough to stream at the copy rate), and one site per lane
-// measured faster than a lane pair.  The black
-// (p = 1) half also sums the functional of the links just written.
-template <typename T, bool GA, int NT, int SPT>
+// measured faster than a lane pair.  The black (p = 1) half also sums the
+// functional of the links just written.  GA: the red half-sweep of the
+// mixed FP32 phase accumulates the site gauge (MODE HALF_RENORM: after
+// normalising it in FP64); MODE HALF_REBUILD (black half closing an FP32
+// period): the swept links are replaced by links rebuilt from the master,
+// and the rebuilt links' F goes to part_rb.
+template <typename T, bool GA, int NT, int SPT, int MODE>
 __device__ __forceinline__ void su2_half_block(T* __restrict__ U, T* __restrict__ G,
                                                const Geom& g, int p, T omega,
-                                               double* __restrict__ part, uint32_t lb) {
+                                               double* __restrict__ part, uint32_t lb,
+                                               const double* __restrict__ U0,
+                                               double* __restrict__ part_rb) {
 #pragma unroll
   for (int k = 0; k < SPT; k++) {
   const uint32_t base = (lb * SPT + k) * NT;
   const uint32_t h = base + threadIdx.x;
-  double fs = 0.0, rs = 0.0, rm = 0.0;
+  double fs = 0.0, rs = 0.0, rm = 0.0, fr = 0.0;
   if (h < g.Vh) {
     const SiteIdx s = decode(g, h, p);
     uint32_t hb[4];
@@ -127,42 +134,71 @@ __device__ __forceinline__ void su2_half_block(T* __restrict__ U, T* __restrict_
     for (int mu = 0; mu < 4; mu++) {
       F[mu] = gmul(gq, F[mu]);
       B[mu] = mul_gdag(B[mu], gq);
-      stq(link_ptr<4>(U, g, p, mu, h), F[mu]);
-      stq(link_ptr<4>(U, g, 1 - p, mu, hb[mu]), B[mu]);
-    }
-    if (GA) {
-      const LRef<T> gp = site_ptr<4>(G, g, p, h);
-      stq(gp, gmul(gq, ldq(gp)));
     }
     if (p == 1) {
 #pragma unroll
       for (int mu = 0; mu < 4; mu++) fs += link_F(F[mu]) + link_F(B[mu]);
     }
+    if constexpr (MODE == HALF_REBUILD) {
+      // the black gauge from the swept W(x,0), then all 8 links in FP64
+      const Q<double> gx = black_gauge(to_d(F[0]), norm_q(site_ptr<4>(G, g, 0, fwd_h(g, h, s, 0))),
+                                       ldq(link_ptr<4>(U0, g, 1, 0, h)));
+#pragma unroll
+      for (int mu = 0; mu < 4; mu++) {
+        const Q<double> wf = qmul_dag(qmul(gx, ldq(link_ptr<4>(U0, g, 1, mu, h))),
+                                      norm_q(site_ptr<4>(G, g, 0, fwd_h(g, h, s, mu))));
+        const Q<double> wb = qmul_dag(qmul(norm_q(site_ptr<4>(G, g, 0, hb[mu])),
+                                           ldq(link_ptr<4>(U0, g, 0, mu, hb[mu]))), gx);
+        fr += link_F(wf) + link_F(wb);
+        F[mu] = to_f(wf);
+        B[mu] = to_f(wb);
+      }
+    }
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) {
+      stq(link_ptr<4>(U, g, p, mu, h), F[mu]);
+      stq(link_ptr<4>(U, g, 1 - p, mu, hb[mu]), B[mu]);
+    }
+    if constexpr (GA) {
+      const LRef<T> gp = site_ptr<4>(G, g, p, h);
+      Q<T> gc = ldq(gp);
+      if constexpr (MODE == HALF_RENORM) gc = to_f(norm_q(to_d(gc)));
+      stq(gp, gmul(gq, gc));
+    }
     rs = (double)res;
     rm = (double)res;
   }
   wave_partial(part, base + (threadIdx.x & ~63u), g.Vh, fs, rs, rm);
+  if constexpr (MODE == HALF_REBUILD) wave_partial(part_rb, base + (threadIdx.x & ~63u), g.Vh, fr, 0.0, 0.0);
   }
 }
 
-template <typename T, bool GA, int NT, int SPT>
+// the sweep of a slot is beyond the cap (or the job is done): return
+__device__ __forceinline__ bool half_skip(const JobCtl* __restrict__ ctl, int slot) {
+  return ctl->done || ctl->sweep + slot >= ctl->max_sweeps;
+}
+
+template <typename T, bool GA, int NT, int SPT, int MODE>
 __global__ __launch_bounds__(NT) void k_su2_half(T* __restrict__ U, T* __restrict__ G, Geom g,
                                                  int p, T omega, double* __restrict__ part,
-                                                 const JobCtl* __restrict__ ctl) {
-  if (ctl->done) return;
-  su2_half_block<T, GA, NT, SPT>(U, G, g, p, omega, part, blockIdx.x);
+                                                 const JobCtl* __restrict__ ctl, int slot,
+                                                 const double* __restrict__ U0,
+                                                 double* __restrict__ part_rb) {
+  if (half_skip(ctl, slot)) return;
+  su2_half_block<T, GA, NT, SPT, MODE>(U, G, g, p, omega, part, blockIdx.x, U0, part_rb);
 }
 
 // batched: workgroup b runs block b - block0 of its job with the class's
 // launch shape (NT, SPT); the job's partials are per wave, so the shape
 // leaves its results unchanged
-template <typename T, bool GA, int NT, int SPT>
+template <typename T, bool GA, int NT, int SPT, int MODE>
 __global__ __launch_bounds__(NT) void k_su2_half_batch(const BatchJob* __restrict__ jobs,
-                                                       int njobs, int p) {
+                                                       int njobs, int p, int slot) {
   const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
-  if (j.ctl->done) return;
-  su2_half_block<T, GA, NT, SPT>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega,
-                                 j.part + (size_t)3 * j.nw * p, blockIdx.x - j.block0);
+  if (half_skip(j.ctl, slot)) return;
+  su2_half_block<T, GA, NT, SPT, MODE>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega,
+                                       j.part + (size_t)slot * j.pstride + (size_t)3 * j.nw * p,
+                                       blockIdx.x - j.block0, j.U0, j.part + (size_t)kSlots * j.pstride);
 }
 
 // launch of a shape (NT, SPT) of the supported set
@@ -178,20 +214,46 @@ __global__ __launch_bounds__(NT) void k_su2_half_batch(const BatchJob* __restric
     default: L_(256, 1); break;                   \
   }
 
+// kernel variant of a launch: 0 plain, 1 plain + site gauge (red FP32
+// half), 2 renormalise + site gauge, 3 rebuild (black half, G = red gauges)
+template <typename T>
+int half_kind(const void* G, int mode) {
+  if (sizeof(T) == 8 || !G) return 0;
+  return mode == HALF_REBUILD ? 3 : mode == HALF_RENORM ? 2 : 1;
+}
+
+// one launch of kernel template K<T, GA, NT, SPT, MODE> for variant kind
+#define MAG_HALF_VARIANTS(K, NT, SPT, grid, st, ...)                                          \
+  do {                                                                                         \
+    if constexpr (sizeof(T) == 4) {                                                            \
+      switch (kind) {                                                                          \
+        case 1: hipLaunchKernelGGL((K<T, true, NT, SPT, HALF_PLAIN>), grid, dim3(NT), 0, st,   \
+                                   __VA_ARGS__); break;                                        \
+        case 2: hipLaunchKernelGGL((K<T, true, NT, SPT, HALF_RENORM>), grid, dim3(NT), 0, st,  \
+                                   __VA_ARGS__); break;                                        \
+        case 3: hipLaunchKernelGGL((K<T, false, NT, SPT, HALF_REBUILD>), grid, dim3(NT), 0,    \
+                                   st, __VA_ARGS__); break;                                    \
+        default: hipLaunchKernelGGL((K<T, false, NT, SPT, HALF_PLAIN>), grid, dim3(NT), 0, st, \
+                                    __VA_ARGS__); break;                                       \
+      }                                                                                        \
+    } else {                                                                                   \
+      hipLaunchKernelGGL((K<T, false, NT, SPT, HALF_PLAIN>), grid, dim3(NT), 0, st,            \
+                         __VA_ARGS__);                                                         \
+    }                                                                                          \
+  } while (0)
+
 template <typename T>
 void launch_su2_half(T* U, T* G, const Geom& g, int parity, double omega, double* part,
-                     const JobCtl* ctl, Shape sh, hipStream_t st) {
-  double* pp = part + (size_t)3 * sweep_waves(g, 2) * parity;
+                     const JobCtl* ctl, Shape sh, hipStream_t st, const HalfOpt& o) {
+  const size_t nw = (size_t)sweep_waves(g, 2);
+  double* pp = part + (size_t)o.slot * 2 * 3 * nw + 3 * nw * parity;
   const uint32_t lanes = g.Vh;
+  const int kind = half_kind<T>(G, o.mode);
 #define L_(NT, SPT)                                                                        \
   do {                                                                                     \
     const dim3 grid((lanes + NT * SPT - 1) / (NT * SPT));                                  \
-    if (G)                                                                                 \
-      hipLaunchKernelGGL((k_su2_half<T, true, NT, SPT>), grid, dim3(NT), 0, st, U, G, g,   \
-                         parity, (T)omega, pp, ctl);                                       \
-    else                                                                                   \
-      hipLaunchKernelGGL((k_su2_half<T, false, NT, SPT>), grid, dim3(NT), 0, st, U, G, g,  \
-                         parity, (T)omega, pp, ctl);                                       \
+    MAG_HALF_VARIANTS(k_su2_half, NT, SPT, grid, st, U, G, g, parity, (T)omega, pp, ctl,   \
+                      o.slot, o.U0, o.part_rb);                                            \
   } while (0)
   MAG_SHAPE_SWITCH(sh, L_)
 #undef L_
@@ -316,14 +378,39 @@ __device__ __forceinline__ T cm_subgroup(M2<T>* F, M2<T>* B, M2<T>& Gx, T omega)
   return res;
 }
 
+// rows 0,1 of a lane's matrix replaced by those of the first lane of its
+// site's lane group (LS = 2 or 4 lanes, DPP within the quad)
+template <int LS>
+__device__ __forceinline__ void group_broadcast(M2<double>& m, int q) {
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      double re = m.r[i][c].re, im = m.r[i][c].im;
+      const double pre = pair_xchg(re), pim = pair_xchg(im);
+      if (q & 1) { re = pre; im = pim; }
+      if (LS == 4) {
+        const double qre = quad_xchg2(re), qim = quad_xchg2(im);
+        if (q & 2) { re = qre; im = qim; }
+      }
+      m.r[i][c] = C<double>{re, im};
+    }
+}
+
 // One workgroup lb of an SU(3) half-sweep, LS = kSu3Lanes lanes per site:
 // lane q of the group holds the forward and backward links of the NL = 4/LS
 // directions mu = NL*q .. NL*q + NL - 1, the group exchanges the partial
-// fields with DPP and every lane derives the same g.
-template <typename T, bool GA, int NT, int SPT>
+// fields with DPP and every lane derives the same g.  GA / MODE as for SU(2):
+// the red FP32 half accumulates the site gauge (renormalised first in
+// HALF_RENORM); HALF_REBUILD recovers the black gauge from the swept
+// W(x,0) on the group's first lane, broadcasts it, and each lane rebuilds
+// its links from the master.
+template <typename T, bool GA, int NT, int SPT, int MODE>
 __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict__ G,
                                                const Geom& g, int p, T omega,
-                                               double* __restrict__ part, uint32_t lb) {
+                                               double* __restrict__ part, uint32_t lb,
+                                               const double* __restrict__ U0,
+                                               double* __restrict__ part_rb) {
   constexpr int LS = kSu3Lanes, NL = 4 / kSu3Lanes;
 #pragma unroll
   for (int kk = 0; kk < SPT; kk++) {
@@ -331,7 +418,7 @@ __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict_
     const uint32_t tid = base + threadIdx.x;
     const uint32_t h = tid / LS;
     const int q = (int)(tid % LS);
-    double fs = 0.0, rs = 0.0, rm = 0.0;
+    double fs = 0.0, rs = 0.0, rm = 0.0, fr = 0.0;
     if (h < g.Vh) {
       const SiteIdx s = decode(g, h, p);
       uint32_t hb[4];
@@ -350,10 +437,37 @@ __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict_
       }
       M2<T> Gx;
       const LRef<T> gp = site_ptr<12>(G, g, p, h);
-      if (GA) Gx = ldm(gp);
+      if constexpr (GA) {
+        Gx = ldm(gp);
+        if constexpr (MODE == HALF_RENORM) Gx = to_f(gs3(to_d(Gx)));
+      }
       T res = cm_subgroup<T, 0, 1, GA, NL>(F, B, Gx, omega);
       res += cm_subgroup<T, 1, 2, GA, NL>(F, B, Gx, omega);
       res += cm_subgroup<T, 0, 2, GA, NL>(F, B, Gx, omega);
+      if constexpr (MODE == HALF_REBUILD) {
+        // F of the swept links for the history, then the links from the master
+#pragma unroll
+        for (int k = 0; k < NL; k++) fs += su3_link_F(F[k]) + su3_link_F(B[k]);
+        const int mu0 = NL * q;
+        const uint32_t hf0 = fwd_h(g, h, s, mu0);
+        M2<double> y = black_gauge_rows(to_d(F[0]), gs3(to_d(ldm(site_ptr<12>(G, g, 0, hf0)))),
+                                        ldm(link_ptr<12>(U0, g, 1, mu0, h)));
+        group_broadcast<LS>(y, q);  // W(x,0) is on the group's first lane
+        const M3<double> gx = gs3(y);
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+          const int mu = NL * q + k;
+          const uint32_t hbm = mu == 0 ? hb[0] : mu == 1 ? hb[1] : mu == 2 ? hb[2] : hb[3];
+          const M2<double> wf =
+              rebuild_link(gx, ldm(link_ptr<12>(U0, g, 1, mu, h)),
+                           gs3(to_d(ldm(site_ptr<12>(G, g, 0, fwd_h(g, h, s, mu))))));
+          const M2<double> wb = rebuild_link(gs3(to_d(ldm(site_ptr<12>(G, g, 0, hbm)))),
+                                             ldm(link_ptr<12>(U0, g, 0, mu, hbm)), gx);
+          fr += su3_link_F(wf) + su3_link_F(wb);
+          F[k] = to_f(wf);
+          B[k] = to_f(wb);
+        }
+      }
 #if MAG_SU3_PTR_REMAT
       // the store addresses again from the 32-bit indices rather than 64-bit
       // pointers kept live through the three subgroups
@@ -377,7 +491,7 @@ __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict_
         }
       }
       if (GA && q == 0) stm(gp, Gx);
-      if (p == 1) {
+      if (MODE != HALF_REBUILD && p == 1) {
 #pragma unroll
         for (int k = 0; k < NL; k++) fs += su3_link_F(F[k]) + su3_link_F(B[k]);
       }
@@ -387,63 +501,68 @@ __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict_
       }
     }
     wave_partial(part, base + (threadIdx.x & ~63u), LS * g.Vh, fs, rs, rm);
+    if constexpr (MODE == HALF_REBUILD)
+      wave_partial(part_rb, base + (threadIdx.x & ~63u), LS * g.Vh, fr, 0.0, 0.0);
   }
 }
 
-template <typename T, bool GA, int NT, int SPT>
+template <typename T, bool GA, int NT, int SPT, int MODE>
 __global__ __launch_bounds__(NT) void k_su3_half(T* __restrict__ U, T* __restrict__ G, Geom g,
                                                  int p, T omega, double* __restrict__ part,
-                                                 const JobCtl* __restrict__ ctl) {
-  if (ctl->done) return;
-  su3_half_block<T, GA, NT, SPT>(U, G, g, p, omega, part, blockIdx.x);
+                                                 const JobCtl* __restrict__ ctl, int slot,
+                                                 const double* __restrict__ U0,
+                                                 double* __restrict__ part_rb) {
+  if (half_skip(ctl, slot)) return;
+  su3_half_block<T, GA, NT, SPT, MODE>(U, G, g, p, omega, part, blockIdx.x, U0, part_rb);
 }
 
-template <typename T, bool GA, int NT, int SPT>
+template <typename T, bool GA, int NT, int SPT, int MODE>
 __global__ __launch_bounds__(NT) void k_su3_half_batch(const BatchJob* __restrict__ jobs,
-                                                       int njobs, int p) {
+                                                       int njobs, int p, int slot) {
   const BatchJob& j = jobs[batch_job(jobs, njobs, blockIdx.x)];
-  if (j.ctl->done) return;
-  su3_half_block<T, GA, NT, SPT>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega,
-                                 j.part + (size_t)3 * j.nw * p, blockIdx.x - j.block0);
+  if (half_skip(j.ctl, slot)) return;
+  su3_half_block<T, GA, NT, SPT, MODE>((T*)j.U, (T*)j.G, j.g, p, (T)j.omega,
+                                       j.part + (size_t)slot * j.pstride + (size_t)3 * j.nw * p,
+                                       blockIdx.x - j.block0, j.U0, j.part + (size_t)kSlots * j.pstride);
 }
 
 template <typename T>
 void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega, double* part,
-                     const JobCtl* ctl, Shape sh, hipStream_t st) {
-  double* pp = part + (size_t)3 * sweep_waves(g, 3) * parity;
+                     const JobCtl* ctl, Shape sh, hipStream_t st, const HalfOpt& o) {
+  const size_t nw = (size_t)sweep_waves(g, 3);
+  double* pp = part + (size_t)o.slot * 2 * 3 * nw + 3 * nw * parity;
   const uint32_t lanes = (uint32_t)sweep_lanes(g, 3);
+  const int kind = half_kind<T>(G, o.mode);
 #define L_(NT, SPT)                                                                        \
   do {                                                                                     \
     const dim3 grid((lanes + NT * SPT - 1) / (NT * SPT));                                  \
-    if (G)                                                                                 \
-      hipLaunchKernelGGL((k_su3_half<T, true, NT, SPT>), grid, dim3(NT), 0, st, U, G, g,   \
-                         parity, (T)omega, pp, ctl);                                       \
-    else                                                                                   \
-      hipLaunchKernelGGL((k_su3_half<T, false, NT, SPT>), grid, dim3(NT), 0, st, U, G, g,  \
-                         parity, (T)omega, pp, ctl);                                       \
+    MAG_HALF_VARIANTS(k_su3_half, NT, SPT, grid, st, U, G, g, parity, (T)omega, pp, ctl,   \
+                      o.slot, o.U0, o.part_rb);                                            \
   } while (0)
   MAG_SHAPE_SWITCH(sh, L_)
 #undef L_
 }
 
 // batched half-sweep over a class table of storage type T (float: the
-// mixed FP32 phase with the site gauge, double: FP64): every job of the
-// table has the same group; block0 / nblocks_total counted in workgroups
-// of the shape sh
+// mixed FP32 phase, double: FP64): every job of the table has the same
+// group; block0 / nblocks_total counted in workgroups of the shape sh; in
+// the FP32 class parity 0 accumulates the site gauge and `mode` selects the
+// renormalising / rebuilding variants as in HalfOpt
 template <typename T>
 void launch_half_batch_t(int group, Shape sh, const BatchJob* jobs, int njobs, int nblocks_total,
-                         int parity, hipStream_t st) {
-  constexpr bool GA = sizeof(T) == 4;
+                         int parity, hipStream_t st, int slot, int mode) {
   if (njobs == 0 || nblocks_total == 0) return;
   const dim3 grid(nblocks_total);
+  // red: site gauge (kind 1 / 2), black: none (0) or rebuild (3)
+  const int kind = sizeof(T) == 8 ? 0
+                   : parity == 0  ? (mode == HALF_RENORM ? 2 : 1)
+                                  : (mode == HALF_REBUILD ? 3 : 0);
 #define L_(NT, SPT)                                                                          \
   do {                                                                                       \
     if (group == 2)                                                                          \
-      hipLaunchKernelGGL((k_su2_half_batch<T, GA, NT, SPT>), grid, dim3(NT), 0, st, jobs,    \
-                         njobs, parity);                                                     \
+      MAG_HALF_VARIANTS(k_su2_half_batch, NT, SPT, grid, st, jobs, njobs, parity, slot);     \
     else                                                                                     \
-      hipLaunchKernelGGL((k_su3_half_batch<T, GA, NT, SPT>), grid, dim3(NT), 0, st, jobs,    \
-                         njobs, parity);                                                     \
+      MAG_HALF_VARIANTS(k_su3_half_batch, NT, SPT, grid, st, jobs, njobs, parity, slot);     \
   } while (0)
   MAG_SHAPE_SWITCH(sh, L_)
 #undef L_

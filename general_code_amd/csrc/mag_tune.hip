@@ -41,15 +41,22 @@ bool valid_shape(int nt, int spt) {
   return false;
 }
 
+// control block of the scratch sweeps: not done, no cap in reach
+const JobCtl kScratchCtl = [] {
+  JobCtl c{};
+  c.max_sweeps = 1 << 30;
+  return c;
+}();
+
 // launch one half-sweep of the job's current kernel with shape sh on the
-// scratch arrays
+// scratch arrays (FP32: the red half accumulates the site gauge)
 void launch_scratch(const mag_job* j, bool fp32, void* U, void* G, double* part, JobCtl* ctl,
                     int parity, Shape sh) {
   if (j->prm.group == MAG_SU2) {
-    if (fp32) launch_su2_half<float>((float*)U, (float*)G, j->g, parity, j->prm.omega, part, ctl, sh, j->st);
+    if (fp32) launch_su2_half<float>((float*)U, parity == 0 ? (float*)G : nullptr, j->g, parity, j->prm.omega, part, ctl, sh, j->st);
     else launch_su2_half<double>((double*)U, nullptr, j->g, parity, j->prm.omega, part, ctl, sh, j->st);
   } else {
-    if (fp32) launch_su3_half<float>((float*)U, (float*)G, j->g, parity, j->prm.omega, part, ctl, sh, j->st);
+    if (fp32) launch_su3_half<float>((float*)U, parity == 0 ? (float*)G : nullptr, j->g, parity, j->prm.omega, part, ctl, sh, j->st);
     else launch_su3_half<double>((double*)U, nullptr, j->g, parity, j->prm.omega, part, ctl, sh, j->st);
   }
 }
@@ -82,7 +89,7 @@ int tune(const mag_job* j, bool fp32, Shape* out) {
     e = hipMemcpyAsync(U, fp32 ? (const void*)j->W32 : (const void*)j->U64, ubytes,
                        hipMemcpyDeviceToDevice, j->st);
   if (e == hipSuccess && fp32) e = hipMemcpyAsync(G, j->G32, gbytes, hipMemcpyDeviceToDevice, j->st);
-  if (e == hipSuccess) e = hipMemsetAsync(ctl, 0, sizeof(JobCtl), j->st);  // done = 0
+  if (e == hipSuccess) e = hipMemcpyAsync(ctl, &kScratchCtl, sizeof(JobCtl), hipMemcpyHostToDevice, j->st);
   if (e != hipSuccess) {
     cleanup();
     return fail(MAG_ERR_NOMEM, "autotune scratch: %s", hipGetErrorString(e));
@@ -165,7 +172,7 @@ int tune_batch(int group, bool fp32, const std::vector<mag_job*>& jobs, Shape* o
       e = hipMemcpyAsync(b.U, fp32 ? (const void*)j->W32 : (const void*)j->U64, ub,
                          hipMemcpyDeviceToDevice, st);
     if (e == hipSuccess && fp32) e = hipMemcpyAsync(b.G, j->G32, gb, hipMemcpyDeviceToDevice, st);
-    if (e == hipSuccess) e = hipMemsetAsync(b.ctl, 0, sizeof(JobCtl), st);  // done = 0
+    if (e == hipSuccess) e = hipMemcpyAsync(b.ctl, &kScratchCtl, sizeof(JobCtl), hipMemcpyHostToDevice, st);
     tab.push_back(b);
   }
   if (e == hipSuccess) e = hipMalloc((void**)&d_tab, tab.size() * sizeof(BatchJob));
