@@ -123,82 +123,116 @@ def lime_read(path):
     return out
 
 
+def _bind(L):
+    """Set the ctypes signatures of every oracle entry point on handle L."""
+    sig = {
+        "or_su2_mul": (None, [_dp, _dp, _dp]),
+        "or_su2_mul_dag": (None, [_dp, _dp, _dp]),
+        "or_su2_dag_mul": (None, [_dp, _dp, _dp]),
+        "or_su2_link_functional": (ctypes.c_double, [_dp]),
+        "or_spin_contribution1": (None, [_dp, _dp, _dp]),
+        "or_spin_contribution1_conj": (None, [_dp, _dp, _dp]),
+        "or_spin_reflect_fast": (None, [_dp, _dp]),
+        "or_spin_parallel": (ctypes.c_double, [_dp, _dp]),
+        "or_spin_gauge_matrix": (None, [_dp, _dp]),
+        "or_su2_contribution_site": (None, [_dp, _dp, _ip, ctypes.c_int64, _dp]),
+        "or_su2_functional": (ctypes.c_double, [_dp, _ip]),
+        "or_su2_functional_spin": (ctypes.c_double, [_dp, _dp, _ip]),
+        "or_su2_gauge_transform_spins": (None, [_dp, _dp, _ip]),
+        "or_su2_or_sweep_lexi": (None, [_dp, _dp, _ip]),
+        "or_su2_or_sweep_lexi_omp": (None, [_dp, _dp, _ip]),
+        "or_su2_relax_sweep_lexi": (None, [_dp, _dp, _ip, _dp, _dp]),
+        "or_su2_maximize_final": (ctypes.c_int, [_dp, _dp, _ip, ctypes.c_int,
+                                                 ctypes.c_double, ctypes.c_double,
+                                                 ctypes.c_int]),
+        "or_atan2_pos": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+        "or_sincos": (None, [ctypes.c_double, _dp, _dp]),
+        "or_approx_same": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+        "or_su2_maximize_approximate": (ctypes.c_int, [_dp, _dp, _ip, ctypes.c_int,
+                                                       ctypes.c_int, ctypes.c_int]),
+        "or_su2_spin_halfsweep": (None, [_dp, _dp, _ip, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, _dp, _dp]),
+        "or_su2_heatbath_halfsweep": (None, [_dp, _dp, _ip, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_uint64,
+                                             ctypes.c_uint32]),
+        "or_su2_link_sweep": (None, [_dp, _ip, ctypes.c_double, _dp, _dp, _dp]),
+        "or_su3_link_sweep": (None, [_dp, _ip, ctypes.c_double, _dp, _dp, _dp]),
+        "or_omega_gauge": (None, [_dp, ctypes.c_double, _dp]),
+        "or_block_field_fwd": (None, [_dp, _dp]),
+        "or_block_field_bwd": (None, [_dp, _dp]),
+        "or_su3_functional": (ctypes.c_double, [_dp, _ip]),
+        "or_su3_reunitarize": (None, [_dp]),
+        "or_su3_proj": (None, [_dp]),
+        "or_su3_mul": (None, [_dp, _dp, _dp]),
+        "or_su3_mul_dag": (None, [_dp, _dp, _dp]),
+        "or_su2_plaquette": (ctypes.c_double, [_dp, _ip]),
+        "or_su3_plaquette": (ctypes.c_double, [_dp, _ip]),
+        "or_su2_wilson_loop": (ctypes.c_double, [_dp, _ip, ctypes.c_int, ctypes.c_int]),
+        "or_su3_wilson_loop": (ctypes.c_double, [_dp, _ip, ctypes.c_int, ctypes.c_int]),
+        "or_su2_abelian_angles": (None, [_dp, _ip, _dp]),
+        "or_su3_abelian_angles": (None, [_dp, _ip, _dp]),
+        "or_u1_monopole_plaquettes": (None, [_dp, _ip, _dp]),
+        "or_su3_monopole_plaquettes": (None, [_dp, _ip, _dp]),
+        "or_monopole_currents": (None, [_dp, _ip, _dp]),
+        "or_monopole_clusters": (ctypes.c_int64, [_dp, _ip, ctypes.POINTER(ctypes.c_int32),
+                                                  ctypes.c_int64]),
+        "or_philox4x32_10": (None, [_u32p, _u32p, _u32p]),
+        "or_gen_su2_config": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64]),
+        "or_gen_su3_config": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64]),
+        "or_gen_copy_spins": (None, [_dp, _ip, ctypes.c_uint64, ctypes.c_uint32]),
+        "or_gen_su2_config_copy": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64,
+                                          ctypes.c_uint32]),
+        "or_gen_su3_config_copy": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64,
+                                          ctypes.c_uint32]),
+        "or_gen_su3_exp": (None, [_dp, ctypes.c_double, _dp]),
+        "or_gen_spins_uniform": (None, [_dp, _ip, ctypes.c_uint64]),
+        "or_set_threads": (None, [ctypes.c_int]),
+        "or_get_threads": (ctypes.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        sig = {
-            "or_su2_mul": (None, [_dp, _dp, _dp]),
-            "or_su2_mul_dag": (None, [_dp, _dp, _dp]),
-            "or_su2_dag_mul": (None, [_dp, _dp, _dp]),
-            "or_su2_link_functional": (ctypes.c_double, [_dp]),
-            "or_spin_contribution1": (None, [_dp, _dp, _dp]),
-            "or_spin_contribution1_conj": (None, [_dp, _dp, _dp]),
-            "or_spin_reflect_fast": (None, [_dp, _dp]),
-            "or_spin_parallel": (ctypes.c_double, [_dp, _dp]),
-            "or_spin_gauge_matrix": (None, [_dp, _dp]),
-            "or_su2_contribution_site": (None, [_dp, _dp, _ip, ctypes.c_int64, _dp]),
-            "or_su2_functional": (ctypes.c_double, [_dp, _ip]),
-            "or_su2_functional_spin": (ctypes.c_double, [_dp, _dp, _ip]),
-            "or_su2_gauge_transform_spins": (None, [_dp, _dp, _ip]),
-            "or_su2_or_sweep_lexi": (None, [_dp, _dp, _ip]),
-            "or_su2_or_sweep_lexi_omp": (None, [_dp, _dp, _ip]),
-            "or_su2_relax_sweep_lexi": (None, [_dp, _dp, _ip, _dp, _dp]),
-            "or_su2_maximize_final": (ctypes.c_int, [_dp, _dp, _ip, ctypes.c_int,
-                                                     ctypes.c_double, ctypes.c_double,
-                                                     ctypes.c_int]),
-            "or_atan2_pos": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
-            "or_sincos": (None, [ctypes.c_double, _dp, _dp]),
-            "or_approx_same": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_int]),
-            "or_su2_maximize_approximate": (ctypes.c_int, [_dp, _dp, _ip, ctypes.c_int,
-                                                           ctypes.c_int, ctypes.c_int]),
-            "or_su2_spin_halfsweep": (None, [_dp, _dp, _ip, ctypes.c_int, ctypes.c_int,
-                                             ctypes.c_double, _dp, _dp]),
-            "or_su2_heatbath_halfsweep": (None, [_dp, _dp, _ip, ctypes.c_int,
-                                                 ctypes.c_double, ctypes.c_uint64,
-                                                 ctypes.c_uint32]),
-            "or_su2_link_sweep": (None, [_dp, _ip, ctypes.c_double, _dp, _dp, _dp]),
-            "or_su3_link_sweep": (None, [_dp, _ip, ctypes.c_double, _dp, _dp, _dp]),
-            "or_omega_gauge": (None, [_dp, ctypes.c_double, _dp]),
-            "or_block_field_fwd": (None, [_dp, _dp]),
-            "or_block_field_bwd": (None, [_dp, _dp]),
-            "or_su3_functional": (ctypes.c_double, [_dp, _ip]),
-            "or_su3_reunitarize": (None, [_dp]),
-            "or_su3_proj": (None, [_dp]),
-            "or_su3_mul": (None, [_dp, _dp, _dp]),
-            "or_su3_mul_dag": (None, [_dp, _dp, _dp]),
-            "or_su2_plaquette": (ctypes.c_double, [_dp, _ip]),
-            "or_su3_plaquette": (ctypes.c_double, [_dp, _ip]),
-            "or_su2_wilson_loop": (ctypes.c_double, [_dp, _ip, ctypes.c_int, ctypes.c_int]),
-            "or_su3_wilson_loop": (ctypes.c_double, [_dp, _ip, ctypes.c_int, ctypes.c_int]),
-            "or_su2_abelian_angles": (None, [_dp, _ip, _dp]),
-            "or_su3_abelian_angles": (None, [_dp, _ip, _dp]),
-            "or_u1_monopole_plaquettes": (None, [_dp, _ip, _dp]),
-            "or_su3_monopole_plaquettes": (None, [_dp, _ip, _dp]),
-            "or_monopole_currents": (None, [_dp, _ip, _dp]),
-            "or_monopole_clusters": (ctypes.c_int64, [_dp, _ip, ctypes.POINTER(ctypes.c_int32),
-                                                      ctypes.c_int64]),
-            "or_philox4x32_10": (None, [_u32p, _u32p, _u32p]),
-            "or_gen_su2_config": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64]),
-            "or_gen_su3_config": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64]),
-            "or_gen_copy_spins": (None, [_dp, _ip, ctypes.c_uint64, ctypes.c_uint32]),
-            "or_gen_su2_config_copy": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64,
-                                              ctypes.c_uint32]),
-            "or_gen_su3_config_copy": (None, [_dp, _ip, ctypes.c_double, ctypes.c_uint64,
-                                              ctypes.c_uint32]),
-            "or_gen_su3_exp": (None, [_dp, ctypes.c_double, _dp]),
-            "or_gen_spins_uniform": (None, [_dp, _ip, ctypes.c_uint64]),
-            "or_set_threads": (None, [ctypes.c_int]),
-            "or_get_threads": (ctypes.c_int, []),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = L
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+_fast = None
+
+
+def fast_lib():
+    """The same C restatement compiled with the reference's own flags
+    (-Ofast -march=native, /root/reference makefile:5) for the CPU-baseline
+    timing only (never as the checker: -Ofast allows reassociation).  Built
+    on first use for the machine it runs on (oracle/Makefile target `fast`)
+    into a temporary directory, so no binary for another CPU is reused."""
+    global _fast
+    if _fast is None:
+        import tempfile
+        out = tempfile.mkdtemp(prefix="mag_oracle_fast_")
+        subprocess.run(["make", "-C", _HERE, "fast", f"FAST_OUT={out}"], check=True,
+                       stdout=subprocess.DEVNULL)
+        _fast = _bind(ctypes.CDLL(os.path.join(out, "libmag_oracle_fast.so")))
+    return _fast
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _d(a: np.ndarray):

@@ -121,6 +121,32 @@ def test_su2_fp64_converges_like_oracle(mag, oracle):
         assert abs(w1 - w0) < 1e-12
 
 
+def test_su3_fp64_converges_like_oracle(mag, oracle):
+    """SU(3) 8^3x16, eps 0.4, omega 1.6, tol 1e-11 run to convergence: the
+    same number of sweeps as the oracle's run of the same sweeps (bit-
+    identical trajectories), the same final links."""
+    dims = (8, 8, 8, 16)
+    links = oracle.gen_su3_config(dims, 0.4, 2024)
+    ref = links.copy()
+    F_prev = oracle.su3_functional(ref, dims)
+    n_ref = 0
+    for _ in range(20000):
+        F, _, _ = oracle.su3_link_sweep(ref, dims, 1.6)
+        n_ref += 1
+        if abs(F - F_prev) < 1e-11 * abs(F):
+            break
+        F_prev = F
+    with mag.MagJob(group=mag.SU3, dims=dims, omega=1.6, tol=1e-11, max_sweeps=20000) as job:
+        job.upload(links)
+        assert job.run()
+        sweeps, conv, Ffin = job.status()
+        got = job.download()
+    print(f"[su3 8^3x16 to tol] oracle {n_ref} sweeps, GPU {sweeps}", flush=True)
+    assert conv and sweeps == n_ref
+    assert np.array_equal(got, ref)
+    assert abs(Ffin - F) < 1e-12 * abs(F)
+
+
 @pytest.mark.parametrize("omega", [1.0, 1.6])
 def test_su3_fp64_sweeps_match_oracle(mag, oracle, omega):
     dims = SU3_DIMS
