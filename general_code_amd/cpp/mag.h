@@ -191,6 +191,31 @@ std::vector<SpinT> generate_spins_uniform(DataPattern& pattern, uint64_t seed) {
   return s;
 }
 
+// read_spins (mag.h:12-22, mag.cpp:54-88): V spins of 3 raw doubles each,
+// lexicographic site order.  The reference prints an error and goes on for
+// a short file; here that is an error.  (Its overloads without a pattern
+// size the read by the process globals x_size.., which apps/mag_su2 never
+// sets, so they read nothing; this one always reads the lattice's V spins.)
+template <class SpinT = spin, class DataPattern>
+std::vector<SpinT> read_spins(const std::string& path, const DataPattern& pattern) {
+  size_t V = 1;
+  for (int i = 0; i < 4; i++) V *= (size_t)pattern.lat_dim[i];
+  std::vector<SpinT> s(V);
+  std::ifstream f(path, std::ios::binary);
+  if (!f || !f.read(reinterpret_cast<char*>(detail::spin_data(s)), (std::streamsize)(3 * V * sizeof(double))))
+    throw error(MAG_ERR_ARG, "read_spins: cannot read " + std::to_string(3 * V) + " doubles from " + path);
+  return s;
+}
+
+// write_spins (mag.h:12-22, mag.cpp:107-141): 3 raw doubles per spin
+template <class SpinT>
+void write_spins(const std::string& path, std::vector<SpinT>& spins) {
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(detail::spin_data(spins)),
+          (std::streamsize)(3 * spins.size() * sizeof(double)));
+  if (!f) throw error(MAG_ERR_ARG, "write_spins: cannot write " + path);
+}
+
 // MAG_functional_su2 (mag.h:69-70, mag.cpp:993-1001)
 template <class Lattice>
 double MAG_functional_su2(const Lattice& conf) {

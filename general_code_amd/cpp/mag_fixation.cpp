@@ -1,10 +1,23 @@
 // mag_fixation -- the MAG fixing app on the MI355X, with the command-line
 // flags of ilyakudrov/general_code apps/mag_su2/mag_fixation.cpp:60-117.
 //
-// New trial (--is_new_trial 1): uniform random spins -> simulated annealing
-// -> final maximisation (--is_final 1) or approximate maximisation -> gauge
-// transformation -> MAG functional and plaquette -> lexicographic output,
-// as in mag_fixation.cpp:165-249.  Extra flags of this build:
+// For each gauge copy (--gauge_copies, mag_fixation.cpp:165-249): the
+// starting spins are new (--is_new_trial 1: uniform random spins, then
+// simulated annealing) or read from --path_previous (--is_new_trial 0,
+// read_spins); then
+//   --is_final 1: final maximisation -> gauge transformation -> MAG
+//       functional and plaquette -> lexicographic output of the links, and
+//       the copy loop ends;
+//   --is_final 0: approximate maximisation; --is_functional_save 1 records
+//       the copy's spin functional (written to --path_functional_output as
+//       "copy,functional" after the loop); the spins go to
+//       --path_spins_output (write_spins), with --is_compare 1 only when
+//       their functional beats the old one: that of the spins in
+//       --path_previous (--is_compare_spins 1) or of the links in
+//       --path_conf read as lexicographical (--is_compare_spins 0).
+// The reference's spin-file calls there size the data by process globals it
+// never sets and so read and write nothing; here they read and write the
+// lattice's V spins (3 doubles each).  Extra flags of this build:
 //   --seed N            Philox seed of the spins and the heat bath
 //   --synthetic_eps E   no --path_conf: generate exp(i E a.sigma) links with
 //                       a Haar gauge transformation from --seed
@@ -24,7 +37,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <fstream>
 #include <iostream>
+#include <map>
 #include <sstream>
 #include <string>
 
@@ -44,6 +59,7 @@ int main(int argc, char** argv) {
   int tolerance_digits = 7, gauge_copies = 1;
   int x_size = 0, y_size = 0, z_size = 0, t_size = 0;
   bool is_new_trial = true, is_final = true, is_functional_save = false;
+  bool is_compare = false, is_compare_spins = false;
   uint64_t seed = 1;
   double synthetic_eps = -1;
   bool link_form = false;
@@ -80,7 +96,9 @@ int main(int argc, char** argv) {
     else if (a == "--is_new_trial") std::istringstream(next()) >> is_new_trial;
     else if (a == "--is_final") std::istringstream(next()) >> is_final;
     else if (a == "--is_functional_save") std::istringstream(next()) >> is_functional_save;
-    else if (a == "--is_compare" || a == "--is_compare_spins" || a == "--ml5_conf_num") next();
+    else if (a == "--is_compare") std::istringstream(next()) >> is_compare;
+    else if (a == "--is_compare_spins") std::istringstream(next()) >> is_compare_spins;
+    else if (a == "--ml5_conf_num") next();  // parsed and unused, as in the reference
     else if (a == "--seed") seed = std::stoull(next());
     else if (a == "--synthetic_eps") synthetic_eps = std::stod(next());
     else if (a == "--link_form") std::istringstream(next()) >> link_form;
@@ -124,7 +142,9 @@ int main(int argc, char** argv) {
     const bool want_clusters = !path_clusters_unwrapped.empty() || !path_clusters_wrapped.empty() ||
                                !path_windings.empty() || !path_monopoles.empty();
     const bool want_J = !path_monopoles_output.empty() || want_clusters;
+    bool fixed = false;  // the links hold the gauge-fixed configuration
     if (link_form) {
+      fixed = true;
       std::vector<double> hist;
       const int sweeps = fix_mag_links(conf, omega, tol, max_sweeps, MAG_PREC_F64, &hist,
                                        want_J ? &J : nullptr);
@@ -132,32 +152,69 @@ int main(int argc, char** argv) {
     } else {
       DataPatternLexicographical pattern(dims);
       std::vector<spin> spins;
+      std::map<int, double> functional;
+      if (!is_final && path_spins_output.empty())
+        throw error(MAG_ERR_ARG, "--is_final 0 writes the spins: give --path_spins_output");
+      if (!is_new_trial && path_previous.empty())
+        throw error(MAG_ERR_ARG, "--is_new_trial 0 reads the spins from --path_previous");
       for (int copy = 1; copy <= gauge_copies; copy++) {
         if (is_new_trial) {
           spins = generate_spins_uniform<spin>(pattern, seed + copy);
           make_simulated_annealing(conf, spins, T_init, T_final, T_step, OR_steps,
                                    thermalization_steps, seed + copy);
         } else {
-          throw error(MAG_ERR_ARG, "--is_new_trial 0 needs spins from --path_previous "
-                                   "(spin files are not supported by this build)");
+          spins = read_spins<spin>(path_previous, pattern);
         }
         if (is_final) {
           const int it = make_maximization_final(conf, spins, OR_steps, tolerance_maximal,
                                                  tolerance_average, max_iter);
           std::cout << "maximization final iterations " << it << std::endl;
-        } else {
-          const int it = make_maximization_approximate(conf, spins, OR_steps, tolerance_digits,
-                                                       max_iter);
-          std::cout << "maximization approximate iterations " << it << std::endl;
+          std::cout << "spin functional " << MAG_functional_su2_spin(conf, spins) << std::endl;
+          gauge_tranformation_spins(conf, spins);
+          fixed = true;
+          break;
         }
-        std::cout << "spin functional " << MAG_functional_su2_spin(conf, spins) << std::endl;
-        gauge_tranformation_spins(conf, spins);
-        break;
+        const int it = make_maximization_approximate(conf, spins, OR_steps, tolerance_digits,
+                                                     max_iter);
+        const double F_new = MAG_functional_su2_spin(conf, spins);
+        std::cout << "copy " << copy << " maximization approximate iterations " << it
+                  << std::endl;
+        std::cout << "spin functional " << F_new << std::endl;
+        if (is_functional_save) functional[copy] = F_new;
+        if (is_compare) {
+          double F_old;
+          if (is_compare_spins) {
+            std::vector<spin> old = read_spins<spin>(path_previous, pattern);
+            F_old = MAG_functional_su2_spin(conf, old);
+          } else {
+            Data::LatticeData<DataPatternLexicographical, su2> conf_old(dims);
+            read_links(conf_old, path_conf, "lexicographical", 0, file_precision);
+            F_old = MAG_functional_su2(conf_old);
+          }
+          std::cout << "new functional is " << F_new << std::endl;
+          std::cout << "old functional is " << F_old << std::endl;
+          if (F_old < F_new) {
+            std::cout << "new functional is higher, saving spin configuration" << std::endl;
+            write_spins(path_spins_output, spins);
+          } else {
+            std::cout << "new functional is lower, spin configuration is not saved" << std::endl;
+          }
+        } else {
+          write_spins(path_spins_output, spins);
+        }
+      }
+      if (is_functional_save) {
+        std::ofstream f(path_functional_output);
+        if (!f) throw error(MAG_ERR_ARG, "cannot write " + path_functional_output);
+        f.precision(17);
+        f << "copy,functional" << std::endl;
+        for (const auto& [c, F] : functional) f << c << "," << F << std::endl;
       }
     }
     const double secs =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::cout << "fixing time " << secs << std::endl;
+    if (!fixed) return 0;  // approximate path: the spins are the output
     std::cout << "final functional is " << MAG_functional_su2(conf) << std::endl;
     std::cout << "plaquette " << plaket(conf) << std::endl;
     if (!path_conf_output.empty()) write_links_lexicographical(conf, path_conf_output);

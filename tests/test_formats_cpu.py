@@ -122,3 +122,24 @@ def test_ildg_wrong_size_is_refused(oracle, reader, tmp_path):
     r = subprocess.run([reader, "2", "4", "2", "2", "2", str(f), "ildg", "double",
                         str(tmp_path / "o.bin")], capture_output=True, text=True)
     assert r.returncode == 1 and "ildg-binary-data holds" in r.stderr
+
+
+def test_spin_files_roundtrip_and_short_file(mag, tmp_path):
+    """read_spins / write_spins (mag.cpp:54-141): V spins of 3 raw doubles,
+    lexicographic; a short file is an error (the reference prints one and
+    goes on with garbage)."""
+    exe = str(tmp_path / "spins")
+    pkg = os.path.join(ROOT, "general_code_amd")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    "-I", os.path.join(pkg, "cpp"), os.path.join(ROOT, "tests", "cpp", "spins_main.cpp"),
+                    "-L", pkg, "-lmag_gpu", f"-Wl,-rpath,{pkg}", "-o", exe], check=True)
+    dims = (4, 2, 4, 6)
+    V = int(np.prod(dims))
+    s = np.random.default_rng(3).normal(size=(V, 3))
+    p, out = tmp_path / "s.bin", tmp_path / "o.bin"
+    s.tofile(p)
+    subprocess.run([exe, *map(str, dims), str(p), str(out)], check=True)
+    assert np.array_equal(np.fromfile(out).reshape(V, 3), s)
+    s[:-1].tofile(p)
+    r = subprocess.run([exe, *map(str, dims), str(p), str(out)], capture_output=True, text=True)
+    assert r.returncode == 1 and "read_spins" in r.stderr

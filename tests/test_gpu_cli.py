@@ -107,3 +107,82 @@ def test_monopole_output(oracle, tmp_path, link_form):
     lines = tabs["monopoles"].read_text().splitlines()
     assert lines[0] == "asymmetry"
     assert float(lines[1]) == pytest.approx(summ["asymmetry"], rel=1e-5)  # 6 digits, as the app
+
+
+BASE = ["--x_size", "8", "--y_size", "8", "--z_size", "4", "--t_size", "8", "--T_init", "2.0",
+        "--T_final", "0.5", "--T_step", "0.5", "--OR_steps", "2", "--thermalization_steps", "2"]
+DIMS = (8, 8, 4, 8)
+
+
+def _conf(oracle, tmp_path):
+    links = oracle.gen_su2_config(DIMS, 0.4, 33)
+    conf = tmp_path / "conf.bin"
+    links.tofile(conf)
+    return links, ["--path_conf", str(conf), "--conf_format", "lexicographical"]
+
+
+def test_approximate_copy_search_writes_spins_and_functionals(oracle, tmp_path):
+    """--is_final 0 (mag_fixation.cpp:205-249): three copies, each written
+    to --path_spins_output (no comparison), the functional of every copy in
+    the "copy,functional" CSV; the spin file holds V spins of the last copy."""
+    links, conf = _conf(oracle, tmp_path)
+    spins_out, csv = tmp_path / "spins.bin", tmp_path / "F.csv"
+    out = _run(conf + BASE + ["--is_final", "0", "--tolerance_digits", "6", "--gauge_copies", "3",
+                              "--is_functional_save", "1", "--path_functional_output", str(csv),
+                              "--path_spins_output", str(spins_out), "--seed", "5"])
+    F = _num(out, "spin functional")
+    assert len(F) == 3 and len(set(F)) == 3  # three different copies
+    lines = csv.read_text().splitlines()
+    assert lines[0] == "copy,functional"
+    assert [int(l.split(",")[0]) for l in lines[1:]] == [1, 2, 3]
+    assert np.allclose([float(l.split(",")[1]) for l in lines[1:]], F, rtol=0, atol=1e-15)
+    s = np.fromfile(spins_out).reshape(-1, 3)
+    assert s.shape[0] == int(np.prod(DIMS))
+    assert abs(oracle.su2_functional_spin(links, s, DIMS) - F[-1]) < 1e-13
+    assert np.max(np.abs(np.linalg.norm(s, axis=1) - 1)) < 1e-12
+
+
+def test_compare_keeps_the_better_spins(oracle, tmp_path):
+    """--is_compare 1 --is_compare_spins 1: a copy's spins replace the file
+    only when their functional beats that of the spins in --path_previous."""
+    links, conf = _conf(oracle, tmp_path)
+    best = tmp_path / "best.bin"
+    _run(conf + BASE + ["--is_final", "0", "--tolerance_digits", "9", "--path_spins_output",
+                        str(best), "--seed", "8"])
+    F_best = oracle.su2_functional_spin(links, np.fromfile(best).reshape(-1, 3), DIMS)
+    worse = tmp_path / "worse.bin"
+    out = _run(conf + BASE + ["--is_final", "0", "--tolerance_digits", "1", "--OR_steps", "0",
+                              "--is_compare", "1", "--is_compare_spins", "1", "--path_previous",
+                              str(best), "--path_spins_output", str(worse), "--seed", "9"])
+    assert _num(out, "old functional is")[0] == pytest.approx(F_best, abs=1e-13)
+    assert "is not saved" in out and not worse.exists()
+    # resuming from the best spins (--is_new_trial 0) does not lower them: saved
+    again = tmp_path / "again.bin"
+    out = _run(conf + BASE + ["--is_new_trial", "0", "--is_final", "0", "--tolerance_digits", "11",
+                              "--is_compare", "1", "--is_compare_spins", "1", "--path_previous",
+                              str(best), "--path_spins_output", str(again)])
+    F_again = _num(out, "new functional is")[0]
+    assert F_again >= F_best - 1e-15
+    if F_again > F_best:
+        assert again.exists()
+
+
+def test_resume_from_spins_equals_the_library_path(mag, oracle, tmp_path):
+    """--is_new_trial 0 --path_previous S --is_final 1: the final maximisation
+    from the spins in S, the gauge transformation and the link output equal
+    the same calls through the library, bit for bit."""
+    links, conf = _conf(oracle, tmp_path)
+    S = oracle.gen_spins_uniform(DIMS, 44)
+    prev = tmp_path / "prev.bin"
+    S.tofile(prev)
+    outp = tmp_path / "fixed.bin"
+    _run(conf + BASE + ["--is_new_trial", "0", "--path_previous", str(prev), "--is_final", "1",
+                        "--tolerance_maximal", "1e-9", "--tolerance_average", "1e-13",
+                        "--path_conf_output", str(outp)])
+    with mag.SpinSession(DIMS) as s:
+        s.upload_links(links)
+        s.set_spins(S)
+        s.maximize_final(2, 1e-9, 1e-13, 1000000)
+        s.gauge_transform()
+        ref = s.download_links()
+    assert np.array_equal(np.fromfile(outp).reshape(-1, 4), ref)
