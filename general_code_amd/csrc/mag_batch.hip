@@ -392,6 +392,7 @@ int mag_batch_add(mag_batch* b, mag_job* j) {
   if (j->batch) return fail(MAG_ERR_STATE, "job is already in a batch");
   if (!j->loaded) return fail(MAG_ERR_STATE, "job has no configuration");
   if (j->prm.device != b->device) return fail(MAG_ERR_ARG, "job and batch on different devices");
+  if (j->prm.track_gauge) return fail(MAG_ERR_ARG, "a job that tracks its gauge runs alone");
   if (j->prm.precision == MAG_PREC_MIXED && !j->spin) {
     if (b->reunit == 0) {
       if (b->replay % j->prm.reunit_every != 0)

@@ -73,6 +73,9 @@ int check_params(const mag_params* p) {
     if (p->fp64_warmup < 0 || p->fp64_warmup % kChunk != 0)
       return fail(MAG_ERR_ARG, "fp64_warmup must be a non-negative multiple of %d", kChunk);
   }
+  if (p->track_gauge != 0 && p->track_gauge != 1) return fail(MAG_ERR_ARG, "track_gauge must be 0 or 1");
+  if (p->track_gauge && (p->group != MAG_SU2 || p->form != MAG_FORM_LINK))
+    return fail(MAG_ERR_ARG, "track_gauge is for the SU(2) link form");
   if (p->form == MAG_FORM_LINK) {
     if (p->copies > 1) return fail(MAG_ERR_ARG, "the link form holds one copy; use MAG_FORM_SPIN");
   } else if (p->form == MAG_FORM_SPIN) {
@@ -146,8 +149,8 @@ void launch_half(mag_job* j, int parity, const mag::HalfOpt& o = mag::HalfOpt{})
       mag::launch_su3_half<float>(j->W32, G, j->g, parity, j->prm.omega, j->part, j->ctl, sh, j->st, oo);
   } else {
     const mag::Shape sh = link_shape(j, false);
-    if (j->prm.group == MAG_SU2)
-      mag::launch_su2_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, j->part, j->ctl, sh, j->st);
+    if (j->prm.group == MAG_SU2)  // G64: the tracked gauge (or nullptr)
+      mag::launch_su2_half<double>(j->U64, j->G64, j->g, parity, j->prm.omega, j->part, j->ctl, sh, j->st);
     else
       mag::launch_su3_half<double>(j->U64, nullptr, j->g, parity, j->prm.omega, j->part, j->ctl, sh, j->st);
   }
@@ -168,7 +171,7 @@ void flush_slots(mag_job* j, bool rebuild) {
 // the FP64 links at the end of the FP32 phase (in place over the master)
 void launch_final_rebuild(mag_job* j) {
   if (j->prm.group == MAG_SU2)
-    mag::launch_su2_rebuild_final(j->U64, j->G32, j->W32, j->U64, j->g, j->st);
+    mag::launch_su2_rebuild_final(j->U64, j->G32, j->W32, j->U64, j->g, j->st, j->G64);
   else
     mag::launch_su3_rebuild_final(j->U64, j->G32, j->W32, j->U64, j->g, j->st);
 }
@@ -335,6 +338,7 @@ int reset_after_upload(mag_job* j) {
   j->fp64_start = mixed ? -1 : 0;
   j->since_rebuild = 0;
   j->pending = 0;
+  if (j->G64) mag::launch_gauge_identity64(j->G64, j->g, j->st);
   // initial functional from the FP64 links (and each copy's site variable)
   double* tmp = nullptr;
   CHK(alloc_tmp(j, &tmp));
@@ -375,6 +379,7 @@ void free_job(mag_job* j) {
   hipFree(j->S32);
   hipFree(j->W32);
   hipFree(j->G32);
+  hipFree(j->G64);
   hipFree(j->part);
   hipFree(j->ctl);
   hipFree(j->hF);
@@ -516,6 +521,7 @@ int mag_job_footprint(const mag_params* p, uint64_t* bytes) {
     }
     if (p->group == MAG_SU3) b += (uint64_t)V * 4 * nr * 8;  // transient FP64 links
   }
+  if (p->track_gauge) b += (uint64_t)V * 4 * 8;  // FP64 site gauges
   const uint64_t lanes = (uint64_t)(V / 2) * (p->group == MAG_SU2 ? 1 : 2);
   const uint64_t nw = (lanes + 63) / 64;  // partial triples per half-sweep
   b += (uint64_t)4 << 20;  // pitch padding of the chunk arrays (4 KB per array)
@@ -577,6 +583,7 @@ int mag_job_create(const mag_params* p, mag_job** out) {
     if (j->spin && p->group == MAG_SU3) ALLOC(j->S32, j->sstride * j->ncopies * sizeof(float));
   }
   if (j->spin) ALLOC(j->S, j->sstride * j->ncopies * sizeof(double));
+  if (p->track_gauge) ALLOC(j->G64, (size_t)2 * j->g.hp * 4 * sizeof(double));
   // mixed link form: kSlots sweep slots and the rebuild partials
   const size_t nslots = (p->precision == MAG_PREC_MIXED && !j->spin) ? mag::kSlots + 1 : 1;
   ALLOC(j->part, j->pstride * j->ncopies * nslots * sizeof(double));
@@ -766,6 +773,62 @@ int mag_job_download_links(mag_job* j, double* links) {
   if (e != hipSuccess || e2 != hipSuccess)
     return fail(MAG_ERR_HIP, "download failed: %s",
                 hipGetErrorString(e != hipSuccess ? e : e2));
+  return MAG_OK;
+}
+
+static int check_canonical(mag_job* j) {
+  if (!j) return fail(MAG_ERR_ARG, "NULL job");
+  if (!j->loaded) return fail(MAG_ERR_STATE, "no configuration uploaded");
+  if (!j->G64) return fail(MAG_ERR_STATE, "the job does not track its gauge (track_gauge = 0)");
+  if (mixed32(j)) return fail(MAG_ERR_STATE, "the job is in its FP32 phase: run or finalize it first");
+  return MAG_OK;
+}
+
+int mag_job_download_spins(mag_job* j, double* spins) {
+  CHK(check_canonical(j));
+  if (!spins) return fail(MAG_ERR_ARG, "NULL spins");
+  CHK(set_device(j));
+  const size_t n = (size_t)j->V * 3;
+  double *S = nullptr, *stage = nullptr;
+  HIPCHK(hipMalloc((void**)&S, n * sizeof(double)));
+  hipError_t e = hipMalloc((void**)&stage, n * sizeof(double));
+  if (e == hipSuccess) {
+    mag::launch_gauge_spins(j->G64, S, j->g, j->st);
+    mag::launch_spins_layout(stage, S, j->g, 0, j->st);
+    e = hipMemcpyAsync(spins, stage, n * sizeof(double), hipMemcpyDeviceToHost, j->st);
+  }
+  hipError_t e2 = hipStreamSynchronize(j->st);
+  hipFree(S);
+  hipFree(stage);
+  if (e != hipSuccess || e2 != hipSuccess)
+    return fail(MAG_ERR_HIP, "download_spins failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+  return MAG_OK;
+}
+
+int mag_job_download_links_canonical(mag_job* j, double* links) {
+  CHK(check_canonical(j));
+  if (!links) return fail(MAG_ERR_ARG, "NULL links");
+  CHK(set_device(j));
+  const size_t n = (size_t)j->V * 16;
+  double *S = nullptr, *soa = nullptr, *stage = nullptr;
+  HIPCHK(hipMalloc((void**)&S, (size_t)j->V * 3 * sizeof(double)));
+  hipError_t e = hipMalloc((void**)&soa, j->nreal * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&stage, n * sizeof(double));
+  if (e == hipSuccess) {
+    mag::launch_gauge_spins(j->G64, S, j->g, j->st);
+    mag::launch_canonical_links(j->U64, S, j->G64, soa, j->g, j->st);
+    mag::launch_su2_soa_to_aos<double>(soa, stage, j->g, j->st);
+    e = hipMemcpyAsync(links, stage, n * sizeof(double), hipMemcpyDeviceToHost, j->st);
+  }
+  hipError_t e2 = hipStreamSynchronize(j->st);
+  hipFree(S);
+  hipFree(soa);
+  hipFree(stage);
+  if (e != hipSuccess || e2 != hipSuccess)
+    return fail(MAG_ERR_HIP, "download_links_canonical failed: %s",
+                hipGetErrorString(e != hipSuccess ? e : e2));
+  if (!all_finite(links, n))
+    return fail(MAG_ERR_NONFINITE, "a site's spin is (0,0,1), the pole of GetGaugeMatrix");
   return MAG_OK;
 }
 

@@ -253,6 +253,16 @@ void launch_su3_aos_to_soa(const double* aos, T* U, const Geom& g, hipStream_t s
 template <typename T>
 void launch_su3_soa_to_aos(const T* U, double* aos, const Geom& g, hipStream_t st);
 
+// FP64 site gauges (SU(2), both parities) set to the identity
+void launch_gauge_identity64(double* G, const Geom& g, hipStream_t st);
+// gauge tracking (SU(2) link form): spins s(x) = row 3 of the rotation of
+// the accumulated gauge, in the spin SoA layout (S[(p*3+c)*Vh + h]); the
+// links in the reference's convention G_c(x) U_in G_c(x+mu)^+ computed from
+// the current links as u(x) U u(x+mu)^+, u = G_c G_acc^+, G_c =
+// GetGaugeMatrix(s) (chunked FP64 links into out)
+void launch_gauge_spins(const double* Gacc, double* S, const Geom& g, hipStream_t st);
+void launch_canonical_links(const double* U, const double* S, const double* Gacc, double* out,
+                            const Geom& g, hipStream_t st);
 // FP64 links -> FP32 links (both in the chunked SoA layout)
 void launch_links_f64_to_f32(const double* a, float* b, const Geom& g, int group, hipStream_t st);
 // site gauges set to the identity (NR reals per site, both parities)
@@ -262,8 +272,10 @@ void launch_gauge_identity(float* G, const Geom& g, int group, hipStream_t st);
 // G(x) U0 G(x+mu)^+, one thread per black site rebuilding its 8 links,
 // with the red gauges G_r normalised in FP64 and the black gauges recovered
 // from the working links W (mag_gauge.h).
+// Gacc (gauge tracking, SU(2)): the accumulated FP64 site gauges, which
+// take the FP32 phase's gauge (G_acc <- G G_acc)
 void launch_su2_rebuild_final(const double* U0, const float* Gr, const float* W, double* out64,
-                              const Geom& g, hipStream_t st);
+                              const Geom& g, hipStream_t st, double* Gacc = nullptr);
 void launch_su3_rebuild_final(const double* U0, const float* Gr, const float* W, double* out64,
                               const Geom& g, hipStream_t st);
 // FP64 reunitarisation of links in place (SU(2) q/|q|, SU(3) Gram-Schmidt)

@@ -24,7 +24,8 @@ struct mag_job {
   hipStream_t st = nullptr;
   double* U64 = nullptr;  // FP64 links (plain) / FP64 master then output (mixed)
   float* W32 = nullptr;   // mixed: FP32 working links
-  float* G32 = nullptr;   // mixed: accumulated site gauge
+  float* G32 = nullptr;   // mixed: accumulated site gauge (red sites)
+  double* G64 = nullptr;  // track_gauge: total gauge of the fixing, FP64
   double* part = nullptr;
   mag::JobCtl* ctl = nullptr;
   double* hF = nullptr;

@@ -210,7 +210,8 @@ void launch_gauge_identity(float* G, const Geom& g, int group, hipStream_t st) {
 __global__ __launch_bounds__(256) void k_su2_rebuild_final(const double* U0,
                                                            const float* __restrict__ Gr,
                                                            const float* __restrict__ W,
-                                                           double* out64, Geom g) {
+                                                           double* out64, Geom g,
+                                                           double* __restrict__ Gacc) {
   const uint32_t h = blockIdx.x * 256 + threadIdx.x;
   if (h >= g.Vh) return;
   const SiteIdx s = decode(g, h, 1);
@@ -236,6 +237,30 @@ __global__ __launch_bounds__(256) void k_su2_rebuild_final(const double* U0,
     stq(link_ptr<4>(out64, g, 1, mu, h), wf[mu]);
     stq(link_ptr<4>(out64, g, 0, mu, hb[mu]), wb[mu]);
   }
+  if (Gacc) {  // gauge tracking: the FP32 phase's black gauge joins the total
+    const LRef<double> r = site_ptr<4>(Gacc, g, 1, h);
+    stq(r, qmul(gx, ldq(r)));
+  }
+}
+
+// gauge tracking, end of the FP32 phase: red sites' total gauge
+// G_acc <- normalise(G_r) G_acc (the black sites' in k_su2_rebuild_final)
+__global__ __launch_bounds__(256) void k_su2_compose_red(const float* __restrict__ Gr,
+                                                         double* __restrict__ Gacc, Geom g) {
+  const uint32_t h = blockIdx.x * 256 + threadIdx.x;
+  if (h >= g.Vh) return;
+  const LRef<double> r = site_ptr<4>(Gacc, g, 0, h);
+  stq(r, qmul(norm_q(site_ptr<4>(Gr, g, 0, h)), ldq(r)));
+}
+
+// every site gauge (both parities, FP64, NR = 4) set to the identity
+__global__ void k_gauge_identity64(double* __restrict__ G, Geom g) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= 2 * g.Vh) return;
+  stq(site_ptr<4>(G, g, (int)(s / g.Vh), s % g.Vh), Q<double>{1.0, 0.0, 0.0, 0.0});
+}
+void launch_gauge_identity64(double* G, const Geom& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_gauge_identity64, dim3((2 * g.Vh + 255) / 256), dim3(256), 0, st, G, g);
 }
 
 __global__ __launch_bounds__(256) void k_su3_rebuild_final(const double* U0,
@@ -263,9 +288,11 @@ __global__ __launch_bounds__(256) void k_su3_rebuild_final(const double* U0,
 }
 
 void launch_su2_rebuild_final(const double* U0, const float* Gr, const float* W, double* out64,
-                              const Geom& g, hipStream_t st) {
+                              const Geom& g, hipStream_t st, double* Gacc) {
   hipLaunchKernelGGL(k_su2_rebuild_final, dim3((g.Vh + 255) / 256), dim3(256), 0, st, U0, Gr, W,
-                     out64, g);
+                     out64, g, Gacc);
+  if (Gacc)
+    hipLaunchKernelGGL(k_su2_compose_red, dim3((g.Vh + 255) / 256), dim3(256), 0, st, Gr, Gacc, g);
 }
 void launch_su3_rebuild_final(const double* U0, const float* Gr, const float* W, double* out64,
                               const Geom& g, hipStream_t st) {

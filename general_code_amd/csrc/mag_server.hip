@@ -271,6 +271,7 @@ int mag_server_submit(mag_server* s, const mag_request* r, uint64_t* id) {
   if (fp > s->budget) return fail(MAG_ERR_NOMEM, "job footprint %llu exceeds the budget %llu",
                                   (unsigned long long)fp, (unsigned long long)s->budget);
   if (r->params.device != s->device) return fail(MAG_ERR_ARG, "request for another device");
+  if (r->params.track_gauge) return fail(MAG_ERR_ARG, "served jobs do not track their gauge");
   Pending p;
   p.req = *r;
   p.req.links = nullptr;

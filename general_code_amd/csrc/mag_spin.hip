@@ -343,4 +343,54 @@ void launch_spin_links(const double* U, const double* S, double* out, const Geom
   hipLaunchKernelGGL(k_spin_links, dim3(2 * nblocks(g)), dim3(kBlock), 0, st, U, S, out, g);
 }
 
+// ------------------------------------------------------------------ gauge tracking
+// the accumulated gauge of a site, normalised
+__device__ __forceinline__ Q<double> acc_gauge(const double* Gacc, const Geom& g, int p,
+                                               uint32_t h) {
+  const Q<double> q = ldq(site_ptr<4>(Gacc, g, p, h));
+  const double n = 1.0 / sqrt(q.a0 * q.a0 + q.a1 * q.a1 + q.a2 * q.a2 + q.a3 * q.a3);
+  return Q<double>{q.a0 * n, q.a1 * n, q.a2 * n, q.a3 * n};
+}
+// s(x) = row 3 of the rotation of G_acc(x): invariant under a diagonal U(1)
+// factor on the left, the freedom the MAG functional leaves
+__global__ __launch_bounds__(kBlock) void k_gauge_spins(const double* __restrict__ Gacc,
+                                                        double* __restrict__ S, Geom g) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= 2 * g.Vh) return;
+  const int p = (int)(i / g.Vh);
+  const uint32_t h = i % g.Vh;
+  st_spin(S, g, p, h, spin_of_gauge(acc_gauge(Gacc, g, p, h)));
+}
+// u(x) = GetGaugeMatrix(s(x)) G_acc(x)^+ (diagonal up to rounding)
+__device__ __forceinline__ Q<double> u1_fix(const double* S, const double* Gacc, const Geom& g,
+                                            int p, uint32_t h) {
+  return qmul_dag(gauge_matrix(ld_spin(S, g, p, h)), acc_gauge(Gacc, g, p, h));
+}
+// out_x,mu = u(x) U_x,mu u(x+mu)^+ = G_c(x) U_in G_c(x+mu)^+; thread per site
+__global__ __launch_bounds__(kBlock) void k_canonical_links(const double* __restrict__ U,
+                                                            const double* __restrict__ S,
+                                                            const double* __restrict__ Gacc,
+                                                            double* __restrict__ out, Geom g) {
+  const int nb = (int)((g.Vh + kBlock - 1) / kBlock);
+  const int p = blockIdx.x / nb;
+  const uint32_t h = (blockIdx.x % nb) * kBlock + threadIdx.x;
+  if (h >= g.Vh) return;
+  const SiteIdx s = decode(g, h, p);
+  const Q<double> u = u1_fix(S, Gacc, g, p, h);
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) {
+    const Q<double> uf = u1_fix(S, Gacc, g, 1 - p, fwd_h(g, h, s, mu));
+    stq(link_ptr<4>(out, g, p, mu, h), qmul_dag(qmul(u, ldq(link_ptr<4>(U, g, p, mu, h))), uf));
+  }
+}
+void launch_gauge_spins(const double* Gacc, double* S, const Geom& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_gauge_spins, dim3((2 * g.Vh + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                     Gacc, S, g);
+}
+void launch_canonical_links(const double* U, const double* S, const double* Gacc, double* out,
+                            const Geom& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_canonical_links, dim3(2 * nblocks(g)), dim3(kBlock), 0, st, U, S, Gacc,
+                     out, g);
+}
+
 }  // namespace mag

@@ -218,12 +218,15 @@ __global__ __launch_bounds__(NT) void k_su2_half_batch(const BatchJob* __restric
 // half), 2 renormalise + site gauge, 3 rebuild (black half, G = red gauges)
 template <typename T>
 int half_kind(const void* G, int mode) {
-  if (sizeof(T) == 8 || !G) return 0;
+  if (!G) return 0;
+  if (sizeof(T) == 8) return 1;
   return mode == HALF_REBUILD ? 3 : mode == HALF_RENORM ? 2 : 1;
 }
 
-// one launch of kernel template K<T, GA, NT, SPT, MODE> for variant kind
-#define MAG_HALF_VARIANTS(K, NT, SPT, grid, st, ...)                                          \
+// one launch of kernel template K<T, GA, NT, SPT, MODE> for variant kind;
+// FP64 launches of DGA kernels (SU(2)) may accumulate the site gauge too
+// (kind 1: gauge tracking of an FP64 job)
+#define MAG_HALF_VARIANTS(K, DGA, NT, SPT, grid, st, ...)                                     \
   do {                                                                                         \
     if constexpr (sizeof(T) == 4) {                                                            \
       switch (kind) {                                                                          \
@@ -236,6 +239,13 @@ int half_kind(const void* G, int mode) {
         default: hipLaunchKernelGGL((K<T, false, NT, SPT, HALF_PLAIN>), grid, dim3(NT), 0, st, \
                                     __VA_ARGS__); break;                                       \
       }                                                                                        \
+    } else if constexpr (DGA) {                                                                \
+      if (kind == 1)                                                                           \
+        hipLaunchKernelGGL((K<T, true, NT, SPT, HALF_PLAIN>), grid, dim3(NT), 0, st,           \
+                           __VA_ARGS__);                                                       \
+      else                                                                                     \
+        hipLaunchKernelGGL((K<T, false, NT, SPT, HALF_PLAIN>), grid, dim3(NT), 0, st,          \
+                           __VA_ARGS__);                                                       \
     } else {                                                                                   \
       hipLaunchKernelGGL((K<T, false, NT, SPT, HALF_PLAIN>), grid, dim3(NT), 0, st,            \
                          __VA_ARGS__);                                                         \
@@ -252,8 +262,8 @@ void launch_su2_half(T* U, T* G, const Geom& g, int parity, double omega, double
 #define L_(NT, SPT)                                                                        \
   do {                                                                                     \
     const dim3 grid((lanes + NT * SPT - 1) / (NT * SPT));                                  \
-    MAG_HALF_VARIANTS(k_su2_half, NT, SPT, grid, st, U, G, g, parity, (T)omega, pp, ctl,   \
-                      o.slot, o.U0, o.part_rb);                                            \
+    MAG_HALF_VARIANTS(k_su2_half, true, NT, SPT, grid, st, U, G, g, parity, (T)omega, pp,  \
+                      ctl, o.slot, o.U0, o.part_rb);                                       \
   } while (0)
   MAG_SHAPE_SWITCH(sh, L_)
 #undef L_
@@ -536,8 +546,8 @@ void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega, double
 #define L_(NT, SPT)                                                                        \
   do {                                                                                     \
     const dim3 grid((lanes + NT * SPT - 1) / (NT * SPT));                                  \
-    MAG_HALF_VARIANTS(k_su3_half, NT, SPT, grid, st, U, G, g, parity, (T)omega, pp, ctl,   \
-                      o.slot, o.U0, o.part_rb);                                            \
+    MAG_HALF_VARIANTS(k_su3_half, false, NT, SPT, grid, st, U, G, g, parity, (T)omega, pp, \
+                      ctl, o.slot, o.U0, o.part_rb);                                       \
   } while (0)
   MAG_SHAPE_SWITCH(sh, L_)
 #undef L_
@@ -560,9 +570,9 @@ void launch_half_batch_t(int group, Shape sh, const BatchJob* jobs, int njobs, i
 #define L_(NT, SPT)                                                                          \
   do {                                                                                       \
     if (group == 2)                                                                          \
-      MAG_HALF_VARIANTS(k_su2_half_batch, NT, SPT, grid, st, jobs, njobs, parity, slot);     \
+      MAG_HALF_VARIANTS(k_su2_half_batch, false, NT, SPT, grid, st, jobs, njobs, parity, slot); \
     else                                                                                     \
-      MAG_HALF_VARIANTS(k_su3_half_batch, NT, SPT, grid, st, jobs, njobs, parity, slot);     \
+      MAG_HALF_VARIANTS(k_su3_half_batch, false, NT, SPT, grid, st, jobs, njobs, parity, slot); \
   } while (0)
   MAG_SHAPE_SWITCH(sh, L_)
 #undef L_

@@ -61,6 +61,7 @@ class _Params(ctypes.Structure):
         ("form", ctypes.c_int),
         ("copies", ctypes.c_int),
         ("fp64_warmup", ctypes.c_int),
+        ("track_gauge", ctypes.c_int),
     ]
 
 
@@ -136,6 +137,8 @@ SIGNATURES = {
     "mag_tune_clear": (ctypes.c_int, []),
     "mag_job_shape": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
     "mag_job_download_links": (ctypes.c_int, [_vp, _dp]),
+    "mag_job_download_links_canonical": (ctypes.c_int, [_vp, _dp]),
+    "mag_job_download_spins": (ctypes.c_int, [_vp, _dp]),
     "mag_job_functional": (ctypes.c_int, [_vp, _dp]),
     "mag_job_plaquette": (ctypes.c_int, [_vp, _dp]),
     "mag_job_sweeps_timed": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
@@ -289,6 +292,7 @@ class MagParams:
     form: int = FORM_LINK
     copies: int = 1
     fp64_warmup: int = 0
+    track_gauge: int = 0
 
     def to_c(self) -> _Params:
         p = _Params()
@@ -306,6 +310,7 @@ class MagParams:
         p.form = int(self.form)
         p.copies = int(self.copies)
         p.fp64_warmup = int(self.fp64_warmup)
+        p.track_gauge = int(self.track_gauge)
         return p
 
     @property
@@ -370,6 +375,18 @@ class MagJob:
         V = self.params.volume
         out = np.empty((4 * V, self.params.reals_in), dtype=np.float64)
         _check(lib().mag_job_download_links(self._h, _dptr(out)))
+        return out
+
+    def download_canonical(self) -> np.ndarray:
+        """SU(2) links in the reference's spin-form convention (track_gauge)."""
+        out = np.empty((4 * self.params.volume, 4), dtype=np.float64)
+        _check(lib().mag_job_download_links_canonical(self._h, _dptr(out)))
+        return out
+
+    def download_spins(self) -> np.ndarray:
+        """Spins s(x) of the tracked gauge, [V, 3] (track_gauge)."""
+        out = np.empty((self.params.volume, 3), dtype=np.float64)
+        _check(lib().mag_job_download_spins(self._h, _dptr(out)))
         return out
 
     # work -----------------------------------------------------------------

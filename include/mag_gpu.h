@@ -84,6 +84,10 @@ typedef struct mag_params {
   int fp64_warmup;    /* mixed: FP64 sweeps before the FP32 phase (a multiple */
                       /*        of 20); the early sweeps pick the maximum     */
                       /*        the run converges to                          */
+  int track_gauge;    /* SU(2) link form: also accumulate the site gauge G(x) */
+                      /*        of the fixing (current links = G U_in G^+),   */
+                      /*        for mag_job_download_links_canonical and      */
+                      /*        mag_job_download_spins; not in batches        */
 } mag_params;
 
 /* Link form: the links carry the gauge transformation and are rewritten by
@@ -159,6 +163,18 @@ int mag_job_fp64_start(mag_job *job, int *sweep);
 /* Download the current links (reference layout, FP64).  A mixed job that is
  * still in its FP32 phase returns its FP32 working links. */
 int mag_job_download_links(mag_job *job, double *links);
+/* SU(2) link form with track_gauge, after the run (FP64 phase): the fixed
+ * links in the convention of the reference's spin form, G_c(x) U_in
+ * G_c(x+mu)^+ with G_c = GetGaugeMatrix(s(x)) (matrix.h:960-963) and s(x)
+ * the spin of the accumulated gauge (row 3 of its rotation, invariant
+ * under the diagonal U(1) that the MAG functional leaves free): exactly
+ * what gauge_tranformation_spins (mag.cpp:1045-1068) writes for the spins
+ * that mag_job_download_spins returns (V x 3 doubles, lexicographic).  The
+ * plain download returns the same links up to a diagonal U(1) gauge.
+ * MAG_ERR_NONFINITE if a spin is exactly (0,0,1), the pole of
+ * GetGaugeMatrix. */
+int mag_job_download_links_canonical(mag_job *job, double *links);
+int mag_job_download_spins(mag_job *job, double *spins);
 /* MAG functional / plaquette of the device-resident links (blocking) */
 int mag_job_functional(mag_job *job, double *F);
 int mag_job_plaquette(mag_job *job, double *P);
