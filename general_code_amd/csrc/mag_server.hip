@@ -33,6 +33,7 @@
 #include <map>
 #include <mutex>
 #include <new>
+#include <set>
 #include <thread>
 #include <vector>
 
@@ -67,6 +68,7 @@ struct mag_server {
   std::condition_variable cv_done;  // waiters: a result was published
   std::deque<Pending> queue;
   std::map<uint64_t, mag_result> results;
+  std::set<uint64_t> cancel;  // admitted requests to cancel (scheduler acts)
   uint64_t next_id = 1;
   uint64_t submitted = 0, finished = 0;
   bool stop = false;
@@ -98,7 +100,7 @@ mag_result result_of(mag_server* s, const Active& a, int status) {
   r.t_submit = a.t_submit;
   r.t_admit = a.t_admit;
   r.status = status;
-  if (status == MAG_OK) {
+  if (status == MAG_OK || status == MAG_ERR_CANCELLED) {
     mag_job* j = a.job;
     int sw = 0, conv = 0;
     double F = 0;
@@ -112,7 +114,7 @@ mag_result result_of(mag_server* s, const Active& a, int status) {
     r.F = F;
     r.best_copy = best;
     r.fp64_start = j->fp64_start;
-    if (!conv) r.status = MAG_ERR_NOT_CONVERGED;
+    if (!conv && status == MAG_OK) r.status = MAG_ERR_NOT_CONVERGED;
   }
   r.t_done = s->now();
   return r;
@@ -184,6 +186,31 @@ void retire_finished(mag_server* s) {
   s->active.swap(still);
 }
 
+// admitted requests the caller cancelled: out of the batch (which first
+// lets a replay that holds the job finish), result published, memory freed
+void cancel_requested(mag_server* s) {
+  std::set<uint64_t> ids;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->cancel.empty()) return;
+    ids.swap(s->cancel);
+  }
+  std::vector<Active> still;
+  for (const Active& a : s->active) {
+    if (ids.count(a.id)) {
+      mag_batch_remove(s->batch, a.job);
+      (void)mag::job_read_ctl(a.job);
+      const mag_result r = result_of(s, a, MAG_ERR_CANCELLED);
+      mag_job_destroy(a.job);
+      s->used -= a.footprint;
+      publish(s, r);
+    } else {
+      still.push_back(a);
+    }
+  }
+  s->active.swap(still);
+}
+
 void fail_all_active(mag_server* s, int rc) {
   for (const Active& a : s->active) {
     const mag_result r = result_of(s, a, rc);
@@ -218,6 +245,7 @@ void scheduler(mag_server* s) {
       continue;
     }
     retire_finished(s);
+    cancel_requested(s);
   }
 }
 
@@ -305,6 +333,32 @@ int mag_server_drain(mag_server* s) {
   if (!s) return fail(MAG_ERR_ARG, "NULL server");
   std::unique_lock<std::mutex> g(s->mu);
   s->cv_done.wait(g, [&] { return s->finished == s->submitted; });
+  return MAG_OK;
+}
+
+int mag_server_cancel(mag_server* s, uint64_t id) {
+  if (!s) return fail(MAG_ERR_ARG, "NULL server");
+  std::lock_guard<std::mutex> g(s->mu);
+  if (id == 0 || id >= s->next_id) return fail(MAG_ERR_ARG, "unknown request %llu", (unsigned long long)id);
+  if (s->results.count(id))
+    return fail(MAG_ERR_STATE, "request %llu has already finished", (unsigned long long)id);
+  for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+    if (it->id != id) continue;
+    mag_result r;
+    std::memset(&r, 0, sizeof(r));
+    r.id = id;
+    r.status = MAG_ERR_CANCELLED;
+    r.t_submit = it->t_submit;
+    r.t_admit = -1.0;  // never admitted
+    r.t_done = s->now();
+    s->queue.erase(it);
+    s->results[id] = r;
+    s->finished++;
+    s->cv_done.notify_all();
+    return MAG_OK;
+  }
+  s->cancel.insert(id);  // admitted (or being admitted): the scheduler removes it
+  s->cv_work.notify_all();
   return MAG_OK;
 }
 

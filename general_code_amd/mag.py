@@ -28,6 +28,7 @@ MAG_ERR_NONFINITE = -4
 MAG_ERR_NOT_CONVERGED = -5
 MAG_ERR_HIP = -6
 MAG_ERR_STATE = -7
+MAG_ERR_CANCELLED = -8
 
 SU2, SU3 = 2, 3
 F64, MIXED = 0, 1
@@ -176,6 +177,7 @@ SIGNATURES = {
                                          ctypes.POINTER(ctypes.c_uint64)]),
     "mag_server_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_Result)]),
     "mag_server_drain": (ctypes.c_int, [_vp]),
+    "mag_server_cancel": (ctypes.c_int, [_vp, ctypes.c_uint64]),
     "mag_server_clock": (ctypes.c_int, [_vp, _dp]),
     "mag_server_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(_ServerStats)]),
     # spin form (reference mag.h surface)
@@ -862,6 +864,10 @@ class MagServer:
 
     def drain(self):
         _check(lib().mag_server_drain(self._h))
+
+    def cancel(self, rid: int):
+        """Cancel a queued or running request (its result: MAG_ERR_CANCELLED)."""
+        _check(lib().mag_server_cancel(self._h, int(rid)))
 
     def clock(self) -> float:
         t = ctypes.c_double()

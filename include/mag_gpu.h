@@ -55,6 +55,7 @@ extern "C" {
 #define MAG_ERR_NOT_CONVERGED (-5) /* sweep cap reached before the tolerance   */
 #define MAG_ERR_HIP (-6)           /* HIP runtime error                        */
 #define MAG_ERR_STATE (-7)         /* call not valid in the job's state        */
+#define MAG_ERR_CANCELLED (-8)     /* served request cancelled                 */
 
 /* ------------------------------------------------------------ parameters */
 #define MAG_SU2 2
@@ -396,6 +397,13 @@ int mag_server_submit(mag_server *s, const mag_request *r, uint64_t *id);
 int mag_server_wait(mag_server *s, uint64_t id, mag_result *out);
 /* blocks until every request submitted so far is finished */
 int mag_server_drain(mag_server *s);
+/* Cancel request id: a queued request is dropped at once, an admitted one
+ * leaves the batch at the scheduler's next step (after the replay that
+ * holds it has finished) and its memory is freed; either way its result
+ * has status MAG_ERR_CANCELLED (with the sweeps and F reached, if it was
+ * running).  MAG_ERR_STATE if it has already finished.  The other requests
+ * are not affected: their results stay those of each job alone. */
+int mag_server_cancel(mag_server *s, uint64_t id);
 int mag_server_clock(mag_server *s, double *seconds);
 int mag_server_get_stats(mag_server *s, mag_server_stats *st);
 

@@ -128,3 +128,49 @@ def test_su3_site_form_request_reaches_fp64_maximum(mag):
         r = srv.wait(srv.submit(p, eps=0.4, seed=6))
     assert conv_l and r["converged"]
     assert abs(r["F"] - F_l) / F_l < 1e-9, (r["F"], F_l)
+
+
+def test_failing_capped_and_cancelled_requests_leave_the_others_unchanged(mag, oracle):
+    """Served beside good requests: one whose links hold a NaN (refused at
+    admission: MAG_ERR_NONFINITE), one that reaches its sweep cap
+    (MAG_ERR_NOT_CONVERGED, its sweeps equal to the cap), one cancelled
+    while it runs and one cancelled while queued (MAG_ERR_CANCELLED).  The
+    good requests' results are bit-identical to each job alone, and the
+    server goes on serving afterwards."""
+    good = REQS[:4]
+    ref = [_alone(mag, r) for r in good]
+    bad_links = oracle.gen_su2_config((4, 4, 4, 4), 0.3, 9)
+    bad_links[17, 2] = np.nan
+    with mag.MagServer(max_active=6) as srv:
+        long_req = mag.MagParams(group=mag.SU2, dims=(16, 16, 16, 16), tol=0.0, max_sweeps=10 ** 6)
+        ids_long = srv.submit(long_req, eps=0.5, seed=77)
+        ids = [srv.submit(_params(mag, r), eps=r["eps"], seed=r["seed"], copy=r.get("copy", 0))
+               for r in good]
+        id_nan = srv.submit(mag.MagParams(group=mag.SU2, dims=(4, 4, 4, 4)), links=bad_links)
+        id_cap = srv.submit(mag.MagParams(group=mag.SU2, dims=(8, 8, 8, 8), tol=1e-14, max_sweeps=40),
+                            eps=0.5, seed=78)
+        res = [srv.wait(i) for i in ids]
+        r_nan, r_cap = srv.wait(id_nan), srv.wait(id_cap)
+        srv.cancel(ids_long)  # running since the start
+        r_long = srv.wait(ids_long)
+        # a request cancelled before it can be admitted (max_active jobs busy)
+        blockers = [srv.submit(long_req, eps=0.5, seed=80 + k) for k in range(6)]
+        queued = srv.submit(_params(mag, good[0]), eps=good[0]["eps"], seed=good[0]["seed"])
+        srv.cancel(queued)
+        r_queued = srv.wait(queued)
+        for b in blockers:
+            srv.cancel(b)
+        for b in blockers:
+            assert srv.wait(b)["status"] == mag.MAG_ERR_CANCELLED
+        with pytest.raises(mag.MagError):
+            srv.cancel(queued)  # already finished
+        # still serving
+        again = srv.wait(srv.submit(_params(mag, good[1]), eps=good[1]["eps"], seed=good[1]["seed"],
+                                    copy=good[1].get("copy", 0)))
+    for r, a in zip(res, ref):
+        assert r["status"] == 0 and r["F"] == a["F"] and r["sweeps"] == a["sweeps"]
+    assert r_nan["status"] == mag.MAG_ERR_NONFINITE
+    assert r_cap["status"] == mag.MAG_ERR_NOT_CONVERGED and r_cap["sweeps"] == 40
+    assert r_long["status"] == mag.MAG_ERR_CANCELLED and r_long["sweeps"] > 0
+    assert r_queued["status"] == mag.MAG_ERR_CANCELLED and r_queued["t_admit"] < 0
+    assert again["status"] == 0 and again["F"] == ref[1]["F"]
