@@ -129,6 +129,49 @@ __device__ __forceinline__ Q<T> omega_gauge(T h0, T h1, T h2, T omega, T& resid)
   return g;
 }
 
+// The same rule in FP32 for the FP32 sweeps, with the hardware reciprocal,
+// square root and reciprocal square root (about 1 ulp) and the short series
+// of mag_trig.h in place of IEEE division and the device library's
+// atan2f / sincosf: about a third of the instructions (the rule was some
+// 290 of the 2450 VALU instructions per wave of the SU(3) FP32 half-sweep
+// per subgroup).  FP32 trajectories are not compared bit for bit with
+// anything; the FP64 phases use the exact-IEEE rule above.
+template <>
+__device__ __forceinline__ Q<float> omega_gauge<float>(float h0, float h1, float h2, float omega,
+                                                       float& resid) {
+  const float hp2 = h0 * h0 + h1 * h1;
+  Q<float> g;
+  g.a3 = 0.0f;
+  if (hp2 == 0.0f) {  // along e3 or zero: as the FP64 rule
+    resid = (h2 >= 0.0f) ? 0.0f : 2.0f;
+    if (h2 >= 0.0f) {
+      g.a0 = 1.0f; g.a1 = 0.0f; g.a2 = 0.0f;
+    } else {
+      float s, c;
+      trig::sincos_0_pi_f(0.5f * omega * 3.14159265358979323846f, &s, &c);
+      g.a0 = c; g.a1 = s; g.a2 = 0.0f;
+    }
+    return g;
+  }
+  const float hp = __builtin_amdgcn_sqrtf(hp2);
+  const float rhn = __builtin_amdgcn_rsqf(hp2 + h2 * h2);
+  resid = (h2 >= 0.0f) ? hp2 * rhn * rhn * __builtin_amdgcn_rcpf(1.0f + h2 * rhn)
+                       : 1.0f - h2 * rhn;
+  // theta = atan2(hp, h2) in (0, pi): the angle to the e3 axis
+  const float ah = fabsf(h2);
+  const bool steep = hp > ah;
+  const float a = trig::atan01_f((steep ? ah : hp) * __builtin_amdgcn_rcpf(steep ? hp : ah));
+  const float th = steep ? 1.57079632679489662f - a : a;
+  const float theta = h2 >= 0.0f ? th : 3.14159265358979323846f - th;
+  float s, c;
+  trig::sincos_0_pi_f(0.5f * omega * theta, &s, &c);
+  const float inv = __builtin_amdgcn_rcpf(hp);
+  g.a0 = c;
+  g.a1 = -s * h1 * inv;
+  g.a2 = s * h0 * inv;
+  return g;
+}
+
 // ----------------------------------------------------------------- complex
 template <typename T> struct C { T re, im; };
 template <typename T> __device__ __forceinline__ C<T> cmk(T r, T i) { return C<T>{r, i}; }

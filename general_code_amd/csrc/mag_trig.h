@@ -94,5 +94,52 @@ __device__ __forceinline__ void sincos_0_2pi(double a, double* s, double* c) {
   }
 }
 
+// FP32: the angle functions of the FP32 sweeps, with hardware reciprocal
+// and square root (mag_math.h, omega_gauge<float>).  atan on [0, 1] is an
+// odd polynomial of degree 15 fitted to atan by weighted least squares
+// (Lawson's iteration towards the minimax fit; max error 5e-8 with these
+// FP32 coefficients); sin and cos of the angle reduced by pi/2 are their
+// Taylor series to r^9 and r^8 (truncation below 3e-8 for |r| <= pi/4).
+// The angle of sincos_0_pi_f lies in [0, pi], so q <= 2 and q * (pi/2)_hi
+// is exact.
+__device__ __forceinline__ float atan01_f(float t) {
+  const float z = t * t;
+  float p = -0.004054536111652851f;
+  p = p * z + 0.021862855181097984f;
+  p = p * z + -0.05591219291090965f;
+  p = p * z + 0.09642188996076584f;
+  p = p * z + -0.1390862762928009f;
+  p = p * z + 0.19946564733982086f;
+  p = p * z + -0.33329859375953674f;
+  p = p * z + 0.9999993443489075f;
+  return t * p;
+}
+__device__ __forceinline__ void sincos_0_pi_f(float a, float* s, float* c) {
+  const int q = (int)(a * 0.63661977f + 0.5f);
+  const float qd = (float)q;
+  const float r = (a - qd * 1.5707963705062866f) - qd * -4.371139006309477e-08f;
+  const float z = r * r;
+  float ps = 1.0f / 362880.0f;
+  ps = ps * z + -1.0f / 5040.0f;
+  ps = ps * z + 1.0f / 120.0f;
+  ps = ps * z + -1.0f / 6.0f;
+  const float sr = r + (r * z) * ps;
+  float pc = 1.0f / 40320.0f;
+  pc = pc * z + -1.0f / 720.0f;
+  pc = pc * z + 1.0f / 24.0f;
+  pc = pc * z + -0.5f;
+  const float cr = 1.0f + z * pc;
+  if (q == 0) {
+    *s = sr;
+    *c = cr;
+  } else if (q == 1) {
+    *s = cr;
+    *c = -sr;
+  } else {
+    *s = -sr;
+    *c = -cr;
+  }
+}
+
 }  // namespace trig
 }  // namespace mag
