@@ -88,11 +88,11 @@ inline int nblocks(const Geom& g) { return (int)((g.Vh + kBlock - 1) / kBlock); 
 constexpr int kSu3Lanes = MAG_SU3_LANES;
 static_assert(kSu3Lanes == 2 || kSu3Lanes == 4, "SU(3) lanes per site: 2 or 4");
 // lanes of a link-form half-sweep: SU(2) one per site, SU(3) kSu3Lanes;
-// the SU(3) site form runs on lane pairs (site_lanes)
+// the SU(3) site form runs one site per lane (site3_lanes)
 inline size_t sweep_lanes(const Geom& g, int group) {
   return (size_t)g.Vh * (group == 2 ? 1 : kSu3Lanes);
 }
-inline size_t site3_lanes(const Geom& g) { return (size_t)g.Vh * 2; }
+inline size_t site3_lanes(const Geom& g) { return (size_t)g.Vh; }
 // workgroups of a half-sweep at the default shape
 inline int sweep_blocks(const Geom& g, int group) {
   return (int)((sweep_lanes(g, group) + kBlock - 1) / kBlock);

@@ -433,72 +433,123 @@ void launch_copies_half_batch(int ncopies, bool fp32, const BatchJob* jobs, int 
 // For the subgroup (I,J), with D = H^(I) - H^(J), the field of the link
 // form's block sums (blk_fwd / blk_bwd) is
 //   h0 - i h1 = g_I D g_J^+,   h2 = (g_I D g_I^+ - g_J D g_J^+) / 2,
-// and the update rotates rows I, J of G.  A site builds H once from its 8
-// links (X and Y each from two products of rows and the unitary row-2 rule)
-// and then updates only G; only G is written.  Copy c follows the
-// link-form trajectory of the links G_c U G_c^+; all copies of a
-// configuration share the link reads.  Two lanes per site as in the link
-// form: each builds H from two directions, the pair adds them.
+// and the update rotates rows I, J of G.  X and Y are unitary, so
+// sum_i x_i x_i^+ = X X^+ = 1 and sum_i conj(y_i) y_i^T = conj(Y^+ Y) = 1
+// for each of the 8 links: H^(2) = 8 - H^(0) - H^(1).  A link therefore
+// contributes only x_0, x_1 (x_i = U conj(g'_i), g'_i the stored row i of
+// the neighbour's gauge) or y_0, y_1 (y_i = g'_i U), and
+//   D01 = H0 - H1,   D12 = H0 + 2 H1 - 8,   D02 = 2 H0 + H1 - 8.
+// One site per lane: a lane builds H0 and H1 from its 8 links and then
+// updates only G; only G is written.  (The first version ran a site on a
+// lane pair, each lane summing four links, and both lanes repeated the
+// three subgroup updates: about 4500 VALU instructions per site update in
+// FP32 against about 1900 here.)  Copy c follows the link-form trajectory
+// of the links G_c U G_c^+; all copies of a configuration share the link
+// reads.
 template <typename T>
 struct Herm3 {
   T d0, d1, d2;       // diagonal
   C<T> o01, o02, o12;  // upper off-diagonal (H_ba = conj(H_ab))
 };
-// H += v v^+  (H_ab += v_a conj(v_b))
+template <typename T>
+__device__ __forceinline__ Herm3<T> herm_zero() {
+  return Herm3<T>{T(0), T(0), T(0), {T(0), T(0)}, {T(0), T(0)}, {T(0), T(0)}};
+}
+// sum_k a_k conj(b_k) over k = 0..2, one fused chain per component
+template <typename T>
+__device__ __forceinline__ C<T> cdot3c(const C<T>* a, const C<T>* b) {
+  T re = a[0].re * b[0].re, im = a[0].im * b[0].re;
+  re = fma(a[0].im, b[0].im, re);
+  im = fma(-a[0].re, b[0].im, im);
+#pragma unroll
+  for (int k = 1; k < 3; k++) {
+    re = fma(a[k].re, b[k].re, re);
+    re = fma(a[k].im, b[k].im, re);
+    im = fma(a[k].im, b[k].re, im);
+    im = fma(-a[k].re, b[k].im, im);
+  }
+  return C<T>{re, im};
+}
+// a0 b0 + a1 b1 + a2 b2 (complex), one fused chain per component
+template <typename T>
+__device__ __forceinline__ C<T> cdot3(C<T> a0, C<T> b0, C<T> a1, C<T> b1, C<T> a2, C<T> b2) {
+  T re = a0.re * b0.re, im = a0.re * b0.im;
+  re = fma(-a0.im, b0.im, re);
+  im = fma(a0.im, b0.re, im);
+  re = fma(a1.re, b1.re, fma(-a1.im, b1.im, re));
+  im = fma(a1.re, b1.im, fma(a1.im, b1.re, im));
+  re = fma(a2.re, b2.re, fma(-a2.im, b2.im, re));
+  im = fma(a2.re, b2.im, fma(a2.im, b2.re, im));
+  return C<T>{re, im};
+}
+// H += v v^+  (H_ab += v_a conj(v_b)), as fused multiply-add chains
 template <typename T>
 __device__ __forceinline__ void herm_add_col(Herm3<T>& H, C<T> v0, C<T> v1, C<T> v2) {
-  H.d0 += cnorm2(v0);
-  H.d1 += cnorm2(v1);
-  H.d2 += cnorm2(v2);
-  H.o01 = cadd(H.o01, cmulc(v0, v1));
-  H.o02 = cadd(H.o02, cmulc(v0, v2));
-  H.o12 = cadd(H.o12, cmulc(v1, v2));
+  H.d0 = fma(v0.re, v0.re, fma(v0.im, v0.im, H.d0));
+  H.d1 = fma(v1.re, v1.re, fma(v1.im, v1.im, H.d1));
+  H.d2 = fma(v2.re, v2.re, fma(v2.im, v2.im, H.d2));
+  H.o01 = C<T>{fma(v0.re, v1.re, fma(v0.im, v1.im, H.o01.re)),
+               fma(v0.im, v1.re, fma(-v0.re, v1.im, H.o01.im))};
+  H.o02 = C<T>{fma(v0.re, v2.re, fma(v0.im, v2.im, H.o02.re)),
+               fma(v0.im, v2.re, fma(-v0.re, v2.im, H.o02.im))};
+  H.o12 = C<T>{fma(v1.re, v2.re, fma(v1.im, v2.im, H.o12.re)),
+               fma(v1.im, v2.re, fma(-v1.re, v2.im, H.o12.im))};
 }
 // H += conj(y) y^T  (H_ab += conj(y_a) y_b)
 template <typename T>
 __device__ __forceinline__ void herm_add_row(Herm3<T>& H, C<T> y0, C<T> y1, C<T> y2) {
-  H.d0 += cnorm2(y0);
-  H.d1 += cnorm2(y1);
-  H.d2 += cnorm2(y2);
-  H.o01 = cadd(H.o01, cmulc(y1, y0));
-  H.o02 = cadd(H.o02, cmulc(y2, y0));
-  H.o12 = cadd(H.o12, cmulc(y2, y1));
+  herm_add_col(H, cconj(y0), cconj(y1), cconj(y2));
 }
-// r = D w^+ (a column: r_a = sum_b D_ab conj(w_b))
+// a A + b B + c 1
+template <typename T>
+__device__ __forceinline__ Herm3<T> herm_comb(const Herm3<T>& A, T a, const Herm3<T>& B, T b,
+                                              T c) {
+  Herm3<T> D;
+  D.d0 = a * A.d0 + b * B.d0 + c;
+  D.d1 = a * A.d1 + b * B.d1 + c;
+  D.d2 = a * A.d2 + b * B.d2 + c;
+  D.o01 = C<T>{a * A.o01.re + b * B.o01.re, a * A.o01.im + b * B.o01.im};
+  D.o02 = C<T>{a * A.o02.re + b * B.o02.re, a * A.o02.im + b * B.o02.im};
+  D.o12 = C<T>{a * A.o12.re + b * B.o12.re, a * A.o12.im + b * B.o12.im};
+  return D;
+}
+// a + d conj(w): (x + iy)(u - iv) = (xu + yv) + i(yu - xv), fused
+template <typename T>
+__device__ __forceinline__ C<T> cacc_c(C<T> a, C<T> d, C<T> w) {
+  return C<T>{fma(d.re, w.re, fma(d.im, w.im, a.re)), fma(d.im, w.re, fma(-d.re, w.im, a.im))};
+}
+// r = D w^+ (a column: r_a = sum_b D_ab conj(w_b)), fused chains
 template <typename T>
 __device__ __forceinline__ void herm_mul_dag(const Herm3<T>& D, const C<T>* w, C<T>* r) {
-  const C<T> c0 = cconj(w[0]), c1 = cconj(w[1]), c2 = cconj(w[2]);
-  r[0] = cadd(cadd(C<T>{D.d0 * c0.re, D.d0 * c0.im}, cmul(D.o01, c1)), cmul(D.o02, c2));
-  r[1] = cadd(cadd(cmul(cconj(D.o01), c0), C<T>{D.d1 * c1.re, D.d1 * c1.im}), cmul(D.o12, c2));
-  r[2] = cadd(cadd(cmul(cconj(D.o02), c0), cmul(cconj(D.o12), c1)), C<T>{D.d2 * c2.re, D.d2 * c2.im});
+  r[0] = cacc_c(cacc_c(C<T>{D.d0 * w[0].re, -D.d0 * w[0].im}, D.o01, w[1]), D.o02, w[2]);
+  r[1] = cacc_c(cacc_c(C<T>{D.d1 * w[1].re, -D.d1 * w[1].im}, cconj(D.o01), w[0]), D.o12, w[2]);
+  r[2] = cacc_c(cacc_c(C<T>{D.d2 * w[2].re, -D.d2 * w[2].im}, cconj(D.o02), w[0]), cconj(D.o12),
+                w[1]);
 }
+// sum_c u_c r_c, one fused chain per component
 template <typename T>
 __device__ __forceinline__ C<T> dot3(const C<T>* u, const C<T>* r) {
-  return cadd(cadd(cmul(u[0], r[0]), cmul(u[1], r[1])), cmul(u[2], r[2]));
+  return cdot3(u[0], r[0], u[1], r[1], u[2], r[2]);
+}
+// g H g^+ of a Hermitian H, in double
+__device__ __forceinline__ double herm_form(const Herm3<double>& H, const C<double>* g) {
+  const double d = H.d0 * cnorm2(g[0]) + H.d1 * cnorm2(g[1]) + H.d2 * cnorm2(g[2]);
+  const double o = cmul(g[0], cmulc(H.o01, g[1])).re + cmul(g[0], cmulc(H.o02, g[2])).re +
+                   cmul(g[1], cmulc(H.o12, g[2])).re;
+  return d + 2.0 * o;
 }
 template <typename T>
-__device__ __forceinline__ void herm_pair_sum(Herm3<T>& H) {
-  H.d0 += pair_xchg(H.d0);
-  H.d1 += pair_xchg(H.d1);
-  H.d2 += pair_xchg(H.d2);
-  H.o01.re += pair_xchg(H.o01.re);
-  H.o01.im += pair_xchg(H.o01.im);
-  H.o02.re += pair_xchg(H.o02.re);
-  H.o02.im += pair_xchg(H.o02.im);
-  H.o12.re += pair_xchg(H.o12.re);
-  H.o12.im += pair_xchg(H.o12.im);
+__device__ __forceinline__ Herm3<double> herm_d(const Herm3<T>& H) {
+  return Herm3<double>{(double)H.d0, (double)H.d1, (double)H.d2,
+                       {(double)H.o01.re, (double)H.o01.im},
+                       {(double)H.o02.re, (double)H.o02.im},
+                       {(double)H.o12.re, (double)H.o12.im}};
 }
 
-// subgroup (I,J) update of the site gauge G (3 rows); returns 1 - cos(theta)
+// subgroup (I,J) update of the site gauge G (3 rows) in the field of
+// D = H^(I) - H^(J); returns 1 - cos(theta)
 template <typename T, int I, int J>
-__device__ __forceinline__ T site_subgroup(const Herm3<T>* H, C<T> (&G)[3][3], T omega) {
-  Herm3<T> D;
-  D.d0 = H[I].d0 - H[J].d0;
-  D.d1 = H[I].d1 - H[J].d1;
-  D.d2 = H[I].d2 - H[J].d2;
-  D.o01 = csub(H[I].o01, H[J].o01);
-  D.o02 = csub(H[I].o02, H[J].o02);
-  D.o12 = csub(H[I].o12, H[J].o12);
+__device__ __forceinline__ T site_subgroup(const Herm3<T>& D, C<T> (&G)[3][3], T omega) {
   C<T> rJ[3], rI[3];
   herm_mul_dag(D, G[J], rJ);
   herm_mul_dag(D, G[I], rI);
@@ -508,60 +559,98 @@ __device__ __forceinline__ T site_subgroup(const Herm3<T>* H, C<T> (&G)[3][3], T
   T res;
   const Q<T> gq = omega_gauge(h0, h1, h2, omega, res);
   // r = [[g0 + i g3, g2 + i g1], [-g2 + i g1, g0 - i g3]], g3 = 0 (as in cm_subgroup)
-  const C<T> r00{gq.a0, T(0)}, r01{gq.a2, gq.a1}, r10{-gq.a2, gq.a1}, r11{gq.a0, T(0)};
+  // row I <- g0 a + (g2 + i g1) b,  row J <- (-g2 + i g1) a + g0 b
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const C<T> a = G[I][c], b = G[J][c];
-    G[I][c] = cadd(cmul(r00, a), cmul(r01, b));
-    G[J][c] = cadd(cmul(r10, a), cmul(r11, b));
+    G[I][c] = C<T>{fma(gq.a0, a.re, fma(gq.a2, b.re, -gq.a1 * b.im)),
+                   fma(gq.a0, a.im, fma(gq.a2, b.im, gq.a1 * b.re))};
+    G[J][c] = C<T>{fma(gq.a0, b.re, fma(-gq.a2, a.re, -gq.a1 * a.im)),
+                   fma(gq.a0, b.im, fma(-gq.a2, a.im, gq.a1 * a.re))};
   }
   return res;
 }
 
-// the contributions of one direction to H^(0..2): the forward link U_f
-// with the site gauge G_f at x + mu, the backward link U_b with G_b at x - mu
+// row 2 = conj(row0 x row1) of an SU(3) matrix, fused chains
 template <typename T>
-__device__ __forceinline__ void site_fwd_field(Herm3<T>* H, const M2<T>& Uf, const M2<T>& Gf) {
-  {
-    // X = U_f G_f^+: rows 0, 1 from the stored rows of U_f and all rows of
-    // G_f, row 2 = conj(row0 x row1)
-    C<T> gf2[3];
-    row2_of(Gf, gf2);
-    M2<T> X;
+__device__ __forceinline__ void row2_fma(const M2<T>& M, C<T>* z) {
+  const C<T>* a = M.r[0];
+  const C<T>* b = M.r[1];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int b = 0; b < 3; b++) {
-        const C<T>* gb = b < 2 ? Gf.r[b] : gf2;
-        X.r[a][b] = cadd(cadd(cmulc(Uf.r[a][0], gb[0]), cmulc(Uf.r[a][1], gb[1])),
-                         cmulc(Uf.r[a][2], gb[2]));
-      }
-    C<T> x2[3];
-    row2_of(X, x2);
-#pragma unroll
-    for (int i = 0; i < 3; i++) herm_add_col(H[i], X.r[0][i], X.r[1][i], x2[i]);
+  for (int k = 0; k < 3; k++) {
+    const int i = (k + 1) % 3, j = (k + 2) % 3;
+    // conj(a_i b_j - a_j b_i)
+    T re = a[i].re * b[j].re, nim = -a[i].re * b[j].im;
+    re = fma(-a[i].im, b[j].im, re);
+    nim = fma(-a[i].im, b[j].re, nim);
+    re = fma(-a[j].re, b[i].re, re);
+    nim = fma(a[j].re, b[i].im, nim);
+    re = fma(a[j].im, b[i].im, re);
+    nim = fma(a[j].im, b[i].re, nim);
+    z[k] = C<T>{re, nim};
   }
 }
+
+// x_i = U conj(g_i), i = 0, 1 (g_i the stored rows of the forward
+// neighbour's gauge; U's row 2 from its rows 0, 1): H_i += x_i x_i^+
 template <typename T>
-__device__ __forceinline__ void site_bwd_field(Herm3<T>* H, const M2<T>& Ub, const M2<T>& Gb) {
-  {
-    // Y = G_b U_b: rows 0, 1 from the stored rows of G_b and all rows of
-    // U_b, row 2 = conj(row0 x row1)
-    C<T> ub2[3];
-    row2_of(Ub, ub2);
-    M2<T> Y;
+__device__ __forceinline__ void site_fwd2(Herm3<T>& H0, Herm3<T>& H1, const M2<T>& U,
+                                          const M2<T>& Gf) {
+  C<T> u2[3];
+  row2_fma(U, u2);
 #pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int b = 0; b < 3; b++)
-        Y.r[a][b] = cadd(cadd(cmul(Gb.r[a][0], Ub.r[0][b]), cmul(Gb.r[a][1], Ub.r[1][b])),
-                         cmul(Gb.r[a][2], ub2[b]));
-    C<T> y2[3];
-    row2_of(Y, y2);
-    herm_add_row(H[0], Y.r[0][0], Y.r[0][1], Y.r[0][2]);
-    herm_add_row(H[1], Y.r[1][0], Y.r[1][1], Y.r[1][2]);
-    herm_add_row(H[2], y2[0], y2[1], y2[2]);
+  for (int i = 0; i < 2; i++) {
+    const C<T>* gi = Gf.r[i];
+    herm_add_col(i == 0 ? H0 : H1, cdot3c(U.r[0], gi), cdot3c(U.r[1], gi), cdot3c(u2, gi));
   }
+}
+// y_i = g_i U, i = 0, 1 (g_i the stored rows of the backward neighbour's
+// gauge): H_i += conj(y_i) y_i^T
+template <typename T>
+__device__ __forceinline__ void site_bwd2(Herm3<T>& H0, Herm3<T>& H1, const M2<T>& U,
+                                          const M2<T>& Gb) {
+  C<T> u2[3];
+  row2_fma(U, u2);
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const C<T>* gi = Gb.r[i];
+    C<T> y[3];
+#pragma unroll
+    for (int b = 0; b < 3; b++) y[b] = cdot3(gi[0], U.r[0][b], gi[1], U.r[1][b], gi[2], u2[b]);
+    herm_add_row(i == 0 ? H0 : H1, y[0], y[1], y[2]);
+  }
+}
+
+// rows 0, 1 back to orthonormal: the FP32 site gauges are never rebuilt in
+// FP64 during their FP32 phase, and every update leaves about 1e-7 in row
+// norm and in the overlap of the rows, which would otherwise accumulate
+// over the sweeps.  Row 0 is scaled by (3 - |row|^2) / 2 (the Newton step
+// of 1/|row| at 1); row 1 loses its component along row 0 and is scaled
+// the same way.
+template <typename T>
+__device__ __forceinline__ void scale_row(C<T>* r) {
+  T n = T(0);
+#pragma unroll
+  for (int c = 0; c < 3; c++) n = fma(r[c].re, r[c].re, fma(r[c].im, r[c].im, n));
+  const T f = T(1.5) - T(0.5) * n;
+#pragma unroll
+  for (int c = 0; c < 3; c++) r[c] = C<T>{r[c].re * f, r[c].im * f};
+}
+template <typename T>
+__device__ __forceinline__ void orthonorm_rows(M2<T>& m) {
+  scale_row(m.r[0]);
+  const C<T> o = cdot3c(m.r[1], m.r[0]);  // <row 0, row 1> = sum r1_c conj(r0_c)
+#pragma unroll
+  for (int c = 0; c < 3; c++) m.r[1][c] = csub(m.r[1][c], cmul(o, m.r[0][c]));
+  scale_row(m.r[1]);
+}
+
+// directions whose 4 links and 4 neighbour gauges are loaded together:
+// FP32 two (122 VGPRs, 4 waves per SIMD; all four: 172 VGPRs), FP64 one
+// (194 VGPRs; two: 256)
+template <typename T>
+constexpr int site_dirs() {
+  return sizeof(T) == 4 ? 2 : 1;
 }
 
 template <typename T>
@@ -575,98 +664,71 @@ __device__ __forceinline__ void su3_site_copies_block(const T* __restrict__ U,
   bool live = false;
   for (int c = 0; c < ncopies; c++) live |= !ctl[c].done;
   if (!live) return;  // uniform over the workgroup
-  const uint32_t tid = lb * kBlock + threadIdx.x;
-  const uint32_t h = tid >> 1;
-  const int q = (int)(tid & 1u);
+  const uint32_t h = lb * kBlock + threadIdx.x;
   const bool in = h < g.Vh;
-  SiteIdx s{};
-  uint32_t hb[4] = {0, 0, 0, 0};
-  if (in) {
-    s = decode(g, h, p);
-    back_all(g, h, s, hb);
-  }
+  const SiteIdx s = in ? decode(g, h, p) : SiteIdx{};
   for (int c = 0; c < ncopies; c++) {
     if (ctl[c].done) continue;  // uniform over the workgroup
     T* Gc = G + (size_t)c * gstride;
     double fs = 0.0, rs = 0.0, rm = 0.0;
     if (in) {
-      Herm3<T> H[3];
+      Herm3<T> H0 = herm_zero<T>(), H1 = herm_zero<T>();
+      // site_dirs() directions per pass: their links and neighbour gauges
+      // are loaded together, then summed
+      constexpr int ND = site_dirs<T>();
+#pragma unroll 1
+      for (int m0 = 0; m0 < 4; m0 += ND) {
+        M2<T> Lf[ND], Gf[ND], Lb[ND], Gb[ND];
 #pragma unroll
-      for (int i = 0; i < 3; i++) H[i] = Herm3<T>{T(0), T(0), T(0), {T(0), T(0)}, {T(0), T(0)}, {T(0), T(0)}};
-      // FP32: the eight loads of the two directions first, then the
-      // arithmetic (16^3x32, one copy: 24.3 -> 11.3 us per half-sweep);
-      // FP64 loads each direction's links where it uses them (its
-      // registers are full: 250 VGPRs)
-      if constexpr (sizeof(T) == 4) {
-        M2<T> Lf[2], Lgf[2], Lb[2], Lgb[2];
-        auto load_dir = [&](int k) {
-          const int mu = 2 * q + k;
-          const uint32_t hfm = fwd_h(g, h, s, mu);
-          const uint32_t hbm = q ? hb[2 + k] : hb[k];
+        for (int k = 0; k < ND; k++) {
+          const int mu = m0 + k;
           Lf[k] = ldm(link_ptr<12>(U, g, p, mu, h));
-          Lgf[k] = ldm(site_ptr<12>(Gc, g, 1 - p, hfm));
+          Gf[k] = ldm(site_ptr<12>(Gc, g, 1 - p, fwd_h(g, h, s, mu)));
+          const uint32_t hbm = back_h(g, h, s, mu);
           Lb[k] = ldm(link_ptr<12>(U, g, 1 - p, mu, hbm));
-          Lgb[k] = ldm(site_ptr<12>(Gc, g, 1 - p, hbm));
-        };
-        load_dir(0);
-        load_dir(1);
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-          site_fwd_field(H, Lf[k], Lgf[k]);
-          site_bwd_field(H, Lb[k], Lgb[k]);
+          Gb[k] = ldm(site_ptr<12>(Gc, g, 1 - p, hbm));
         }
-      } else {
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
-          const int mu = 2 * q + k;
-          const uint32_t hbm = q ? hb[2 + k] : hb[k];
-          site_fwd_field(H, ldm(link_ptr<12>(U, g, p, mu, h)),
-                         ldm(site_ptr<12>(Gc, g, 1 - p, fwd_h(g, h, s, mu))));
-          site_bwd_field(H, ldm(link_ptr<12>(U, g, 1 - p, mu, hbm)), ldm(site_ptr<12>(Gc, g, 1 - p, hbm)));
+        for (int k = 0; k < ND; k++) {
+          site_fwd2(H0, H1, Lf[k], Gf[k]);
+          site_bwd2(H0, H1, Lb[k], Gb[k]);
         }
       }
-#pragma unroll
-      for (int i = 0; i < 3; i++) herm_pair_sum(H[i]);
       const LRef<T> gp = site_ptr<12>(Gc, g, p, h);
-      M2<T> Gx2 = ldm(gp);
+      M2<T> G2 = ldm(gp);
       C<T> Gx[3][3];
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 3; b++) Gx[a][b] = Gx2.r[a][b];
-      row2_of(Gx2, Gx[2]);
-      T res = site_subgroup<T, 0, 1>(H, Gx, omega);
-      res += site_subgroup<T, 1, 2>(H, Gx, omega);
-      res += site_subgroup<T, 0, 2>(H, Gx, omega);
+        for (int b = 0; b < 3; b++) Gx[a][b] = G2.r[a][b];
+      row2_of(G2, Gx[2]);
+      T res = site_subgroup<T, 0, 1>(herm_comb(H0, T(1), H1, T(-1), T(0)), Gx, omega);
+      res += site_subgroup<T, 1, 2>(herm_comb(H0, T(1), H1, T(2), T(-8)), Gx, omega);
+      res += site_subgroup<T, 0, 2>(herm_comb(H0, T(2), H1, T(1), T(-8)), Gx, omega);
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 3; b++) Gx2.r[a][b] = Gx[a][b];
-      if constexpr (sizeof(T) == 4) renorm_rows(Gx2);
-      if (q == 0) {
-        stm(gp, Gx2);
-        rs = (double)res;
-        rm = (double)res;
-        if (p == 1) {
-          // sum_i g_i H^(i) g_i^+ = sum over the site's 8 links of
-          // sum_i |W_ii|^2, in double
+        for (int b = 0; b < 3; b++) G2.r[a][b] = Gx[a][b];
+      if constexpr (sizeof(T) == 4) orthonorm_rows(G2);
+      stm(gp, G2);
+      rs = (double)res;
+      rm = (double)res;
+      if (p == 1) {
+        // sum_i g_i H^(i) g_i^+ (= sum over the site's 8 links of
+        // sum_i |W_ii|^2) with H^(2) = 8 - H0 - H1, in double
+        const Herm3<double> A = herm_d(H0), B = herm_d(H1);
+        C<double> gd[3][3];
 #pragma unroll
-          for (int i = 0; i < 3; i++) {
-            Herm3<double> Hd{(double)H[i].d0, (double)H[i].d1, (double)H[i].d2,
-                             {(double)H[i].o01.re, (double)H[i].o01.im},
-                             {(double)H[i].o02.re, (double)H[i].o02.im},
-                             {(double)H[i].o12.re, (double)H[i].o12.im}};
-            C<double> gi[3], r[3];
+        for (int a = 0; a < 3; a++)
 #pragma unroll
-            for (int b = 0; b < 3; b++) gi[b] = C<double>{(double)Gx[i][b].re, (double)Gx[i][b].im};
-            herm_mul_dag(Hd, gi, r);
-            fs += dot3(gi, r).re;
-          }
-        }
+          for (int b = 0; b < 3; b++) gd[a][b] = C<double>{(double)Gx[a][b].re, (double)Gx[a][b].im};
+        fs = herm_form(A, gd[0]) + herm_form(B, gd[1]) +
+             (8.0 * (cnorm2(gd[2][0]) + cnorm2(gd[2][1]) + cnorm2(gd[2][2])) -
+              herm_form(A, gd[2]) - herm_form(B, gd[2]));
       }
     }
-    wave_partial(part + (size_t)c * pstride, lb * kBlock + (threadIdx.x & ~63u), 2 * g.Vh, fs,
-                 rs, rm);
+    wave_partial(part + (size_t)c * pstride, lb * kBlock + (threadIdx.x & ~63u), g.Vh, fs, rs,
+                 rm);
   }
 }
 

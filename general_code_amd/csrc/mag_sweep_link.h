@@ -270,22 +270,6 @@ void launch_su2_half(T* U, T* G, const Geom& g, int parity, double omega, double
 }
 
 // ================================================================== SU(3)
-// rows 0, 1 scaled by (3 - |row|^2) / 2, the Newton step of 1/|row| at 1
-// (the FP32 site gauges of the site form, which are never rebuilt in FP64
-// during their FP32 phase, lose about 1e-7 of row norm per sweep without it)
-template <typename T>
-__device__ __forceinline__ void renorm_rows(M2<T>& m) {
-#pragma unroll
-  for (int i = 0; i < 2; i++) {
-    T n = T(0);
-#pragma unroll
-    for (int c = 0; c < 3; c++) n += m.r[i][c].re * m.r[i][c].re + m.r[i][c].im * m.r[i][c].im;
-    const T f = T(1.5) - T(0.5) * n;
-#pragma unroll
-    for (int c = 0; c < 3; c++) m.r[i][c] = C<T>{m.r[i][c].re * f, m.r[i][c].im * f};
-  }
-}
-
 #ifndef MAG_SU3_PTR_REMAT
 #define MAG_SU3_PTR_REMAT 1
 #endif
