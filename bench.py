@@ -118,6 +118,43 @@ def measure(G, group, dims, precision, eps, omega, steps, warmup, seed=2024):
     }
 
 
+# SU(3) site form (read-only links, one site gauge per configuration): per
+# site update the 8 links are read (8 x 12 FP32 reals, 384 B) and the
+# site's gauge read and written (2 x 12 x 4 B); the neighbours' gauges are
+# read as well but shared by up to 8 sites through the caches
+SITE3_B_SITE_MIN = 8 * 12 * 4 + 2 * 12 * 4
+
+
+def measure_site3(G, dims, eps, omega, steps, warmup, seed=2024):
+    """Sweep rate of the SU(3) site form, one configuration, FP32 phase (the
+    served form of an SU(3) request), priced two ways: against the link-form
+    roofline of the same site updates (B_site 768, the SU(3) FP32 line of
+    the roofline table) and by the bytes it moves at least (links read once,
+    site gauge read and written)."""
+    V = 1
+    for d in dims:
+        V *= d
+    job = G.MagJob(group=3, dims=dims, precision=G.MIXED, omega=omega, tol=0.0,
+                   max_sweeps=2 * steps + warmup + 4, form=G.FORM_SPIN, copies=1, tol_mixed=0.0,
+                   fp64_warmup=0)
+    try:
+        job.generate(eps, seed)
+        job.sweeps_timed(max(warmup, 1))
+        total_ms = job.sweeps_timed(steps)
+        kern_ms = job.halfsweep_timed(2 * steps)
+    finally:
+        job.close()
+    step_ms = total_ms / steps
+    rate = V / (step_ms * 1e-3)
+    link_peak = PEAK_HBM_GBS * 1e9 / b_site(3, True)
+    moved = (V // 2) * SITE3_B_SITE_MIN / (kern_ms * 1e-3) / 1e9
+    return {"site_updates_per_s": rate, "ms_per_step": step_ms, "kernel_ms": kern_ms,
+            "link_roofline_frac": round(rate / link_peak, 4),
+            "b_site_min": SITE3_B_SITE_MIN, "min_moved_GBs": round(moved, 1),
+            "min_moved_frac": round(moved / PEAK_HBM_GBS, 4),
+            "form": "site form, one site gauge over read-only FP32 links, FP32 phase"}
+
+
 # algorithmic bytes per site of abelian projection + plaquettes + currents:
 # SU(2) angles 4 links x 4 doubles read + 4 angles written, plaquettes 4
 # angles read + 6 written, currents 6 read + 4 written; SU(3) 12-real links
@@ -360,14 +397,17 @@ SU2_WARMUP, SU3_WARMUP = 600, 100
 LINK_TOL_MIXED = 10.0
 
 
-def run_batched(G, specs, max_sweeps, su3_precision, su3_warmup, su3_site=False,
+def run_batched(G, specs, max_sweeps, su3_precision, su3_warmup, su3_site=True,
                 su2_warmup=SU2_WARMUP, su2_precision=None):
     """All configurations in one graph-captured batch: each SU(2)
     configuration is one spin-form copy set (links read once for all its
-    copies; mixed: FP32-rounded links after the FP64 warm-up, spins and
-    arithmetic in FP64), each SU(3) copy a link-form job (mixed: FP32 sweeps
-    after the warm-up); `su3_precision` applies to both unless
-    `su2_precision` is given."""
+    copies; mixed: FP32-rounded links and FP32 spins after the FP64
+    warm-up, field and functional summed in FP64), each SU(3) configuration
+    one site-form copy set (read-only links, one site gauge per copy;
+    su3_site=False: each SU(3) copy a link-form job); `su3_precision`
+    applies to both unless `su2_precision` is given.  Measured on the
+    config-3 workload: SU(3) in the site form 1.57 s, in the link form
+    1.94 s."""
     su2_precision = su3_precision if su2_precision is None else su2_precision
     jobs = []
     for sp in specs:
@@ -433,6 +473,7 @@ def batch_bench(G, max_sweeps=WORKLOAD_CAP, su3_precision=None, su3_warmup=SU3_W
         "batched_configs_per_s": n / t_batch,
         "speedup": t_plain / t_batch,
         "su3_precision": "mixed" if su3_precision == G.MIXED else "fp64",
+        "su3_form": "site form (copy sets over read-only links)",
         "fp64_warmup": {"su2": SU2_WARMUP, "su3": su3_warmup},
         "converged_plain": sum(1 for p in plain if p[1]),
         "converged_batched": sum(1 for b in batched if b[1]),
@@ -470,10 +511,9 @@ def serve(G, reqs, plain, max_sweeps, su3_tol_mixed=1e-11, su3_warmup=SU3_WARMUP
     """Replay the arrival stream against a MagServer.  plain: one job at a
     time, FP64 link form (the reference's one-configuration-per-run use);
     else continuous batching, mixed precision after an FP64 warm-up: SU(2)
-    in the spin form (FP32-rounded links, FP64 spins), SU(3) in the site
-    form (read-only links, one site gauge; su3_site=False: the link form).
-    Measured on this workload: site form 2.0x the plain throughput, link
-    form 1.75x."""
+    in the spin form (FP32-rounded links and FP32 spins, field and
+    functional summed in FP64), SU(3) in the site form (read-only links,
+    one site gauge; su3_site=False: the link form)."""
     def params(r):
         p = G.MagParams(group=r["group"], dims=r["dims"], omega=r["omega"], tol=r["tol"],
                         max_sweeps=max_sweeps)
@@ -596,6 +636,9 @@ def main():
                 "b_site": r["roofline"]["b_site"],
                 "launch_shape": r["shape"],
             }
+        for name, d in [("su3_32^4_mixed_site", dims), ("su3_24^3x48_mixed_site", (24, 24, 24, 48))]:
+            print(f"[bench] {name}", file=sys.stderr, flush=True)
+            extra[name] = measure_site3(G, d, 0.4, 1.6, args.steps, args.warmup)
 
         for name, group, d, e in [("monopoles_su2_32^4", 2, dims, 1.5),
                                   ("monopoles_su3_24^3x48", 3, (24, 24, 24, 48), 1.2)]:

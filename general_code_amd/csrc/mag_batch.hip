@@ -63,8 +63,9 @@ struct ClassRun {
   size_t cap_red = 0;
   JobCtl* ctl_out = nullptr;  // pinned host memory, one entry per red[i]
   size_t cap_out = 0;
-  int nblocks = 0;   // half-sweep grid
-  mag::Shape shape;  // link-form classes: launch shape of the half-sweep
+  int nblocks = 0;     // half-sweep grid
+  int red_blocks = 0;  // per-sweep reduction grid (red_blocks per entry)
+  mag::Shape shape;    // link-form classes: launch shape of the half-sweep
   hipStream_t s = nullptr;
   hipEvent_t ev = nullptr;
   hipGraph_t graph = nullptr;
@@ -133,7 +134,8 @@ BatchJob describe(mag_job* j) {
   b.nw = j->nw;
   b.nparts = 2 * j->nw;
   b.block0 = 0;
-  b.S = (j->spin && j->prm.group == MAG_SU3 && fp32) ? (void*)j->S32 : (void*)j->S;
+  b.red0 = 0;
+  b.S = (j->spin && fp32) ? (void*)j->S32 : (void*)j->S;  // FP32 phase: FP32 spins / gauges
   b.sstride = j->sstride;
   b.pstride = j->pstride;
   b.ncopies = j->spin ? j->ncopies : 0;
@@ -183,7 +185,7 @@ void enqueue_class_replay(const mag_batch* b, int c, hipStream_t s) {
     }
     launch_class_half(b, c, 0, s);
     launch_class_half(b, c, 1, s);
-    mag::launch_reduce_batch(t.d_red, nred, mag::RED_SWEEP, s);
+    mag::launch_reduce_batch(t.d_red, nred, t.red_blocks, s);
   }
   mag::launch_gather_ctl(t.d_red, (int)t.red.size(), t.ctl_out, s);
 }
@@ -209,6 +211,7 @@ int refresh_class(mag_batch* b, int c) {
   t.fp32 = c < kSpinClass && (c % 2) == 0;
   t.h.clear();
   t.red.clear();
+  t.red_blocks = 0;
   t.owner.clear();
   t.cap_jobs = t.jobs;
   t.nblocks = 0;
@@ -225,6 +228,8 @@ int refresh_class(mag_batch* b, int c) {
     t.nblocks += d.nb;
     t.h.push_back(d);
     if (!j->spin) {
+      d.red0 = t.red_blocks;
+      t.red_blocks += mag::red_blocks(d.nparts);
       t.red.push_back(d);
       t.owner.emplace_back(j, 0);
     } else {
@@ -237,6 +242,8 @@ int refresh_class(mag_batch* b, int c) {
         e.hF = j->hF + hs;
         e.hR = j->hR + hs;
         e.hA = j->hA + hs;
+        e.red0 = t.red_blocks;
+        t.red_blocks += mag::red_blocks(e.nparts);
         t.red.push_back(e);
         t.owner.emplace_back(j, k);
       }

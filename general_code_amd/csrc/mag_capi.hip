@@ -100,6 +100,15 @@ namespace {
 
 bool mixed32(const mag_job* j) { return j->prm.precision == MAG_PREC_MIXED && j->phase == 0; }
 
+bool su2_spin(const mag_job* j) { return j->spin && j->prm.group == MAG_SU2; }
+
+// SU(2) spin form: the FP64 spins of copy c, refreshed from the FP32 spins
+// while the job is in its FP32 phase (which sweeps only those)
+double* spins64(mag_job* j, int c) {
+  double* S = j->S + (size_t)c * j->sstride;
+  if (su2_spin(j) && mixed32(j)) mag::launch_spins_to64(j->S32 + (size_t)c * j->sstride, S, j->g, j->st);
+  return S;
+}
 int set_device(const mag_job* j) {
   HIPCHK(hipSetDevice(j->prm.device));
   return MAG_OK;
@@ -129,7 +138,7 @@ void launch_half(mag_job* j, int parity, const mag::HalfOpt& o = mag::HalfOpt{})
                                           j->prm.omega, j->part, j->pstride, j->ctl, j->st);
   } else if (j->spin) {
     if (mixed32(j))
-      mag::launch_su2_copies_half<float>(j->W32, j->S, j->sstride, j->ncopies, j->g, parity,
+      mag::launch_su2_copies_half<float>(j->W32, j->S32, j->sstride, j->ncopies, j->g, parity,
                                          j->prm.omega, j->part, j->pstride, j->ctl, j->st);
     else
       mag::launch_su2_copies_half<double>(j->U64, j->S, j->sstride, j->ncopies, j->g, parity,
@@ -209,7 +218,7 @@ void enqueue_sweep(mag_job* j) {
 // functional partials of the current working links
 void launch_functional(mag_job* j) {
   if (j->spin && j->prm.group == MAG_SU2) {
-    mag::launch_spin_functional(j->U64, j->S, j->g, j->part, nullptr, j->st);
+    mag::launch_spin_functional(j->U64, spins64(j, 0), j->g, j->part, nullptr, j->st);
   } else if (mixed32(j) && !j->spin) {
     if (j->prm.group == MAG_SU2) mag::launch_su2_functional<float>(j->W32, j->g, j->part, j->st);
     else mag::launch_su3_functional<float>(j->W32, j->g, j->part, j->st);
@@ -245,6 +254,17 @@ bool all_converged(const mag_job* j) {
 }
 
 bool su3_site(const mag_job* j) { return j->spin && j->prm.group == MAG_SU3; }
+// FP64 spins (or SU(3) site gauges) of every copy -> their FP32 working copies
+void sites_to32(mag_job* j) {
+  for (int c = 0; c < j->ncopies; c++) {
+    if (su3_site(j))
+      mag::launch_su3_sites_to32(j->S + (size_t)c * j->sstride, j->S32 + (size_t)c * j->sstride,
+                                 j->g, j->st);
+    else if (su2_spin(j))
+      mag::launch_spins_to32(j->S + (size_t)c * j->sstride, j->S32 + (size_t)c * j->sstride, j->g,
+                             j->st);
+  }
+}
 
 // SU(3) site form: the FP64 links of copy c, G_c U G_c^+, into tmp (FP64
 // chunked links); in the FP32 phase the FP64 gauges are refreshed first
@@ -293,10 +313,7 @@ void init_copy(mag_job* j, int c, double* tmp) {
 int start_fp32_phase(mag_job* j) {
   mag::launch_links_f64_to_f32(j->U64, j->W32, j->g, j->prm.group, j->st);
   if (!j->spin) mag::launch_gauge_identity(j->G32, j->g, j->prm.group, j->st);
-  if (su3_site(j))
-    for (int c = 0; c < j->ncopies; c++)
-      mag::launch_su3_sites_to32(j->S + (size_t)c * j->sstride, j->S32 + (size_t)c * j->sstride,
-                                 j->g, j->st);
+  sites_to32(j);
   j->phase = 0;
   j->since_rebuild = 0;
   j->pending = 0;
@@ -330,10 +347,7 @@ int reset_after_upload(mag_job* j) {
   if (mixed && j->phase == 0) {
     mag::launch_links_f64_to_f32(j->U64, j->W32, j->g, j->prm.group, j->st);
     if (!j->spin) mag::launch_gauge_identity(j->G32, j->g, j->prm.group, j->st);
-    if (su3_site(j))
-      for (int c = 0; c < j->ncopies; c++)
-        mag::launch_su3_sites_to32(j->S + (size_t)c * j->sstride,
-                                   j->S32 + (size_t)c * j->sstride, j->g, j->st);
+    sites_to32(j);
   }
   j->fp64_start = mixed ? -1 : 0;
   j->since_rebuild = 0;
@@ -406,8 +420,7 @@ int finalize_mixed_spin(mag_job* j) {
       su3_copy_links(j, c, tmp);  // FP32 gauges -> FP64 (Gram-Schmidt), then the links
       mag::launch_su3_functional<double>(tmp, j->g, part, j->st);
     } else {
-      mag::launch_spin_functional(j->U64, j->S + (size_t)c * j->sstride, j->g, part, nullptr,
-                                  j->st);
+      mag::launch_spin_functional(j->U64, spins64(j, c), j->g, part, nullptr, j->st);
     }
     mag::launch_reduce(part, 2 * j->nf, mag::RED_RESET, j->invF, j->invV, j->ctl + c,
                        j->hF + hs, j->hR + hs, j->hA + hs, nullptr, j->st);
@@ -467,7 +480,7 @@ int run_until_done(mag_job* j) {
 
 void mag::job_copy_links64(mag_job* j, int c, double* soa) {
   if (su3_site(j)) su3_copy_links(j, c, soa);
-  else mag::launch_spin_links(j->U64, j->S + (size_t)c * j->sstride, soa, j->g, j->st);
+  else mag::launch_spin_links(j->U64, spins64(j, c), soa, j->g, j->st);
 }
 int mag::job_read_ctl(mag_job* j) { return read_ctl(j); }
 int mag::job_finalize_mixed(mag_job* j) { return finalize_mixed(j); }
@@ -517,16 +530,16 @@ int mag_job_footprint(const mag_params* p, uint64_t* bytes) {
     b += nc * (uint64_t)V * sr * 8;
     if (p->precision == MAG_PREC_MIXED) {
       b -= (uint64_t)V * nr * 4;  // no accumulated link-form gauge
-      if (p->group == MAG_SU3) b += nc * (uint64_t)V * sr * 4;
+      b += nc * (uint64_t)V * sr * 4;  // FP32 spins / site gauges
     }
     if (p->group == MAG_SU3) b += (uint64_t)V * 4 * nr * 8;  // transient FP64 links
   }
   if (p->track_gauge) b += (uint64_t)V * 4 * 8;  // FP64 site gauges
   const uint64_t lanes = (uint64_t)(V / 2) * (p->group == MAG_SU2 ? 1 : 2);
-  const uint64_t nw = (lanes + 63) / 64;  // partial triples per half-sweep
+  const int nw = (int)((lanes + 63) / 64);  // partial triples per half-sweep (at most)
   b += (uint64_t)4 << 20;  // pitch padding of the chunk arrays (4 KB per array)
   const uint64_t slots = (p->precision == MAG_PREC_MIXED && p->form == MAG_FORM_LINK) ? mag::kSlots + 1 : 1;
-  b += nc * (nw * 2 * 3 * 8 * slots + (uint64_t)p->max_sweeps * 3 * 8 + sizeof(JobCtl)) + 3 * 8;
+  b += nc * (mag::part_stride(nw) * 8 * slots + (uint64_t)p->max_sweeps * 3 * 8 + sizeof(JobCtl)) + 3 * 8;
   *bytes = b;
   return MAG_OK;
 }
@@ -561,7 +574,7 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   j->ncopies = j->spin ? p->copies : 1;
   // SU(2) spins: plain SoA at pitch Vh; SU(3) site gauges: chunk arrays
   j->sstride = p->group == MAG_SU2 ? (size_t)2 * j->g.Vh * 3 : (size_t)2 * j->g.hp * 12;
-  j->pstride = (size_t)2 * j->nw * 3;
+  j->pstride = mag::part_stride(j->nw);
 #define ALLOC(ptr, bytes)                                                          \
   do {                                                                             \
     hipError_t e_ = hipMalloc((void**)&(ptr), (bytes));                            \
@@ -580,7 +593,7 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   if (p->precision == MAG_PREC_MIXED) {
     ALLOC(j->W32, j->nreal * sizeof(float));
     if (!j->spin) ALLOC(j->G32, (size_t)2 * j->g.hp * j->nr * sizeof(float));
-    if (j->spin && p->group == MAG_SU3) ALLOC(j->S32, j->sstride * j->ncopies * sizeof(float));
+    if (j->spin) ALLOC(j->S32, j->sstride * j->ncopies * sizeof(float));
   }
   if (j->spin) ALLOC(j->S, j->sstride * j->ncopies * sizeof(double));
   if (p->track_gauge) ALLOC(j->G64, (size_t)2 * j->g.hp * 4 * sizeof(double));
@@ -946,6 +959,9 @@ int mag_job_set_copy_spins(mag_job* j, int c, const double* spins) {
   hipError_t e = hipMemcpyAsync(stage, spins, n * sizeof(double), hipMemcpyHostToDevice, j->st);
   if (e == hipSuccess) {
     mag::launch_spins_layout(stage, j->S + (size_t)c * j->sstride, j->g, 1, j->st);
+    if (mixed32(j))
+      mag::launch_spins_to32(j->S + (size_t)c * j->sstride, j->S32 + (size_t)c * j->sstride, j->g,
+                             j->st);
     init_copy(j, c, nullptr);
     e = hipStreamSynchronize(j->st);
   }
@@ -962,7 +978,7 @@ int mag_job_get_copy_spins(mag_job* j, int c, double* spins) {
   const size_t n = (size_t)j->V * 3;
   double* stage = nullptr;
   HIPCHK(hipMalloc((void**)&stage, n * sizeof(double)));
-  mag::launch_spins_layout(stage, j->S + (size_t)c * j->sstride, j->g, 0, j->st);
+  mag::launch_spins_layout(stage, spins64(j, c), j->g, 0, j->st);
   hipError_t e = hipMemcpyAsync(spins, stage, n * sizeof(double), hipMemcpyDeviceToHost, j->st);
   hipError_t e2 = hipStreamSynchronize(j->st);
   hipFree(stage);
@@ -1031,7 +1047,7 @@ int mag_job_download_copy_links(mag_job* j, int c, double* links) {
     su3_copy_links(j, c, soa);
     mag::launch_su3_soa_to_aos<double>(soa, stage, j->g, j->st);
   } else {
-    mag::launch_spin_links(j->U64, j->S + (size_t)c * j->sstride, soa, j->g, j->st);
+    mag::launch_spin_links(j->U64, spins64(j, c), soa, j->g, j->st);
     mag::launch_su2_soa_to_aos<double>(soa, stage, j->g, j->st);
   }
   e = hipMemcpyAsync(links, stage, n * sizeof(double), hipMemcpyDeviceToHost, j->st);

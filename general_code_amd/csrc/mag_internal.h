@@ -65,7 +65,9 @@ struct JobCtl {
                    // at the FP64 rebuilds instead); else on the last sweep
   int rb_every;    // sweeps between two FP64 rebuilds (mixed link form)
   double F_ref;    // F of the links of the previous rebuild (0: none yet)
-  double pad;
+  int arrive;      // workgroups of the running per-sweep reduction that have
+                   // folded their share (the last one resets it to 0)
+  int pad;
 };
 
 // reduction kernel modes
@@ -117,9 +119,10 @@ struct Shape {
 // recovered from the swept links (HALF_REBUILD, mag_gauge.h) and writes
 // the functional partials of the rebuilt links to part_rb.  The per-sweep
 // partials of the FP32 phase are kept in kSlots slots (slot s at
-// part + s * 2*nw*3) and reduced together at the end of a period or when
-// the slots are full (launch_reduce_slots / launch_reduce_period); the
-// rebuild partials sit after the slots, at part + kSlots * 2*nw*3.
+// part + s * part_stride(nw)) and reduced together at the end of a period
+// or when the slots are full (launch_reduce_slots / launch_reduce_period);
+// the rebuild partials sit after the slots, at part + kSlots *
+// part_stride(nw).
 enum { HALF_PLAIN = 0, HALF_RENORM = 1, HALF_REBUILD = 2 };
 constexpr int kSlots = 20;
 
@@ -131,7 +134,7 @@ struct HalfOpt {
 };
 
 // One half-sweep of parity `parity`; partial triples go to
-// part + slot*2*3*nw + 3*nw*parity, nw = sweep_waves(g, group); G is the
+// part + slot*part_stride(nw) + 3*nw*parity, nw = sweep_waves(g, group); G is the
 // red site gauge (mixed FP32 phase: accumulated by red PLAIN / RENORM
 // half-sweeps, read by HALF_REBUILD) or nullptr.  The kernels return at
 // once when ctl->done is set or ctl->sweep + slot reaches ctl->max_sweeps.
@@ -159,10 +162,10 @@ void launch_reduce_period(const double* part_rb, int nrb, int n, int rebuild, do
 // every copy not yet done updates its spins, copy c at S + c*sstride; its
 // partial triples (one per wave) go to part + c*pstride + 3*nw*parity
 // (nw = sweep_waves(g, 2)) and it is skipped once ctl[c].done.
-// TL = float: the FP32-rounded links of the mixed form (converted to FP64
-// on load; spins and arithmetic stay FP64).
+// TL = float: the FP32 phase of the mixed form, FP32-rounded links and
+// FP32 spins (field and functional summed in FP64, FP32 omega step).
 template <typename TL>
-void launch_su2_copies_half(const TL* U, double* S, size_t sstride, int ncopies,
+void launch_su2_copies_half(const TL* U, TL* S, size_t sstride, int ncopies,
                             const Geom& g, int parity, double omega, double* part,
                             size_t pstride, const JobCtl* ctl, hipStream_t st);
 // SU(3) site form: links U read-only, site gauge of copy c at G + c*gstride
@@ -180,6 +183,27 @@ void launch_reduce_copies(const double* part, size_t pstride, int nparts, int nc
 
 // per-sweep (or initial) reduction of the partials into the job history
 constexpr int kRedBlock = 1024;  // threads of the one-workgroup reduction
+// The per-sweep reductions (RED_SWEEP) run on two levels in one launch:
+// workgroup b folds partial triples [b kFold, (b+1) kFold) into fold triple
+// b (fixed order), and the workgroup that arrives last (ctl->arrive) sums
+// the fold triples in index order and updates the control block; the fold
+// triples of a job (or copy) sit right after its 2*nw sweep partials.
+// One workgroup reading every partial took up to a fifth of a sweep (2 MB
+// of partials at 48^4); the device-scope fences of the two levels cost a
+// few microseconds, more than they save below kFoldMin fold blocks, where
+// one workgroup reduces everything (red_blocks = 1).  Either way the
+// summation order depends only on nparts.
+constexpr int kFold = kRedBlock;
+constexpr int kFoldMin = 16;
+__host__ __device__ inline int fold_blocks(int nparts) { return (nparts + kFold - 1) / kFold; }
+__host__ __device__ inline int red_blocks(int nparts) {
+  return fold_blocks(nparts) > kFoldMin ? fold_blocks(nparts) : 1;
+}
+// doubles per job (or copy, or FP32-phase sweep slot) of a partial buffer:
+// the 2*nw sweep triples, then their fold triples
+__host__ __device__ inline size_t part_stride(int nw) {
+  return (size_t)3 * (2 * nw + fold_blocks(2 * nw));
+}
 void launch_reduce(const double* part, int nparts, int mode, double invF,
                    double invV, JobCtl* ctl, double* hist_F, double* hist_rmax,
                    double* hist_ravg, double* out3, hipStream_t st);
@@ -204,8 +228,9 @@ struct BatchJob {
   double invV;
   int nb;      // workgroups of one half-sweep of this job (default shape)
   int nw;      // partial triples of one half-sweep (one per wave)
-  int nparts;  // partial triples of a sweep (2*nw)
+  int nparts;  // partial triples of a sweep (2*nw); their fold triples follow them
   int block0;  // first workgroup of this job in a batched half-sweep grid
+  int red0;    // first workgroup of this entry in a batched per-sweep reduction
   // spin form (gauge copies): spins of copy c at S + c*sstride, partials
   // at part + c*pstride, control block ctl + c; ncopies = 0 in the link form
   void* S;  // spins (double) / site gauges (the link type)
@@ -232,8 +257,10 @@ void launch_copies_half_batch(int ncopies, bool fp32, const BatchJob* jobs, int 
 // batched SU(3) site-form half-sweep (FP32 or FP64 class)
 void launch_su3_site_copies_batch(bool fp32, const BatchJob* jobs, int njobs, int nblocks_total,
                                   int parity, hipStream_t st);
-// batched reduction, one workgroup per job: mode RED_SWEEP (end of sweep)
-void launch_reduce_batch(const BatchJob* jobs, int njobs, int mode, hipStream_t st);
+// batched per-sweep reduction (RED_SWEEP) of every entry of a table:
+// red_blocks(nparts) workgroups per entry (red0 ascending, nblocks_total
+// in all)
+void launch_reduce_batch(const BatchJob* jobs, int njobs, int nblocks_total, hipStream_t st);
 // every entry's control block into out (host-visible memory, n entries)
 void launch_gather_ctl(const BatchJob* jobs, int n, JobCtl* out, hipStream_t st);
 
@@ -347,6 +374,10 @@ void launch_spins_layout(const double* aos, double* S, const Geom& g, int to_soa
                          hipStream_t st);
 // every spin of n copies (nreal = n*6*Vh reals) set to e3
 void launch_spins_e3(double* S, size_t nreal, const Geom& g, hipStream_t st);
+// spins of one copy FP64 -> FP32 / FP32 -> FP64, normalised in FP64
+void launch_spins_to32(const double* a, float* b, const Geom& g, hipStream_t st);
+void launch_spins_to64(const float* a, double* b, const Geom& g, hipStream_t st);
+
 // initial spins of gauge copies 0..ncopies-1 of a synthetic configuration
 // (copy c at S + c*sstride): spin of g_c(x) g_0(x)^+, copy 0 exactly e3
 void launch_gen_copy_spins(double* S, size_t sstride, int ncopies, const Geom& g,

@@ -46,7 +46,9 @@ struct mag_job {
   // part + c*pstride, control ctl[c] / ctl_h[c] and history at
   // hF/hR/hA + c*max_sweeps
   // SU(3) in the site form: S holds the FP64 site gauges of the copies
-  // (2 rows, chunked site layout), S32 their FP32 working copies (mixed)
+  // (2 rows, chunked site layout).  S32: the FP32 working copies of the
+  // spins / site gauges that the FP32 phase of the mixed form sweeps (S is
+  // refreshed from them when read during that phase and at its end)
   bool spin = false;
   int ncopies = 1;
   double* S = nullptr;

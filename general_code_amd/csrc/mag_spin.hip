@@ -332,6 +332,34 @@ __global__ __launch_bounds__(kBlock) void k_spin_links(const double* __restrict_
   }
 }
 
+// the spins of one copy (2 Vh sites) FP64 -> FP32 and FP32 -> FP64, each
+// put back on the unit sphere in FP64 (the FP32 phase of the gauge-copy
+// spin form)
+__global__ void k_spins_to32(const double* __restrict__ a, float* __restrict__ b, Geom g) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * g.Vh) return;
+  const int p = (int)(i / g.Vh);
+  const uint32_t h = i % g.Vh;
+  const V3 v = ld_spin(a, g, p, h);
+  const double r = 1.0 / sqrt(v.a1 * v.a1 + v.a2 * v.a2 + v.a3 * v.a3);
+  st_spin(b, g, p, h, V3f{(float)(v.a1 * r), (float)(v.a2 * r), (float)(v.a3 * r)});
+}
+__global__ void k_spins_to64(const float* __restrict__ a, double* __restrict__ b, Geom g) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * g.Vh) return;
+  const int p = (int)(i / g.Vh);
+  const uint32_t h = i % g.Vh;
+  const V3 v = to_d(ld_spin(a, g, p, h));
+  const double r = 1.0 / sqrt(v.a1 * v.a1 + v.a2 * v.a2 + v.a3 * v.a3);
+  st_spin(b, g, p, h, V3{v.a1 * r, v.a2 * r, v.a3 * r});
+}
+void launch_spins_to32(const double* a, float* b, const Geom& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_spins_to32, dim3((2 * g.Vh + 255) / 256), dim3(256), 0, st, a, b, g);
+}
+void launch_spins_to64(const float* a, double* b, const Geom& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_spins_to64, dim3((2 * g.Vh + 255) / 256), dim3(256), 0, st, a, b, g);
+}
+
 void launch_spins_e3(double* S, size_t nreal, const Geom& g, hipStream_t st) {
   if (nreal == 0) return;
   // nreal is a whole number of copies: ncopies * 2 parities * 3 * Vh

@@ -27,6 +27,32 @@ __device__ __forceinline__ void st_spin(double* S, const Geom& g, int p, uint32_
   b[2 * (size_t)g.Vh] = s.a3;
 }
 
+// FP32 spins (the FP32 phase of the gauge-copy spin form)
+struct V3f {
+  float a1, a2, a3;
+};
+__device__ __forceinline__ V3f ld_spin(const float* S, const Geom& g, int p, uint32_t h) {
+  const float* b = S + (size_t)p * 3 * g.Vh + h;
+  return V3f{b[0], b[g.Vh], b[2 * (size_t)g.Vh]};
+}
+__device__ __forceinline__ void st_spin(float* S, const Geom& g, int p, uint32_t h, const V3f& s) {
+  float* b = S + (size_t)p * 3 * g.Vh + h;
+  b[0] = s.a1;
+  b[g.Vh] = s.a2;
+  b[2 * (size_t)g.Vh] = s.a3;
+}
+__device__ __forceinline__ V3 to_d(const V3f& s) { return V3{s.a1, s.a2, s.a3}; }
+// an FP32 spin in FP64, scaled to unit length by the Newton step
+// (3 - |s|^2) / 2 (|s|^2 - 1 is of the order of FP32 rounding, so the
+// result is a unit vector to about 1e-14): the functional is linear in each
+// spin's length, and summed from unscaled FP32 spins it would carry their
+// rounding of length (about 6e-8 per spin) at first order
+__device__ __forceinline__ V3 to_unit_d(const V3f& s) {
+  const V3 v = to_d(s);
+  const double c = fma(-0.5, fma(v.a1, v.a1, fma(v.a2, v.a2, v.a3 * v.a3)), 1.5);
+  return V3{v.a1 * c, v.a2 * c, v.a3 * c};
+}
+
 // h += M(A) b (spin::contribution1, matrix.h:909-926)
 __device__ __forceinline__ void contribution1(V3& h, const Q<double>& A, const V3& b) {
 #pragma clang fp contract(off)
@@ -111,6 +137,51 @@ __device__ __forceinline__ void spin_omega(V3& s, const V3& h, double omega) {
 }
 
 
+// The geodesic omega step of the FP32 phase: the field h is summed in
+// FP64 (from products of FP32 values, exact), its direction and the spin
+// step in FP32 with the hardware reciprocal, square root and reciprocal
+// square root and the short series of mag_trig.h.  The step direction is
+// good to about 1e-7, so the spins settle within about 1e-7 of the
+// maximum of the FP32-rounded problem and its functional within about
+// 1e-14; the FP64 phase that follows polishes from there.  Returns
+// 1 - s.h_hat before the step.
+__device__ __forceinline__ float spin_omega_f(V3f& s, const V3& hd, float omega) {
+  const float h1 = (float)hd.a1, h2 = (float)hd.a2, h3 = (float)hd.a3;
+  const float hn2 = h1 * h1 + h2 * h2 + h3 * h3;
+  if (hn2 == 0.0f) return 0.0f;
+  const float ih = __builtin_amdgcn_rsqf(hn2);
+  const float u1 = h1 * ih, u2 = h2 * ih, u3 = h3 * ih;
+  const float c = s.a1 * u1 + s.a2 * u2 + s.a3 * u3;
+  float t1 = u1 - c * s.a1, t2 = u2 - c * s.a2, t3 = u3 - c * s.a3;
+  float tn2 = t1 * t1 + t2 * t2 + t3 * t3;
+  if (tn2 == 0.0f) {
+    if (c >= 0.0f) return 1.0f - c;
+    // antiparallel: rotate in the plane of s and an axis not along it
+    float e1 = 1.0f, e2 = 0.0f;
+    if (fabsf(s.a1) > 0.9f) { e1 = 0.0f; e2 = 1.0f; }
+    const float d = e1 * s.a1 + e2 * s.a2;
+    t1 = e1 - d * s.a1;
+    t2 = e2 - d * s.a2;
+    t3 = -d * s.a3;
+    tn2 = t1 * t1 + t2 * t2 + t3 * t3;
+  }
+  const float rtn = __builtin_amdgcn_rsqf(tn2);
+  const float tn = tn2 * rtn;
+  // theta = atan2(tn, c) in (0, pi)
+  const float ac = fabsf(c);
+  const bool steep = tn > ac;
+  const float a = trig::atan01_f((steep ? ac : tn) * __builtin_amdgcn_rcpf(steep ? tn : ac));
+  const float th = steep ? 1.57079632679489662f - a : a;
+  const float theta = c >= 0.0f ? th : 3.14159265358979323846f - th;
+  float sa, ca;
+  trig::sincos_0_2pi_f(omega * theta, &sa, &ca);
+  const float st = sa * rtn;
+  float n1 = ca * s.a1 + st * t1, n2 = ca * s.a2 + st * t2, n3 = ca * s.a3 + st * t3;
+  const float r = __builtin_amdgcn_rsqf(n1 * n1 + n2 * n2 + n3 * n3);
+  s = V3f{n1 * r, n2 * r, n3 * r};
+  return 1.0f - c;
+}
+
 // Row 3 of the rotation of the site gauge q: the spin of q, invariant under
 // a diagonal U(1) factor on the left (the link functional is invariant
 // under the same U(1)).
@@ -156,6 +227,17 @@ __device__ __forceinline__ Adj adj_of(const Q<double>& A) {
   return Adj{A5 + 2 * q1,       A5 + 2 * q2,       A5 + 2 * q3,
              2 * (q12 + q03),   2 * (q12 - q03),   2 * (q13 - q02),
              2 * (q13 + q02),   2 * (q23 + q01),   2 * (q23 - q01)};
+}
+// The same entries as fused chains (the FP32 phase: A holds FP32 values,
+// so every product is exact and each entry is rounded about once)
+__device__ __forceinline__ Adj adj_of_fma(const Q<double>& A) {
+  const double b0 = 2 * A.a0, b1 = 2 * A.a1, b2 = 2 * A.a2;
+  const double t23 = fma(A.a2, A.a2, A.a3 * A.a3), t13 = fma(A.a1, A.a1, A.a3 * A.a3),
+               t12 = fma(A.a1, A.a1, A.a2 * A.a2);
+  const double u3 = b0 * A.a3, u2 = b0 * A.a2, u1 = b0 * A.a1;
+  return Adj{fma(A.a0, A.a0, fma(A.a1, A.a1, -t23)), fma(A.a0, A.a0, fma(A.a2, A.a2, -t13)),
+             fma(A.a0, A.a0, fma(A.a3, A.a3, -t12)), fma(b1, A.a2, u3), fma(b1, A.a2, -u3),
+             fma(b1, A.a3, -u2), fma(b1, A.a3, u2), fma(b2, A.a3, u1), fma(b2, A.a3, -u1)};
 }
 // h += M(A) b
 __device__ __forceinline__ void add_M(V3& h, const Adj& M, const V3& b) {

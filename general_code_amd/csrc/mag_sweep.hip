@@ -101,28 +101,87 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce(const double* __restrict__
                                                       int nparts, int mode, double invF,
                                                       double invV, JobCtl* ctl, double* hF,
                                                       double* hR, double* hA, double* out3) {
-  if (mode == RED_SWEEP && ctl->done) return;
   reduce_job(part, nparts, mode, invF, invV, ctl, hF, hR, hA, out3);
 }
 
-// one workgroup per job of the table
+// Per-sweep reduction, workgroup b of the nb = red_blocks(nparts) of one
+// job (or copy); with nb = 1 the one workgroup reduces every partial.
+// Else: fold partial triples [b kFold, (b+1) kFold) into fold
+// triple b at part + 3 nparts (one triple per thread, the fixed-order
+// workgroup tree); the workgroup that arrives last on ctl->arrive (after a
+// device-scope release of its fold triple) acquires the others', reduces
+// the nb fold triples in index order (reduce_job, RED_SWEEP) and resets
+// the counter.  The result depends only on the partials.  Every workgroup
+// reads ctl->done before the last one can write it.
+__device__ void reduce_sweep2(const double* __restrict__ part, int nparts, double invF,
+                              double invV, JobCtl* ctl, double* hF, double* hR, double* hA,
+                              int b) {
+  if (ctl->done) return;  // uniform over the launch
+  const int nb = red_blocks(nparts);
+  if (nb == 1) {  // one workgroup: every partial in index order
+    reduce_job(part, nparts, RED_SWEEP, invF, invV, ctl, hF, hR, hA, nullptr);
+    return;
+  }
+  double* fold = const_cast<double*>(part) + 3 * (size_t)nparts;
+  double a = 0.0, r = 0.0, m = 0.0;
+  const int i = b * kFold + (int)threadIdx.x;
+  if (i < nparts) {
+    a = part[3 * i];
+    r = part[3 * i + 1];
+    m = part[3 * i + 2];
+  }
+  block_reduce3<kRedBlock / 64>(a, r, m);
+  __shared__ int last;
+  if (threadIdx.x == 0) {
+    fold[3 * b] = a;
+    fold[3 * b + 1] = r;
+    fold[3 * b + 2] = m;
+    __threadfence();
+    last = atomicAdd(&ctl->arrive, 1) == nb - 1;
+  }
+  __syncthreads();
+  if (!last) return;  // uniform over the workgroup
+  __threadfence();
+  reduce_job(fold, nb, RED_SWEEP, invF, invV, ctl, hF, hR, hA, nullptr);
+  if (threadIdx.x == 0) ctl->arrive = 0;
+}
+__global__ __launch_bounds__(kRedBlock) void k_reduce_sweep(const double* __restrict__ part,
+                                                            int nparts, double invF, double invV,
+                                                            JobCtl* ctl, double* hF, double* hR,
+                                                            double* hA) {
+  reduce_sweep2(part, nparts, invF, invV, ctl, hF, hR, hA, (int)blockIdx.x);
+}
+
+// per-sweep reduction of a table: red_blocks(nparts) workgroups per entry
+__device__ __forceinline__ int red_entry(const BatchJob* __restrict__ jobs, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].red0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 __global__ __launch_bounds__(kRedBlock) void k_reduce_batch(const BatchJob* __restrict__ jobs,
-                                                            int mode) {
-  const BatchJob& j = jobs[blockIdx.x];
-  if (j.ctl->done) return;
-  reduce_job(j.part, j.nparts, mode, j.invF, j.invV, j.ctl,
-             j.hF, j.hR, j.hA, nullptr);
+                                                            int njobs) {
+  const BatchJob& j = jobs[red_entry(jobs, njobs, (int)blockIdx.x)];
+  reduce_sweep2(j.part, j.nparts, j.invF, j.invV, j.ctl, j.hF, j.hR, j.hA,
+                (int)blockIdx.x - j.red0);
 }
 
 void launch_reduce(const double* part, int nparts, int mode, double invF, double invV,
                    JobCtl* ctl, double* hF, double* hR, double* hA, double* out3,
                    hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedBlock), 0, st, part, nparts, mode, invF, invV,
-                     ctl, hF, hR, hA, out3);
+  if (mode == RED_SWEEP)
+    hipLaunchKernelGGL(k_reduce_sweep, dim3(red_blocks(nparts)), dim3(kRedBlock), 0, st, part,
+                       nparts, invF, invV, ctl, hF, hR, hA);
+  else
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kRedBlock), 0, st, part, nparts, mode, invF, invV,
+                       ctl, hF, hR, hA, out3);
 }
-void launch_reduce_batch(const BatchJob* jobs, int njobs, int mode, hipStream_t st) {
-  if (njobs == 0) return;
-  hipLaunchKernelGGL(k_reduce_batch, dim3(njobs), dim3(kRedBlock), 0, st, jobs, mode);
+void launch_reduce_batch(const BatchJob* jobs, int njobs, int nblocks_total, hipStream_t st) {
+  if (njobs == 0 || nblocks_total == 0) return;
+  hipLaunchKernelGGL(k_reduce_batch, dim3(nblocks_total), dim3(kRedBlock), 0, st, jobs, njobs);
 }
 
 // ------------------------------------------------------ FP32-phase slots
@@ -260,16 +319,34 @@ void launch_half_batch(int group, bool fp32, Shape sh, const BatchJob* jobs, int
 // (mag.cpp:925-953) after the sweep.  A copy's arithmetic and partition do
 // not depend on the other copies, so its trajectory is the same in a set
 // of any size.
+// TL = double: FP64 links and spins, the FP64 omega step.  TL = float (the
+// FP32 phase): FP32-rounded links and FP32 spins; M is formed in FP64 from
+// the FP32 link and h summed in FP64 from the spins scaled to unit length
+// in FP64 (to_unit_d), so h and s'.h are functions of the FP32 state good
+// to about 1e-14 and the functional resolves changes far below FP32
+// rounding; the omega step is FP32 (spin_omega_f).
 template <typename TL>
 __device__ __forceinline__ Q<double> ld_link64(const TL* U, const Geom& g, int p, int mu,
                                                uint32_t h) {
   const Q<TL> q = ldq(link_ptr<4>(U, g, p, mu, h));
   return Q<double>{(double)q.a0, (double)q.a1, (double)q.a2, (double)q.a3};
 }
+__device__ __forceinline__ V3 ld_spin64(const double* S, const Geom& g, int p, uint32_t h) {
+  return ld_spin(S, g, p, h);
+}
+__device__ __forceinline__ V3 ld_spin64(const float* S, const Geom& g, int p, uint32_t h) {
+  return to_unit_d(ld_spin(S, g, p, h));
+}
+
+template <typename TL>
+__device__ __forceinline__ Adj adj_of_t(const Q<double>& A) {
+  if constexpr (sizeof(TL) == 4) return adj_of_fma(A);
+  else return adj_of(A);
+}
 
 template <int K, typename TL>
 __device__ __forceinline__ void su2_copies_block(const TL* __restrict__ U,
-                                                 double* __restrict__ S, size_t sstride,
+                                                 TL* __restrict__ S, size_t sstride,
                                                  const Geom& g, int p, double omega,
                                                  double* __restrict__ part, size_t pstride,
                                                  const JobCtl* __restrict__ ctl, uint32_t lb) {
@@ -294,27 +371,37 @@ __device__ __forceinline__ void su2_copies_block(const TL* __restrict__ U,
 #pragma unroll
     for (int mu = 0; mu < 4; mu++) {
       const uint32_t hf = fwd_h(g, h, s, mu);
-      const Adj Mf = adj_of(ld_link64(U, g, p, mu, h));
+      const Adj Mf = adj_of_t<TL>(ld_link64(U, g, p, mu, h));
 #pragma unroll
       for (int k = 0; k < K; k++)
-        if (act[k]) add_M(H[k], Mf, ld_spin(S + k * sstride, g, 1 - p, hf));
+        if (act[k]) add_M(H[k], Mf, ld_spin64(S + k * sstride, g, 1 - p, hf));
       const uint32_t hb = back_h(g, h, s, mu);
-      const Adj Mb = adj_of(ld_link64(U, g, 1 - p, mu, hb));
+      const Adj Mb = adj_of_t<TL>(ld_link64(U, g, 1 - p, mu, hb));
 #pragma unroll
       for (int k = 0; k < K; k++)
-        if (act[k]) add_MT(H[k], Mb, ld_spin(S + k * sstride, g, 1 - p, hb));
+        if (act[k]) add_MT(H[k], Mb, ld_spin64(S + k * sstride, g, 1 - p, hb));
     }
 #pragma unroll
     for (int k = 0; k < K; k++) {
       if (!act[k]) continue;
-      double* Sk = S + k * sstride;
-      V3 sp = ld_spin(Sk, g, p, h);
+      TL* Sk = S + k * sstride;
       const V3& Hk = H[k];
-      const double hn = sqrt(Hk.a1 * Hk.a1 + Hk.a2 * Hk.a2 + Hk.a3 * Hk.a3);
-      const double d = hn > 0 ? 1 - (sp.a1 * Hk.a1 + sp.a2 * Hk.a2 + sp.a3 * Hk.a3) / hn : 0.0;
-      spin_omega(sp, Hk, omega);
-      st_spin(Sk, g, p, h, sp);
-      if (p == 1) fs[k] = sp.a1 * Hk.a1 + sp.a2 * Hk.a2 + sp.a3 * Hk.a3;
+      double d;
+      V3 sn;
+      if constexpr (sizeof(TL) == 4) {
+        V3f sp = ld_spin(Sk, g, p, h);
+        d = (double)spin_omega_f(sp, Hk, (float)omega);
+        st_spin(Sk, g, p, h, sp);
+        sn = to_unit_d(sp);
+      } else {
+        V3 sp = ld_spin(Sk, g, p, h);
+        const double hn = sqrt(Hk.a1 * Hk.a1 + Hk.a2 * Hk.a2 + Hk.a3 * Hk.a3);
+        d = hn > 0 ? 1 - (sp.a1 * Hk.a1 + sp.a2 * Hk.a2 + sp.a3 * Hk.a3) / hn : 0.0;
+        spin_omega(sp, Hk, omega);
+        st_spin(Sk, g, p, h, sp);
+        sn = sp;
+      }
+      if (p == 1) fs[k] = sn.a1 * Hk.a1 + sn.a2 * Hk.a2 + sn.a3 * Hk.a3;
       ds[k] = d;
       dm[k] = d;
     }
@@ -328,7 +415,7 @@ __device__ __forceinline__ void su2_copies_block(const TL* __restrict__ U,
 
 template <int K, typename TL>
 __global__ __launch_bounds__(kBlock) void k_su2_copies_half(const TL* __restrict__ U,
-                                                            double* __restrict__ S,
+                                                            TL* __restrict__ S,
                                                             size_t sstride, Geom g, int p,
                                                             double omega,
                                                             double* __restrict__ part,
@@ -344,26 +431,26 @@ __global__ __launch_bounds__(kBlock) void k_copies_half_batch(const BatchJob* __
                                                               int njobs, int p) {
   const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
   const BatchJob& j = jobs[batch_job(jobs, njobs, (int)lb)];
-  su2_copies_block<K, TL>((const TL*)j.U, (double*)j.S, j.sstride, j.g, p, j.omega,
+  su2_copies_block<K, TL>((const TL*)j.U, (TL*)j.S, j.sstride, j.g, p, j.omega,
                           j.part + (size_t)3 * j.nw * p, j.pstride, j.ctl,
                           lb - j.block0);
 }
 
-// one workgroup per copy
+// red_blocks(nparts) workgroups per copy
 __global__ __launch_bounds__(kRedBlock) void k_reduce_copies(const double* __restrict__ part,
                                                              size_t pstride, int nparts,
                                                              double invF, double invV,
                                                              JobCtl* ctl, double* hF, double* hR,
                                                              double* hA, size_t hstride) {
-  const int c = blockIdx.x;
-  if (ctl[c].done) return;
-  reduce_job(part + (size_t)c * pstride, nparts, RED_SWEEP, invF, invV, ctl + c,
-             hF + (size_t)c * hstride, hR + (size_t)c * hstride, hA + (size_t)c * hstride,
-             nullptr);
+  const int nb = red_blocks(nparts);
+  const int c = (int)blockIdx.x / nb;
+  reduce_sweep2(part + (size_t)c * pstride, nparts, invF, invV, ctl + c,
+                hF + (size_t)c * hstride, hR + (size_t)c * hstride, hA + (size_t)c * hstride,
+                (int)blockIdx.x % nb);
 }
 
 template <typename TL>
-void launch_su2_copies_half(const TL* U, double* S, size_t sstride, int ncopies, const Geom& g,
+void launch_su2_copies_half(const TL* U, TL* S, size_t sstride, int ncopies, const Geom& g,
                             int parity, double omega, double* part, size_t pstride,
                             const JobCtl* ctl, hipStream_t st) {
   const int nb = nblocks(g);
@@ -386,14 +473,14 @@ void launch_su2_copies_half(const TL* U, double* S, size_t sstride, int ncopies,
 template void launch_su2_copies_half<double>(const double*, double*, size_t, int, const Geom&,
                                              int, double, double*, size_t, const JobCtl*,
                                              hipStream_t);
-template void launch_su2_copies_half<float>(const float*, double*, size_t, int, const Geom&, int,
+template void launch_su2_copies_half<float>(const float*, float*, size_t, int, const Geom&, int,
                                             double, double*, size_t, const JobCtl*,
                                             hipStream_t);
 void launch_reduce_copies(const double* part, size_t pstride, int nparts, int ncopies,
                           double invF, double invV, JobCtl* ctl, double* hF, double* hR,
                           double* hA, size_t hstride, hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce_copies, dim3(ncopies), dim3(kRedBlock), 0, st, part, pstride,
-                     nparts, invF, invV, ctl, hF, hR, hA, hstride);
+  hipLaunchKernelGGL(k_reduce_copies, dim3(ncopies * red_blocks(nparts)), dim3(kRedBlock), 0, st,
+                     part, pstride, nparts, invF, invV, ctl, hF, hR, hA, hstride);
 }
 void launch_copies_half_batch(int ncopies, bool fp32, const BatchJob* jobs, int njobs,
                               int nblocks_total, int parity, hipStream_t st) {

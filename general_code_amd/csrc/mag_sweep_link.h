@@ -256,7 +256,7 @@ template <typename T>
 void launch_su2_half(T* U, T* G, const Geom& g, int parity, double omega, double* part,
                      const JobCtl* ctl, Shape sh, hipStream_t st, const HalfOpt& o) {
   const size_t nw = (size_t)sweep_waves(g, 2);
-  double* pp = part + (size_t)o.slot * 2 * 3 * nw + 3 * nw * parity;
+  double* pp = part + (size_t)o.slot * part_stride((int)nw) + 3 * nw * parity;
   const uint32_t lanes = g.Vh;
   const int kind = half_kind<T>(G, o.mode);
 #define L_(NT, SPT)                                                                        \
@@ -524,7 +524,7 @@ template <typename T>
 void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega, double* part,
                      const JobCtl* ctl, Shape sh, hipStream_t st, const HalfOpt& o) {
   const size_t nw = (size_t)sweep_waves(g, 3);
-  double* pp = part + (size_t)o.slot * 2 * 3 * nw + 3 * nw * parity;
+  double* pp = part + (size_t)o.slot * part_stride((int)nw) + 3 * nw * parity;
   const uint32_t lanes = (uint32_t)sweep_lanes(g, 3);
   const int kind = half_kind<T>(G, o.mode);
 #define L_(NT, SPT)                                                                        \

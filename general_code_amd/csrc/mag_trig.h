@@ -141,5 +141,29 @@ __device__ __forceinline__ void sincos_0_pi_f(float a, float* s, float* c) {
   }
 }
 
+// the same for an angle in [0, 2 pi] (q <= 4; q * (pi/2)_hi exact)
+__device__ __forceinline__ void sincos_0_2pi_f(float a, float* s, float* c) {
+  const int q = (int)(a * 0.63661977f + 0.5f);
+  const float qd = (float)q;
+  const float r = (a - qd * 1.5707963705062866f) - qd * -4.371139006309477e-08f;
+  const float z = r * r;
+  float ps = 1.0f / 362880.0f;
+  ps = ps * z + -1.0f / 5040.0f;
+  ps = ps * z + 1.0f / 120.0f;
+  ps = ps * z + -1.0f / 6.0f;
+  const float sr = r + (r * z) * ps;
+  float pc = 1.0f / 40320.0f;
+  pc = pc * z + -1.0f / 720.0f;
+  pc = pc * z + 1.0f / 24.0f;
+  pc = pc * z + -0.5f;
+  const float cr = 1.0f + z * pc;
+  switch (q & 3) {
+    case 0: *s = sr; *c = cr; break;
+    case 1: *s = cr; *c = -sr; break;
+    case 2: *s = -sr; *c = -cr; break;
+    default: *s = -cr; *c = sr; break;
+  }
+}
+
 }  // namespace trig
 }  // namespace mag

@@ -92,12 +92,15 @@ typedef struct mag_params {
 } mag_params;
 
 /* Link form: the links carry the gauge transformation and are rewritten by
- * every half-sweep.  Spin form (SU(2), FP64): the links stay read-only and
- * each gauge copy is a field of unit 3-vectors s(x) (the reference's spin
- * form, mag.cpp:569-626, 1375-1512); all copies of one configuration are
- * swept together, so the links are read once per half-sweep for all of
- * them.  Both forms run the same checkerboard omega step and give the same
- * functional per sweep to rounding. */
+ * every half-sweep.  Spin form (SU(2)): the links stay read-only and each
+ * gauge copy is a field of unit 3-vectors s(x) (the reference's spin form,
+ * mag.cpp:569-626, 1375-1512); SU(3): each copy is a site gauge field G(x)
+ * (the site form).  All copies of one configuration are swept together,
+ * so the links are read once per half-sweep for all of them.  Both forms
+ * run the same checkerboard omega step and give the same functional per
+ * sweep to rounding.  In mixed precision the spin form's FP32 phase sweeps
+ * FP32 spins over FP32-rounded links (field and functional summed in
+ * FP64); the FP64 spins follow at the end of the phase. */
 #define MAG_FORM_LINK 0
 #define MAG_FORM_SPIN 1
 #define MAG_MAX_COPIES 8
