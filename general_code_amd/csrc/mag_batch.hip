@@ -2,7 +2,8 @@
 //
 // The active members are split into classes: SU(2) / SU(3) x FP32 working
 // links of the mixed phase / FP64 links, spin-form gauge-copy sets by copy
-// count and link precision, SU(3) site-form sets by precision.  One sweep of
+// count and link precision, SU(3) site-form sets by precision (FP32: and
+// copy count).  One sweep of
 // a class is
 //   for parity 0, 1: one half-sweep launch whose grid concatenates the
 //                    member jobs' grids
@@ -42,10 +43,11 @@ namespace {
 // 0..3: link form (SU(2)/SU(3) x FP32/FP64); kSpinClass + K - 1 (+
 // MAG_MAX_COPIES with FP32 links): spin-form copy sets of K copies (one
 // kernel instance per K and link type)
-// kSiteClass3 (+1 with FP32 links): SU(3) site-form copy sets
+// kSiteClass3: SU(3) site-form copy sets with FP64 links (any copy count);
+// kSiteClass3 + K: FP32 sets of K copies (one kernel instance per K)
 constexpr int kSpinClass = 4;
 constexpr int kSiteClass3 = kSpinClass + 2 * MAG_MAX_COPIES;
-constexpr int kClasses = kSiteClass3 + 2;
+constexpr int kClasses = kSiteClass3 + 1 + MAG_MAX_COPIES;
 
 struct ClassRun {
   int group = 2;
@@ -108,7 +110,7 @@ struct mag_batch {
 namespace {
 
 int class_of(const mag_job* j) {
-  if (j->spin && j->prm.group == MAG_SU3) return kSiteClass3 + (mag::job_mixed32(j) ? 1 : 0);
+  if (j->spin && j->prm.group == MAG_SU3) return kSiteClass3 + (mag::job_mixed32(j) ? j->ncopies : 0);
   if (j->spin) return kSpinClass + j->ncopies - 1 + (mag::job_mixed32(j) ? MAG_MAX_COPIES : 0);
   return (j->prm.group == MAG_SU2 ? 0 : 2) + (mag::job_mixed32(j) ? 0 : 1);
 }
@@ -155,8 +157,8 @@ void launch_class_half(const mag_batch* b, int c, int p, hipStream_t st) {
   if (c < kSpinClass) {
     mag::launch_half_batch(t.group, t.fp32, t.shape, t.d, (int)t.h.size(), t.nblocks, p, st);
   } else if (c >= kSiteClass3) {
-    mag::launch_su3_site_copies_batch(c == kSiteClass3 + 1, t.d, (int)t.h.size(), t.nblocks, p,
-                                      st);
+    mag::launch_su3_site_copies_batch(c - kSiteClass3, c > kSiteClass3, t.d, (int)t.h.size(),
+                                      t.nblocks, p, st);
   } else {
     const int k = (c - kSpinClass) % MAG_MAX_COPIES + 1;
     const bool fp32 = c >= kSpinClass + MAG_MAX_COPIES;

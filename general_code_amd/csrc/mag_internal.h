@@ -254,9 +254,10 @@ void launch_reduce_period_batch(const BatchJob* jobs, int njobs, int n, int rebu
 // all hold ncopies copies
 void launch_copies_half_batch(int ncopies, bool fp32, const BatchJob* jobs, int njobs,
                               int nblocks_total, int parity, hipStream_t st);
-// batched SU(3) site-form half-sweep (FP32 or FP64 class)
-void launch_su3_site_copies_batch(bool fp32, const BatchJob* jobs, int njobs, int nblocks_total,
-                                  int parity, hipStream_t st);
+// batched SU(3) site-form half-sweep: the FP64 class (any copy counts) or
+// the FP32 class of sets of ncopies copies
+void launch_su3_site_copies_batch(int ncopies, bool fp32, const BatchJob* jobs, int njobs,
+                                  int nblocks_total, int parity, hipStream_t st);
 // batched per-sweep reduction (RED_SWEEP) of every entry of a table:
 // red_blocks(nparts) workgroups per entry (red0 ascending, nblocks_total
 // in all)

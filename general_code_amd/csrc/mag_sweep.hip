@@ -679,12 +679,10 @@ __device__ __forceinline__ void row2_fma(const M2<T>& M, C<T>* z) {
 }
 
 // x_i = U conj(g_i), i = 0, 1 (g_i the stored rows of the forward
-// neighbour's gauge; U's row 2 from its rows 0, 1): H_i += x_i x_i^+
+// neighbour's gauge; u2 the row 2 of U): H_i += x_i x_i^+
 template <typename T>
 __device__ __forceinline__ void site_fwd2(Herm3<T>& H0, Herm3<T>& H1, const M2<T>& U,
-                                          const M2<T>& Gf) {
-  C<T> u2[3];
-  row2_fma(U, u2);
+                                          const C<T>* u2, const M2<T>& Gf) {
 #pragma unroll
   for (int i = 0; i < 2; i++) {
     const C<T>* gi = Gf.r[i];
@@ -692,12 +690,10 @@ __device__ __forceinline__ void site_fwd2(Herm3<T>& H0, Herm3<T>& H1, const M2<T
   }
 }
 // y_i = g_i U, i = 0, 1 (g_i the stored rows of the backward neighbour's
-// gauge): H_i += conj(y_i) y_i^T
+// gauge; u2 the row 2 of U): H_i += conj(y_i) y_i^T
 template <typename T>
 __device__ __forceinline__ void site_bwd2(Herm3<T>& H0, Herm3<T>& H1, const M2<T>& U,
-                                          const M2<T>& Gb) {
-  C<T> u2[3];
-  row2_fma(U, u2);
+                                          const C<T>* u2, const M2<T>& Gb) {
 #pragma unroll
   for (int i = 0; i < 2; i++) {
     const C<T>* gi = Gb.r[i];
@@ -732,14 +728,49 @@ __device__ __forceinline__ void orthonorm_rows(M2<T>& m) {
   scale_row(m.r[1]);
 }
 
-// directions whose 4 links and 4 neighbour gauges are loaded together:
-// FP32 two (122 VGPRs, 4 waves per SIMD; all four: 172 VGPRs), FP64 one
-// (194 VGPRs; two: 256)
+// The end of a site's update for one copy: its gauge G (at gp) rotated in
+// the three subgroups by the fields of H0, H1 and stored; residual and (on
+// the black half) the site's functional sum_i g_i H^(i) g_i^+ (= sum over
+// its 8 links of sum_i |W_ii|^2, H^(2) = 8 - H0 - H1), in double
 template <typename T>
-constexpr int site_dirs() {
-  return sizeof(T) == 4 ? 2 : 1;
+__device__ __forceinline__ void site_finish(const Herm3<T>& H0, const Herm3<T>& H1,
+                                            LRef<T> gp, int p, T omega, double& fs, double& rs,
+                                            double& rm) {
+  M2<T> G2 = ldm(gp);
+  C<T> Gx[3][3];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) Gx[a][b] = G2.r[a][b];
+  row2_of(G2, Gx[2]);
+  T res = site_subgroup<T, 0, 1>(herm_comb(H0, T(1), H1, T(-1), T(0)), Gx, omega);
+  res += site_subgroup<T, 1, 2>(herm_comb(H0, T(1), H1, T(2), T(-8)), Gx, omega);
+  res += site_subgroup<T, 0, 2>(herm_comb(H0, T(2), H1, T(1), T(-8)), Gx, omega);
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) G2.r[a][b] = Gx[a][b];
+  if constexpr (sizeof(T) == 4) orthonorm_rows(G2);
+  stm(gp, G2);
+  rs = (double)res;
+  rm = (double)res;
+  fs = 0.0;
+  if (p == 1) {
+    const Herm3<double> A = herm_d(H0), B = herm_d(H1);
+    C<double> gd[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) gd[a][b] = C<double>{(double)Gx[a][b].re, (double)Gx[a][b].im};
+    fs = herm_form(A, gd[0]) + herm_form(B, gd[1]) +
+         (8.0 * (cnorm2(gd[2][0]) + cnorm2(gd[2][1]) + cnorm2(gd[2][2])) -
+          herm_form(A, gd[2]) - herm_form(B, gd[2]));
+  }
 }
 
+// FP64: the copies one after the other, each reading the links again (the
+// FP64 field matrices of every copy would not fit in registers); the FP64
+// phases are the short warm-up and polish.
 template <typename T>
 __device__ __forceinline__ void su3_site_copies_block(const T* __restrict__ U,
                                                       T* __restrict__ G, size_t gstride,
@@ -760,63 +791,103 @@ __device__ __forceinline__ void su3_site_copies_block(const T* __restrict__ U,
     double fs = 0.0, rs = 0.0, rm = 0.0;
     if (in) {
       Herm3<T> H0 = herm_zero<T>(), H1 = herm_zero<T>();
-      // site_dirs() directions per pass: their links and neighbour gauges
-      // are loaded together, then summed
-      constexpr int ND = site_dirs<T>();
 #pragma unroll 1
-      for (int m0 = 0; m0 < 4; m0 += ND) {
-        M2<T> Lf[ND], Gf[ND], Lb[ND], Gb[ND];
-#pragma unroll
-        for (int k = 0; k < ND; k++) {
-          const int mu = m0 + k;
-          Lf[k] = ldm(link_ptr<12>(U, g, p, mu, h));
-          Gf[k] = ldm(site_ptr<12>(Gc, g, 1 - p, fwd_h(g, h, s, mu)));
-          const uint32_t hbm = back_h(g, h, s, mu);
-          Lb[k] = ldm(link_ptr<12>(U, g, 1 - p, mu, hbm));
-          Gb[k] = ldm(site_ptr<12>(Gc, g, 1 - p, hbm));
-        }
-#pragma unroll
-        for (int k = 0; k < ND; k++) {
-          site_fwd2(H0, H1, Lf[k], Gf[k]);
-          site_bwd2(H0, H1, Lb[k], Gb[k]);
-        }
+      for (int mu = 0; mu < 4; mu++) {
+        const uint32_t hbm = back_h(g, h, s, mu);
+        const M2<T> Lf = ldm(link_ptr<12>(U, g, p, mu, h));
+        const M2<T> Gf = ldm(site_ptr<12>(Gc, g, 1 - p, fwd_h(g, h, s, mu)));
+        const M2<T> Lb = ldm(link_ptr<12>(U, g, 1 - p, mu, hbm));
+        const M2<T> Gb = ldm(site_ptr<12>(Gc, g, 1 - p, hbm));
+        C<T> u2[3];
+        row2_fma(Lf, u2);
+        site_fwd2(H0, H1, Lf, u2, Gf);
+        row2_fma(Lb, u2);
+        site_bwd2(H0, H1, Lb, u2, Gb);
       }
-      const LRef<T> gp = site_ptr<12>(Gc, g, p, h);
-      M2<T> G2 = ldm(gp);
-      C<T> Gx[3][3];
-#pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++) Gx[a][b] = G2.r[a][b];
-      row2_of(G2, Gx[2]);
-      T res = site_subgroup<T, 0, 1>(herm_comb(H0, T(1), H1, T(-1), T(0)), Gx, omega);
-      res += site_subgroup<T, 1, 2>(herm_comb(H0, T(1), H1, T(2), T(-8)), Gx, omega);
-      res += site_subgroup<T, 0, 2>(herm_comb(H0, T(2), H1, T(1), T(-8)), Gx, omega);
-#pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++) G2.r[a][b] = Gx[a][b];
-      if constexpr (sizeof(T) == 4) orthonorm_rows(G2);
-      stm(gp, G2);
-      rs = (double)res;
-      rm = (double)res;
-      if (p == 1) {
-        // sum_i g_i H^(i) g_i^+ (= sum over the site's 8 links of
-        // sum_i |W_ii|^2) with H^(2) = 8 - H0 - H1, in double
-        const Herm3<double> A = herm_d(H0), B = herm_d(H1);
-        C<double> gd[3][3];
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-          for (int b = 0; b < 3; b++) gd[a][b] = C<double>{(double)Gx[a][b].re, (double)Gx[a][b].im};
-        fs = herm_form(A, gd[0]) + herm_form(B, gd[1]) +
-             (8.0 * (cnorm2(gd[2][0]) + cnorm2(gd[2][1]) + cnorm2(gd[2][2])) -
-              herm_form(A, gd[2]) - herm_form(B, gd[2]));
-      }
+      site_finish(H0, H1, site_ptr<12>(Gc, g, p, h), p, omega, fs, rs, rm);
     }
     wave_partial(part + (size_t)c * pstride, lb * kBlock + (threadIdx.x & ~63u), g.Vh, fs, rs,
                  rm);
   }
+}
+
+// FP32 with the K copies of the set inside the link loop: each link is
+// loaded, and its row 2 formed, once for all copies, which then add their
+// terms with their own neighbour gauges (with the copies outside, four
+// copies at 32^4 read 4.1 times the links of one: the 4 MB L2 of an XCD
+// does not keep them from one copy to the next).  K = 1 loads two
+// directions per pass (100 VGPRs, 5 waves per SIMD), more copies one.  A
+// copy's operations and their order are the same for every K.  The link
+// loads carry no nontemporal hint: it cost 10% at 32^4 (the next
+// half-sweep reads the links again from the MALL) and gained 3% at 48^4.
+template <int K>
+__device__ __forceinline__ void su3_site_set_block(const float* __restrict__ U,
+                                                   float* __restrict__ G, size_t gstride,
+                                                   const Geom& g, int p, float omega,
+                                                   double* __restrict__ part, size_t pstride,
+                                                   const JobCtl* __restrict__ ctl, uint32_t lb) {
+  bool act[K];
+  bool live = false;
+#pragma unroll
+  for (int c = 0; c < K; c++) {
+    act[c] = !ctl[c].done;  // uniform over the workgroup
+    live |= act[c];
+  }
+  if (!live) return;
+  const uint32_t h = lb * kBlock + threadIdx.x;
+  const bool in = h < g.Vh;
+  double fs[K], rs[K], rm[K];
+#pragma unroll
+  for (int c = 0; c < K; c++) fs[c] = rs[c] = rm[c] = 0.0;
+  if (in) {
+    const SiteIdx s = decode(g, h, p);
+    Herm3<float> H0[K], H1[K];
+#pragma unroll
+    for (int c = 0; c < K; c++) H0[c] = H1[c] = herm_zero<float>();
+    constexpr int ND = K == 1 ? 2 : 1;
+#pragma unroll 1
+    for (int m0 = 0; m0 < 4; m0 += ND) {
+      M2<float> Lf[ND], Lb[ND];
+      uint32_t hf[ND], hb[ND];
+#pragma unroll
+      for (int k = 0; k < ND; k++) {
+        const int mu = m0 + k;
+        hf[k] = fwd_h(g, h, s, mu);
+        hb[k] = back_h(g, h, s, mu);
+        Lf[k] = ldm(link_ptr<12>(U, g, p, mu, h));
+        Lb[k] = ldm(link_ptr<12>(U, g, 1 - p, mu, hb[k]));
+      }
+#pragma unroll
+      for (int c = 0; c < K; c++) {
+        if (!act[c]) continue;
+        const float* Gc = G + (size_t)c * gstride;
+        M2<float> Gf[ND], Gb[ND];
+#pragma unroll
+        for (int k = 0; k < ND; k++) {
+          Gf[k] = ldm(site_ptr<12>(Gc, g, 1 - p, hf[k]));
+          Gb[k] = ldm(site_ptr<12>(Gc, g, 1 - p, hb[k]));
+        }
+#pragma unroll
+        for (int k = 0; k < ND; k++) {
+          C<float> u2[3];
+          row2_fma(Lf[k], u2);
+          site_fwd2(H0[c], H1[c], Lf[k], u2, Gf[k]);
+          row2_fma(Lb[k], u2);
+          site_bwd2(H0[c], H1[c], Lb[k], u2, Gb[k]);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < K; c++)
+      if (act[c])
+        site_finish(H0[c], H1[c], site_ptr<12>(G + (size_t)c * gstride, g, p, h), p, omega, fs[c],
+                    rs[c], rm[c]);
+  }
+#pragma unroll
+  for (int c = 0; c < K; c++)
+    if (act[c])  // uniform
+      wave_partial(part + (size_t)c * pstride, lb * kBlock + (threadIdx.x & ~63u), g.Vh, fs[c],
+                   rs[c], rm[c]);
 }
 
 template <typename T>
@@ -829,15 +900,33 @@ __global__ __launch_bounds__(kBlock) void k_su3_site_copies(const T* __restrict_
   su3_site_copies_block<T>(U, G, gstride, ncopies, g, p, omega, part, pstride, ctl,
                            xcd_block(blockIdx.x, gridDim.x));
 }
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_su3_site_set(const float* __restrict__ U,
+                                                         float* __restrict__ G, size_t gstride,
+                                                         Geom g, int p, float omega,
+                                                         double* __restrict__ part,
+                                                         size_t pstride,
+                                                         const JobCtl* __restrict__ ctl) {
+  su3_site_set_block<K>(U, G, gstride, g, p, omega, part, pstride, ctl,
+                        xcd_block(blockIdx.x, gridDim.x));
+}
 
-template <typename T>
+// FP64 table: any copy counts; FP32 table: sets of K copies
 __global__ __launch_bounds__(kBlock) void k_su3_site_copies_batch(
     const BatchJob* __restrict__ jobs, int njobs, int p) {
   const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
   const BatchJob& j = jobs[batch_job(jobs, njobs, (int)lb)];
-  su3_site_copies_block<T>((const T*)j.U, (T*)j.S, j.sstride, j.ncopies, j.g, p, (T)j.omega,
-                           j.part + (size_t)3 * j.nw * p, j.pstride, j.ctl,
-                           lb - j.block0);
+  su3_site_copies_block<double>((const double*)j.U, (double*)j.S, j.sstride, j.ncopies, j.g, p,
+                                j.omega, j.part + (size_t)3 * j.nw * p, j.pstride, j.ctl,
+                                lb - j.block0);
+}
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_su3_site_set_batch(const BatchJob* __restrict__ jobs,
+                                                               int njobs, int p) {
+  const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const BatchJob& j = jobs[batch_job(jobs, njobs, (int)lb)];
+  su3_site_set_block<K>((const float*)j.U, (float*)j.S, j.sstride, j.g, p, (float)j.omega,
+                        j.part + (size_t)3 * j.nw * p, j.pstride, j.ctl, lb - j.block0);
 }
 
 template <typename T>
@@ -846,8 +935,26 @@ void launch_su3_site_copies(const T* U, T* G, size_t gstride, int ncopies, const
                             const JobCtl* ctl, hipStream_t st) {
   const int nb = (int)((site3_lanes(g) + kBlock - 1) / kBlock);
   const size_t nw = (site3_lanes(g) + 63) / 64;
-  hipLaunchKernelGGL((k_su3_site_copies<T>), dim3(nb), dim3(kBlock), 0, st, U, G, gstride,
-                     ncopies, g, parity, (T)omega, part + 3 * nw * parity, pstride, ctl);
+  double* pp = part + 3 * nw * parity;
+  if constexpr (sizeof(T) == 4) {
+#define L_(K)                                                                                  \
+  hipLaunchKernelGGL((k_su3_site_set<K>), dim3(nb), dim3(kBlock), 0, st, U, G, gstride, g,     \
+                     parity, (float)omega, pp, pstride, ctl)
+    switch (ncopies) {
+      case 1: L_(1); break;
+      case 2: L_(2); break;
+      case 3: L_(3); break;
+      case 4: L_(4); break;
+      case 5: L_(5); break;
+      case 6: L_(6); break;
+      case 7: L_(7); break;
+      default: L_(8); break;
+    }
+#undef L_
+  } else {
+    hipLaunchKernelGGL((k_su3_site_copies<T>), dim3(nb), dim3(kBlock), 0, st, U, G, gstride,
+                       ncopies, g, parity, (T)omega, pp, pstride, ctl);
+  }
 }
 template void launch_su3_site_copies<float>(const float*, float*, size_t, int, const Geom&, int,
                                             double, double*, size_t, const JobCtl*,
@@ -856,15 +963,28 @@ template void launch_su3_site_copies<double>(const double*, double*, size_t, int
                                              int, double, double*, size_t, const JobCtl*,
                                              hipStream_t);
 
-void launch_su3_site_copies_batch(bool fp32, const BatchJob* jobs, int njobs, int nblocks_total,
-                                  int parity, hipStream_t st) {
+void launch_su3_site_copies_batch(int ncopies, bool fp32, const BatchJob* jobs, int njobs,
+                                  int nblocks_total, int parity, hipStream_t st) {
   if (njobs == 0 || nblocks_total == 0) return;
-  if (fp32)
-    hipLaunchKernelGGL((k_su3_site_copies_batch<float>), dim3(nblocks_total), dim3(kBlock), 0,
-                       st, jobs, njobs, parity);
-  else
-    hipLaunchKernelGGL((k_su3_site_copies_batch<double>), dim3(nblocks_total), dim3(kBlock), 0,
-                       st, jobs, njobs, parity);
+  if (!fp32) {
+    hipLaunchKernelGGL(k_su3_site_copies_batch, dim3(nblocks_total), dim3(kBlock), 0, st, jobs,
+                       njobs, parity);
+    return;
+  }
+#define L_(K)                                                                                  \
+  hipLaunchKernelGGL((k_su3_site_set_batch<K>), dim3(nblocks_total), dim3(kBlock), 0, st, jobs, \
+                     njobs, parity)
+  switch (ncopies) {
+    case 1: L_(1); break;
+    case 2: L_(2); break;
+    case 3: L_(3); break;
+    case 4: L_(4); break;
+    case 5: L_(5); break;
+    case 6: L_(6); break;
+    case 7: L_(7); break;
+    default: L_(8); break;
+  }
+#undef L_
 }
 
 }  // namespace mag

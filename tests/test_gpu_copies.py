@@ -198,6 +198,35 @@ def test_spin_form_argument_errors(mag):
             j.generate(0.3, 1, copy=1)
 
 
+def test_spin_form_fp32_phase_reads(mag, oracle):
+    """The FP32 phase of the spin form sweeps FP32 spins.  Spins and copy
+    links read during that phase come from them: the spins (unit length in
+    FP64) give the oracle's spin functional on the FP32-rounded links equal
+    to the job's last recorded functional, and the copy links equal the
+    oracle's gauge_tranformation_spins of the FP64 links with those spins
+    (up to the diagonal U(1) gauge: the a0^2 + a3^2 of every link)."""
+    dims = (8, 8, 8, 8)
+    n = 30
+    with mag.MagJob(group=mag.SU2, dims=dims, form=mag.FORM_SPIN, copies=2, precision=mag.MIXED,
+                    omega=OMEGA, tol=0.0, max_sweeps=4 * n, tol_mixed=0.0,
+                    fp64_warmup=0) as j:
+        j.generate(EPS, 91)
+        j.sweeps(n)
+        assert j.fp64_start() < 0  # still in the FP32 phase
+        for c in range(2):
+            F = j.copy_history(c)[0]
+            assert len(F) == n
+            s = j.copy_spins(c)
+            got = j.download_copy(c)
+            links = oracle.gen_su2_config(dims, EPS, 91)
+            assert np.max(np.abs(np.linalg.norm(s, axis=1) - 1)) < 1e-14
+            links32 = links.astype(np.float32).astype(np.float64)
+            Fo = oracle.su2_functional_spin(links32, s, dims)
+            assert abs(Fo - F[-1]) / abs(F[-1]) < 1e-12, c
+            ref = oracle.su2_gauge_transform_spins(links, s, dims)
+            assert np.max(np.abs(_u1_invariant(got) - _u1_invariant(ref))) < 1e-12, c
+
+
 def test_spin_form_mixed_reaches_fp64_maximum(mag):
     """Mixed spin form: FP64 warm-up, then sweeps that read FP32-rounded
     links (spins and arithmetic FP64), then FP64 polish.  Every copy ends at
