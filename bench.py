@@ -396,6 +396,15 @@ SU2_WARMUP, SU3_WARMUP = 600, 100
 # F above the plain result); with 10 tol max dF/F 2.3e-13, same sweep counts.
 LINK_TOL_MIXED = 10.0
 
+# SU(3) site form: its FP32 phase tests F, evaluated in FP64 every 10
+# sweeps, on the mean change per sweep since the previous evaluation and
+# hands over to FP64 at 2 tol; the FP64 sweeps then stop on the plain run's
+# rule.  (A lower hand-over runs the FP32 phase past the plain stopping
+# point; a higher one leaves longer FP64 polishes on slowly converging
+# lattices: 24^3x48 at omega 1.6 contracts the change per sweep by only
+# 0.9992.)
+SITE_TOL_MIXED = 2.0
+
 
 def run_batched(G, specs, max_sweeps, su3_precision, su3_warmup, su3_site=True,
                 su2_warmup=SU2_WARMUP, su2_precision=None):
@@ -421,7 +430,8 @@ def run_batched(G, specs, max_sweeps, su3_precision, su3_warmup, su3_site=True,
         elif su3_site:
             j = G.MagJob(group=3, dims=sp["dims"], omega=sp["omega"], tol=sp["tol"],
                          max_sweeps=max_sweeps, form=G.FORM_SPIN, copies=sp["copies"],
-                         precision=su3_precision, tol_mixed=1e-11, fp64_warmup=su3_warmup)
+                         precision=su3_precision, tol_mixed=SITE_TOL_MIXED * sp["tol"],
+                         fp64_warmup=su3_warmup)
             j.generate(sp["eps"], sp["seed"])
             jobs.append((j, list(range(sp["copies"]))))
         else:
@@ -506,7 +516,7 @@ def serve_workload(n=40, rate=20.0, seed=4):
     return reqs
 
 
-def serve(G, reqs, plain, max_sweeps, su3_tol_mixed=1e-11, su3_warmup=SU3_WARMUP,
+def serve(G, reqs, plain, max_sweeps, su3_tol_mixed=SITE_TOL_MIXED * 1e-11, su3_warmup=SU3_WARMUP,
           su2_warmup=SU2_WARMUP, su3_site=True):
     """Replay the arrival stream against a MagServer.  plain: one job at a
     time, FP64 link form (the reference's one-configuration-per-run use);

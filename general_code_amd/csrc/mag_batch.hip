@@ -188,6 +188,10 @@ void enqueue_class_replay(const mag_batch* b, int c, hipStream_t s) {
     launch_class_half(b, c, 0, s);
     launch_class_half(b, c, 1, s);
     mag::launch_reduce_batch(t.d_red, nred, t.red_blocks, s);
+    // FP32 SU(3) site-form classes: the stopping test on F in FP64 every
+    // reunit sweeps, as a job alone (mag_capi.hip enqueue_sweep)
+    if (c > kSiteClass3 && i % b->reunit == b->reunit - 1)
+      mag::launch_su3_site_check_batch(t.d, (int)t.h.size(), t.nblocks, t.d_red, nred, s);
   }
   mag::launch_gather_ctl(t.d_red, (int)t.red.size(), t.ctl_out, s);
 }
@@ -287,7 +291,7 @@ int launch_class(mag_batch* b, int c, int n) {
   HIPCHK(hipEventRecord(t.ev, t.s));
   t.inflight = true;
   for (mag_job* j : t.cap_jobs)
-    if (mag::job_mixed32(j) && !j->spin)
+    if (mag::job_mixed32(j) && (!j->spin || j->prm.group == MAG_SU3))
       j->since_rebuild = (j->since_rebuild + n * b->replay) % j->prm.reunit_every;
   return MAG_OK;
 }
@@ -402,7 +406,7 @@ int mag_batch_add(mag_batch* b, mag_job* j) {
   if (!j->loaded) return fail(MAG_ERR_STATE, "job has no configuration");
   if (j->prm.device != b->device) return fail(MAG_ERR_ARG, "job and batch on different devices");
   if (j->prm.track_gauge) return fail(MAG_ERR_ARG, "a job that tracks its gauge runs alone");
-  if (j->prm.precision == MAG_PREC_MIXED && !j->spin) {
+  if (j->prm.precision == MAG_PREC_MIXED && (!j->spin || j->prm.group == MAG_SU3)) {
     if (b->reunit == 0) {
       if (b->replay % j->prm.reunit_every != 0)
         return fail(MAG_ERR_ARG, "reunit_every %d does not divide the replay length %d",

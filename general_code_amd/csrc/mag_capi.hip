@@ -100,6 +100,7 @@ namespace {
 
 bool mixed32(const mag_job* j) { return j->prm.precision == MAG_PREC_MIXED && j->phase == 0; }
 
+bool su3_site(const mag_job* j) { return j->spin && j->prm.group == MAG_SU3; }
 bool su2_spin(const mag_job* j) { return j->spin && j->prm.group == MAG_SU2; }
 
 // SU(2) spin form: the FP64 spins of copy c, refreshed from the FP32 spins
@@ -209,6 +210,13 @@ void enqueue_sweep(mag_job* j) {
   if (j->spin) {
     mag::launch_reduce_copies(j->part, j->pstride, 2 * j->nw, j->ncopies, j->invF, j->invV,
                               j->ctl, j->hF, j->hR, j->hA, (size_t)j->prm.max_sweeps, j->st);
+    // SU(3) site form, FP32 phase: the stopping test on F evaluated in FP64
+    // every reunit_every sweeps
+    if (su3_site(j) && mixed32(j) && ++j->since_rebuild == j->prm.reunit_every) {
+      j->since_rebuild = 0;
+      mag::launch_su3_site_check(j->W32, j->S32, j->sstride, j->ncopies, j->g, j->part,
+                                 j->pstride, j->invF, j->ctl, j->st);
+    }
     return;
   }
   mag::launch_reduce(j->part, 2 * j->nw, mag::RED_SWEEP, j->invF, j->invV, j->ctl, j->hF,
@@ -253,7 +261,6 @@ bool all_converged(const mag_job* j) {
   return true;
 }
 
-bool su3_site(const mag_job* j) { return j->spin && j->prm.group == MAG_SU3; }
 // FP64 spins (or SU(3) site gauges) of every copy -> their FP32 working copies
 void sites_to32(mag_job* j) {
   for (int c = 0; c < j->ncopies; c++) {
@@ -292,7 +299,9 @@ void init_copy(mag_job* j, int c, double* tmp) {
   h.max_sweeps = j->prm.max_sweeps;
   const bool fp32 = j->prm.precision == MAG_PREC_MIXED && j->phase == 0;
   h.tol = fp32 ? j->prm.tol_mixed : j->prm.tol;
-  h.window = (fp32 && !j->spin) ? -1 : 0;
+  // no per-sweep test in the FP32 phase of the link form and of the SU(3)
+  // site form (the test runs on FP64 evaluations every reunit_every sweeps)
+  h.window = (fp32 && (!j->spin || su3_site(j))) ? -1 : 0;
   h.rb_every = j->prm.reunit_every;
   hipMemcpyAsync(j->ctl + c, &h, sizeof(JobCtl), hipMemcpyHostToDevice, j->st);
   double* part = j->part + (size_t)c * j->pstride;
@@ -319,7 +328,7 @@ int start_fp32_phase(mag_job* j) {
   j->pending = 0;
   for (int c = 0; c < j->ncopies; c++) {
     j->ctl_h[c].tol = j->prm.tol_mixed;
-    j->ctl_h[c].window = j->spin ? 0 : -1;
+    j->ctl_h[c].window = (j->spin && !su3_site(j)) ? 0 : -1;
     j->ctl_h[c].rb_every = j->prm.reunit_every;
     j->ctl_h[c].F_ref = 0.0;
   }

@@ -175,6 +175,13 @@ template <typename T>
 void launch_su3_site_copies(const T* U, T* G, size_t gstride, int ncopies, const Geom& g,
                             int parity, double omega, double* part, size_t pstride,
                             const JobCtl* ctl, hipStream_t st);
+// SU(3) site form, FP32 phase, every reunit_every sweeps: F of every copy's
+// links in FP64 from the FP32 state (gauges put on SU(3) in FP64), then the
+// stopping test on it (apply_rebuild: the change per sweep since the
+// previous evaluation below tol); partials at part + c*pstride
+void launch_su3_site_check(const float* U, const float* G, size_t gstride, int ncopies,
+                           const Geom& g, double* part, size_t pstride, double invF, JobCtl* ctl,
+                           hipStream_t st);
 // end-of-sweep reduction of every copy (one workgroup per copy): copy c
 // reduces part + c*pstride into ctl[c] and the history at hF + c*hstride
 void launch_reduce_copies(const double* part, size_t pstride, int nparts, int ncopies,
@@ -258,6 +265,10 @@ void launch_copies_half_batch(int ncopies, bool fp32, const BatchJob* jobs, int 
 // the FP32 class of sets of ncopies copies
 void launch_su3_site_copies_batch(int ncopies, bool fp32, const BatchJob* jobs, int njobs,
                                   int nblocks_total, int parity, hipStream_t st);
+// launch_su3_site_check for an FP32 site-form class: jobs (half-sweep
+// table, grid nblocks_total) and red (one entry per copy)
+void launch_su3_site_check_batch(const BatchJob* jobs, int njobs, int nblocks_total,
+                                 const BatchJob* red, int nred, hipStream_t st);
 // batched per-sweep reduction (RED_SWEEP) of every entry of a table:
 // red_blocks(nparts) workgroups per entry (red0 ascending, nblocks_total
 // in all)

@@ -929,6 +929,104 @@ __global__ __launch_bounds__(kBlock) void k_su3_site_set_batch(const BatchJob* _
                         j.part + (size_t)3 * j.nw * p, j.pstride, j.ctl, lb - j.block0);
 }
 
+// The stopping test of the SU(3) site form's FP32 phase.  The functional
+// that the FP32 half-sweeps sum comes from FP32 field matrices, whose
+// rounding moves it by ~1e-7 / sqrt(V) (relative) from one sweep to the
+// next -- more than the tolerance -- so it cannot decide convergence.
+// Every reunit_every sweeps this kernel evaluates, for every copy not done,
+// F of the links G_c U G_c^+ in FP64 from the FP32 state, with the FP32
+// links taken exactly and every gauge put on SU(3) in FP64 (gs3): a
+// function of the state free of FP32 rounding.  Lane h covers the sites
+// (p, h), p = 0, 1, and their 4 forward links (every link once); one
+// partial triple (F, 0, 0) per wave at part + c * pstride.
+__device__ __forceinline__ void su3_site_F_block(const float* __restrict__ U,
+                                                 const float* __restrict__ G, size_t gstride,
+                                                 int ncopies, const Geom& g,
+                                                 double* __restrict__ part, size_t pstride,
+                                                 const JobCtl* __restrict__ ctl, uint32_t lb) {
+  bool live = false;
+  for (int c = 0; c < ncopies; c++) live |= !ctl[c].done;
+  if (!live) return;  // uniform over the workgroup
+  const uint32_t h = lb * kBlock + threadIdx.x;
+  const bool in = h < g.Vh;
+  for (int c = 0; c < ncopies; c++) {
+    if (ctl[c].done) continue;  // uniform over the workgroup
+    const float* Gc = G + (size_t)c * gstride;
+    double fs = 0.0;
+    if (in) {
+#pragma unroll 1
+      for (int p = 0; p < 2; p++) {
+        const SiteIdx s = decode(g, h, p);
+        const M3<double> gx = gs3(to_d(ldm(site_ptr<12>(Gc, g, p, h))));
+#pragma unroll 1
+        for (int mu = 0; mu < 4; mu++) {
+          const M3<double> gy = gs3(to_d(ldm(site_ptr<12>(Gc, g, 1 - p, fwd_h(g, h, s, mu)))));
+          const M3<double> u = full3(to_d(ldm(link_ptr<12>(U, g, p, mu, h))));
+          // W_ii = g_i U g'_i^+
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            C<double> v[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) v[a] = cdot3c(u.m[a], gy.m[i]);
+            fs += cnorm2(cdot3(gx.m[i][0], v[0], gx.m[i][1], v[1], gx.m[i][2], v[2]));
+          }
+        }
+      }
+    }
+    wave_partial(part + (size_t)c * pstride, lb * kBlock + (threadIdx.x & ~63u), g.Vh, fs, 0.0,
+                 0.0);
+  }
+}
+__global__ __launch_bounds__(kBlock) void k_su3_site_F(const float* __restrict__ U,
+                                                       const float* __restrict__ G,
+                                                       size_t gstride, int ncopies, Geom g,
+                                                       double* __restrict__ part, size_t pstride,
+                                                       const JobCtl* __restrict__ ctl) {
+  su3_site_F_block(U, G, gstride, ncopies, g, part, pstride, ctl, blockIdx.x);
+}
+__global__ __launch_bounds__(kBlock) void k_su3_site_F_batch(const BatchJob* __restrict__ jobs,
+                                                             int njobs) {
+  const BatchJob& j = jobs[batch_job(jobs, njobs, (int)blockIdx.x)];
+  su3_site_F_block((const float*)j.U, (const float*)j.S, j.sstride, j.ncopies, j.g, j.part,
+                   j.pstride, j.ctl, blockIdx.x - j.block0);
+}
+// the F of that evaluation (nparts triples at part) tested against the
+// previous one: converged when their difference per sweep is below tol
+// (apply_rebuild); the sweep count does not change
+__device__ __forceinline__ void reduce_check(const double* __restrict__ part, int nparts,
+                                             double invF, JobCtl* ctl) {
+  if (ctl->done) return;
+  double a, b, c;
+  reduce_triples(part, nparts, a, b, c);
+  if (threadIdx.x == 0) apply_rebuild(a * invF, ctl);
+}
+__global__ __launch_bounds__(kRedBlock) void k_reduce_check(const double* __restrict__ part,
+                                                            size_t pstride, int nparts,
+                                                            double invF, JobCtl* ctl) {
+  reduce_check(part + (size_t)blockIdx.x * pstride, nparts, invF, ctl + blockIdx.x);
+}
+__global__ __launch_bounds__(kRedBlock) void k_reduce_check_batch(const BatchJob* __restrict__ jobs) {
+  const BatchJob& j = jobs[blockIdx.x];
+  reduce_check(j.part, j.nw, j.invF, j.ctl);
+}
+
+void launch_su3_site_check(const float* U, const float* G, size_t gstride, int ncopies,
+                           const Geom& g, double* part, size_t pstride, double invF, JobCtl* ctl,
+                           hipStream_t st) {
+  const int nb = (int)((site3_lanes(g) + kBlock - 1) / kBlock);
+  const int nw = (int)((site3_lanes(g) + 63) / 64);
+  hipLaunchKernelGGL(k_su3_site_F, dim3(nb), dim3(kBlock), 0, st, U, G, gstride, ncopies, g, part,
+                     pstride, ctl);
+  hipLaunchKernelGGL(k_reduce_check, dim3(ncopies), dim3(kRedBlock), 0, st, part, pstride, nw,
+                     invF, ctl);
+}
+void launch_su3_site_check_batch(const BatchJob* jobs, int njobs, int nblocks_total,
+                                 const BatchJob* red, int nred, hipStream_t st) {
+  if (njobs == 0 || nblocks_total == 0) return;
+  hipLaunchKernelGGL(k_su3_site_F_batch, dim3(nblocks_total), dim3(kBlock), 0, st, jobs, njobs);
+  hipLaunchKernelGGL(k_reduce_check_batch, dim3(nred), dim3(kRedBlock), 0, st, red);
+}
+
 template <typename T>
 void launch_su3_site_copies(const T* U, T* G, size_t gstride, int ncopies, const Geom& g,
                             int parity, double omega, double* part, size_t pstride,
