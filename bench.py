@@ -125,6 +125,35 @@ def measure(G, group, dims, precision, eps, omega, steps, warmup, seed=2024):
 SITE3_B_SITE_MIN = 8 * 12 * 4 + 2 * 12 * 4
 
 
+def measure_copies(G, group, dims, precision, copies, eps, omega, steps, warmup, seed=2024):
+    """The site forms' half-sweep kernel alone (mag_job_halfsweep_timed):
+    SU(2) spin form or SU(3) site form, `copies` gauge copies of one
+    configuration, FP32 phase (MIXED) or FP64.  Rate in site updates of all
+    copies per second; bytes moved at least per site update and copy: the
+    links once for all copies, each copy's own site variable read and
+    written (SU(2) spins 3 reals, SU(3) gauges 12)."""
+    V = 1
+    for d in dims:
+        V *= d
+    job = G.MagJob(group=group, dims=dims, precision=precision, omega=omega, tol=0.0,
+                   max_sweeps=2 * steps + warmup + 4, form=G.FORM_SPIN, copies=copies,
+                   tol_mixed=0.0, fp64_warmup=0)
+    try:
+        job.generate(eps, seed)
+        job.sweeps_timed(max(warmup, 1))
+        kern_ms = job.halfsweep_timed(2 * steps)
+    finally:
+        job.close()
+    w = 4 if precision == G.MIXED else 8
+    nr, sr = (4, 3) if group == 2 else (12, 12)
+    b_min = 8 * nr * w + copies * 2 * sr * w
+    moved = (V // 2) * b_min / (kern_ms * 1e-3) / 1e9
+    return {"copies": copies, "kernel_ms": kern_ms,
+            "site_updates_per_s": copies * (V // 2) / (kern_ms * 1e-3),
+            "b_min_per_site": b_min, "min_moved_GBs": round(moved, 1),
+            "min_moved_frac": round(moved / PEAK_HBM_GBS, 4)}
+
+
 def measure_site3(G, dims, eps, omega, steps, warmup, seed=2024):
     """Sweep rate of the SU(3) site form, one configuration, FP32 phase (the
     served form of an SU(3) request), priced two ways: against the link-form
@@ -600,6 +629,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="extra lines: the sweep and site-form kernels only (profiling runs)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=40)
@@ -649,7 +680,21 @@ def main():
         for name, d in [("su3_32^4_mixed_site", dims), ("su3_24^3x48_mixed_site", (24, 24, 24, 48))]:
             print(f"[bench] {name}", file=sys.stderr, flush=True)
             extra[name] = measure_site3(G, d, 0.4, 1.6, args.steps, args.warmup)
+        # the half-sweep kernels of the batched and served forms
+        print("[bench] site forms", file=sys.stderr, flush=True)
+        extra["site_form_kernels"] = {
+            name: measure_copies(G, group, d, prec, k, e, om, args.steps, args.warmup)
+            for name, group, d, prec, k, e, om in [
+                ("su2_32^4_spin_fp32_1copy", 2, dims, G.MIXED, 1, eps, omega),
+                ("su2_32^4_spin_fp32_4copies", 2, dims, G.MIXED, 4, eps, omega),
+                ("su2_48^4_spin_fp32_1copy", 2, (48,) * 4, G.MIXED, 1, eps, omega),
+                ("su2_48^4_spin_fp64_1copy", 2, (48,) * 4, G.F64, 1, eps, omega),
+                ("su3_32^4_site_fp32_1copy", 3, dims, G.MIXED, 1, 0.4, 1.6),
+                ("su3_32^4_site_fp32_4copies", 3, dims, G.MIXED, 4, 0.4, 1.6),
+                ("su3_32^4_site_fp64_1copy", 3, dims, G.F64, 1, 0.4, 1.6),
+            ]}
 
+    if not args.no_extra and not args.kernels_only:
         for name, group, d, e in [("monopoles_su2_32^4", 2, dims, 1.5),
                                   ("monopoles_su3_24^3x48", 3, (24, 24, 24, 48), 1.2)]:
             print(f"[bench] {name}", file=sys.stderr, flush=True)

@@ -137,6 +137,21 @@ __device__ __forceinline__ void stm_stream(LRef<double> r, const M2<double>& m) 
   }
 }
 
+// The same load with the nontemporal hint: the links of an SU(3) site-form
+// set of several copies, each read once per half-sweep while the copies'
+// neighbour gauges are read by up to 8 sites each; marked for early
+// eviction, the links leave the caches to the gauges.
+typedef float mag_f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ M2<float> ldm_nt(LRef<const float> r) {
+  const mag_f4v a = __builtin_nontemporal_load(reinterpret_cast<const mag_f4v*>(r.p));
+  const mag_f4v b = __builtin_nontemporal_load(reinterpret_cast<const mag_f4v*>(r.p + r.cs));
+  const mag_f4v c = __builtin_nontemporal_load(reinterpret_cast<const mag_f4v*>(r.p + 2 * r.cs));
+  M2<float> m;
+  m.r[0][0] = {a.x, a.y}; m.r[0][1] = {a.z, a.w}; m.r[0][2] = {b.x, b.y};
+  m.r[1][0] = {b.z, b.w}; m.r[1][1] = {c.x, c.y}; m.r[1][2] = {c.z, c.w};
+  return m;
+}
+
 // ------------------------------------------------------------------ geometry
 struct SiteIdx {
   uint32_t x, y, z, t, row;

@@ -817,9 +817,12 @@ __device__ __forceinline__ void su3_site_copies_block(const T* __restrict__ U,
 // copies at 32^4 read 4.1 times the links of one: the 4 MB L2 of an XCD
 // does not keep them from one copy to the next).  K = 1 loads two
 // directions per pass (100 VGPRs, 5 waves per SIMD), more copies one.  A
-// copy's operations and their order are the same for every K.  The link
-// loads carry no nontemporal hint: it cost 10% at 32^4 (the next
-// half-sweep reads the links again from the MALL) and gained 3% at 48^4.
+// copy's operations and their order are the same for every K.  Sets of
+// two or more copies load the links with the nontemporal hint (32^4, four
+// copies: 140 -> 126 us per half-sweep; the copies' neighbour gauges, not
+// the links, are then what the caches hold); one copy loads them plainly
+// (the hint cost 10% at 32^4, where the next half-sweep reads the links
+// again from the MALL).
 template <int K>
 __device__ __forceinline__ void su3_site_set_block(const float* __restrict__ U,
                                                    float* __restrict__ G, size_t gstride,
@@ -854,8 +857,13 @@ __device__ __forceinline__ void su3_site_set_block(const float* __restrict__ U,
         const int mu = m0 + k;
         hf[k] = fwd_h(g, h, s, mu);
         hb[k] = back_h(g, h, s, mu);
-        Lf[k] = ldm(link_ptr<12>(U, g, p, mu, h));
-        Lb[k] = ldm(link_ptr<12>(U, g, 1 - p, mu, hb[k]));
+        if constexpr (K == 1) {
+          Lf[k] = ldm(link_ptr<12>(U, g, p, mu, h));
+          Lb[k] = ldm(link_ptr<12>(U, g, 1 - p, mu, hb[k]));
+        } else {
+          Lf[k] = ldm_nt(link_ptr<12>(U, g, p, mu, h));
+          Lb[k] = ldm_nt(link_ptr<12>(U, g, 1 - p, mu, hb[k]));
+        }
       }
 #pragma unroll
       for (int c = 0; c < K; c++) {
