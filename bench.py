@@ -36,16 +36,19 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 
 
-def pmc_traffic(kernel: str, lattice: str):
-    """HBM bytes per launch of a half-sweep kernel from the committed PMC
-    summary, or None when it holds no entry for that kernel and lattice."""
+def pmc_traffic(kernels, lattice: str):
+    """HBM bytes per launch of half-sweep kernels from the committed PMC
+    summary: the mean over `kernels` (the red and the black half of the
+    mixed FP32 phase are different kernels launched alternately), or None
+    when it holds no entry for one of them at that lattice."""
     path = os.path.join(ROOT, "profiles", "halfsweep_hbm_traffic_pmc.md")
     try:
         text = open(path).read()
-        block = text.split("```json", 1)[1].split("```", 1)[0]
-        return json.loads(block).get(f"{kernel}|{lattice}", {}).get("traffic_bytes")
+        block = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
+        vals = [block.get(f"{k}|{lattice}", {}).get("traffic_bytes") for k in kernels]
     except (OSError, IndexError, ValueError):
         return None
+    return None if None in vals else sum(vals) / len(vals)
 
 
 def b_site(group: int, mixed: bool) -> int:
@@ -93,8 +96,10 @@ def measure(G, group, dims, precision, eps, omega, steps, warmup, seed=2024):
     bytes_launch = (V // 2) * b_site(group, mixed)
     achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
     step_ms = total_ms / steps
-    kname = f"void mag::k_su{group}_half<{'float' if mixed else 'double'}, false>"
-    traffic = pmc_traffic(kname, lattice_name(dims))
+    # the mixed FP32 phase: red halves accumulate the site gauge, black halves do not
+    knames = ([f"void mag::k_su{group}_half<float, true>", f"void mag::k_su{group}_half<float, false>"]
+              if mixed else [f"void mag::k_su{group}_half<double, false>"])
+    traffic = pmc_traffic(knames, lattice_name(dims))
     return {
         "value": V * steps / (total_ms * 1e-3),
         "ms_per_step": step_ms,
