@@ -9,7 +9,9 @@
 //                    member jobs' grids
 //   one reduction launch (a workgroup per job, per copy of a copy set);
 //   link-form FP32 classes instead reduce once per reunit_every sweeps,
-//   after the black half-sweep that rebuilds the links from the FP64 master
+//   after the black half-sweep that rebuilds the links from the FP64
+//   master, and FP32 SU(3) site-form classes after the F evaluation in FP64
+//   that ends their period
 // and `replay` sweeps (default 20) plus a gather of every member's control
 // block into pinned host memory are captured in one hipGraph per class.
 //
@@ -152,13 +154,13 @@ void drop_graph(ClassRun& t) {
 }
 
 // half-sweep of one class
-void launch_class_half(const mag_batch* b, int c, int p, hipStream_t st) {
+void launch_class_half(const mag_batch* b, int c, int p, hipStream_t st, int slot = 0) {
   const ClassRun& t = b->cls[c];
   if (c < kSpinClass) {
     mag::launch_half_batch(t.group, t.fp32, t.shape, t.d, (int)t.h.size(), t.nblocks, p, st);
   } else if (c >= kSiteClass3) {
     mag::launch_su3_site_copies_batch(c - kSiteClass3, c > kSiteClass3, t.d, (int)t.h.size(),
-                                      t.nblocks, p, st);
+                                      t.nblocks, p, st, slot);
   } else {
     const int k = (c - kSpinClass) % MAG_MAX_COPIES + 1;
     const bool fp32 = c >= kSpinClass + MAG_MAX_COPIES;
@@ -185,13 +187,23 @@ void enqueue_class_replay(const mag_batch* b, int c, hipStream_t s) {
       }
       continue;
     }
+    if (c > kSiteClass3) {
+      // FP32 SU(3) site-form sets, as a job alone (mag_capi.hip
+      // enqueue_sweep): sweep partials in slots, and at the end of each
+      // period of reunit sweeps F in FP64, the slots and the stopping test
+      const int R = b->reunit, k = i % R;
+      launch_class_half(b, c, 0, s, k);
+      launch_class_half(b, c, 1, s, k);
+      if (k == R - 1) {
+        mag::launch_su3_site_F_batch(t.d, (int)t.h.size(), t.nblocks, s);
+        mag::launch_reduce_slots_batch(t.d_red, nred, R, s);
+        mag::launch_reduce_period_batch(t.d_red, nred, R, 1, s);
+      }
+      continue;
+    }
     launch_class_half(b, c, 0, s);
     launch_class_half(b, c, 1, s);
     mag::launch_reduce_batch(t.d_red, nred, t.red_blocks, s);
-    // FP32 SU(3) site-form classes: the stopping test on F in FP64 every
-    // reunit sweeps, as a job alone (mag_capi.hip enqueue_sweep)
-    if (c > kSiteClass3 && i % b->reunit == b->reunit - 1)
-      mag::launch_su3_site_check_batch(t.d, (int)t.h.size(), t.nblocks, t.d_red, nred, s);
   }
   mag::launch_gather_ctl(t.d_red, (int)t.red.size(), t.ctl_out, s);
 }

@@ -131,9 +131,10 @@ mag::Shape link_shape(mag_job* j, bool fp32) {
 
 void launch_half(mag_job* j, int parity, const mag::HalfOpt& o = mag::HalfOpt{}) {
   if (j->spin && j->prm.group == MAG_SU3) {
-    if (mixed32(j))
+    if (mixed32(j))  // FP32 phase: partials in sweep slot o.slot
       mag::launch_su3_site_copies<float>(j->W32, j->S32, j->sstride, j->ncopies, j->g, parity,
-                                         j->prm.omega, j->part, j->pstride, j->ctl, j->st);
+                                         j->prm.omega, j->part, j->pstride, j->ctl, j->st,
+                                         o.slot);
     else
       mag::launch_su3_site_copies<double>(j->U64, j->S, j->sstride, j->ncopies, j->g, parity,
                                           j->prm.omega, j->part, j->pstride, j->ctl, j->st);
@@ -167,14 +168,24 @@ void launch_half(mag_job* j, int parity, const mag::HalfOpt& o = mag::HalfOpt{})
 }
 
 // mixed FP32 phase: reduce the slots swept since the last reduction into
-// the history and the sweep count; after a rebuilding sweep also test
-// convergence on the rebuilt links' F
+// the history and the sweep count; after a rebuilding sweep (SU(3) site
+// form: after the F evaluation that ends a period) also test convergence
+// on that F
 void flush_slots(mag_job* j, bool rebuild) {
   if (j->pending == 0) return;
-  mag::launch_reduce_slots(j->part, j->pstride, 2 * j->nw, j->pending, j->invF, j->invV, j->ctl,
-                           j->hF, j->hR, j->hA, j->st);
-  mag::launch_reduce_period(j->part + (size_t)mag::kSlots * j->pstride, j->nw, j->pending,
-                            rebuild ? 1 : 0, j->invF, j->ctl, j->hF, j->st);
+  if (j->spin) {  // SU(3) site form: every copy of the set
+    const size_t hs = (size_t)j->prm.max_sweeps;
+    mag::launch_reduce_slots_copies(j->part, j->pstride, 2 * j->nw, j->ncopies, j->pending,
+                                    j->invF, j->invV, j->ctl, j->hF, j->hR, j->hA, hs, j->st);
+    mag::launch_reduce_period_copies(j->part + (size_t)mag::kSlots * j->ncopies * j->pstride,
+                                     j->pstride, j->nw, j->ncopies, j->pending, rebuild ? 1 : 0,
+                                     j->invF, j->ctl, j->hF, hs, j->st);
+  } else {
+    mag::launch_reduce_slots(j->part, j->pstride, 2 * j->nw, j->pending, j->invF, j->invV,
+                             j->ctl, j->hF, j->hR, j->hA, j->st);
+    mag::launch_reduce_period(j->part + (size_t)mag::kSlots * j->pstride, j->nw, j->pending,
+                              rebuild ? 1 : 0, j->invF, j->ctl, j->hF, j->st);
+  }
   j->pending = 0;
 }
 
@@ -205,18 +216,31 @@ void enqueue_sweep(mag_job* j) {
     if (rebuild || j->pending == mag::kSlots) flush_slots(j, rebuild);
     return;
   }
+  if (su3_site(j) && mixed32(j)) {
+    // SU(3) site form, FP32 phase: the sweeps' partials go to slots; every
+    // reunit_every sweeps F is evaluated in FP64 from the FP32 state and the
+    // period's reduction tests it (the FP32-summed F of the sweeps is too
+    // noisy to stop on)
+    mag::HalfOpt o;
+    o.slot = j->pending;
+    launch_half(j, 0, o);
+    launch_half(j, 1, o);
+    j->pending++;
+    const bool check = ++j->since_rebuild == j->prm.reunit_every;
+    if (check) {
+      j->since_rebuild = 0;
+      mag::launch_su3_site_F(j->W32, j->S32, j->sstride, j->ncopies, j->g,
+                             j->part + (size_t)mag::kSlots * j->ncopies * j->pstride, j->pstride,
+                             j->ctl, j->st);
+    }
+    if (check || j->pending == mag::kSlots) flush_slots(j, check);
+    return;
+  }
   launch_half(j, 0);
   launch_half(j, 1);
   if (j->spin) {
     mag::launch_reduce_copies(j->part, j->pstride, 2 * j->nw, j->ncopies, j->invF, j->invV,
                               j->ctl, j->hF, j->hR, j->hA, (size_t)j->prm.max_sweeps, j->st);
-    // SU(3) site form, FP32 phase: the stopping test on F evaluated in FP64
-    // every reunit_every sweeps
-    if (su3_site(j) && mixed32(j) && ++j->since_rebuild == j->prm.reunit_every) {
-      j->since_rebuild = 0;
-      mag::launch_su3_site_check(j->W32, j->S32, j->sstride, j->ncopies, j->g, j->part,
-                                 j->pstride, j->invF, j->ctl, j->st);
-    }
     return;
   }
   mag::launch_reduce(j->part, 2 * j->nw, mag::RED_SWEEP, j->invF, j->invV, j->ctl, j->hF,
@@ -419,6 +443,7 @@ void free_job(mag_job* j) {
 // spin form: the FP32 links were only read; FP64 sweeps go on from the
 // spins with the FP64 links
 int finalize_mixed_spin(mag_job* j) {
+  flush_slots(j, false);
   CHK(read_ctl(j));
   double* tmp = nullptr;
   CHK(alloc_tmp(j, &tmp));
@@ -547,7 +572,11 @@ int mag_job_footprint(const mag_params* p, uint64_t* bytes) {
   const uint64_t lanes = (uint64_t)(V / 2) * (p->group == MAG_SU2 ? 1 : 2);
   const int nw = (int)((lanes + 63) / 64);  // partial triples per half-sweep (at most)
   b += (uint64_t)4 << 20;  // pitch padding of the chunk arrays (4 KB per array)
-  const uint64_t slots = (p->precision == MAG_PREC_MIXED && p->form == MAG_FORM_LINK) ? mag::kSlots + 1 : 1;
+  // mixed link form and SU(3) site form: kSlots sweep slots and one more
+  const uint64_t slots =
+      (p->precision == MAG_PREC_MIXED && (p->form == MAG_FORM_LINK || p->group == MAG_SU3))
+          ? mag::kSlots + 1
+          : 1;
   b += nc * (mag::part_stride(nw) * 8 * slots + (uint64_t)p->max_sweeps * 3 * 8 + sizeof(JobCtl)) + 3 * 8;
   *bytes = b;
   return MAG_OK;
@@ -606,8 +635,10 @@ int mag_job_create(const mag_params* p, mag_job** out) {
   }
   if (j->spin) ALLOC(j->S, j->sstride * j->ncopies * sizeof(double));
   if (p->track_gauge) ALLOC(j->G64, (size_t)2 * j->g.hp * 4 * sizeof(double));
-  // mixed link form: kSlots sweep slots and the rebuild partials
-  const size_t nslots = (p->precision == MAG_PREC_MIXED && !j->spin) ? mag::kSlots + 1 : 1;
+  // mixed link form: kSlots sweep slots and the rebuild partials; SU(3)
+  // site form: kSlots sweep slots and the partials of the FP64 F check
+  const size_t nslots =
+      (p->precision == MAG_PREC_MIXED && (!j->spin || p->group == MAG_SU3)) ? mag::kSlots + 1 : 1;
   ALLOC(j->part, j->pstride * j->ncopies * nslots * sizeof(double));
   ALLOC(j->ctl, j->ncopies * sizeof(JobCtl));
   ALLOC(j->hF, (size_t)p->max_sweeps * j->ncopies * sizeof(double));

@@ -170,18 +170,35 @@ void launch_su2_copies_half(const TL* U, TL* S, size_t sstride, int ncopies,
                             size_t pstride, const JobCtl* ctl, hipStream_t st);
 // SU(3) site form: links U read-only, site gauge of copy c at G + c*gstride
 // (2 rows, chunked site layout, the links' type); partials and control as
-// for the spin form, one triple per wave of the site3_lanes(g) lanes
+// for the spin form, one triple per wave of the site3_lanes(g) lanes.
+// The FP32 phase (T = float) keeps the partials of its sweeps in slots:
+// sweep slot s of copy c at part + (s*ncopies + c)*pstride, a sweep whose
+// slot lies beyond the cap (ctl[c].sweep + s >= max_sweeps) is skipped;
+// the slots are reduced together at the end of a period of reunit_every
+// sweeps (launch_reduce_slots_copies / launch_reduce_period_copies), as in
+// the mixed link form.  Outside the FP32 phase slot = 0.
 template <typename T>
 void launch_su3_site_copies(const T* U, T* G, size_t gstride, int ncopies, const Geom& g,
                             int parity, double omega, double* part, size_t pstride,
-                            const JobCtl* ctl, hipStream_t st);
-// SU(3) site form, FP32 phase, every reunit_every sweeps: F of every copy's
-// links in FP64 from the FP32 state (gauges put on SU(3) in FP64), then the
-// stopping test on it (apply_rebuild: the change per sweep since the
-// previous evaluation below tol); partials at part + c*pstride
-void launch_su3_site_check(const float* U, const float* G, size_t gstride, int ncopies,
-                           const Geom& g, double* part, size_t pstride, double invF, JobCtl* ctl,
-                           hipStream_t st);
+                            const JobCtl* ctl, hipStream_t st, int slot = 0);
+// SU(3) site form, FP32 phase, at the end of every period of reunit_every
+// sweeps: F of every copy's links in FP64 from the FP32 state (gauges put
+// on SU(3) in FP64), partials of copy c at part + c*pstride (part: slot
+// kSlots of the buffer); the period reduction then tests it (apply_rebuild:
+// the change per sweep since the previous evaluation below tol)
+void launch_su3_site_F(const float* U, const float* G, size_t gstride, int ncopies,
+                       const Geom& g, double* part, size_t pstride, const JobCtl* ctl,
+                       hipStream_t st);
+// the slots of one copy set: n slots of each of the ncopies copies into the
+// copies' histories (hF + c*hstride ...), then per copy the period
+// (reduce_period: sweep count, F_prev, cap, and with check = 1 the
+// stopping test on the F partials at part_check + c*pstride, nw of them)
+void launch_reduce_slots_copies(const double* part, size_t pstride, int nparts, int ncopies, int n,
+                                double invF, double invV, const JobCtl* ctl, double* hF,
+                                double* hR, double* hA, size_t hstride, hipStream_t st);
+void launch_reduce_period_copies(const double* part_check, size_t pstride, int nw, int ncopies,
+                                 int n, int check, double invF, JobCtl* ctl, const double* hF,
+                                 size_t hstride, hipStream_t st);
 // end-of-sweep reduction of every copy (one workgroup per copy): copy c
 // reduces part + c*pstride into ctl[c] and the history at hF + c*hstride
 void launch_reduce_copies(const double* part, size_t pstride, int nparts, int ncopies,
@@ -253,7 +270,9 @@ struct BatchJob {
 void launch_half_batch(int group, bool fp32, Shape sh, const BatchJob* jobs, int njobs,
                        int nblocks_total, int parity, hipStream_t st, int slot = 0,
                        int mode = HALF_PLAIN);
-// batched launch_reduce_slots / launch_reduce_period over a table
+// batched launch_reduce_slots / launch_reduce_period over a table (link-form
+// jobs, or the copy entries of FP32 SU(3) site-form sets: slot s of copy c
+// at the set's part + (s*ncopies + c)*pstride, check partials in slot kSlots)
 void launch_reduce_slots_batch(const BatchJob* jobs, int njobs, int n, hipStream_t st);
 void launch_reduce_period_batch(const BatchJob* jobs, int njobs, int n, int rebuild,
                                 hipStream_t st);
@@ -264,11 +283,11 @@ void launch_copies_half_batch(int ncopies, bool fp32, const BatchJob* jobs, int 
 // batched SU(3) site-form half-sweep: the FP64 class (any copy counts) or
 // the FP32 class of sets of ncopies copies
 void launch_su3_site_copies_batch(int ncopies, bool fp32, const BatchJob* jobs, int njobs,
-                                  int nblocks_total, int parity, hipStream_t st);
-// launch_su3_site_check for an FP32 site-form class: jobs (half-sweep
-// table, grid nblocks_total) and red (one entry per copy)
-void launch_su3_site_check_batch(const BatchJob* jobs, int njobs, int nblocks_total,
-                                 const BatchJob* red, int nred, hipStream_t st);
+                                  int nblocks_total, int parity, hipStream_t st, int slot = 0);
+// launch_su3_site_F for an FP32 site-form class (half-sweep table, grid
+// nblocks_total): F partials of each set's copies in its slot kSlots; the
+// period reduction of the class's copy entries tests them
+void launch_su3_site_F_batch(const BatchJob* jobs, int njobs, int nblocks_total, hipStream_t st);
 // batched per-sweep reduction (RED_SWEEP) of every entry of a table:
 // red_blocks(nparts) workgroups per entry (red0 ascending, nblocks_total
 // in all)

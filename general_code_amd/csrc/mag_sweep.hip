@@ -212,7 +212,9 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_slots(const double* __rest
 }
 // After the slots: the sweep count advances by n (up to the cap), F_prev is
 // the last recorded F, and a period that ended with a rebuild (its
-// rebuilding sweep ran) tests convergence on the rebuilt links' F.
+// rebuilding sweep ran) tests convergence on the F of its end state: the
+// rebuilt links' F in the mixed link form, the F evaluated in FP64 from the
+// FP32 state in the SU(3) site form (part_rb: the partials of that F).
 __device__ __forceinline__ void reduce_period(const double* __restrict__ part_rb, int nrb, int n,
                                               int rebuild, double invF, JobCtl* ctl,
                                               const double* hF) {
@@ -247,17 +249,54 @@ void launch_reduce_period(const double* part_rb, int nrb, int n, int rebuild, do
   hipLaunchKernelGGL(k_reduce_period, dim3(1), dim3(kRedBlock), 0, st, part_rb, nrb, n, rebuild,
                      invF, ctl, hF);
 }
-// batched: slots of job b / n, slot b % n; the period: one block per job
+// batched: slots of entry b / n, slot b % n; the period: one block per
+// entry.  A link-form entry's slot s is at part + s * pstride; a copy-set
+// entry (copy c of an SU(3) site-form set of K copies, part = the set's
+// buffer + c * pstride) has its slot s at part + s * K * pstride, and its
+// check partials (the F of the stopping test) in slot kSlots.
+__device__ __forceinline__ size_t slot_stride(const BatchJob& j) {
+  return (size_t)(j.ncopies > 0 ? j.ncopies : 1) * j.pstride;
+}
 __global__ __launch_bounds__(kRedBlock) void k_reduce_slots_batch(const BatchJob* __restrict__ jobs,
                                                                   int n) {
   const BatchJob& j = jobs[blockIdx.x / n];
-  reduce_slot(j.part, j.pstride, j.nparts, j.invF, j.invV, j.ctl, j.hF, j.hR, j.hA,
+  reduce_slot(j.part, slot_stride(j), j.nparts, j.invF, j.invV, j.ctl, j.hF, j.hR, j.hA,
               (int)(blockIdx.x % n));
 }
 __global__ __launch_bounds__(kRedBlock) void k_reduce_period_batch(
     const BatchJob* __restrict__ jobs, int n, int rebuild) {
   const BatchJob& j = jobs[blockIdx.x];
-  reduce_period(j.part + (size_t)kSlots * j.pstride, j.nw, n, rebuild, j.invF, j.ctl, j.hF);
+  reduce_period(j.part + (size_t)kSlots * slot_stride(j), j.nw, n, rebuild, j.invF, j.ctl, j.hF);
+}
+// one job's copy set: block b reduces slot b / ncopies of copy b % ncopies
+// (history of copy c at hF + c * hstride); the period: one block per copy
+__global__ __launch_bounds__(kRedBlock) void k_reduce_slots_copies(
+    const double* __restrict__ part, size_t pstride, int nparts, int ncopies, double invF,
+    double invV, const JobCtl* ctl, double* hF, double* hR, double* hA, size_t hstride) {
+  const int c = (int)blockIdx.x % ncopies, k = (int)blockIdx.x / ncopies;
+  const size_t hs = (size_t)c * hstride;
+  reduce_slot(part + (size_t)c * pstride, (size_t)ncopies * pstride, nparts, invF, invV, ctl + c,
+              hF + hs, hR + hs, hA + hs, k);
+}
+__global__ __launch_bounds__(kRedBlock) void k_reduce_period_copies(
+    const double* __restrict__ part_check, size_t pstride, int nw, int n, int check,
+    double invF, JobCtl* ctl, const double* hF, size_t hstride) {
+  const int c = (int)blockIdx.x;
+  reduce_period(part_check + (size_t)c * pstride, nw, n, check, invF, ctl + c,
+                hF + (size_t)c * hstride);
+}
+void launch_reduce_slots_copies(const double* part, size_t pstride, int nparts, int ncopies, int n,
+                                double invF, double invV, const JobCtl* ctl, double* hF,
+                                double* hR, double* hA, size_t hstride, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_reduce_slots_copies, dim3(n * ncopies), dim3(kRedBlock), 0, st, part,
+                     pstride, nparts, ncopies, invF, invV, ctl, hF, hR, hA, hstride);
+}
+void launch_reduce_period_copies(const double* part_check, size_t pstride, int nw, int ncopies,
+                                 int n, int check, double invF, JobCtl* ctl, const double* hF,
+                                 size_t hstride, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_period_copies, dim3(ncopies), dim3(kRedBlock), 0, st, part_check,
+                     pstride, nw, n, check, invF, ctl, hF, hstride);
 }
 void launch_reduce_slots_batch(const BatchJob* jobs, int njobs, int n, hipStream_t st) {
   if (njobs == 0 || n <= 0) return;
@@ -828,12 +867,14 @@ __device__ __forceinline__ void su3_site_set_block(const float* __restrict__ U,
                                                    float* __restrict__ G, size_t gstride,
                                                    const Geom& g, int p, float omega,
                                                    double* __restrict__ part, size_t pstride,
-                                                   const JobCtl* __restrict__ ctl, uint32_t lb) {
+                                                   const JobCtl* __restrict__ ctl, int slot,
+                                                   uint32_t lb) {
   bool act[K];
   bool live = false;
 #pragma unroll
   for (int c = 0; c < K; c++) {
-    act[c] = !ctl[c].done;  // uniform over the workgroup
+    // uniform over the workgroup; a slot's sweep beyond the cap is skipped
+    act[c] = !ctl[c].done && ctl[c].sweep + slot < ctl[c].max_sweeps;
     live |= act[c];
   }
   if (!live) return;
@@ -914,8 +955,9 @@ __global__ __launch_bounds__(kBlock) void k_su3_site_set(const float* __restrict
                                                          Geom g, int p, float omega,
                                                          double* __restrict__ part,
                                                          size_t pstride,
-                                                         const JobCtl* __restrict__ ctl) {
-  su3_site_set_block<K>(U, G, gstride, g, p, omega, part, pstride, ctl,
+                                                         const JobCtl* __restrict__ ctl,
+                                                         int slot) {
+  su3_site_set_block<K>(U, G, gstride, g, p, omega, part, pstride, ctl, slot,
                         xcd_block(blockIdx.x, gridDim.x));
 }
 
@@ -930,11 +972,12 @@ __global__ __launch_bounds__(kBlock) void k_su3_site_copies_batch(
 }
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_su3_site_set_batch(const BatchJob* __restrict__ jobs,
-                                                               int njobs, int p) {
+                                                               int njobs, int p, int slot) {
   const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
   const BatchJob& j = jobs[batch_job(jobs, njobs, (int)lb)];
   su3_site_set_block<K>((const float*)j.U, (float*)j.S, j.sstride, j.g, p, (float)j.omega,
-                        j.part + (size_t)3 * j.nw * p, j.pstride, j.ctl, lb - j.block0);
+                        j.part + (size_t)slot * K * j.pstride + (size_t)3 * j.nw * p, j.pstride,
+                        j.ctl, slot, lb - j.block0);
 }
 
 // The stopping test of the SU(3) site form's FP32 phase.  The functional
@@ -995,57 +1038,38 @@ __global__ __launch_bounds__(kBlock) void k_su3_site_F(const float* __restrict__
 __global__ __launch_bounds__(kBlock) void k_su3_site_F_batch(const BatchJob* __restrict__ jobs,
                                                              int njobs) {
   const BatchJob& j = jobs[batch_job(jobs, njobs, (int)blockIdx.x)];
-  su3_site_F_block((const float*)j.U, (const float*)j.S, j.sstride, j.ncopies, j.g, j.part,
-                   j.pstride, j.ctl, blockIdx.x - j.block0);
+  su3_site_F_block((const float*)j.U, (const float*)j.S, j.sstride, j.ncopies, j.g,
+                   j.part + (size_t)kSlots * j.ncopies * j.pstride, j.pstride, j.ctl,
+                   blockIdx.x - j.block0);
 }
 // the F of that evaluation (nparts triples at part) tested against the
 // previous one: converged when their difference per sweep is below tol
 // (apply_rebuild); the sweep count does not change
-__device__ __forceinline__ void reduce_check(const double* __restrict__ part, int nparts,
-                                             double invF, JobCtl* ctl) {
-  if (ctl->done) return;
-  double a, b, c;
-  reduce_triples(part, nparts, a, b, c);
-  if (threadIdx.x == 0) apply_rebuild(a * invF, ctl);
-}
-__global__ __launch_bounds__(kRedBlock) void k_reduce_check(const double* __restrict__ part,
-                                                            size_t pstride, int nparts,
-                                                            double invF, JobCtl* ctl) {
-  reduce_check(part + (size_t)blockIdx.x * pstride, nparts, invF, ctl + blockIdx.x);
-}
-__global__ __launch_bounds__(kRedBlock) void k_reduce_check_batch(const BatchJob* __restrict__ jobs) {
-  const BatchJob& j = jobs[blockIdx.x];
-  reduce_check(j.part, j.nw, j.invF, j.ctl);
-}
-
-void launch_su3_site_check(const float* U, const float* G, size_t gstride, int ncopies,
-                           const Geom& g, double* part, size_t pstride, double invF, JobCtl* ctl,
-                           hipStream_t st) {
+// F partials of every copy (check region of the partial buffer); the
+// period reduction (launch_reduce_period_copies / _batch) applies the test
+void launch_su3_site_F(const float* U, const float* G, size_t gstride, int ncopies,
+                       const Geom& g, double* part, size_t pstride, const JobCtl* ctl,
+                       hipStream_t st) {
   const int nb = (int)((site3_lanes(g) + kBlock - 1) / kBlock);
-  const int nw = (int)((site3_lanes(g) + 63) / 64);
   hipLaunchKernelGGL(k_su3_site_F, dim3(nb), dim3(kBlock), 0, st, U, G, gstride, ncopies, g, part,
                      pstride, ctl);
-  hipLaunchKernelGGL(k_reduce_check, dim3(ncopies), dim3(kRedBlock), 0, st, part, pstride, nw,
-                     invF, ctl);
 }
-void launch_su3_site_check_batch(const BatchJob* jobs, int njobs, int nblocks_total,
-                                 const BatchJob* red, int nred, hipStream_t st) {
+void launch_su3_site_F_batch(const BatchJob* jobs, int njobs, int nblocks_total, hipStream_t st) {
   if (njobs == 0 || nblocks_total == 0) return;
   hipLaunchKernelGGL(k_su3_site_F_batch, dim3(nblocks_total), dim3(kBlock), 0, st, jobs, njobs);
-  hipLaunchKernelGGL(k_reduce_check_batch, dim3(nred), dim3(kRedBlock), 0, st, red);
 }
 
 template <typename T>
 void launch_su3_site_copies(const T* U, T* G, size_t gstride, int ncopies, const Geom& g,
                             int parity, double omega, double* part, size_t pstride,
-                            const JobCtl* ctl, hipStream_t st) {
+                            const JobCtl* ctl, hipStream_t st, int slot) {
   const int nb = (int)((site3_lanes(g) + kBlock - 1) / kBlock);
   const size_t nw = (site3_lanes(g) + 63) / 64;
-  double* pp = part + 3 * nw * parity;
+  double* pp = part + (size_t)slot * ncopies * pstride + 3 * nw * parity;
   if constexpr (sizeof(T) == 4) {
 #define L_(K)                                                                                  \
   hipLaunchKernelGGL((k_su3_site_set<K>), dim3(nb), dim3(kBlock), 0, st, U, G, gstride, g,     \
-                     parity, (float)omega, pp, pstride, ctl)
+                     parity, (float)omega, pp, pstride, ctl, slot)
     switch (ncopies) {
       case 1: L_(1); break;
       case 2: L_(2); break;
@@ -1064,13 +1088,13 @@ void launch_su3_site_copies(const T* U, T* G, size_t gstride, int ncopies, const
 }
 template void launch_su3_site_copies<float>(const float*, float*, size_t, int, const Geom&, int,
                                             double, double*, size_t, const JobCtl*,
-                                            hipStream_t);
+                                            hipStream_t, int);
 template void launch_su3_site_copies<double>(const double*, double*, size_t, int, const Geom&,
                                              int, double, double*, size_t, const JobCtl*,
-                                             hipStream_t);
+                                             hipStream_t, int);
 
 void launch_su3_site_copies_batch(int ncopies, bool fp32, const BatchJob* jobs, int njobs,
-                                  int nblocks_total, int parity, hipStream_t st) {
+                                  int nblocks_total, int parity, hipStream_t st, int slot) {
   if (njobs == 0 || nblocks_total == 0) return;
   if (!fp32) {
     hipLaunchKernelGGL(k_su3_site_copies_batch, dim3(nblocks_total), dim3(kBlock), 0, st, jobs,
@@ -1079,7 +1103,7 @@ void launch_su3_site_copies_batch(int ncopies, bool fp32, const BatchJob* jobs, 
   }
 #define L_(K)                                                                                  \
   hipLaunchKernelGGL((k_su3_site_set_batch<K>), dim3(nblocks_total), dim3(kBlock), 0, st, jobs, \
-                     njobs, parity)
+                     njobs, parity, slot)
   switch (ncopies) {
     case 1: L_(1); break;
     case 2: L_(2); break;
