@@ -432,12 +432,15 @@ LINK_TOL_MIXED = 10.0
 
 # SU(3) site form: its FP32 phase tests F, evaluated in FP64 every 10
 # sweeps, on the mean change per sweep since the previous evaluation and
-# hands over to FP64 at 2 tol; the FP64 sweeps then stop on the plain run's
-# rule.  (A lower hand-over runs the FP32 phase past the plain stopping
-# point; a higher one leaves longer FP64 polishes on slowly converging
-# lattices: 24^3x48 at omega 1.6 contracts the change per sweep by only
-# 0.9992.)
-SITE_TOL_MIXED = 2.0
+# hands over to FP64 at 1.2 tol; the FP64 sweeps then stop on the plain
+# run's rule.  Measured on the 13 SU(3) requests of the serving workload
+# (16^4 - 32^4, omega 1.6, tol 1e-11), each alone: hand-over at 2 / 1.5 /
+# 1.2 tol: 6.12 / 5.93 / 5.78 s, FP64 polishes of up to 2648 / 1458 / 618
+# sweeps, sweep counts the plain run's (+-1), max dF/F 9.7e-12 / 9.8e-12 /
+# 9.8e-12; at 1.0 tol the FP32 phase itself meets the plain rule and runs
+# to the end of its 10-sweep period past the plain stopping point (5-15
+# sweeps more, max dF/F 1.5e-10).
+SITE_TOL_MIXED = 1.2
 
 
 def run_batched(G, specs, max_sweeps, su3_precision, su3_warmup, su3_site=True,
