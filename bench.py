@@ -700,6 +700,7 @@ def main():
                 ("su3_32^4_site_fp32_1copy", 3, dims, G.MIXED, 1, 0.4, 1.6),
                 ("su3_32^4_site_fp32_4copies", 3, dims, G.MIXED, 4, 0.4, 1.6),
                 ("su3_32^4_site_fp64_1copy", 3, dims, G.F64, 1, 0.4, 1.6),
+                ("su3_48^3x24_site_fp32_1copy", 3, (48, 48, 48, 24), G.MIXED, 1, 0.4, 1.6),
             ]}
 
     if not args.no_extra and not args.kernels_only:

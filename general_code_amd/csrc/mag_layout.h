@@ -152,6 +152,40 @@ __device__ __forceinline__ M2<float> ldm_nt(LRef<const float> r) {
   return m;
 }
 
+// su2 links with the nontemporal hint
+__device__ __forceinline__ Q<float> ldq_nt(LRef<const float> r) {
+  const mag_f4v v = __builtin_nontemporal_load(reinterpret_cast<const mag_f4v*>(r.p));
+  return Q<float>{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ Q<double> ldq_nt(LRef<const double> r) {
+  const mag_d2v a = __builtin_nontemporal_load(reinterpret_cast<const mag_d2v*>(r.p));
+  const mag_d2v b = __builtin_nontemporal_load(reinterpret_cast<const mag_d2v*>(r.p + r.cs));
+  return Q<double>{a.x, a.y, b.x, b.y};
+}
+__device__ __forceinline__ M2<double> ldm_nt(LRef<const double> r) {
+  M2<double> m;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const mag_d2v d = __builtin_nontemporal_load(reinterpret_cast<const mag_d2v*>(r.p + k * r.cs));
+    m.r[k / 3][k % 3] = {d.x, d.y};
+  }
+  return m;
+}
+
+// The site-form kernels read every link once per half-sweep and the
+// neighbours' site variables up to 8 times.  Where the links of both
+// parities (8 Vh links of link_bytes) exceed the 256 MB MALL, the next
+// half-sweep reads them from HBM whatever the hint, and loading them with
+// the nontemporal hint leaves the caches to the site variables: per
+// half-sweep SU(2) 48^4 FP64 177 -> 158 us, FP32 87 -> 83 us, SU(3) FP64
+// 32^4 114 -> 105 us, SU(3) FP32 48^3x24 152 -> 126 us.  Where they fit, the
+// plain load lets the next half-sweep find them in the MALL (the hint cost
+// 10% at SU(3) FP32 32^4).
+constexpr size_t kMallBytes = (size_t)256 << 20;
+__device__ __forceinline__ bool stream_links(const Geom& g, size_t link_bytes) {
+  return (size_t)g.Vh * 8 * link_bytes > kMallBytes;
+}
+
 // ------------------------------------------------------------------ geometry
 struct SiteIdx {
   uint32_t x, y, z, t, row;

@@ -366,8 +366,8 @@ void launch_half_batch(int group, bool fp32, Shape sh, const BatchJob* jobs, int
 // rounding; the omega step is FP32 (spin_omega_f).
 template <typename TL>
 __device__ __forceinline__ Q<double> ld_link64(const TL* U, const Geom& g, int p, int mu,
-                                               uint32_t h) {
-  const Q<TL> q = ldq(link_ptr<4>(U, g, p, mu, h));
+                                               uint32_t h, bool nt) {
+  const Q<TL> q = nt ? ldq_nt(link_ptr<4>(U, g, p, mu, h)) : ldq(link_ptr<4>(U, g, p, mu, h));
   return Q<double>{(double)q.a0, (double)q.a1, (double)q.a2, (double)q.a3};
 }
 __device__ __forceinline__ V3 ld_spin64(const double* S, const Geom& g, int p, uint32_t h) {
@@ -383,12 +383,12 @@ __device__ __forceinline__ Adj adj_of_t(const Q<double>& A) {
   else return adj_of(A);
 }
 
-template <int K, typename TL>
-__device__ __forceinline__ void su2_copies_block(const TL* __restrict__ U,
-                                                 TL* __restrict__ S, size_t sstride,
-                                                 const Geom& g, int p, double omega,
-                                                 double* __restrict__ part, size_t pstride,
-                                                 const JobCtl* __restrict__ ctl, uint32_t lb) {
+template <int K, typename TL, bool NT>
+__device__ __forceinline__ void su2_copies_block_nt(const TL* __restrict__ U,
+                                                    TL* __restrict__ S, size_t sstride,
+                                                    const Geom& g, int p, double omega,
+                                                    double* __restrict__ part, size_t pstride,
+                                                    const JobCtl* __restrict__ ctl, uint32_t lb) {
 #pragma clang fp contract(off)
   bool act[K];
   bool live = false;
@@ -410,12 +410,12 @@ __device__ __forceinline__ void su2_copies_block(const TL* __restrict__ U,
 #pragma unroll
     for (int mu = 0; mu < 4; mu++) {
       const uint32_t hf = fwd_h(g, h, s, mu);
-      const Adj Mf = adj_of_t<TL>(ld_link64(U, g, p, mu, h));
+      const Adj Mf = adj_of_t<TL>(ld_link64(U, g, p, mu, h, NT));
 #pragma unroll
       for (int k = 0; k < K; k++)
         if (act[k]) add_M(H[k], Mf, ld_spin64(S + k * sstride, g, 1 - p, hf));
       const uint32_t hb = back_h(g, h, s, mu);
-      const Adj Mb = adj_of_t<TL>(ld_link64(U, g, 1 - p, mu, hb));
+      const Adj Mb = adj_of_t<TL>(ld_link64(U, g, 1 - p, mu, hb, NT));
 #pragma unroll
       for (int k = 0; k < K; k++)
         if (act[k]) add_MT(H[k], Mb, ld_spin64(S + k * sstride, g, 1 - p, hb));
@@ -450,6 +450,19 @@ __device__ __forceinline__ void su2_copies_block(const TL* __restrict__ U,
     if (act[k])  // uniform
       wave_partial(part + k * pstride, lb * kBlock + (threadIdx.x & ~63u), g.Vh, fs[k], ds[k],
                    dm[k]);
+}
+
+// links with the nontemporal hint where they exceed the MALL (uniform)
+template <int K, typename TL>
+__device__ __forceinline__ void su2_copies_block(const TL* __restrict__ U, TL* __restrict__ S,
+                                                 size_t sstride, const Geom& g, int p,
+                                                 double omega, double* __restrict__ part,
+                                                 size_t pstride, const JobCtl* __restrict__ ctl,
+                                                 uint32_t lb) {
+  if (stream_links(g, 4 * sizeof(TL)))
+    su2_copies_block_nt<K, TL, true>(U, S, sstride, g, p, omega, part, pstride, ctl, lb);
+  else
+    su2_copies_block_nt<K, TL, false>(U, S, sstride, g, p, omega, part, pstride, ctl, lb);
 }
 
 template <int K, typename TL>
@@ -810,14 +823,14 @@ __device__ __forceinline__ void site_finish(const Herm3<T>& H0, const Herm3<T>& 
 // FP64: the copies one after the other, each reading the links again (the
 // FP64 field matrices of every copy would not fit in registers); the FP64
 // phases are the short warm-up and polish.
-template <typename T>
-__device__ __forceinline__ void su3_site_copies_block(const T* __restrict__ U,
-                                                      T* __restrict__ G, size_t gstride,
-                                                      int ncopies, const Geom& g, int p,
-                                                      T omega, double* __restrict__ part,
-                                                      size_t pstride,
-                                                      const JobCtl* __restrict__ ctl,
-                                                      uint32_t lb) {
+template <typename T, bool NT>
+__device__ __forceinline__ void su3_site_copies_block_nt(const T* __restrict__ U,
+                                                         T* __restrict__ G, size_t gstride,
+                                                         int ncopies, const Geom& g, int p,
+                                                         T omega, double* __restrict__ part,
+                                                         size_t pstride,
+                                                         const JobCtl* __restrict__ ctl,
+                                                         uint32_t lb) {
   bool live = false;
   for (int c = 0; c < ncopies; c++) live |= !ctl[c].done;
   if (!live) return;  // uniform over the workgroup
@@ -833,9 +846,10 @@ __device__ __forceinline__ void su3_site_copies_block(const T* __restrict__ U,
 #pragma unroll 1
       for (int mu = 0; mu < 4; mu++) {
         const uint32_t hbm = back_h(g, h, s, mu);
-        const M2<T> Lf = ldm(link_ptr<12>(U, g, p, mu, h));
+        const M2<T> Lf = NT ? ldm_nt(link_ptr<12>(U, g, p, mu, h)) : ldm(link_ptr<12>(U, g, p, mu, h));
         const M2<T> Gf = ldm(site_ptr<12>(Gc, g, 1 - p, fwd_h(g, h, s, mu)));
-        const M2<T> Lb = ldm(link_ptr<12>(U, g, 1 - p, mu, hbm));
+        const M2<T> Lb =
+            NT ? ldm_nt(link_ptr<12>(U, g, 1 - p, mu, hbm)) : ldm(link_ptr<12>(U, g, 1 - p, mu, hbm));
         const M2<T> Gb = ldm(site_ptr<12>(Gc, g, 1 - p, hbm));
         C<T> u2[3];
         row2_fma(Lf, u2);
@@ -850,20 +864,34 @@ __device__ __forceinline__ void su3_site_copies_block(const T* __restrict__ U,
   }
 }
 
+// links with the nontemporal hint where they exceed the MALL (uniform)
+template <typename T>
+__device__ __forceinline__ void su3_site_copies_block(const T* __restrict__ U, T* __restrict__ G,
+                                                      size_t gstride, int ncopies, const Geom& g,
+                                                      int p, T omega, double* __restrict__ part,
+                                                      size_t pstride,
+                                                      const JobCtl* __restrict__ ctl,
+                                                      uint32_t lb) {
+  if (stream_links(g, 12 * sizeof(T)))
+    su3_site_copies_block_nt<T, true>(U, G, gstride, ncopies, g, p, omega, part, pstride, ctl, lb);
+  else
+    su3_site_copies_block_nt<T, false>(U, G, gstride, ncopies, g, p, omega, part, pstride, ctl, lb);
+}
+
 // FP32 with the K copies of the set inside the link loop: each link is
 // loaded, and its row 2 formed, once for all copies, which then add their
 // terms with their own neighbour gauges (with the copies outside, four
 // copies at 32^4 read 4.1 times the links of one: the 4 MB L2 of an XCD
 // does not keep them from one copy to the next).  K = 1 loads two
-// directions per pass (100 VGPRs, 5 waves per SIMD), more copies one.  A
+// directions per pass, more copies one.  A
 // copy's operations and their order are the same for every K.  Sets of
 // two or more copies load the links with the nontemporal hint (32^4, four
 // copies: 140 -> 126 us per half-sweep; the copies' neighbour gauges, not
 // the links, are then what the caches hold); one copy loads them plainly
-// (the hint cost 10% at 32^4, where the next half-sweep reads the links
-// again from the MALL).
-template <int K>
-__device__ __forceinline__ void su3_site_set_block(const float* __restrict__ U,
+// where the links fit the MALL (the hint cost 10% at 32^4, where the next
+// half-sweep reads the links again from the MALL), with the hint beyond it.
+template <int K, bool NT>
+__device__ __forceinline__ void su3_site_set_block_nt(const float* __restrict__ U,
                                                    float* __restrict__ G, size_t gstride,
                                                    const Geom& g, int p, float omega,
                                                    double* __restrict__ part, size_t pstride,
@@ -898,7 +926,7 @@ __device__ __forceinline__ void su3_site_set_block(const float* __restrict__ U,
         const int mu = m0 + k;
         hf[k] = fwd_h(g, h, s, mu);
         hb[k] = back_h(g, h, s, mu);
-        if constexpr (K == 1) {
+        if constexpr (!NT) {
           Lf[k] = ldm(link_ptr<12>(U, g, p, mu, h));
           Lb[k] = ldm(link_ptr<12>(U, g, 1 - p, mu, hb[k]));
         } else {
@@ -937,6 +965,21 @@ __device__ __forceinline__ void su3_site_set_block(const float* __restrict__ U,
     if (act[c])  // uniform
       wave_partial(part + (size_t)c * pstride, lb * kBlock + (threadIdx.x & ~63u), g.Vh, fs[c],
                    rs[c], rm[c]);
+}
+
+template <int K>
+__device__ __forceinline__ void su3_site_set_block(const float* __restrict__ U,
+                                                   float* __restrict__ G, size_t gstride,
+                                                   const Geom& g, int p, float omega,
+                                                   double* __restrict__ part, size_t pstride,
+                                                   const JobCtl* __restrict__ ctl, int slot,
+                                                   uint32_t lb) {
+  // both variants in one kernel (K = 1: 165 VGPRs, 3 waves per SIMD) ran
+  // 4% faster at 32^4 than the plain-load variant alone (100 VGPRs)
+  if (K > 1 || stream_links(g, 48))  // uniform
+    su3_site_set_block_nt<K, true>(U, G, gstride, g, p, omega, part, pstride, ctl, slot, lb);
+  else
+    su3_site_set_block_nt<K, false>(U, G, gstride, g, p, omega, part, pstride, ctl, slot, lb);
 }
 
 template <typename T>
