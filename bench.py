@@ -96,9 +96,13 @@ def measure(G, group, dims, precision, eps, omega, steps, warmup, seed=2024):
     bytes_launch = (V // 2) * b_site(group, mixed)
     achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
     step_ms = total_ms / steps
-    # the mixed FP32 phase: red halves accumulate the site gauge, black halves do not
-    knames = ([f"void mag::k_su{group}_half<float, true>", f"void mag::k_su{group}_half<float, false>"]
-              if mixed else [f"void mag::k_su{group}_half<double, false>"])
+    # the mixed FP32 phase: red halves accumulate the site gauge, black halves
+    # do not; lattices whose links exceed the 256 MB MALL run the streaming
+    # kernels (k_su2_half_st: SU(2) in both precisions, k_su3_half_st: SU(3) FP32)
+    link_bytes = 8 * (V // 2) * (4 if group == 2 else 12) * (4 if mixed else 8)
+    st = "_st" if link_bytes > (256 << 20) and (group == 2 or mixed) else ""
+    knames = ([f"void mag::k_su{group}_half{st}<float, true>", f"void mag::k_su{group}_half{st}<float, false>"]
+              if mixed else [f"void mag::k_su{group}_half{st}<double, false>"])
     traffic = pmc_traffic(knames, lattice_name(dims))
     return {
         "value": V * steps / (total_ms * 1e-3),

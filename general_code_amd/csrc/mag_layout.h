@@ -137,6 +137,16 @@ __device__ __forceinline__ void stm_stream(LRef<double> r, const M2<double>& m) 
   }
 }
 
+__device__ __forceinline__ void stm_stream(LRef<float> r, const M2<float>& m) {
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  const f4v a = {m.r[0][0].re, m.r[0][0].im, m.r[0][1].re, m.r[0][1].im};
+  const f4v b = {m.r[0][2].re, m.r[0][2].im, m.r[1][0].re, m.r[1][0].im};
+  const f4v c = {m.r[1][1].re, m.r[1][1].im, m.r[1][2].re, m.r[1][2].im};
+  __builtin_nontemporal_store(a, reinterpret_cast<f4v*>(r.p));
+  __builtin_nontemporal_store(b, reinterpret_cast<f4v*>(r.p + r.cs));
+  __builtin_nontemporal_store(c, reinterpret_cast<f4v*>(r.p + 2 * r.cs));
+}
+
 // The same load with the nontemporal hint: the links of an SU(3) site-form
 // set of several copies, each read once per half-sweep while the copies'
 // neighbour gauges are read by up to 8 sites each; marked for early
@@ -162,6 +172,15 @@ __device__ __forceinline__ Q<double> ldq_nt(LRef<const double> r) {
   const mag_d2v b = __builtin_nontemporal_load(reinterpret_cast<const mag_d2v*>(r.p + r.cs));
   return Q<double>{a.x, a.y, b.x, b.y};
 }
+__device__ __forceinline__ void stq_nt(LRef<float> r, const Q<float>& q) {
+  const mag_f4v v = {q.a0, q.a1, q.a2, q.a3};
+  __builtin_nontemporal_store(v, reinterpret_cast<mag_f4v*>(r.p));
+}
+__device__ __forceinline__ void stq_nt(LRef<double> r, const Q<double>& q) {
+  const mag_d2v a = {q.a0, q.a1}, b = {q.a2, q.a3};
+  __builtin_nontemporal_store(a, reinterpret_cast<mag_d2v*>(r.p));
+  __builtin_nontemporal_store(b, reinterpret_cast<mag_d2v*>(r.p + r.cs));
+}
 __device__ __forceinline__ M2<double> ldm_nt(LRef<const double> r) {
   M2<double> m;
 #pragma unroll
@@ -182,7 +201,7 @@ __device__ __forceinline__ M2<double> ldm_nt(LRef<const double> r) {
 // plain load lets the next half-sweep find them in the MALL (the hint cost
 // 10% at SU(3) FP32 32^4).
 constexpr size_t kMallBytes = (size_t)256 << 20;
-__device__ __forceinline__ bool stream_links(const Geom& g, size_t link_bytes) {
+__host__ __device__ inline bool stream_links(const Geom& g, size_t link_bytes) {
   return (size_t)g.Vh * 8 * link_bytes > kMallBytes;
 }
 

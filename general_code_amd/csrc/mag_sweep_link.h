@@ -99,12 +99,12 @@ __device__ __forceinline__ void wave_partial(double* __restrict__ part, uint32_t
 // normalising it in FP64); MODE HALF_REBUILD (black half closing an FP32
 // period): the swept links are replaced by links rebuilt from the master,
 // and the rebuilt links' F goes to part_rb.
-template <typename T, bool GA, int NT, int SPT, int MODE>
-__device__ __forceinline__ void su2_half_block(T* __restrict__ U, T* __restrict__ G,
-                                               const Geom& g, int p, T omega,
-                                               double* __restrict__ part, uint32_t lb,
-                                               const double* __restrict__ U0,
-                                               double* __restrict__ part_rb) {
+template <typename T, bool GA, int NT, int SPT, int MODE, bool ST>
+__device__ __forceinline__ void su2_half_block_st(T* __restrict__ U, T* __restrict__ G,
+                                                  const Geom& g, int p, T omega,
+                                                  double* __restrict__ part, uint32_t lb,
+                                                  const double* __restrict__ U0,
+                                                  double* __restrict__ part_rb) {
 #pragma unroll
   for (int k = 0; k < SPT; k++) {
   const uint32_t base = (lb * SPT + k) * NT;
@@ -117,8 +117,13 @@ __device__ __forceinline__ void su2_half_block(T* __restrict__ U, T* __restrict_
     Q<T> F[4], B[4];
 #pragma unroll
     for (int mu = 0; mu < 4; mu++) {
-      F[mu] = ldq(link_ptr<4>(U, g, p, mu, h));
-      B[mu] = ldq(link_ptr<4>(U, g, 1 - p, mu, hb[mu]));
+      if constexpr (ST) {
+        F[mu] = ldq_nt(link_ptr<4>((const T*)U, g, p, mu, h));
+        B[mu] = ldq_nt(link_ptr<4>((const T*)U, g, 1 - p, mu, hb[mu]));
+      } else {
+        F[mu] = ldq(link_ptr<4>(U, g, p, mu, h));
+        B[mu] = ldq(link_ptr<4>(U, g, 1 - p, mu, hb[mu]));
+      }
     }
     T a0 = T(0), a1 = T(0), a2 = T(0), b0 = T(0), b1 = T(0), b2 = T(0);
 #pragma unroll
@@ -156,8 +161,13 @@ __device__ __forceinline__ void su2_half_block(T* __restrict__ U, T* __restrict_
     }
 #pragma unroll
     for (int mu = 0; mu < 4; mu++) {
-      stq(link_ptr<4>(U, g, p, mu, h), F[mu]);
-      stq(link_ptr<4>(U, g, 1 - p, mu, hb[mu]), B[mu]);
+      if constexpr (ST) {
+        stq_nt(link_ptr<4>(U, g, p, mu, h), F[mu]);
+        stq_nt(link_ptr<4>(U, g, 1 - p, mu, hb[mu]), B[mu]);
+      } else {
+        stq(link_ptr<4>(U, g, p, mu, h), F[mu]);
+        stq(link_ptr<4>(U, g, 1 - p, mu, hb[mu]), B[mu]);
+      }
     }
     if constexpr (GA) {
       const LRef<T> gp = site_ptr<4>(G, g, p, h);
@@ -173,11 +183,29 @@ __device__ __forceinline__ void su2_half_block(T* __restrict__ U, T* __restrict_
   }
 }
 
+// Links past the MALL (8 Vh links of 4 T, 48^4 SU(2) in either precision)
+// are loaded and stored with the nontemporal hint: each is read and written
+// once per half-sweep and the next half-sweep reads it from HBM whatever the
+// hint (48^4 per half-sweep FP64 267 -> 251 us, mixed 133 -> 125 us); below,
+// the plain accesses let the next half-sweep hit the MALL.
+template <typename T, bool GA, int NT, int SPT, int MODE>
+__device__ __forceinline__ void su2_half_block(T* __restrict__ U, T* __restrict__ G,
+                                               const Geom& g, int p, T omega,
+                                               double* __restrict__ part, uint32_t lb,
+                                               const double* __restrict__ U0,
+                                               double* __restrict__ part_rb) {
+  if (stream_links(g, 4 * sizeof(T)))  // uniform
+    su2_half_block_st<T, GA, NT, SPT, MODE, true>(U, G, g, p, omega, part, lb, U0, part_rb);
+  else
+    su2_half_block_st<T, GA, NT, SPT, MODE, false>(U, G, g, p, omega, part, lb, U0, part_rb);
+}
+
 // the sweep of a slot is beyond the cap (or the job is done): return
 __device__ __forceinline__ bool half_skip(const JobCtl* __restrict__ ctl, int slot) {
   return ctl->done || ctl->sweep + slot >= ctl->max_sweeps;
 }
 
+// one job: the launcher picks the plain or the streaming kernel by lattice
 template <typename T, bool GA, int NT, int SPT, int MODE>
 __global__ __launch_bounds__(NT) void k_su2_half(T* __restrict__ U, T* __restrict__ G, Geom g,
                                                  int p, T omega, double* __restrict__ part,
@@ -185,12 +213,23 @@ __global__ __launch_bounds__(NT) void k_su2_half(T* __restrict__ U, T* __restric
                                                  const double* __restrict__ U0,
                                                  double* __restrict__ part_rb) {
   if (half_skip(ctl, slot)) return;
-  su2_half_block<T, GA, NT, SPT, MODE>(U, G, g, p, omega, part, blockIdx.x, U0, part_rb);
+  su2_half_block_st<T, GA, NT, SPT, MODE, false>(U, G, g, p, omega, part, blockIdx.x, U0,
+                                                 part_rb);
+}
+template <typename T, bool GA, int NT, int SPT, int MODE>
+__global__ __launch_bounds__(NT) void k_su2_half_st(T* __restrict__ U, T* __restrict__ G, Geom g,
+                                                    int p, T omega, double* __restrict__ part,
+                                                    const JobCtl* __restrict__ ctl, int slot,
+                                                    const double* __restrict__ U0,
+                                                    double* __restrict__ part_rb) {
+  if (half_skip(ctl, slot)) return;
+  su2_half_block_st<T, GA, NT, SPT, MODE, true>(U, G, g, p, omega, part, blockIdx.x, U0,
+                                                part_rb);
 }
 
 // batched: workgroup b runs block b - block0 of its job with the class's
 // launch shape (NT, SPT); the job's partials are per wave, so the shape
-// leaves its results unchanged
+// leaves its results unchanged; plain or streaming accesses per job
 template <typename T, bool GA, int NT, int SPT, int MODE>
 __global__ __launch_bounds__(NT) void k_su2_half_batch(const BatchJob* __restrict__ jobs,
                                                        int njobs, int p, int slot) {
@@ -265,7 +304,18 @@ void launch_su2_half(T* U, T* G, const Geom& g, int parity, double omega, double
     MAG_HALF_VARIANTS(k_su2_half, true, NT, SPT, grid, st, U, G, g, parity, (T)omega, pp,  \
                       ctl, o.slot, o.U0, o.part_rb);                                       \
   } while (0)
-  MAG_SHAPE_SWITCH(sh, L_)
+#define LS_(NT, SPT)                                                                         \
+  do {                                                                                       \
+    const dim3 grid((lanes + NT * SPT - 1) / (NT * SPT));                                    \
+    MAG_HALF_VARIANTS(k_su2_half_st, true, NT, SPT, grid, st, U, G, g, parity, (T)omega, pp, \
+                      ctl, o.slot, o.U0, o.part_rb);                                         \
+  } while (0)
+  if (stream_links(g, 4 * sizeof(T))) {
+    MAG_SHAPE_SWITCH(sh, LS_)
+  } else {
+    MAG_SHAPE_SWITCH(sh, L_)
+  }
+#undef LS_
 #undef L_
 }
 
@@ -399,12 +449,12 @@ __device__ __forceinline__ void group_broadcast(M2<double>& m, int q) {
 // HALF_RENORM); HALF_REBUILD recovers the black gauge from the swept
 // W(x,0) on the group's first lane, broadcasts it, and each lane rebuilds
 // its links from the master.
-template <typename T, bool GA, int NT, int SPT, int MODE>
-__device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict__ G,
-                                               const Geom& g, int p, T omega,
-                                               double* __restrict__ part, uint32_t lb,
-                                               const double* __restrict__ U0,
-                                               double* __restrict__ part_rb) {
+template <typename T, bool GA, int NT, int SPT, int MODE, bool ST>
+__device__ __forceinline__ void su3_half_block_st(T* __restrict__ U, T* __restrict__ G,
+                                                  const Geom& g, int p, T omega,
+                                                  double* __restrict__ part, uint32_t lb,
+                                                  const double* __restrict__ U0,
+                                                  double* __restrict__ part_rb) {
   constexpr int LS = kSu3Lanes, NL = 4 / kSu3Lanes;
 #pragma unroll
   for (int kk = 0; kk < SPT; kk++) {
@@ -426,8 +476,13 @@ __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict_
         const uint32_t hbm = mu == 0 ? hb[0] : mu == 1 ? hb[1] : mu == 2 ? hb[2] : hb[3];
         pf[k] = link_ptr<12>(U, g, p, mu, h);
         pb[k] = link_ptr<12>(U, g, 1 - p, mu, hbm);
-        F[k] = ldm(pf[k]);
-        B[k] = ldm(pb[k]);
+        if constexpr (ST) {
+          F[k] = ldm_nt(LRef<const T>{pf[k].p, pf[k].cs});
+          B[k] = ldm_nt(LRef<const T>{pb[k].p, pb[k].cs});
+        } else {
+          F[k] = ldm(pf[k]);
+          B[k] = ldm(pb[k]);
+        }
       }
       M2<T> Gx;
       const LRef<T> gp = site_ptr<12>(G, g, p, h);
@@ -476,7 +531,7 @@ __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict_
 #endif
 #pragma unroll
       for (int k = 0; k < NL; k++) {
-        if constexpr (sizeof(T) == 8) {
+        if constexpr (sizeof(T) == 8 || ST) {
           stm_stream(pf[k], F[k]);
           stm_stream(pb[k], B[k]);
         } else {
@@ -500,6 +555,27 @@ __device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict_
   }
 }
 
+// FP32 links past the MALL (48^3x24) are loaded and stored with the
+// nontemporal hint, as for SU(2): 212 -> 191 us per half-sweep.  FP64
+// links are always stored with it and always loaded plainly (the hinted
+// loads cost 11% at 32^4, 768 MiB of links).
+template <typename T>
+__host__ __device__ inline bool su3_stream(const Geom& g) {
+  return sizeof(T) == 4 && stream_links(g, 12 * sizeof(T));
+}
+template <typename T, bool GA, int NT, int SPT, int MODE>
+__device__ __forceinline__ void su3_half_block(T* __restrict__ U, T* __restrict__ G,
+                                               const Geom& g, int p, T omega,
+                                               double* __restrict__ part, uint32_t lb,
+                                               const double* __restrict__ U0,
+                                               double* __restrict__ part_rb) {
+  if (su3_stream<T>(g))  // uniform
+    su3_half_block_st<T, GA, NT, SPT, MODE, true>(U, G, g, p, omega, part, lb, U0, part_rb);
+  else
+    su3_half_block_st<T, GA, NT, SPT, MODE, false>(U, G, g, p, omega, part, lb, U0, part_rb);
+}
+
+// one job: the launcher picks the plain or the streaming kernel by lattice
 template <typename T, bool GA, int NT, int SPT, int MODE>
 __global__ __launch_bounds__(NT) void k_su3_half(T* __restrict__ U, T* __restrict__ G, Geom g,
                                                  int p, T omega, double* __restrict__ part,
@@ -507,7 +583,18 @@ __global__ __launch_bounds__(NT) void k_su3_half(T* __restrict__ U, T* __restric
                                                  const double* __restrict__ U0,
                                                  double* __restrict__ part_rb) {
   if (half_skip(ctl, slot)) return;
-  su3_half_block<T, GA, NT, SPT, MODE>(U, G, g, p, omega, part, blockIdx.x, U0, part_rb);
+  su3_half_block_st<T, GA, NT, SPT, MODE, false>(U, G, g, p, omega, part, blockIdx.x, U0,
+                                                 part_rb);
+}
+template <typename T, bool GA, int NT, int SPT, int MODE>
+__global__ __launch_bounds__(NT) void k_su3_half_st(T* __restrict__ U, T* __restrict__ G, Geom g,
+                                                    int p, T omega, double* __restrict__ part,
+                                                    const JobCtl* __restrict__ ctl, int slot,
+                                                    const double* __restrict__ U0,
+                                                    double* __restrict__ part_rb) {
+  if (half_skip(ctl, slot)) return;
+  su3_half_block_st<T, GA, NT, SPT, MODE, true>(U, G, g, p, omega, part, blockIdx.x, U0,
+                                                part_rb);
 }
 
 template <typename T, bool GA, int NT, int SPT, int MODE>
@@ -533,7 +620,18 @@ void launch_su3_half(T* U, T* G, const Geom& g, int parity, double omega, double
     MAG_HALF_VARIANTS(k_su3_half, false, NT, SPT, grid, st, U, G, g, parity, (T)omega, pp, \
                       ctl, o.slot, o.U0, o.part_rb);                                       \
   } while (0)
-  MAG_SHAPE_SWITCH(sh, L_)
+#define LS_(NT, SPT)                                                                          \
+  do {                                                                                        \
+    const dim3 grid((lanes + NT * SPT - 1) / (NT * SPT));                                     \
+    MAG_HALF_VARIANTS(k_su3_half_st, false, NT, SPT, grid, st, U, G, g, parity, (T)omega, pp, \
+                      ctl, o.slot, o.U0, o.part_rb);                                          \
+  } while (0)
+  if (su3_stream<T>(g)) {
+    MAG_SHAPE_SWITCH(sh, LS_)
+  } else {
+    MAG_SHAPE_SWITCH(sh, L_)
+  }
+#undef LS_
 #undef L_
 }
 
