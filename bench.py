@@ -557,6 +557,24 @@ def serve_workload(n=40, rate=20.0, seed=4):
     return reqs
 
 
+def serve_params(G, r, plain, max_sweeps, su3_tol_mixed=SITE_TOL_MIXED * 1e-11,
+                 su3_warmup=SU3_WARMUP, su2_warmup=SU2_WARMUP, su3_site=True):
+    """Job parameters of request r: plain FP64 link form, or the served
+    form (see serve)."""
+    p = G.MagParams(group=r["group"], dims=r["dims"], omega=r["omega"], tol=r["tol"],
+                    max_sweeps=max_sweeps)
+    if not plain:
+        p.precision = G.MIXED
+        if r["group"] == 2:
+            p.form, p.copies, p.tol_mixed = G.FORM_SPIN, 1, r["tol"]
+            p.fp64_warmup = su2_warmup
+        else:
+            p.tol_mixed, p.fp64_warmup = su3_tol_mixed, su3_warmup
+            if su3_site:
+                p.form, p.copies = G.FORM_SPIN, 1
+    return p
+
+
 def serve(G, reqs, plain, max_sweeps, su3_tol_mixed=SITE_TOL_MIXED * 1e-11, su3_warmup=SU3_WARMUP,
           su2_warmup=SU2_WARMUP, su3_site=True):
     """Replay the arrival stream against a MagServer.  plain: one job at a
@@ -566,18 +584,8 @@ def serve(G, reqs, plain, max_sweeps, su3_tol_mixed=SITE_TOL_MIXED * 1e-11, su3_
     functional summed in FP64), SU(3) in the site form (read-only links,
     one site gauge; su3_site=False: the link form)."""
     def params(r):
-        p = G.MagParams(group=r["group"], dims=r["dims"], omega=r["omega"], tol=r["tol"],
-                        max_sweeps=max_sweeps)
-        if not plain:
-            p.precision = G.MIXED
-            if r["group"] == 2:
-                p.form, p.copies, p.tol_mixed = G.FORM_SPIN, 1, r["tol"]
-                p.fp64_warmup = su2_warmup
-            else:
-                p.tol_mixed, p.fp64_warmup = su3_tol_mixed, su3_warmup
-                if su3_site:
-                    p.form, p.copies = G.FORM_SPIN, 1
-        return p
+        return serve_params(G, r, plain, max_sweeps, su3_tol_mixed, su3_warmup, su2_warmup,
+                            su3_site)
 
     with G.MagServer(max_active=1 if plain else 0) as srv:
         t0 = time.perf_counter()
@@ -604,7 +612,26 @@ def serve(G, reqs, plain, max_sweeps, su3_tol_mixed=SITE_TOL_MIXED * 1e-11, su3_
     }
 
 
-def serve_bench(G, n=40, max_sweeps=WORKLOAD_CAP):
+def serve_alone(G, reqs, max_sweeps):
+    """Every request run alone, one after the other, in the served form:
+    the time the served run would take if concurrency bought nothing."""
+    out = {}
+    for r in reqs:
+        with G.MagJob(params=serve_params(G, r, False, max_sweeps)) as j:
+            j.generate(r["eps"], r["seed"])
+            t0 = time.perf_counter()
+            j.run()
+            dt = time.perf_counter() - t0
+        key = f"su{r['group']}_{lattice_name(r['dims'])}"
+        e = out.setdefault(key, {"requests": 0, "s": 0.0})
+        e["requests"] += 1
+        e["s"] = round(e["s"] + dt, 4)
+    tot = lambda g: round(sum(v["s"] for k, v in out.items() if k.startswith(g)), 3)  # noqa: E731
+    return {"by_group_and_size": out, "su2_s": tot("su2"), "su3_s": tot("su3"),
+            "total_s": tot("su")}
+
+
+def serve_bench(G, n=40, max_sweeps=WORKLOAD_CAP, alone=False):
     reqs = serve_workload(n)
     plain_res, plain = serve(G, reqs, True, max_sweeps)
     res, served = serve(G, reqs, False, max_sweeps)
@@ -630,6 +657,7 @@ def serve_bench(G, n=40, max_sweeps=WORKLOAD_CAP):
         "throughput_ratio_converged_only":
             (served["configs_per_s_converged"] / plain["configs_per_s_converged"])
             if plain["configs_per_s_converged"] else None,
+        **({"served_form_alone": serve_alone(G, reqs, max_sweeps)} if alone else {}),
     }
 
 
@@ -645,6 +673,9 @@ def main():
                     help="extra lines: the sweep and site-form kernels only (profiling runs)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-alone", action="store_true",
+                    help="also run every served request alone in its served form "
+                         "(serving.served_form_alone)")
     ap.add_argument("--serve-requests", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -724,7 +755,7 @@ def main():
     served = None
     if not args.no_serve:
         print("[bench] serving", file=sys.stderr, flush=True)
-        served = serve_bench(G, args.serve_requests)
+        served = serve_bench(G, args.serve_requests, alone=args.serve_alone)
 
     cpu = None
     if not args.no_cpu:
