@@ -236,10 +236,13 @@ __global__ __launch_bounds__(kMonoBlock) void k_su3_plaquettes(const double* __r
 }
 
 // ------------------------------------------------------------------ currents
+// colour blockIdx.y: planes at P + 6 V colour, currents at J + 4 V colour
 __global__ __launch_bounds__(kMonoBlock) void k_monopole_currents(const double* __restrict__ P,
                                                                  Lat l, double* __restrict__ J) {
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kMonoBlock + threadIdx.x;
   if (s >= l.V) return;
+  P += (size_t)blockIdx.y * 6 * l.V;
+  J += (size_t)blockIdx.y * 4 * l.V;
   int c[4];
   coords_of(l, s, c);
   const size_t V = l.V;
@@ -279,9 +282,9 @@ void launch_currents(int group, const double* th, const Lat& l, double* P, doubl
     hipLaunchKernelGGL(k_monopole_currents, dim3(nb), dim3(kMonoBlock), 0, st, P, l, J);
   } else {
     hipLaunchKernelGGL(k_su3_plaquettes, dim3(nb), dim3(kMonoBlock), 0, st, th, l, P);
-    for (int k = 0; k < 3; k++)
-      hipLaunchKernelGGL(k_monopole_currents, dim3(nb), dim3(kMonoBlock), 0, st,
-                         P + (size_t)k * 6 * l.V, l, J + (size_t)k * 4 * l.V);
+    // the three colours in one launch (blockIdx.y): 24^3x48 angles +
+    // plaquettes + currents 133 -> 125 us against three launches
+    hipLaunchKernelGGL(k_monopole_currents, dim3(nb, 3), dim3(kMonoBlock), 0, st, P, l, J);
   }
 }
 

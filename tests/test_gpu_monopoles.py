@@ -77,7 +77,9 @@ def test_u1_host_angles_bit_exact(mag, oracle, dims):
     assert np.count_nonzero(np.round(J)) > 0
 
 
-@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (4, 6, 5, 3)])
+# L3 = 3 and 6: runs of t shorter than the kernel's run length, and a
+# partial last run
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (4, 6, 5, 3), (6, 4, 5, 6)])
 def test_su3_host_angles_bit_exact(mag, oracle, dims):
     rng = np.random.default_rng(12)
     n = 4 * int(np.prod(dims))
