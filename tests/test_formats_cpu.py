@@ -60,6 +60,29 @@ def test_su3_lexicographical_roundtrip(oracle, reader, tmp_path):
     assert np.array_equal(np.fromfile(out).reshape(-1, 18), links)
 
 
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_su3_qcdstag_roundtrip(oracle, reader, tmp_path, precision):
+    """The generic QCDSTAG pattern (indexing.h:249-284) for su3: direction
+    t first (mu1 = (mu + 1) % 4), the 18 reals of a link in their own order
+    (no reordering, unlike the su2 specialisation)."""
+    dims = (2, 4, 2, 6)
+    V = int(np.prod(dims))
+    links = oracle.gen_su3_config(dims, 0.4, 9)
+    if precision == "float":
+        links = links.astype(np.float32).astype(np.float64)
+    dt = np.float64 if precision == "double" else np.float32
+    L = links.reshape(V, 4, 18)
+    stag = np.zeros((4, V, 18))
+    for mu in range(4):
+        stag[(mu + 1) % 4] = L[:, mu, :]
+    p = tmp_path / "stag3.bin"
+    stag.astype(dt).tofile(p)
+    out = tmp_path / "out3.bin"
+    subprocess.run([reader, "3", *map(str, dims), str(p), "qcdstag", precision, str(out)],
+                   check=True)
+    assert np.array_equal(np.fromfile(out).reshape(-1, 18), links.reshape(-1, 18))
+
+
 def test_bad_file_reports_error(reader, tmp_path):
     p = tmp_path / "short.bin"
     np.zeros(10).tofile(p)
