@@ -294,6 +294,46 @@ def test_su3_site_copies_match_link_form(mag, oracle):
     assert np.max(np.abs(links[2] - W)) < 1e-11
 
 
+def test_su3_site_fp32_phase_cap_mid_period(mag):
+    """The FP32 phase of the SU(3) site form keeps its sweeps' partials in
+    slots reduced once per period of reunit_every sweeps.  With the sweep
+    cap inside a period (no warm-up, periods ending at 10, 20, ..., 50, cap
+    45) every copy stops exactly at the cap, and run(), sweeps in chunks
+    that end mid-period, and a batch record bit-identical histories."""
+    dims = (8, 8, 8, 8)
+
+    def make():
+        j = mag.MagJob(group=mag.SU3, dims=dims, form=mag.FORM_SPIN, copies=2, omega=1.6,
+                       tol=0.0, max_sweeps=45, precision=mag.MIXED, reunit_every=10,
+                       tol_mixed=0.0, fp64_warmup=0)
+        j.generate(0.4, 808)
+        return j
+
+    with make() as a:
+        a.run()
+        ref = [a.copy_history(c) for c in range(2)]
+        for c in range(2):
+            sw, conv, _ = a.copy_status(c)
+            assert sw == 45 and not conv
+            assert len(ref[c][0]) == 45
+            assert np.all(np.isfinite(ref[c][0]))
+    with make() as b:
+        for _ in range(7):
+            b.sweeps(7)  # 49 sweeps asked for; the cap holds at 45
+        for c in range(2):
+            assert b.copy_status(c)[0] == 45
+            for x, y in zip(b.copy_history(c), ref[c]):
+                assert np.array_equal(x, y)
+    with make() as d, mag.MagBatch() as bt:
+        bt.add(d)
+        bt.run()
+        for c in range(2):
+            assert d.copy_status(c)[0] == 45
+            for x, y in zip(d.copy_history(c), ref[c]):
+                assert np.array_equal(x, y)
+        bt.remove(d)
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_su3_site_copies_batch_equals_alone(mag, prec):
     def make():
