@@ -175,9 +175,11 @@ def test_su3_fp64_sweeps_match_oracle(mag, oracle, omega):
 # against the oracle: SU(2) 32^4 (eps 0.5, omega 1.7) and SU(3) 24^3x48
 # (eps 0.4, omega 1.6); the gauge-invariant 1x1 and 4x4 Wilson loops of the
 # swept links equal those of the input.
+# 40^4: links past the 256 MB MALL (327 MB), the streaming kernel variant
 @pytest.mark.parametrize("group,dims,eps,omega,n", [
     (2, (32, 32, 32, 32), 0.5, 1.7, 30),
     (3, (24, 24, 24, 48), 0.4, 1.6, 20),
+    (2, (40, 40, 40, 40), 0.5, 1.7, 6),
 ])
 def test_full_size_fp64_sweeps_match_oracle(mag, oracle, group, dims, eps, omega, n):
     print(f"[full-size parity] SU({group}) {dims} omega {omega}, {n} sweeps", flush=True)
@@ -251,6 +253,32 @@ def test_su3_mixed_functional_close_to_fp64(mag, oracle):
     assert conv
     assert abs(Fm - F) / abs(F) < 1e-10
     assert abs(oracle.su3_plaquette(got, dims) - oracle.su3_plaquette(links, dims)) < 1e-12
+
+
+@pytest.mark.parametrize("group,dims,eps,omega", [
+    (2, (48, 48, 48, 48), 0.5, 1.7),
+    (3, (40, 40, 40, 24), 0.4, 1.6),
+])
+def test_streaming_mixed_sweeps_stay_gauge_equivalent(mag, group, dims, eps, omega):
+    """Mixed link form on lattices whose FP32 links exceed the 256 MB MALL
+    (SU(2) 48^4 340 MB, SU(3) 40^3x24 295 MB), where the half-sweeps load
+    and store the links with the nontemporal hint: after two periods of FP32
+    sweeps with FP64 rebuilds the final FP64 links are gauge-equivalent to
+    the input (plaquette to 1e-12) and the functional has gone up."""
+    print(f"[streaming mixed] SU({group}) {dims}", flush=True)
+    with mag.MagJob(group=group, dims=dims, precision=mag.MIXED, omega=omega, tol=0.0,
+                    max_sweeps=40, reunit_every=10, tol_mixed=0.0) as job:
+        job.generate(eps, 31)
+        P0 = job.plaquette()
+        F0 = job.functional()
+        job.sweeps(20)
+        job.finalize()
+        P1 = job.plaquette()
+        F1 = job.functional()
+        s, _, _ = job.status()
+    assert s == 20
+    assert abs(P1 - P0) < 1e-12
+    assert F1 > F0
 
 
 def test_one_shot_functionals(mag, oracle):
