@@ -50,6 +50,50 @@ def test_copies_match_oracle_spin_sweeps(mag, oracle):
         assert np.max(np.abs(spins[c] - s)) < 1e-11, c
 
 
+def test_streaming_spin_sweeps_match_oracle(mag, oracle):
+    """SU(2) spin form at 48^4, FP64: the links (680 MB) exceed the 256 MB
+    MALL and the half-sweeps load them with the nontemporal hint.  The
+    sweeps equal the oracle's: functional per sweep to 1e-12, spins to
+    1e-10 (the largest difference over 5.3M sites; the 8^4 test's is below
+    1e-11)."""
+    dims, n = (48, 48, 48, 48), 3
+    print(f"[streaming spin] SU(2) {dims}, {n} sweeps", flush=True)
+    with _spin_job(mag, 1, n_max=n, dims=dims) as job:
+        job.generate(EPS, SEED)
+        job.sweeps(n)
+        hist = job.copy_history(0)[0]
+        spins = job.copy_spins(0)
+    U0 = oracle.gen_su2_config(dims, EPS, SEED)
+    s = oracle.gen_copy_spins(dims, SEED, 0)
+    ref = []
+    for _ in range(n):
+        oracle.su2_spin_halfsweep(U0, s, dims, 0, 2, OMEGA)
+        oracle.su2_spin_halfsweep(U0, s, dims, 1, 2, OMEGA)
+        ref.append(oracle.su2_functional_spin(U0, s, dims))
+    assert _rel(hist, ref) < 1e-12
+    assert np.max(np.abs(spins - s)) < 1e-10
+
+
+def test_streaming_su3_site_fp32_gauge_equivalent(mag):
+    """SU(3) site form, FP32 phase, at 40^3x24: FP32 links of 295 MB, past
+    the MALL, loaded with the nontemporal hint.  After 20 sweeps the copy's
+    links G U G^+ (gauges put on SU(3) in FP64) keep the input's plaquette
+    to 1e-12, and the functional has gone up."""
+    dims = (40, 40, 40, 24)
+    print(f"[streaming site] SU(3) {dims}", flush=True)
+    with mag.MagJob(group=mag.SU3, dims=dims, form=mag.FORM_SPIN, copies=1, omega=1.6, tol=0.0,
+                    max_sweeps=40, precision=mag.MIXED, reunit_every=10, tol_mixed=0.0,
+                    fp64_warmup=0) as j:
+        j.generate(0.4, 45)
+        P0 = j.plaquette()
+        j.sweeps(20)
+        assert j.fp64_start() < 0  # still in the FP32 phase
+        F = j.copy_history(0)[0]
+        links = j.download_copy(0)
+    assert len(F) == 20 and F[-1] > F[0]
+    assert abs(mag.plaquette(mag.SU3, dims, links) - P0) < 1e-12
+
+
 def test_copies_match_link_form_copies(mag):
     n = 40
     with _spin_job(mag, 3, n_max=n) as job:
