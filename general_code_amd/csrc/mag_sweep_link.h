@@ -99,15 +99,15 @@ __device__ __forceinline__ void wave_partial(double* __restrict__ part, uint32_t
 // normalising it in FP64); MODE HALF_REBUILD (black half closing an FP32
 // period): the swept links are replaced by links rebuilt from the master,
 // and the rebuilt links' F goes to part_rb.
-// SPT tiles per thread with every tile's links loaded before the first
-// update (no two sites of one parity share a link, so a later tile's loads
-// never alias an earlier tile's stores): the loads of the second site are
-// in flight while the first is updated.  Measured at 32^4 against the same
-// code tile by tile: FP64 256x2 40.1 us per half-sweep (best one-tile shape
-// 41.0), at 181 VGPRs (2 waves per SIMD).  FP64 only: the FP64 kernels are
-// compiled without contraction, so the two codes agree bit for bit; the FP32
-// kernels' fused multiply-adds came out differently here (FP32 128x2 ran
-// 21.8 us against 23.2, but its sweeps differed from the other shapes').
+// Plain half-sweeps (every shape): SPT tiles per thread with every tile's
+// links loaded before the first update (no two sites of one parity share a
+// link, so a later tile's loads never alias an earlier tile's stores), so
+// the loads of the second site are in flight while the first is updated.
+// Measured at 32^4 against the same update tile by tile: FP64 256x2 40.0 us
+// per half-sweep (best one-tile shape 41.0; 181 VGPRs, 2 waves per SIMD),
+// FP32 128x2 21.8 us (23.2; 96 VGPRs).  All shapes of a plain half-sweep run
+// this code: the FP32 kernels are compiled with contraction, and the
+// one-tile code's fused multiply-adds came out differently from these.
 template <typename T, bool GA, int NT, int SPT, bool ST>
 __device__ __forceinline__ void su2_half_tiles(T* __restrict__ U, T* __restrict__ G,
                                                const Geom& g, int p, T omega,
@@ -184,7 +184,7 @@ __device__ __forceinline__ void su2_half_block_st(T* __restrict__ U, T* __restri
                                                   double* __restrict__ part, uint32_t lb,
                                                   const double* __restrict__ U0,
                                                   double* __restrict__ part_rb) {
-  if constexpr (sizeof(T) == 8 && SPT > 1 && MODE == HALF_PLAIN) {
+  if constexpr (MODE == HALF_PLAIN) {
     su2_half_tiles<T, GA, NT, SPT, ST>(U, G, g, p, omega, part, lb);
     return;
   }
