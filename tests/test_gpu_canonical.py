@@ -72,3 +72,35 @@ def test_track_gauge_arguments(mag):
         job.generate(0.3, 1)
         with pytest.raises(mag.MagError):
             batch.add(job)
+
+
+def test_tracked_gauge_every_shape(mag, oracle):
+    """Every launch shape accumulates the same gauge (the FP64 half-sweeps
+    with the tracked gauge, one or two tiles per thread): spins and
+    canonical links bit-identical across shapes and equal to the oracle's
+    spin-form sweeps."""
+    links = oracle.gen_su2_config(DIMS, 0.4, 63)
+    V = int(np.prod(DIMS))
+    n, omega = 8, 1.7
+    spins = np.tile([0.0, 0.0, 1.0], (V, 1))
+    for _ in range(n):
+        oracle.su2_spin_halfsweep(links, spins, DIMS, 0, 2, omega)
+        oracle.su2_spin_halfsweep(links, spins, DIMS, 1, 2, omega)
+    ref = None
+    try:
+        for nt, spt in mag.SHAPES:
+            print(f"[canonical] shape {nt}x{spt}", flush=True)
+            mag.tune_set(mag.TUNE_FIXED, nt, spt)
+            with mag.MagJob(group=mag.SU2, dims=DIMS, omega=omega, tol=0.0, max_sweeps=n,
+                            track_gauge=1) as job:
+                job.upload(links)
+                job.sweeps(n)
+                got = (job.download_spins(), job.download_canonical(), job.download())
+                assert job.shape() == (nt, spt)
+            assert np.max(np.abs(got[0] - spins)) < 1e-11
+            if ref is None:
+                ref = got
+            for x, y in zip(got, ref):
+                assert np.array_equal(x, y), (nt, spt)
+    finally:
+        mag.tune_set(mag.TUNE_AUTO)
