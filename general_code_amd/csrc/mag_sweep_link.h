@@ -90,15 +90,6 @@ __device__ __forceinline__ void wave_partial(double* __restrict__ part, uint32_t
 }
 
 // ================================================================== SU(2)
-// Tile k of workgroup lb of a half-sweep over the sites of parity p, one
-// site per lane; the 8 links of an su2 site fit in registers (3 waves/SIMD
-// in FP64, enough to stream at the copy rate), and one site per lane
-// measured faster than a lane pair.  The black (p = 1) half also sums the
-// functional of the links just written.  GA: the red half-sweep of the
-// mixed FP32 phase accumulates the site gauge (MODE HALF_RENORM: after
-// normalising it in FP64); MODE HALF_REBUILD (black half closing an FP32
-// period): the swept links are replaced by links rebuilt from the master,
-// and the rebuilt links' F goes to part_rb.
 // Plain half-sweeps (every shape): SPT tiles per thread with every tile's
 // links loaded before the first update (no two sites of one parity share a
 // link, so a later tile's loads never alias an earlier tile's stores), so
@@ -178,6 +169,14 @@ __device__ __forceinline__ void su2_half_tiles(T* __restrict__ U, T* __restrict_
   }
 }
 
+// Tile k of workgroup lb of a half-sweep over the sites of parity p, one
+// site per lane (one site per lane measured faster than a lane pair).  The
+// black (p = 1) half also sums the functional of the links just written.
+// GA: the red half-sweep of the mixed FP32 phase accumulates the site gauge
+// (MODE HALF_RENORM: after normalising it in FP64); MODE HALF_REBUILD
+// (black half closing an FP32 period): the swept links are replaced by
+// links rebuilt from the master, and the rebuilt links' F goes to part_rb.
+// Plain half-sweeps go to su2_half_tiles; this loop runs the other modes.
 template <typename T, bool GA, int NT, int SPT, int MODE, bool ST>
 __device__ __forceinline__ void su2_half_block_st(T* __restrict__ U, T* __restrict__ G,
                                                   const Geom& g, int p, T omega,
@@ -531,14 +530,6 @@ __device__ __forceinline__ void su3_store(LRef<T> r, const M2<T>& m) {
   else stm(r, m);
 }
 
-// One workgroup lb of an SU(3) half-sweep, LS = kSu3Lanes lanes per site:
-// lane q of the group holds the forward and backward links of the NL = 4/LS
-// directions mu = NL*q .. NL*q + NL - 1, the group exchanges the partial
-// fields with DPP and every lane derives the same g.  GA / MODE as for SU(2):
-// the red FP32 half accumulates the site gauge (renormalised first in
-// HALF_RENORM); HALF_REBUILD recovers the black gauge from the swept
-// W(x,0) on the group's first lane, broadcasts it, and each lane rebuilds
-// its links from the master.
 // Plain half-sweeps (every shape), as for SU(2): every tile's links (and
 // red gauge) are loaded before the first tile's update.  Against the
 // one-tile-at-a-time code, 32^4: FP64 256x1 135 us per half-sweep (137),
@@ -608,6 +599,15 @@ __device__ __forceinline__ void su3_half_tiles(T* __restrict__ U, T* __restrict_
   }
 }
 
+// One workgroup lb of an SU(3) half-sweep, LS = kSu3Lanes lanes per site:
+// lane q of the group holds the forward and backward links of the NL = 4/LS
+// directions mu = NL*q .. NL*q + NL - 1, the group exchanges the partial
+// fields with DPP and every lane derives the same g.  GA / MODE as for SU(2):
+// the red FP32 half accumulates the site gauge (renormalised first in
+// HALF_RENORM); HALF_REBUILD recovers the black gauge from the swept
+// W(x,0) on the group's first lane, broadcasts it, and each lane rebuilds
+// its links from the master.
+// Plain half-sweeps go to su3_half_tiles; this loop runs the other modes.
 template <typename T, bool GA, int NT, int SPT, int MODE, bool ST>
 __device__ __forceinline__ void su3_half_block_st(T* __restrict__ U, T* __restrict__ G,
                                                   const Geom& g, int p, T omega,
