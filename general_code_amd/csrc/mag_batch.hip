@@ -136,7 +136,8 @@ BatchJob describe(mag_job* j) {
   b.invV = j->invV;
   b.nb = j->nb;
   b.nw = j->nw;
-  b.nparts = 2 * j->nw;
+  b.nparts = j->spin ? 2 * j->nw : j->nw;  // link form: the black triples (red folded in)
+  b.roff = j->spin ? 0 : 3 * j->nw;
   b.block0 = 0;
   b.red0 = 0;
   b.S = (j->spin && fp32) ? (void*)j->S32 : (void*)j->S;  // FP32 phase: FP32 spins / gauges

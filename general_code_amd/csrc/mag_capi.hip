@@ -181,8 +181,9 @@ void flush_slots(mag_job* j, bool rebuild) {
                                      j->pstride, j->nw, j->ncopies, j->pending, rebuild ? 1 : 0,
                                      j->invF, j->ctl, j->hF, hs, j->st);
   } else {
-    mag::launch_reduce_slots(j->part, j->pstride, 2 * j->nw, j->pending, j->invF, j->invV,
-                             j->ctl, j->hF, j->hR, j->hA, j->st);
+    // the black triples (the red ones folded in) of each slot
+    mag::launch_reduce_slots(j->part + 3 * (size_t)j->nw, j->pstride, j->nw, j->pending,
+                             j->invF, j->invV, j->ctl, j->hF, j->hR, j->hA, j->st);
     mag::launch_reduce_period(j->part + (size_t)mag::kSlots * j->pstride, j->nw, j->pending,
                               rebuild ? 1 : 0, j->invF, j->ctl, j->hF, j->st);
   }
@@ -243,7 +244,8 @@ void enqueue_sweep(mag_job* j) {
                               j->ctl, j->hF, j->hR, j->hA, (size_t)j->prm.max_sweeps, j->st);
     return;
   }
-  mag::launch_reduce(j->part, 2 * j->nw, mag::RED_SWEEP, j->invF, j->invV, j->ctl, j->hF,
+  // link form: the black triples (the red ones folded in)
+  mag::launch_reduce(j->part + 3 * (size_t)j->nw, j->nw, mag::RED_SWEEP, j->invF, j->invV, j->ctl, j->hF,
                      j->hR, j->hA, nullptr, j->st);
 }
 

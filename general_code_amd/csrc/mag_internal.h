@@ -78,7 +78,9 @@ enum { RED_INIT = 0, RED_SWEEP = 1, RED_PLAIN = 2, RED_RESET = 3 };
 // ---------------------------------------------------------------- launchers
 // Partial buffers hold triples of doubles (F sum, residual sum, residual
 // max).  The half-sweep kernels write one triple per wave of 64 lanes
-// (sweep_waves below): a sweep uses 2*nw triples, parity p at 3*nw*p.
+// (sweep_waves below): a sweep uses 2*nw triples, parity p at 3*nw*p.  In
+// the link form the black half folds the red triple of each wave into its
+// own, and the sweep's reductions read the nw black triples at 3*nw.
 // nblocks(g) is the workgroup count of a kBlock-thread launch over Vh
 // sites (the functional, spin-form and copy launches).
 inline int nblocks(const Geom& g) { return (int)((g.Vh + kBlock - 1) / kBlock); }
@@ -252,7 +254,10 @@ struct BatchJob {
   double invV;
   int nb;      // workgroups of one half-sweep of this job (default shape)
   int nw;      // partial triples of one half-sweep (one per wave)
-  int nparts;  // partial triples of a sweep (2*nw); their fold triples follow them
+  int nparts;  // partial triples a sweep's reduction reads, from part + roff: the
+               // nw black triples (link form: they hold the red residuals too),
+               // or 2*nw (site forms); their fold triples follow them
+  int roff;    // offset in doubles of those triples from part (3*nw or 0)
   int block0;  // first workgroup of this job in a batched half-sweep grid
   int red0;    // first workgroup of this entry in a batched per-sweep reduction
   // spin form (gauge copies): spins of copy c at S + c*sstride, partials

@@ -165,7 +165,7 @@ __device__ __forceinline__ int red_entry(const BatchJob* __restrict__ jobs, int 
 __global__ __launch_bounds__(kRedBlock) void k_reduce_batch(const BatchJob* __restrict__ jobs,
                                                             int njobs) {
   const BatchJob& j = jobs[red_entry(jobs, njobs, (int)blockIdx.x)];
-  reduce_sweep2(j.part, j.nparts, j.invF, j.invV, j.ctl, j.hF, j.hR, j.hA,
+  reduce_sweep2(j.part + j.roff, j.nparts, j.invF, j.invV, j.ctl, j.hF, j.hR, j.hA,
                 (int)blockIdx.x - j.red0);
 }
 
@@ -260,7 +260,7 @@ __device__ __forceinline__ size_t slot_stride(const BatchJob& j) {
 __global__ __launch_bounds__(kRedBlock) void k_reduce_slots_batch(const BatchJob* __restrict__ jobs,
                                                                   int n) {
   const BatchJob& j = jobs[blockIdx.x / n];
-  reduce_slot(j.part, slot_stride(j), j.nparts, j.invF, j.invV, j.ctl, j.hF, j.hR, j.hA,
+  reduce_slot(j.part + j.roff, slot_stride(j), j.nparts, j.invF, j.invV, j.ctl, j.hF, j.hR, j.hA,
               (int)(blockIdx.x % n));
 }
 __global__ __launch_bounds__(kRedBlock) void k_reduce_period_batch(

@@ -88,6 +88,30 @@ __device__ __forceinline__ void wave_partial(double* __restrict__ part, uint32_t
     part[3 * w + 2] = m;
   }
 }
+// Link-form half-sweeps: the black half (p = 1) folds the residuals of the
+// red half's triple of the same wave (written by the previous launch, at
+// part - 3 nw, nw = the half-sweep's waves) into its own; the red half
+// sums no functional.  A sweep's reduction then reads the nw black triples
+// alone, half the partials (the functional's sum is unchanged: the red
+// triples' F is exactly zero).
+__device__ __forceinline__ void wave_partial_sweep(double* __restrict__ part, int p,
+                                                   uint32_t lane0, uint32_t nlanes, double f,
+                                                   double r, double m) {
+  f = wave_sum(f);
+  r = wave_sum(r);
+  m = wave_max(m);
+  if ((threadIdx.x & 63u) == 0 && lane0 < nlanes) {
+    const uint32_t w = lane0 >> 6;
+    if (p == 1) {
+      const double* red = part - 3 * (size_t)((nlanes + 63) / 64);
+      r += red[3 * w + 1];
+      m = fmax(m, red[3 * w + 2]);
+    }
+    part[3 * w] = f;
+    part[3 * w + 1] = r;
+    part[3 * w + 2] = m;
+  }
+}
 
 // ================================================================== SU(2)
 // Plain half-sweeps (every shape): SPT tiles per thread with every tile's
@@ -165,7 +189,7 @@ __device__ __forceinline__ void su2_half_tiles(T* __restrict__ U, T* __restrict_
       rs = (double)res;
       rm = (double)res;
     }
-    wave_partial(part, base + (threadIdx.x & ~63u), g.Vh, fs, rs, rm);
+    wave_partial_sweep(part, p, base + (threadIdx.x & ~63u), g.Vh, fs, rs, rm);
   }
 }
 
@@ -260,7 +284,7 @@ __device__ __forceinline__ void su2_half_block_st(T* __restrict__ U, T* __restri
     rs = (double)res;
     rm = (double)res;
   }
-  wave_partial(part, base + (threadIdx.x & ~63u), g.Vh, fs, rs, rm);
+  wave_partial_sweep(part, p, base + (threadIdx.x & ~63u), g.Vh, fs, rs, rm);
   if constexpr (MODE == HALF_REBUILD) wave_partial(part_rb, base + (threadIdx.x & ~63u), g.Vh, fr, 0.0, 0.0);
   }
 }
@@ -595,7 +619,7 @@ __device__ __forceinline__ void su3_half_tiles(T* __restrict__ U, T* __restrict_
         rm = (double)res;
       }
     }
-    wave_partial(part, base + (threadIdx.x & ~63u), LS * g.Vh, fs, rs, rm);
+    wave_partial_sweep(part, p, base + (threadIdx.x & ~63u), LS * g.Vh, fs, rs, rm);
   }
 }
 
@@ -713,7 +737,7 @@ __device__ __forceinline__ void su3_half_block_st(T* __restrict__ U, T* __restri
         rm = (double)res;
       }
     }
-    wave_partial(part, base + (threadIdx.x & ~63u), LS * g.Vh, fs, rs, rm);
+    wave_partial_sweep(part, p, base + (threadIdx.x & ~63u), LS * g.Vh, fs, rs, rm);
     if constexpr (MODE == HALF_REBUILD)
       wave_partial(part_rb, base + (threadIdx.x & ~63u), LS * g.Vh, fr, 0.0, 0.0);
   }
