@@ -686,8 +686,12 @@ __device__ __forceinline__ void su3_half_block_st(T* __restrict__ U, T* __restri
         for (int k = 0; k < NL; k++) fs += su3_link_F(F[k]) + su3_link_F(B[k]);
         const int mu0 = NL * q;
         const uint32_t hf0 = fwd_h(g, h, s, mu0);
+        // the master is read once per period: loaded with the nontemporal
+        // hint it leaves the MALL to the FP32 links and the red gauges the
+        // next half-sweeps read (32^4 per sweep 146.6 -> 144.4 us; the same
+        // hint cost SU(2) 4%, which keeps plain loads)
         M2<double> y = black_gauge_rows(to_d(F[0]), gs3(to_d(ldm(site_ptr<12>(G, g, 0, hf0)))),
-                                        ldm(link_ptr<12>(U0, g, 1, mu0, h)));
+                                        ldm_nt(link_ptr<12>(U0, g, 1, mu0, h)));
         group_broadcast<LS>(y, q);  // W(x,0) is on the group's first lane
         const M3<double> gx = gs3(y);
 #pragma unroll
@@ -695,7 +699,7 @@ __device__ __forceinline__ void su3_half_block_st(T* __restrict__ U, T* __restri
           const int mu = NL * q + k;
           const uint32_t hbm = mu == 0 ? hb[0] : mu == 1 ? hb[1] : mu == 2 ? hb[2] : hb[3];
           const M2<double> wf =
-              rebuild_link(gx, ldm(link_ptr<12>(U0, g, 1, mu, h)),
+              rebuild_link(gx, ldm_nt(link_ptr<12>(U0, g, 1, mu, h)),
                            gs3(to_d(ldm(site_ptr<12>(G, g, 0, fwd_h(g, h, s, mu))))));
           // each rebuilt link is stored at once: the swept links are dead
           // here (holding the rebuilt ones to the common stores below took
@@ -704,7 +708,7 @@ __device__ __forceinline__ void su3_half_block_st(T* __restrict__ U, T* __restri
           const double Ff = su3_link_F(wf);
           su3_store<T, ST>(link_ptr<12>(U, g, p, mu, h), to_f(wf));
           const M2<double> wb = rebuild_link(gs3(to_d(ldm(site_ptr<12>(G, g, 0, hbm)))),
-                                             ldm(link_ptr<12>(U0, g, 0, mu, hbm)), gx);
+                                             ldm_nt(link_ptr<12>(U0, g, 0, mu, hbm)), gx);
           fr += Ff + su3_link_F(wb);
           su3_store<T, ST>(link_ptr<12>(U, g, 1 - p, mu, hbm), to_f(wb));
         }
